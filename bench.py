@@ -1,0 +1,157 @@
+"""Benchmark: 5-D permutohedral joint bilateral at 2048x2048 FP32 on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+
+A step is one JointBilateral::apply (PERMUTOHEDRAL) of a 2048x2048 RGB image
+by an RGB guide through libgtlattice.so: positions built on the device,
+splat, sort, blur, slice and normalisation, inputs and output resident in
+HBM.  Prints one JSON line (metric, value in Mpix/s, roofline of the dominant
+kernel, CPU baseline of the reference on this host).
+"""
+import argparse
+import json
+import os
+import statistics
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Mpix/s, 5D permutohedral bilateral 2048² fp32; served p50/p99 ms at fixed QPS"
+HBM_PEAK_GBS = 8000.0
+WORKLOAD = ("joint bilateral 2048x2048 RGB: guide = 256 seeded rectangles + N(0,0.03), image = U[0,1]; "
+            "sigma_s=16, sigma_r=0.1; positions (r,g,b,x,y) d=5, values (r,g,b,1) vd=4; FP32")
+
+
+def kernel_bytes(kind: str, N: int, M: int, d: int, Cr: int, C: int, sv: int) -> int:
+    """Algorithmic HBM bytes of one launch of each lattice kernel (bilateral
+    form: positions are computed from the Cr guide planes, values are the C
+    image planes plus an implicit 1).  Neighbour value reads in the blur are
+    cache hits; probes, atomics, sort passes are not algorithmic."""
+    vd = C + 1
+    kw8 = 8 * ((d + 3) // 4)
+    if kind == "splat":
+        return 4 * N * (Cr + C) + M * (kw8 + 4 * vd)
+    if kind == "slice":
+        return 4 * N * (Cr + C) + M * sv * vd
+    if kind == "blur_pass":
+        return M * (2 * sv * vd + 8)
+    if kind == "nbr":
+        return M * (kw8 + 8 * (d + 1))
+    raise ValueError(kind)
+
+
+def cpu_baseline_leg(out_gpu, im, guide, ss, sr):
+    """Reference CPU timing (child process, 1 thread) and the GPU result's
+    error against the FP64 oracle and against the reference's own output."""
+    import numpy as np
+    from oracle import oracle as O
+    from tests.util import rel_err, rel_l2
+    env = dict(os.environ, OMP_NUM_THREADS="1")
+    r = subprocess.run([sys.executable, "-m", "oracle.cpu_baseline", "--side", str(im.shape[-1]), "--reps", "3"],
+                       cwd=ROOT, env=env, capture_output=True, text=True, timeout=600)
+    base = json.loads(r.stdout.strip().splitlines()[-1])
+    o64 = O.joint_bilateral(im, guide, ss, ss, 0.0, sr, mode=O.F64_LAT)
+    err = {"rel_err_vs_fp64": rel_err(out_gpu, o64), "rel_l2_vs_fp64": rel_l2(out_gpu, o64)}
+    if O.ref_available("strict"):
+        ref = O.ref_joint_bilateral(im, guide, ss, ss, 0.0, sr, kind="strict")
+        err["rel_err_ref_fp32_vs_fp64"] = rel_err(ref, o64)
+        err["rel_err_vs_reference_fp32"] = rel_err(out_gpu, ref)
+    cpu = {"value": round(base["mpix_s"], 4), "unit": "Mpix/s", "cores": base["cores"], "kind": base["kind"],
+           "sample": f"one {base['side']}x{base['side']} frame of the same workload, median of 3 after 1 warm-up, "
+                     f"OMP_NUM_THREADS=1 (the reference lattice is serial)",
+           "median_s": round(base["median_s"], 4)}
+    return cpu, err
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--side", type=int, default=2048)
+    ap.add_argument("--storage", default="f32", choices=["f32", "bf16", "f16"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(json.dumps({"error": f"--gpus {a.gpus}: this benchmark runs on exactly one GPU; "
+                                   "multi-GPU runs are not part of this build"}))
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from imagestack_amd import gtlattice, synth
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    im, guide, ss, sr = synth.cfg2(side=a.side)
+    C, F, H, W = im.shape
+    N = W * H * F
+    d_im = torch.from_numpy(im).cuda()
+    d_guide = torch.from_numpy(guide).cuda()
+    d_out = torch.empty_like(d_im)
+    torch.cuda.synchronize()
+
+    ctx = gtlattice.Context(0, profiling=1)
+    for _ in range(a.warmup):
+        ctx.joint_bilateral(d_im, d_guide, ss, ss, 0.0, sr, out=d_out, storage=a.storage)
+    torch.cuda.synchronize()
+    stats = []
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        _, st = ctx.joint_bilateral(d_im, d_guide, ss, ss, 0.0, sr, out=d_out, storage=a.storage, stats=True)
+        stats.append(st)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_step = (t1 - t0) * 1e3 / a.steps
+    value = N * a.steps / (t1 - t0) / 1e6
+    out_gpu = d_out.cpu().numpy()
+
+    # Counters (probes / inserts / atomics) in a separate, untimed call.
+    ctx.set_profiling(2)
+    _, cst = ctx.joint_bilateral(d_im, d_guide, ss, ss, 0.0, sr, out=d_out, storage=a.storage, stats=True)
+    ctx.set_profiling(1)
+
+    d = 5
+    M = stats[-1]["M"]
+    sv = 4 if a.storage == "f32" else 2
+    avg = {k: statistics.mean(s[k] for s in stats) for k in
+           ("ms_splat", "ms_sort", "ms_nbr", "ms_blur", "ms_slice", "ms_total")}
+    kernels = {
+        "splat": (avg["ms_splat"], kernel_bytes("splat", N, M, d, 3, 3, sv)),
+        "slice": (avg["ms_slice"], kernel_bytes("slice", N, M, d, 3, 3, sv)),
+        "blur_pass": (avg["ms_blur"] / (d + 1), kernel_bytes("blur_pass", N, M, d, 3, 3, sv)),
+        "nbr": (avg["ms_nbr"], kernel_bytes("nbr", N, M, d, 3, 3, sv)),
+    }
+    dom = max(kernels, key=lambda k: kernels[k][0] * (d + 1 if k == "blur_pass" else 1))
+    ms, nbytes = kernels[dom]
+    achieved = nbytes / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                "algorithmic_bytes": nbytes, "avg_launch_ms": round(ms, 5)}
+
+    line = {
+        "metric": METRIC, "value": round(value, 3), "unit": "Mpix/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": round(ms_step, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32" if a.storage == "f32" else f"f32 accumulate, {a.storage} storage",
+        "data": "synthetic",
+        "config": {"workload": WORKLOAD, "pixels": N, "d": d, "vd": 4, "storage": a.storage},
+        "roofline": roofline,
+        "stages_ms": {k: round(v, 4) for k, v in avg.items()},
+        "kernels": {k: {"ms": round(v[0], 5), "bytes": v[1], "GB/s": round(v[1] / (v[0] * 1e-3) / 1e9, 1)
+                        if v[0] > 0 else None} for k, v in kernels.items()},
+        "lattice": {"M": M, "capacity": stats[-1]["capacity"], "sort_passes": stats[-1]["sort_passes"],
+                    "probes": cst["probes"], "global_inserts": cst["inserts"], "atomics": cst["atomics"]},
+    }
+    if not a.no_cpu:
+        cpu, err = cpu_baseline_leg(out_gpu, im, guide, ss, sr)
+        line["cpu_baseline"] = cpu
+        line["gpu_over_cpu"] = round(value / cpu["value"], 1)
+        line["accuracy"] = err
+    ctx.close()
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,209 @@
+"""ctypes binding of libgtlattice.so (include/gtlattice.h).
+
+Device-pointer entry points take torch tensors on a ROCm device (torch is used
+only for device memory and streams); the *_host entry points take numpy
+arrays.  There is no CPU fallback: if the library or a GPU is missing, the
+calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgtlattice.so")
+
+GT_STORE_F32, GT_STORE_BF16, GT_STORE_F16 = 0, 1, 2
+STORAGE = {"f32": GT_STORE_F32, "bf16": GT_STORE_BF16, "f16": GT_STORE_F16}
+ERRORS = {1: "GT_EINVAL", 2: "GT_ENOMEM", 3: "GT_ERANGE", 4: "GT_EDEVICE", 5: "GT_ESTATE"}
+
+EXPORTS = [
+    "gt_ctx_create", "gt_ctx_destroy", "gt_ctx_set_stream", "gt_ctx_set_profiling", "gt_last_error",
+    "gt_gauss_transform", "gt_gauss_transform_host", "gt_joint_bilateral", "gt_joint_bilateral_host",
+    "gt_build_lattice", "gt_lattice_export",
+]
+
+
+class GTError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class Tuning(C.Structure):
+    _fields_ = [("block", C.c_int), ("load_factor", C.c_float), ("waves_per_cu", C.c_int),
+                ("dedup", C.c_int), ("tile_w", C.c_int)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("M", C.c_int), ("capacity", C.c_int), ("retries", C.c_int), ("sort_passes", C.c_int),
+                ("ms_splat", C.c_float), ("ms_sort", C.c_float), ("ms_nbr", C.c_float), ("ms_blur", C.c_float),
+                ("ms_slice", C.c_float), ("ms_total", C.c_float), ("probes", C.c_ulonglong),
+                ("inserts", C.c_ulonglong), ("atomics", C.c_ulonglong)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+_fp = C.POINTER(C.c_float)
+_vp = C.c_void_p
+
+
+def lib():
+    """Load libgtlattice.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run __graft_entry__.build() first")
+        L = C.CDLL(LIB_PATH)
+        L.gt_last_error.restype = C.c_char_p
+        L.gt_ctx_create.argtypes = [C.POINTER(_vp), C.c_int]
+        L.gt_ctx_destroy.argtypes = [_vp]
+        L.gt_ctx_destroy.restype = None
+        L.gt_ctx_set_stream.argtypes = [_vp, _vp]
+        L.gt_ctx_set_profiling.argtypes = [_vp, C.c_int]
+        sig = [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp, C.c_int, _vp, _fp, _vp, C.c_int,
+               C.POINTER(Tuning), C.POINTER(Stats)]
+        L.gt_gauss_transform.argtypes = sig
+        L.gt_gauss_transform_host.argtypes = sig
+        jsig = [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp, C.c_int, _vp, C.c_float, C.c_float,
+                C.c_float, C.c_float, _vp, C.c_int, C.POINTER(Tuning), C.POINTER(Stats)]
+        L.gt_joint_bilateral.argtypes = jsig
+        L.gt_joint_bilateral_host.argtypes = jsig
+        L.gt_build_lattice.argtypes = [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp, _fp, C.c_int,
+                                       C.POINTER(Tuning), C.POINTER(C.c_int)]
+        L.gt_lattice_export.argtypes = [_vp, _vp, _vp, _vp, _vp]
+        _lib = L
+    return _lib
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        raise GTError(rc, lib().gt_last_error().decode())
+
+
+def _dptr(t) -> int:
+    if t is None:
+        return None
+    if not t.is_cuda or not t.is_contiguous() or t.dtype.itemsize != 4:
+        raise ValueError("expected a contiguous 32-bit device tensor")
+    return t.data_ptr()
+
+
+def _tuning(tune) -> "Tuning | None":
+    if tune is None:
+        return None
+    if isinstance(tune, Tuning):
+        return tune
+    t = Tuning()
+    for k, v in dict(tune).items():
+        setattr(t, k, v)
+    return t
+
+
+class Context:
+    """One gt_ctx: device buffers reused across calls, one HIP stream."""
+
+    def __init__(self, device: int = 0, profiling: int = 0):
+        self._h = _vp()
+        _check(lib().gt_ctx_create(C.byref(self._h), device))
+        if profiling:
+            _check(lib().gt_ctx_set_profiling(self._h, int(profiling)))
+
+    def close(self) -> None:
+        if self._h:
+            lib().gt_ctx_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_profiling(self, level: int) -> None:
+        """0 off, 1 per-stage HIP-event times, 2 also probe/atomic counters."""
+        _check(lib().gt_ctx_set_profiling(self._h, int(level)))
+
+    def set_stream(self, stream_ptr: int | None) -> None:
+        _check(lib().gt_ctx_set_stream(self._h, stream_ptr))
+
+    # --- device-pointer API (torch tensors) -----------------------------
+    def gauss_transform(self, splat, values, inv_sigma, slice_pos=None, out=None, row_w: int = 0,
+                        storage: str = "f32", tune=None, stats: bool = False):
+        """splat: (d, N) device f32, values: (vd, N); returns (vd, Ns)."""
+        import torch
+        d, n = splat.shape[0], splat[0].numel()
+        vd = values.shape[0]
+        ns = n if slice_pos is None else slice_pos[0].numel()
+        if out is None:
+            out = torch.empty((vd, ns), dtype=torch.float32, device=splat.device)
+        inv = np.ascontiguousarray(inv_sigma, dtype=np.float32)
+        st = Stats()
+        _check(lib().gt_gauss_transform(self._h, n, row_w, d, vd, _dptr(splat), _dptr(values), ns,
+                                        _dptr(slice_pos), inv.ctypes.data_as(_fp), _dptr(out),
+                                        STORAGE[storage], _tuning(tune), C.byref(st)))
+        return (out, st.as_dict()) if stats else out
+
+    def joint_bilateral(self, im, ref, fw, fh, ff, color_sigma, out=None, storage: str = "f32",
+                        tune=None, stats: bool = False):
+        """im: (C, F, H, W) device f32, ref: (Cr, F, H, W); returns (C, F, H, W)."""
+        import torch
+        Cc, F, H, W = im.shape
+        if out is None:
+            out = torch.empty_like(im)
+        st = Stats()
+        _check(lib().gt_joint_bilateral(self._h, W, H, F, Cc, _dptr(im), ref.shape[0], _dptr(ref), fw, fh,
+                                        ff, color_sigma, _dptr(out), STORAGE[storage], _tuning(tune),
+                                        C.byref(st)))
+        return (out, st.as_dict()) if stats else out
+
+    def build_lattice(self, splat, values, inv_sigma, row_w: int = 0, storage: str = "f32", tune=None):
+        """Splat + blur and export: dict(keys (M,d) int16, nbr (d+1,M,2) int32,
+        splat (M,vd) f32, blur (M,vd) f32)."""
+        d, n = splat.shape[0], splat[0].numel()
+        vd = values.shape[0]
+        inv = np.ascontiguousarray(inv_sigma, dtype=np.float32)
+        M = C.c_int(0)
+        _check(lib().gt_build_lattice(self._h, n, row_w, d, vd, _dptr(splat), _dptr(values),
+                                      inv.ctypes.data_as(_fp), STORAGE[storage], _tuning(tune), C.byref(M)))
+        M = M.value
+        keys = np.zeros((M, d), np.int16)
+        nbr = np.zeros((d + 1, M, 2), np.int32)
+        sv = np.zeros((M, vd), np.float32)
+        bv = np.zeros((M, vd), np.float32)
+        _check(lib().gt_lattice_export(self._h, keys.ctypes.data, nbr.ctypes.data, sv.ctypes.data,
+                                       bv.ctypes.data))
+        return {"M": M, "keys": keys, "nbr": nbr, "splat": sv, "blur": bv}
+
+    # --- host-pointer API (numpy) -----------------------------------------
+    def gauss_transform_host(self, splat, values, inv_sigma, slice_pos=None, row_w: int = 0,
+                             storage: str = "f32", tune=None):
+        sp = np.ascontiguousarray(splat, dtype=np.float32)
+        va = np.ascontiguousarray(values, dtype=np.float32)
+        d = sp.shape[0]
+        n = sp[0].size
+        vd = va.shape[0]
+        sl = None if slice_pos is None else np.ascontiguousarray(slice_pos, dtype=np.float32)
+        ns = n if sl is None else sl[0].size
+        out = np.zeros((vd, ns), np.float32)
+        inv = np.ascontiguousarray(inv_sigma, dtype=np.float32)
+        st = Stats()
+        _check(lib().gt_gauss_transform_host(self._h, n, row_w, d, vd, sp.ctypes.data, va.ctypes.data, ns,
+                                             None if sl is None else sl.ctypes.data, inv.ctypes.data_as(_fp),
+                                             out.ctypes.data, STORAGE[storage], _tuning(tune), C.byref(st)))
+        return out
+
+    def joint_bilateral_host(self, im, ref, fw, fh, ff, color_sigma, storage: str = "f32", tune=None):
+        im = np.ascontiguousarray(im, dtype=np.float32)
+        ref = np.ascontiguousarray(ref, dtype=np.float32)
+        Cc, F, H, W = im.shape
+        out = np.zeros_like(im)
+        st = Stats()
+        _check(lib().gt_joint_bilateral_host(self._h, W, H, F, Cc, im.ctypes.data, ref.shape[0],
+                                             ref.ctypes.data, fw, fh, ff, color_sigma, out.ctypes.data,
+                                             STORAGE[storage], _tuning(tune), C.byref(st)))
+        return out

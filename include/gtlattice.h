@@ -1,0 +1,161 @@
+/*
+ * gtlattice.h - C ABI of the MI355X (gfx950) permutohedral-lattice Gauss
+ * transform library, libgtlattice.so.
+ *
+ * The library replaces the lattice stages of ImageStack's
+ * GaussTransform::apply, PERMUTOHEDRAL case (src/GaussTransform.cpp:163-226,
+ * driving src/Permutohedral.h) with hand-written HIP kernels:
+ *   splat  : elevation, enclosing simplex, barycentric weights
+ *            (src/Permutohedral.h:290-393), GPU open-addressing hash insert and
+ *            FP32 accumulation of weighted values;
+ *   sort   : compaction of the vertex keys and LSD radix sort, so vertex ids
+ *            are the lexicographic rank of the key (deterministic);
+ *   blur   : neighbour table, then d+1 Jacobi passes of [1/4 1/2 1/4]
+ *            (src/Permutohedral.h:430-488);
+ *   slice  : recompute simplex, gather, optional homogeneous normalisation
+ *            (src/Permutohedral.h:405-427, src/GaussTransform.cpp:673-676).
+ *
+ * Conventions
+ *   - Every function returns 0 on success and a nonzero GT_E* code on
+ *     failure; gt_last_error() then returns a message (thread-local).  No
+ *     function throws.  The C++ adapter turns a failure into ImageStack's
+ *     panic(), matching the reference's error behaviour.
+ *   - Images are dense planar float32: channel c of pixel p is at
+ *     base[c * plane + p] with plane = W*H*F and p = (t*H + y)*W + x, the
+ *     layout of a dense ImageStack Image (src/Image.h:34-37, :57-67).
+ *   - Functions without the _host suffix take DEVICE pointers and run on the
+ *     context's stream; they return after the results are complete.
+ *   - Limits: 1 <= d <= GT_MAX_D, 1 <= vd <= GT_MAX_VD, lattice key
+ *     coordinates within int16 (the reference's key type,
+ *     src/Permutohedral.h:28), N < 2^31.
+ */
+#ifndef GTLATTICE_H
+#define GTLATTICE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GT_MAX_D 16
+#define GT_MAX_VD 16
+
+enum {
+    GT_OK = 0,
+    GT_EINVAL = 1,      /* bad argument (sizes, dims, null pointers)          */
+    GT_ENOMEM = 2,      /* device allocation failed / over the memory budget  */
+    GT_ERANGE = 3,      /* lattice key outside int16, or NaN/Inf position     */
+    GT_EDEVICE = 4,     /* HIP runtime error                                  */
+    GT_ESTATE = 5       /* call out of order (e.g. export before build)       */
+};
+
+/* Storage type of lattice values.  Accumulation is always FP32. */
+typedef enum { GT_STORE_F32 = 0, GT_STORE_BF16 = 1, GT_STORE_F16 = 2 } gt_storage;
+
+/* Tuning knobs.  Zero fields select defaults (or the autotuner's choice). */
+typedef struct {
+    int block;           /* threads per workgroup of the pixel kernels: 64..1024   */
+    float load_factor;   /* target hash-table load, 0.05..0.7 (default 0.35)       */
+    int waves_per_cu;    /* occupancy of the persistent batch kernels (default 8)  */
+    int dedup;           /* workgroup LDS key dedup / pre-reduction: -1 auto, 0, 1 */
+    int tile_w;          /* pixel-tile width of a workgroup (default: whole rows)  */
+} gt_tuning;
+
+/* Per-call statistics (all optional). */
+typedef struct {
+    int M;               /* lattice vertices                                       */
+    int capacity;        /* hash-table slots used                                  */
+    int retries;         /* table regrowths in this call                           */
+    int sort_passes;     /* radix passes run                                       */
+    /* HIP-event times on the context's stream (profiling on):                   */
+    float ms_splat;      /* the splat kernel alone                                 */
+    float ms_sort;       /* compaction, host read-back of M, radix sort, finalize  */
+    float ms_nbr;        /* the neighbour-table kernel                             */
+    float ms_blur;       /* the d+1 blur-pass kernels                              */
+    float ms_slice;      /* the slice kernel alone                                 */
+    float ms_total;      /* first to last                                          */
+    unsigned long long probes;    /* hash probes (slot visits), splat+slice+nbr    */
+    unsigned long long inserts;   /* global insert attempts (after LDS dedup)      */
+    unsigned long long atomics;   /* global FP32 atomic adds issued by splat       */
+} gt_stats;
+
+typedef struct gt_ctx gt_ctx;
+
+/* Context: device buffers, stream, events.  One per host thread. */
+int gt_ctx_create(gt_ctx **ctx, int device);
+void gt_ctx_destroy(gt_ctx *ctx);
+/* Use an existing hipStream_t (NULL = the context's own stream). */
+int gt_ctx_set_stream(gt_ctx *ctx, void *hip_stream);
+/* 0: off; 1: per-stage HIP-event times in gt_stats; 2: also count hash
+ * probes, global inserts and FP32 atomics (slower). */
+int gt_ctx_set_profiling(gt_ctx *ctx, int level);
+const char *gt_last_error(void);
+
+/*
+ * GaussTransform::apply(slice, splat, values, sigmas, PERMUTOHEDRAL)
+ * (src/GaussTransform.cpp:115-226).
+ *   splat    : d planes of n pixels (device)     positions = splat * inv_sigma
+ *   values   : vd planes of n pixels (device)
+ *   slice    : d planes of n_slice pixels, or NULL to slice at the splat
+ *              positions (the reference's replay slice, :196-207)
+ *   inv_sigma: d host floats, 1.0f/sigma_c as the reference computes (:131)
+ *   out      : vd planes of n_slice pixels (device), unnormalised, as the
+ *              reference returns them (:194)
+ *   row_w    : image width W (pixels per row) for 2-D tiling, or 0
+ */
+int gt_gauss_transform(gt_ctx *ctx, int n, int row_w, int d, int vd,
+                       const float *splat, const float *values,
+                       int n_slice, const float *slice, const float *inv_sigma,
+                       float *out, gt_storage storage, const gt_tuning *tune,
+                       gt_stats *stats);
+
+/* Same with host pointers: copies in, runs, copies out. */
+int gt_gauss_transform_host(gt_ctx *ctx, int n, int row_w, int d, int vd,
+                            const float *splat, const float *values,
+                            int n_slice, const float *slice, const float *inv_sigma,
+                            float *out, gt_storage storage, const gt_tuning *tune,
+                            gt_stats *stats);
+
+/*
+ * JointBilateral::apply(im, ref, fw, fh, ff, colorSigma, PERMUTOHEDRAL)
+ * (src/GaussTransform.cpp:504-678) with the position image built on the
+ * device: (ref_c * (1.0f/colorSigma), x * (1.0f/fw), y * (1.0f/fh),
+ * t * (1.0f/ff)), values [im, 1], output im_c / homogeneous.  Zero spatial
+ * sigmas filter columns/rows/frames independently (:516-538) as one batch.
+ *   im  : C planes of W*H*F pixels (device); ref: Cr planes (device)
+ *   out : C planes (device); may alias im.
+ */
+int gt_joint_bilateral(gt_ctx *ctx, int W, int H, int F, int C, const float *im,
+                       int Cr, const float *ref, float fw, float fh, float ff,
+                       float color_sigma, float *out, gt_storage storage,
+                       const gt_tuning *tune, gt_stats *stats);
+
+int gt_joint_bilateral_host(gt_ctx *ctx, int W, int H, int F, int C, const float *im,
+                            int Cr, const float *ref, float fw, float fh, float ff,
+                            float color_sigma, float *out, gt_storage storage,
+                            const gt_tuning *tune, gt_stats *stats);
+
+/*
+ * Lattice inspection for parity checks: splat (and blur) the given points and
+ * keep the lattice in the context.  Replaces reading
+ * PermutohedralLattice::hashTable (src/Permutohedral.h:505-507).
+ */
+int gt_build_lattice(gt_ctx *ctx, int n, int row_w, int d, int vd,
+                     const float *splat, const float *values, const float *inv_sigma,
+                     gt_storage storage, const gt_tuning *tune, int *M_out);
+/*
+ * Copy the built lattice to host memory (any pointer may be NULL):
+ *   keys      : M x d int16, sorted lexicographically (signed)
+ *   nbr       : (d+1) x M x 2 int32, [axis][vertex] = {id(k+1-(d+1)e_j),
+ *               id(k-1+(d+1)e_j)} or -1 (src/Permutohedral.h:446-451)
+ *   splat_vals: M x vd float, values after splat (sorted order)
+ *   blur_vals : M x vd float, values after the d+1 blur passes
+ */
+int gt_lattice_export(gt_ctx *ctx, int16_t *keys, int32_t *nbr,
+                      float *splat_vals, float *blur_vals);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GTLATTICE_H */

@@ -1,0 +1,77 @@
+"""CPU baseline for bench.py: times ImageStack's own JointBilateral::apply
+(PERMUTOHEDRAL) on the host cores.  TEST/MEASUREMENT INFRASTRUCTURE ONLY.
+
+Kinds:
+  reference : oracle/_ref/libref_fast.so, the reference compiled from its own
+              sources with its own optimisation flags (-O3 -ffast-math -fopenmp,
+              portable ISA).  Used whenever it was built.
+  port      : oracle/liboracle.so, our strict scalar restatement (fallback).
+
+Run as a child process so the OpenMP thread count is fixed before the
+library loads:
+    OMP_NUM_THREADS=1 python -m oracle.cpu_baseline --side 2048 --reps 3
+Prints one JSON object.  With --procs P, P processes each filter their own
+copy concurrently (independent images, one per core) and the aggregate rate
+is reported.
+"""
+import argparse
+import json
+import os
+import statistics
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def run_one(side: int, reps: int, kind: str) -> dict:
+    from imagestack_amd import synth as S
+    from oracle import oracle as O
+    im, guide, ss, sr = S.cfg2(side=side)
+    if kind == "reference":
+        fn = lambda: O.ref_joint_bilateral(im, guide, ss, ss, 0.0, sr, kind="fast")  # noqa: E731
+    else:
+        fn = lambda: O.joint_bilateral(im, guide, ss, ss, 0.0, sr, mode=O.F32)  # noqa: E731
+    fn()  # warm-up
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn()
+        times.append(time.perf_counter() - t0)
+    med = statistics.median(times)
+    return {"median_s": med, "times_s": times, "mpix_s": side * side / med / 1e6}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--side", type=int, default=2048)
+    ap.add_argument("--reps", type=int, default=3)
+    ap.add_argument("--kind", default="auto", choices=["auto", "reference", "port"])
+    ap.add_argument("--procs", type=int, default=1)
+    a = ap.parse_args()
+    from oracle import oracle as O
+    kind = a.kind
+    if kind == "auto":
+        kind = "reference" if O.ref_available("fast") else "port"
+    if a.procs <= 1:
+        r = run_one(a.side, a.reps, kind)
+        r.update({"kind": kind, "cores": 1, "side": a.side})
+        print(json.dumps(r))
+        return
+    env = dict(os.environ, OMP_NUM_THREADS="1")
+    cmd = [sys.executable, "-m", "oracle.cpu_baseline", "--side", str(a.side), "--reps", str(a.reps),
+           "--kind", kind]
+    t0 = time.perf_counter()
+    procs = [subprocess.Popen(cmd, cwd=ROOT, env=env, stdout=subprocess.PIPE, text=True) for _ in range(a.procs)]
+    outs = [json.loads(p.communicate()[0].strip().splitlines()[-1]) for p in procs]
+    wall = time.perf_counter() - t0
+    agg = sum(o["mpix_s"] for o in outs)
+    print(json.dumps({"kind": kind, "cores": a.procs, "side": a.side, "mpix_s": agg, "wall_s": wall,
+                      "per_proc_mpix_s": [o["mpix_s"] for o in outs]}))
+
+
+if __name__ == "__main__":
+    main()

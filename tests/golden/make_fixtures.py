@@ -1,0 +1,128 @@
+"""Writes the golden fixtures in tests/golden/ from the reference itself,
+compiled from its own sources by oracle/Makefile (oracle/_ref/libref_strict.so,
+strict IEEE flags).  Run in a container that has the reference sources:
+
+    make -C oracle all && python tests/golden/make_fixtures.py
+
+Fixtures are data only: seeded inputs, reference outputs, sorted lattice keys
+and digests.  Small cases hold full arrays; the 512^2 and 2048^2 cases hold
+M and SHA-256 digests of the sorted key set and of the output.
+"""
+import hashlib
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from imagestack_amd import synth as S  # noqa: E402
+from oracle import oracle as O  # noqa: E402
+
+
+def digest(a) -> str:
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()[:16]
+
+
+def sorted_ref_lattice(pos, val, inv, blur):
+    keys, vals = O.ref_lattice(pos, val, inv, blur=blur)
+    order = O.sort_keys(keys)
+    return keys[order], vals[order]
+
+
+def gauss_testcase():
+    """GaussTransform::test shape (src/GaussTransform.cpp:39-77): 56x34x2,
+    3-D positions U[0,1], values [U[0,1], 1], sigmas (0.2, 0.3, 0.4), slice
+    positions distinct from splat positions (non-replay slice)."""
+    s = S.Stream(11)
+    W, H, F = 56, 34, 2
+    n = W * H * F
+    splat = s.uniform(3 * n).reshape(3, F, H, W).astype(np.float32)
+    slc = s.uniform(3 * n).reshape(3, F, H, W).astype(np.float32)
+    vals = np.stack([s.uniform(n), np.ones(n)]).reshape(2, F, H, W).astype(np.float32)
+    sig = np.array([0.2, 0.3, 0.4], np.float32)
+    out = O.ref_gauss_transform(splat, vals, sig, slice_pos=slc)
+    out_replay = O.ref_gauss_transform(splat, vals, sig)
+    inv = (np.float32(1) / sig).astype(np.float32)
+    keys, splat_v = sorted_ref_lattice(splat, vals, inv, blur=False)
+    _, blur_v = sorted_ref_lattice(splat, vals, inv, blur=True)
+    np.savez_compressed(os.path.join(HERE, "gausstransform_56x34x2.npz"), splat=splat, slice=slc, values=vals,
+                        sigmas=sig, out=out, out_replay=out_replay, keys=keys, splat_vals=splat_v,
+                        blur_vals=blur_v)
+
+
+def bilateral_small():
+    """64x48 RGB rectangles + N(0,0.05), -bilateral 0.1 8: d=5, vd=4."""
+    im = S.rect_image(64, 48, 3, K=8, noise=0.05, seed=64)
+    out = O.ref_joint_bilateral(im, im, 8.0, 8.0, 0.0, 0.1)
+    pos = S.bilateral_positions(im, 8.0, 0.1)
+    val = S.homogeneous_values(im)
+    inv = np.ones(5, np.float32)
+    raw = O.ref_gauss_transform(pos, val, np.ones(5, np.float32))
+    keys, splat_v = sorted_ref_lattice(pos, val, inv, blur=False)
+    _, blur_v = sorted_ref_lattice(pos, val, inv, blur=True)
+    np.savez_compressed(os.path.join(HERE, "bilateral_64x48.npz"), im=im, out=out, raw=raw, keys=keys,
+                        splat_vals=splat_v, blur_vals=blur_v)
+
+
+def jointbilateral_testcase():
+    """JointBilateral::test shape (src/GaussTransform.cpp:442-471): 60x50x3
+    frames, RGB image and guide U[0,1], sigmas (3, 4, 1), colour sigma 0.25:
+    a d=6 lattice with a temporal axis."""
+    s = S.Stream(12)
+    im = s.uniform(3 * 3 * 50 * 60).reshape(3, 3, 50, 60).astype(np.float32)
+    ref = s.uniform(3 * 3 * 50 * 60).reshape(3, 3, 50, 60).astype(np.float32)
+    out = O.ref_joint_bilateral(im, ref, 3.0, 4.0, 1.0, 0.25)
+    np.savez_compressed(os.path.join(HERE, "jointbilateral_60x50x3.npz"), im=im, ref=ref, out=out)
+
+
+def zero_sigma_case():
+    """sigma_y = 0: rows filtered independently (src/GaussTransform.cpp:524-530)."""
+    im = S.rect_image(40, 12, 3, K=6, noise=0.05, seed=21)
+    out = O.ref_joint_bilateral(im, im, 4.0, 0.0, 0.0, 0.2)
+    np.savez_compressed(os.path.join(HERE, "bilateral_rows_40x12.npz"), im=im, out=out)
+
+
+def highdim_small():
+    """d=16 Gauss transform at 32x32 (14 N(0,1) features + x/12, y/12)."""
+    pos = S.highdim_positions(32, 32, nfeat=14, sigma_s=12.0, seed=3)
+    s = S.Stream(4)
+    val = s.uniform(4 * 32 * 32).reshape(4, 1, 32, 32).astype(np.float32)
+    out = O.ref_gauss_transform(pos, val, np.ones(16, np.float32))
+    keys, _ = sorted_ref_lattice(pos, val, np.ones(16, np.float32), blur=False)
+    np.savez_compressed(os.path.join(HERE, "highdim_d16_32x32.npz"), pos=pos, values=val, out=out, keys=keys)
+
+
+def digests():
+    """cfg1 (512^2 bilateral) and cfg2 (2048^2 joint bilateral): M and digests."""
+    rec = {}
+    im, ref, ss, sr = S.cfg1()
+    pos = S.bilateral_positions(ref, ss, sr)
+    val = S.homogeneous_values(im)
+    keys, _ = sorted_ref_lattice(pos, val, np.ones(5, np.float32), blur=False)
+    out = O.ref_joint_bilateral(im, ref, ss, ss, 0.0, sr)
+    rec["cfg1_bilateral_512"] = {"M": int(keys.shape[0]), "keys_sha": digest(keys), "out_sha": digest(out),
+                                 "input_sha": digest(im)}
+    im, ref, ss, sr = S.cfg2()
+    pos = S.bilateral_positions(ref, ss, sr)
+    val = S.homogeneous_values(im)
+    keys, _ = sorted_ref_lattice(pos, val, np.ones(5, np.float32), blur=False)
+    rec["cfg2_jointbilateral_2048"] = {"M": int(keys.shape[0]), "keys_sha": digest(keys),
+                                       "guide_sha": digest(ref), "image_sha": digest(im)}
+    with open(os.path.join(HERE, "digests.json"), "w") as f:
+        json.dump(rec, f, indent=1, sort_keys=True)
+
+
+if __name__ == "__main__":
+    if not O.ref_available("strict"):
+        sys.exit("oracle/_ref/libref_strict.so is missing: run `make -C oracle all` with the reference present")
+    gauss_testcase()
+    bilateral_small()
+    jointbilateral_testcase()
+    zero_sigma_case()
+    highdim_small()
+    digests()
+    print("fixtures written to", HERE)

@@ -1,0 +1,344 @@
+"""Pins the oracle (oracle/lattice_oracle.cpp) before it is trusted as the
+checker of the GPU path.
+
+Each oracle function is pinned by at least one of:
+  (R) the reference compiled from its own sources (oracle/_ref, when built);
+  (G) golden fixtures written from that reference (tests/golden/);
+  (M) what the mathematics fixes: closed forms, invariants, special cases and
+      brute force, chosen so that a dropped term, wrong sign/index or
+      transposed operand fails.
+
+  simplex / vertex keys / weights ... (G) keys, (M) remainder classes, weight
+                                      sum, p=0, exact vertices, ties, brute force
+  scale factors ..................... (M) closed form
+  key set (splat) ................... (R)(G) sorted keys, (M) union of vertices
+  neighbour table ................... (M) closed form by dictionary lookup
+  splat values ...................... (G), (M) mass conservation
+  blur .............................. (G), (M) isolated vertex, linearity
+  slice / GaussTransform ............ (R)(G) bit-exact outputs
+  JointBilateral .................... (R)(G) bit-exact, incl. zero-sigma rows
+  FP64 oracle ....................... (M) agrees with FP32 to FP32 rounding,
+                                      constant field gives constant output,
+                                      1% agreement with brute-force Gaussian
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from imagestack_amd import synth as S
+from tests.util import digest, rel_err
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def load(name):
+    return np.load(os.path.join(GOLD, name), allow_pickle=False)
+
+
+def planar(a):
+    a = np.ascontiguousarray(a, np.float32)
+    return a.reshape(a.shape[0], -1)
+
+
+# --- simplex: closed forms and special cases -------------------------------
+
+def test_scale_factors_closed_form(oracle_lib):
+    for d in (1, 3, 5, 16):
+        sf = oracle_lib.scale_factors(d).astype(np.float64)
+        exact = (d + 1) * np.sqrt(2.0 / 3.0) / np.sqrt((np.arange(d) + 1.0) * (np.arange(d) + 2.0))
+        assert np.allclose(sf, exact, rtol=3e-7, atol=0)
+
+
+def test_origin_is_a_vertex(oracle_lib):
+    for d in (1, 2, 5, 16):
+        keys, w, g, r, ovf = oracle_lib.simplex(np.zeros(d, np.float32))
+        assert not ovf
+        assert np.all(g == 0)
+        assert w[0] == 1.0 and np.all(w[1:] == 0.0)
+        assert np.all(keys[0] == 0)
+
+
+@pytest.mark.parametrize("d", [1, 2, 3, 5, 8, 16])
+def test_simplex_invariants(oracle_lib, d):
+    rng = np.random.default_rng(d)
+    pts = (rng.normal(size=(400, d)) * 5).astype(np.float32)
+    for p in pts:
+        keys, w, g, rank, ovf = oracle_lib.simplex(p)
+        assert not ovf
+        # ranks are a permutation
+        assert sorted(rank.tolist()) == list(range(d + 1))
+        # vertex r: with the implicit coordinate -sum, every coordinate is = r mod (d+1)
+        full = np.concatenate([keys.astype(np.int64), -keys.sum(axis=1, keepdims=True)], axis=1)
+        for r in range(d + 1):
+            assert np.all(np.mod(full[r], d + 1) == r)
+        # d+1 distinct vertices, weights a partition of unity (FP32 rounding)
+        assert len({tuple(k) for k in keys.tolist()}) == d + 1
+        assert abs(float(w.astype(np.float64).sum()) - 1.0) < 1e-4
+        assert w.min() > -1e-4
+
+
+def test_exact_lattice_points(oracle_lib):
+    """A position that elevates exactly onto a remainder-0 vertex gets weight 1."""
+    d = 3
+    sf = oracle_lib.scale_factors(d).astype(np.float64)
+    # elevation of p is E p with E from src/Permutohedral.h:293-298; the
+    # remainder-0 point (4, -4, 0, 0) is reached from p solving E p = e.
+    # build E by accumulation to mirror the recurrence
+    E = np.zeros((d + 1, d))
+    E[d, d - 1] = -d * sf[d - 1]
+    for i in range(d - 1, 0, -1):
+        E[i] = E[i + 1]
+        E[i, i - 1] -= i * sf[i - 1]
+        E[i, i] += (i + 2) * sf[i]
+    E[0] = E[1]
+    E[0, 0] += 2 * sf[0]
+    target = np.array([4.0, -4.0, 0.0, 0.0])
+    p = np.linalg.lstsq(E, target, rcond=None)[0].astype(np.float32)
+    keys, w, g, r, ovf = oracle_lib.simplex(p)
+    assert w.max() > 1 - 1e-5
+    assert tuple(keys[int(np.argmax(w))]) == (4, -4, 0)
+
+
+def test_tie_rules_match_reference_semantics(oracle_lib):
+    """d = 1, elevation exactly e = (1, -1): both greedy roundings are ties
+    (up - e == e - down) and go down (strict <, src/Permutohedral.h:312),
+    giving g = (0, -2), sum = -1; the rank tie diff0 == diff1 increments the
+    later index (:324), rank = (0, 1); the fix-up (:337-347) lifts coordinate
+    0: g = (2, -2), rank = (1, 0).  Worked by hand: vertex 0 is key 2 with
+    weight 0 and vertex 1 is key 1 with weight 1.  Rounding ties up instead
+    would create a different zero-weight vertex (key 0)."""
+    sf = np.float32(oracle_lib.scale_factors(1)[0])
+    p = np.float32(1.0) / sf
+    found = None
+    for _ in range(64):
+        e1 = np.float32(np.float32(-1.0) * p) * sf
+        e0 = np.float32(e1 + np.float32(np.float32(2.0) * p) * sf)
+        if e0 == np.float32(1.0) and e1 == np.float32(-1.0):
+            found = p
+            break
+        p = np.nextafter(p, np.float32(np.inf) if e0 < 1 else np.float32(-np.inf))
+    if found is None:
+        pytest.skip("no float32 position elevates exactly to (1, -1)")
+    keys, w, g, r, _ = oracle_lib.simplex(np.array([found], np.float32))
+    assert g.tolist() == [2, -2]
+    assert r.tolist() == [1, 0]
+    assert keys[:, 0].tolist() == [2, 1]
+    assert w.tolist() == [0.0, 1.0]
+
+
+def test_vertex_union_is_key_set(oracle_lib):
+    """Brute force: the lattice key set is the union of every pixel's d+1
+    vertex keys."""
+    im = S.rect_image(48, 32, 3, K=6, noise=0.05, seed=7)
+    pos = planar(S.bilateral_positions(im, 6.0, 0.15))
+    val = planar(S.homogeneous_values(im))
+    inv = np.ones(5, np.float32)
+    keys, w = oracle_lib.simplex_batch(pos, inv)
+    uni = np.unique(keys.reshape(-1, 5), axis=0)
+    lat = oracle_lib.lattice(pos, val, inv)
+    assert lat["M"] == uni.shape[0]
+    assert np.array_equal(lat["keys"], uni[oracle_lib.sort_keys(uni)])
+
+
+def test_neighbour_table_closed_form(oracle_lib):
+    im = S.rect_image(40, 30, 3, K=5, noise=0.05, seed=8)
+    pos = planar(S.bilateral_positions(im, 5.0, 0.2))
+    val = planar(S.homogeneous_values(im))
+    lat = oracle_lib.lattice(pos, val, np.ones(5, np.float32))
+    d = 5
+    index = {tuple(k): i for i, k in enumerate(lat["keys"].tolist())}
+    for i in range(0, lat["M"], 7):
+        k = np.array(lat["keys"][i], np.int64)
+        for j in range(d + 1):
+            a = k + 1
+            b = k - 1
+            if j < d:
+                a[j] = k[j] - d
+                b[j] = k[j] + d
+            assert lat["nbr"][j, i, 0] == index.get(tuple(a.tolist()), -1)
+            assert lat["nbr"][j, i, 1] == index.get(tuple(b.tolist()), -1)
+
+
+def test_splat_mass_conservation(oracle_lib):
+    im = S.rect_image(48, 32, 3, K=6, noise=0.05, seed=9)
+    pos = planar(S.bilateral_positions(im, 6.0, 0.15))
+    val = planar(S.homogeneous_values(im))
+    lat = oracle_lib.lattice(pos, val, np.ones(5, np.float32), mode=oracle_lib.F64_LAT)
+    assert np.allclose(lat["splat"].sum(axis=0), val.astype(np.float64).sum(axis=1), rtol=1e-5)
+
+
+def exact_d1_position(oracle_lib):
+    """A float32 p whose d=1 elevation is exactly (1, -1) (vertex key 1)."""
+    sf = np.float32(oracle_lib.scale_factors(1)[0])
+    p = np.float32(1.0) / sf
+    for _ in range(64):
+        e1 = np.float32(np.float32(-1.0) * p) * sf
+        e0 = np.float32(e1 + np.float32(np.float32(2.0) * p) * sf)
+        if e0 == np.float32(1.0) and e1 == np.float32(-1.0):
+            return p
+        p = np.nextafter(p, np.float32(np.inf) if e0 < 1 else np.float32(-np.inf))
+    pytest.skip("no float32 position elevates exactly to (1, -1)")
+
+
+def test_blur_stencil_d1(oracle_lib):
+    """d = 1: both axes pair vertex k with k-1 and k+1 (src/Permutohedral.h:
+    446-451), so the two [1/4 1/2 1/4] passes turn a unit value at one vertex
+    into [1 4 6 4 1]/16 when the neighbouring vertices exist."""
+    sf = float(oracle_lib.scale_factors(1)[0])
+    bg = (np.arange(-40, 41, dtype=np.float64) * 0.5 / sf).astype(np.float32)
+    pts = np.concatenate([bg, [exact_d1_position(oracle_lib)]]).astype(np.float32)[None, :]
+    vals = np.zeros((1, pts.shape[1]), np.float32)
+    vals[0, -1] = 1.0
+    lat = oracle_lib.lattice(pts, vals, np.ones(1, np.float32), mode=oracle_lib.F64_LAT)
+    keys = lat["keys"][:, 0].tolist()
+    for k in range(-1, 4):
+        assert k in keys
+    spl = {k: lat["splat"][i, 0] for i, k in enumerate(keys)}
+    assert spl[1] == 1.0 and all(v == 0.0 for k, v in spl.items() if k != 1)
+    out = {k: lat["blur"][i, 0] for i, k in enumerate(keys)}
+    expect = {-1: 1 / 16, 0: 4 / 16, 1: 6 / 16, 2: 4 / 16, 3: 1 / 16}
+    for k, v in out.items():
+        assert v == pytest.approx(expect.get(k, 0.0), abs=1e-15)
+
+
+def test_blur_linearity(oracle_lib):
+    """blur(2u + 3v) = 2 blur(u) + 3 blur(v) on the same lattice."""
+    im = S.rect_image(32, 24, 3, K=4, noise=0.05, seed=10)
+    pos = planar(S.bilateral_positions(im, 4.0, 0.2))
+    rng = np.random.default_rng(0)
+    u = rng.uniform(size=(2, pos.shape[1])).astype(np.float32)
+    v = rng.uniform(size=(2, pos.shape[1])).astype(np.float32)
+    inv = np.ones(5, np.float32)
+    lu = oracle_lib.lattice(pos, u, inv, mode=oracle_lib.F64_LAT)
+    lv = oracle_lib.lattice(pos, v, inv, mode=oracle_lib.F64_LAT)
+    lw = oracle_lib.lattice(pos, (2 * u + 3 * v).astype(np.float32), inv, mode=oracle_lib.F64_LAT)
+    assert np.allclose(lw["blur"], 2 * lu["blur"] + 3 * lv["blur"], rtol=1e-6, atol=1e-9)
+
+
+# --- bit-exact against the reference (golden fixtures) ---------------------
+
+def test_golden_gausstransform_testcase(oracle_lib):
+    f = load("gausstransform_56x34x2.npz")
+    inv = (np.float32(1) / f["sigmas"]).astype(np.float32)
+    out, M = oracle_lib.gauss_transform(f["splat"], f["values"], inv, slice_pos=f["slice"])
+    assert np.array_equal(out.reshape(f["out"].shape), f["out"])
+    outr, _ = oracle_lib.gauss_transform(f["splat"], f["values"], inv)
+    assert np.array_equal(outr.reshape(f["out_replay"].shape), f["out_replay"])
+    lat = oracle_lib.lattice(f["splat"], f["values"], inv)
+    assert np.array_equal(lat["keys"], f["keys"])
+    assert np.array_equal(lat["splat"].astype(np.float32), f["splat_vals"])
+    assert np.array_equal(lat["blur"].astype(np.float32), f["blur_vals"])
+
+
+def test_golden_bilateral_small(oracle_lib):
+    f = load("bilateral_64x48.npz")
+    out = oracle_lib.joint_bilateral(f["im"], f["im"], 8.0, 8.0, 0.0, 0.1)
+    assert np.array_equal(out, f["out"])
+    pos = S.bilateral_positions(f["im"], 8.0, 0.1)
+    val = S.homogeneous_values(f["im"])
+    raw, M = oracle_lib.gauss_transform(pos, val, np.ones(5, np.float32))
+    assert np.array_equal(raw.reshape(f["raw"].shape), f["raw"])
+    lat = oracle_lib.lattice(pos, val, np.ones(5, np.float32))
+    assert np.array_equal(lat["keys"], f["keys"])
+    assert np.array_equal(lat["splat"].astype(np.float32), f["splat_vals"])
+    assert np.array_equal(lat["blur"].astype(np.float32), f["blur_vals"])
+
+
+def test_golden_jointbilateral_testcase(oracle_lib):
+    f = load("jointbilateral_60x50x3.npz")
+    out = oracle_lib.joint_bilateral(f["im"], f["ref"], 3.0, 4.0, 1.0, 0.25)
+    assert np.array_equal(out, f["out"])
+
+
+def test_golden_zero_sigma_rows(oracle_lib):
+    f = load("bilateral_rows_40x12.npz")
+    out = oracle_lib.joint_bilateral(f["im"], f["im"], 4.0, 0.0, 0.0, 0.2)
+    assert np.array_equal(out, f["out"])
+
+
+def test_golden_highdim(oracle_lib):
+    f = load("highdim_d16_32x32.npz")
+    out, M = oracle_lib.gauss_transform(f["pos"], f["values"], np.ones(16, np.float32))
+    assert np.array_equal(out.reshape(f["out"].shape), f["out"])
+    lat = oracle_lib.lattice(f["pos"], f["values"], np.ones(16, np.float32))
+    assert np.array_equal(lat["keys"], f["keys"])
+
+
+def test_golden_cfg1_digest(oracle_lib):
+    rec = json.load(open(os.path.join(GOLD, "digests.json")))["cfg1_bilateral_512"]
+    im, ref, ss, sr = S.cfg1()
+    assert digest(im) == rec["input_sha"]
+    pos = S.bilateral_positions(ref, ss, sr)
+    val = S.homogeneous_values(im)
+    lat = oracle_lib.lattice(pos, val, np.ones(5, np.float32))
+    assert lat["M"] == rec["M"]
+    assert digest(lat["keys"]) == rec["keys_sha"]
+    out = oracle_lib.joint_bilateral(im, ref, ss, ss, 0.0, sr)
+    assert digest(out) == rec["out_sha"]
+
+
+def test_reference_build_agrees(oracle_lib):
+    """(R): oracle vs the reference library on fresh seeded inputs."""
+    O = oracle_lib
+    if not O.ref_available("strict"):
+        pytest.skip("oracle/_ref not built (reference sources absent)")
+    for seed, (W, H) in enumerate([(33, 17), (70, 41)]):
+        im = S.rect_image(W, H, 3, K=5, noise=0.08, seed=100 + seed)
+        g = S.uniform_image(W, H, 3, seed=200 + seed)
+        assert np.array_equal(O.joint_bilateral(g, im, 3.0, 5.0, 0.0, 0.15),
+                              O.ref_joint_bilateral(g, im, 3.0, 5.0, 0.0, 0.15))
+        pos = S.bilateral_positions(im, 3.0, 0.15)
+        val = S.homogeneous_values(g)
+        a, _ = O.gauss_transform(pos, val, np.ones(5, np.float32))
+        b = O.ref_gauss_transform(pos, val, np.ones(5, np.float32))
+        assert np.array_equal(a.reshape(b.shape), b)
+
+
+# --- FP64 oracle ------------------------------------------------------------
+
+def test_f64_oracle_close_to_f32(oracle_lib):
+    O = oracle_lib
+    im = S.rect_image(64, 48, 3, K=8, noise=0.05, seed=64)
+    o32 = O.joint_bilateral(im, im, 8.0, 8.0, 0.0, 0.1, mode=O.F32)
+    o64 = O.joint_bilateral(im, im, 8.0, 8.0, 0.0, 0.1, mode=O.F64_LAT)
+    op = O.joint_bilateral(im, im, 8.0, 8.0, 0.0, 0.1, mode=O.F64_PURE)
+    assert rel_err(o32, o64) < 1e-5
+    assert rel_err(o64, op) < 1e-3        # FP32 elevation rounding only
+    assert not np.array_equal(o32, o64)   # FP64 really accumulates in FP64
+
+
+def test_constant_field_gives_constant_output(oracle_lib):
+    O = oracle_lib
+    im = S.rect_image(40, 30, 3, K=5, noise=0.05, seed=3)
+    const = np.full_like(im, 0.37)
+    out = O.joint_bilateral(const, im, 4.0, 4.0, 0.0, 0.1, mode=O.F64_LAT)
+    assert np.allclose(out, 0.37, rtol=1e-12)
+
+
+def nearly_equal(a, b):
+    """src/main.cpp:45-48 with src/macros.h:47-50: Stats(a-b) mean and
+    variance both below 0.01."""
+    diff = (np.asarray(a, np.float64) - np.asarray(b, np.float64)).ravel()
+    return abs(diff.mean()) < 0.01 and diff.var() < 0.01
+
+
+def test_reference_style_exact_vs_lattice(oracle_lib):
+    """GaussTransform::test (src/GaussTransform.cpp:39-77): the lattice agrees
+    with the brute-force Gauss transform in the homogeneous sense at the
+    reference's 1% statistical tolerance."""
+    O = oracle_lib
+    f = load("gausstransform_56x34x2.npz")
+    sig = f["sigmas"].astype(np.float64)
+    sp = planar(f["splat"]).astype(np.float64)
+    sl = planar(f["slice"]).astype(np.float64)
+    va = planar(f["values"]).astype(np.float64)
+    inv_var = 0.5 / sig ** 2
+    d2 = ((sl.T[:, None, :] - sp.T[None, :, :]) ** 2 * inv_var).sum(-1)
+    ex = np.exp(-d2) @ va.T
+    exact = ex[:, 0] / ex[:, 1]
+    inv = (np.float32(1) / f["sigmas"]).astype(np.float32)
+    out, _ = O.gauss_transform(f["splat"], f["values"], inv, slice_pos=f["slice"], mode=O.F64_LAT)
+    lattice = out[0] / out[1]
+    assert nearly_equal(lattice, exact)
