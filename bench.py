@@ -74,6 +74,8 @@ def main():
     ap.add_argument("--side", type=int, default=2048)
     ap.add_argument("--storage", default="f32", choices=["f32", "bf16", "f16"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="also run a plain device copy of MIB MiB (hardware-counter calibration)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({"error": f"--gpus {a.gpus}: this benchmark runs on exactly one GPU; "
@@ -92,6 +94,17 @@ def main():
     d_guide = torch.from_numpy(guide).cuda()
     d_out = torch.empty_like(d_im)
     torch.cuda.synchronize()
+
+    if a.calibrate_copy:
+        # A plain copy of known size: its FETCH_SIZE / WRITE_SIZE readings
+        # calibrate the counters against the lattice kernels'.
+        n = a.calibrate_copy * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     ctx = gtlattice.Context(0, profiling=1)
     for _ in range(a.warmup):

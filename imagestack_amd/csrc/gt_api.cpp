@@ -1,7 +1,13 @@
 // C-ABI implementation of libgtlattice.so: context, device buffers, the
 // splat -> sort -> blur -> slice pipeline, and the JointBilateral adapter.
-// Host arithmetic that feeds the kernels (scale factors, reciprocals) is
-// compiled with -ffp-contract=off and follows the reference's expressions.
+//
+// One call runs the whole pipeline on the context's stream with a single
+// host synchronisation at the end (the vertex count M stays on the device).
+// The end-of-call flags can ask for a redo: a table that filled up is grown,
+// a key outside the packing plan triggers a new plan from fresh position
+// bounds.  Host arithmetic that feeds the kernels (scale factors,
+// reciprocals) is compiled with -ffp-contract=off and follows the
+// reference's expressions.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -13,6 +19,7 @@
 #include <vector>
 
 #include "gt_internal.h"
+#include "gt_plan.h"
 #include "../../include/gtlattice.h"
 
 namespace {
@@ -29,15 +36,15 @@ int fail(int code, const char *fmt, ...) {
     return code;
 }
 
-#define GT_HIP(x)                                                                     \
-    do {                                                                              \
-        hipError_t e_ = (x);                                                          \
-        if (e_ != hipSuccess) return fail(GT_EDEVICE, "%s failed: %s", #x, hipGetErrorString(e_)); \
+#define GT_HIP(x)                                                                                      \
+    do {                                                                                               \
+        hipError_t e_ = (x);                                                                           \
+        if (e_ != hipSuccess) return fail(GT_EDEVICE, "%s failed: %s", #x, hipGetErrorString(e_));     \
     } while (0)
 
-#define GT_TRY(x)                  \
-    do {                           \
-        int rc_ = (x);             \
+#define GT_TRY(x)                     \
+    do {                              \
+        int rc_ = (x);                \
         if (rc_ != GT_OK) return rc_; \
     } while (0)
 
@@ -52,19 +59,8 @@ uint32_t next_pow2(uint64_t v) {
     return c > (1ull << 31) ? (1u << 31) : (uint32_t)c;
 }
 
-int value_stride(int vd) { return vd <= 1 ? 1 : vd <= 2 ? 2 : vd <= 4 ? 4 : vd <= 8 ? 8 : 16; }
+int value_stride(int vd) { return vd <= 2 ? 2 : vd <= 4 ? 4 : vd <= 8 ? 8 : 16; }
 size_t store_bytes(int storage) { return storage == GT_STORE_F32 ? 4 : 2; }
-int bitlen(uint32_t v) { int b = 0; while (v) { b++; v >>= 1; } return b; }
-
-// Scale factors of src/Permutohedral.h:257-274, same expressions in FP32.
-void scale_factors(int d, float *sf) {
-    const float s23 = sqrtf((float)(2.0 / 3));
-    for (int i = 0; i < d; i++) {
-        float v = 1.0f / (sqrtf((float)(i + 1) * (float)(i + 2)));
-        v *= (float)(d + 1) * s23;
-        sf[i] = v;
-    }
-}
 
 } // namespace
 
@@ -74,15 +70,23 @@ struct gt_ctx {
     hipStream_t own = nullptr;
     int profiling = 0;   // 0 off, 1 HIP events, 2 events + probe/atomic counters
     hipEvent_t ev[8] = {};
-    DevBuf tkeys, tvals, ctr, slot_list, slot_id, pk_a, pk_b, pay_b, hist, vkeys, vals_a, vals_b, nbr, splat_copy;
+    DevBuf tkeys, tacc, ctr, counts, slot_list, slot_id, pk_a, pk_b, pay_b, hist, vkeys, vals_a, vals_b, nbr,
+        splat_copy, mm_part, mm_out;
     DevBuf stage_in, stage_out, view_a, view_b, view_o;
     gt::Counters *h_ctr = nullptr;   // pinned
-    // sizing memory: last lattice size seen for (n, d)
+    float *h_mm = nullptr;           // pinned, 3 * 2 * kMaxDH
+    // capacity memory: last lattice size seen for (n, d)
     int last_n = -1, last_d = -1;
     unsigned last_M = 0;
+    // packing plan cache (position-source kind, d): reused until a key falls outside it
+    bool plan_valid = false;
+    int plan_d = 0, plan_kind = -1, plan_vd = 0;
+    gt::KeyPlan plan{};
+    float vmax[gt::kMaxDH] = {};   // value bounds behind the fixed-point scales
     // lattice kept for export
     bool have = false;
     int M = 0, d = 0, vd = 0, vs = 0, storage = 0;
+    bool packed = false;
     uint32_t cap = 0;
     void *vals_cur = nullptr;
 };
@@ -110,16 +114,14 @@ void free_buf(DevBuf &b) {
 }
 
 struct Tune {
-    int block = 256;
     float lf = 0.35f;
-    int dedup = -1;
+    int dedup = 0;      // 0 auto (LDS tiles when d <= 6 and vd <= 4), 1 on, -1 off
     int tile_w = 32;
 };
 
 Tune resolve(const gt_tuning *t) {
     Tune r;
     if (!t) return r;
-    if (t->block > 0) r.block = std::min(256, std::max(64, t->block & ~63));
     if (t->load_factor > 0) r.lf = std::min(0.7f, std::max(0.05f, t->load_factor));
     r.dedup = t->dedup;
     if (t->tile_w != 0) r.tile_w = t->tile_w;
@@ -132,143 +134,227 @@ float elapsed(hipEvent_t a, hipEvent_t b) {
     return ms;
 }
 
-// Splat, sort, neighbour table and blur.  On return ctx holds the blurred
-// lattice (ctx->vals_cur) and the table for the slice.
-int build(gt_ctx *c, const gt::PixSrc &src, int n, int row_w, int d, int vd, int storage, const Tune &tu,
-          gt::LatConst &lc, gt::TableArgs &ta, bool keep_splat, gt_stats *st) {
-    hipStream_t s = c->stream;
-    const int kw = (d + 3) / 4;
-    const bool count = c->profiling >= 2;
-    scale_factors(d, lc.sf);
-    for (int i = d; i < gt::kMaxDH; i++) lc.sf[i] = 0.f;
-
-    // Table capacity: from the last lattice of the same shape, else a
-    // generous first guess; regrown (and the splat redone) if it fills.
-    uint64_t est;
-    if (c->last_n == n && c->last_d == d && c->last_M > 0) est = (uint64_t)(c->last_M * 1.05) + 64;
-    else est = std::min<uint64_t>((uint64_t)n * (d + 1), std::max<uint64_t>(65536, (uint64_t)n));
-    uint32_t cap = next_pow2((uint64_t)(est / tu.lf));
-    const bool dedup = (tu.dedup < 0) ? (d <= 6 && vd <= 4) : (tu.dedup != 0);
-    int retries = 0;
-    while (true) {
-        GT_TRY(ensure(c->tkeys, (size_t)cap * kw * 8));
-        GT_TRY(ensure(c->tvals, (size_t)cap * vd * 4));
-        GT_TRY(ensure(c->ctr, sizeof(gt::Counters)));
-        GT_TRY(ensure(c->slot_list, (size_t)cap * 4));
-        GT_TRY(ensure(c->slot_id, (size_t)cap * 4));
-        ta.tkeys = (uint64_t *)c->tkeys.p;
-        ta.tvals = (float *)c->tvals.p;
-        ta.mask = cap - 1;
-        ta.max_fill = (uint32_t)(cap * 0.75);
-        ta.slot_id = (int *)c->slot_id.p;
-        gt::Counters *ctr = (gt::Counters *)c->ctr.p;
-        GT_HIP(gt::launch_clear(ta.tkeys, (size_t)cap * kw, ta.tvals, (size_t)cap * vd, ctr, s));
-        if (c->profiling) GT_HIP(hipEventRecord(c->ev[0], s));
-        GT_HIP(gt::launch_splat(d, src.ref != nullptr, dedup, src, n, row_w, tu.tile_w, tu.block, vd, lc, ta, ctr,
-                                count, s));
-        if (c->profiling) GT_HIP(hipEventRecord(c->ev[1], s));
-        GT_HIP(gt::launch_compact(d, ta, cap, (uint32_t *)c->slot_list.p, ctr, s));
-        GT_HIP(hipMemcpyAsync(c->h_ctr, ctr, sizeof(gt::Counters), hipMemcpyDeviceToHost, s));
-        GT_HIP(hipStreamSynchronize(s));
-        if (c->h_ctr->flags & gt::kFlagBad)
-            return fail(GT_ERANGE, "position is NaN/Inf or a lattice key leaves the int16 range");
-        if (!(c->h_ctr->flags & gt::kFlagFull)) break;
-        if (++retries > 8 || cap >= (1u << 31))
-            return fail(GT_ENOMEM, "lattice does not fit a hash table of %u slots", cap);
-        const double done = std::max(1u, c->h_ctr->done_pixels);
-        const double proj = (double)c->h_ctr->created * ((double)n / done) * 1.25;
-        cap = std::max<uint32_t>(cap * 2, next_pow2((uint64_t)(proj / tu.lf)));
-    }
-    const gt::Counters hc = *c->h_ctr;
-    const int M = (int)hc.listed;
-    c->last_n = n;
-    c->last_d = d;
-    c->last_M = (unsigned)M;
-
-    // Radix sort of the packed keys: only the bits that vary.
-    int umin[gt::kMaxDH], off[gt::kMaxDH], bits[gt::kMaxDH];
-    int total = 0;
+// Per-channel bounds of the positions and values on the device (one host
+// round trip), then the key packing plan and the value bounds for the
+// fixed-point scales.
+int make_plan(gt_ctx *c, const gt::PixSrc &src, int n, int d, int vd, bool bilat) {
+    const int nplanes = bilat ? src.cr : d;
+    const float *planes = bilat ? src.ref : src.pos;
+    const int64_t cs = bilat ? src.ref_cs : src.pos_cs;
+    const int nval = src.nval;
+    const int nblk = (int)std::min<int64_t>(1024, std::max<int64_t>(1, ((int64_t)n + 4095) / 4096));
+    GT_TRY(ensure(c->mm_part, (size_t)nblk * (nplanes + nval) * 3 * 4));
+    GT_TRY(ensure(c->mm_out, (size_t)(nplanes + nval) * 3 * 4));
+    float *part = (float *)c->mm_part.p, *mout = (float *)c->mm_out.p;
+    GT_HIP(gt::launch_minmax(planes, cs, nplanes, n, part, nblk, mout, c->stream));
+    GT_HIP(gt::launch_minmax(src.val, src.val_cs, nval, n, part + (size_t)nblk * nplanes * 3, nblk,
+                             mout + (size_t)nplanes * 3, c->stream));
+    GT_HIP(hipMemcpyAsync(c->h_mm, mout, (size_t)(nplanes + nval) * 3 * 4, hipMemcpyDeviceToHost, c->stream));
+    GT_HIP(hipStreamSynchronize(c->stream));
+    const float *mm = c->h_mm, *vm = c->h_mm + (size_t)nplanes * 3;
+    for (int k = 0; k < nval; k++)
+        if (vm[3 * k + 2] != 0.0f) return fail(GT_ERANGE, "value channel %d holds NaN or Inf", k);
+    float lo[gt::kMaxDH], hi[gt::kMaxDH];
     for (int i = 0; i < d; i++) {
-        umin[i] = M > 0 ? hc.kmin[i] : 0;
-        bits[i] = M > 0 ? bitlen((uint32_t)(hc.kmax[i] - hc.kmin[i])) : 0;
-        total += bits[i];
+        float a, b;
+        if (i < nplanes) {
+            const float sc = bilat ? src.ic : src.inv[i];
+            a = mm[3 * i] * sc;
+            b = mm[3 * i + 1] * sc;
+            if (mm[3 * i + 2] != 0.0f || !(mm[3 * i] <= mm[3 * i + 1])) a = b = INFINITY;  // no usable bound
+        } else {
+            // spatial channel k of the bilateral form: float(x) * (1/sigma), x in [0, extent-1]
+            const int k = i - src.cr;
+            int ext[3], m = 0;
+            float inv[3];
+            if (src.fx) { ext[m] = src.W; inv[m++] = src.ix; }
+            if (src.fy) { ext[m] = src.H; inv[m++] = src.iy; }
+            if (src.ft) { ext[m] = n / (src.W * src.H); inv[m++] = src.it; }
+            a = 0.0f * inv[k];
+            b = (float)(ext[k] - 1) * inv[k];
+        }
+        lo[i] = std::min(a, b);
+        hi[i] = std::max(a, b);
     }
-    for (int i = d - 1, acc = 0; i >= 0; i--) { off[i] = acc; acc += bits[i]; }
-    const int pw = std::max(1, (total + 63) / 64);
-    GT_TRY(ensure(c->pk_a, (size_t)pw * M * 8));
-    GT_TRY(ensure(c->pk_b, (size_t)pw * M * 8));
-    GT_TRY(ensure(c->pay_b, (size_t)M * 4));
-    const size_t he = gt::radix_hist_entries(M);
-    GT_TRY(ensure(c->hist, he * 4));
-    GT_HIP(gt::launch_pack(d, ta, (const uint32_t *)c->slot_list.p, M, umin, off, pw, (uint64_t *)c->pk_a.p, s));
-    gt::SortBufs sb{(uint64_t *)c->pk_a.p, (uint64_t *)c->pk_b.p, (uint32_t *)c->slot_list.p,
-                    (uint32_t *)c->pay_b.p, (uint32_t *)c->hist.p, he};
-    bool in_b = false;
-    int passes = 0;
-    GT_HIP(gt::radix_sort(sb, M, pw, total, &in_b, &passes, s));
-    const uint32_t *sorted = in_b ? sb.pay_b : sb.pay_a;
+    float sf[gt::kMaxDH];
+    gt::plan_scale_factors(d, sf);
+    gt::plan_from_bounds(d, sf, lo, hi, c->plan);
+    for (int k = 0; k < gt::kMaxDH; k++) c->vmax[k] = 0.0f;
+    for (int k = 0; k < nval && k < vd; k++) c->vmax[k] = std::max(std::fabs(vm[3 * k]), std::fabs(vm[3 * k + 1]));
+    if (src.homog && nval < vd) c->vmax[nval] = 1.0f;
+    c->plan_valid = true;
+    c->plan_d = d;
+    c->plan_vd = vd;
+    c->plan_kind = bilat ? 1 : 0;
+    return GT_OK;
+}
 
-    const int vs = value_stride(vd);
-    const size_t vbytes = (size_t)M * vs * store_bytes(storage);
-    GT_TRY(ensure(c->vkeys, (size_t)M * kw * 8));
-    GT_TRY(ensure(c->vals_a, vbytes));
-    GT_TRY(ensure(c->vals_b, vbytes));
-    float *splat_copy = nullptr;
-    if (keep_splat) {
-        GT_TRY(ensure(c->splat_copy, (size_t)M * vd * 4));
-        splat_copy = (float *)c->splat_copy.p;
-    }
-    GT_HIP(gt::launch_finalize(d, storage, ta, sorted, M, vd, vs, (uint64_t *)c->vkeys.p, c->vals_a.p, splat_copy,
-                               s));
-    if (c->profiling) GT_HIP(hipEventRecord(c->ev[2], s));
-
-    // Neighbour table, then the d+1 blur passes (ping-pong).
-    GT_TRY(ensure(c->nbr, (size_t)(d + 1) * M * 8));
-    int2 *nbr = (int2 *)c->nbr.p;
-    GT_HIP(gt::launch_neighbors(d, ta, (const uint64_t *)c->vkeys.p, M, nbr, (gt::Counters *)c->ctr.p, count, s));
-    if (c->profiling) GT_HIP(hipEventRecord(c->ev[3], s));
-    void *cur = c->vals_a.p, *nxt = c->vals_b.p;
-    for (int j = 0; j <= d; j++) {
-        GT_HIP(gt::launch_blur(storage, vs, nbr + (size_t)j * M, M, cur, nxt, s));
-        std::swap(cur, nxt);
-    }
-    if (c->profiling) GT_HIP(hipEventRecord(c->ev[4], s));
-
-    c->have = true;
-    c->M = M;
-    c->d = d;
-    c->vd = vd;
-    c->vs = vs;
-    c->storage = storage;
-    c->cap = cap;
-    c->vals_cur = cur;
-    if (st) {
-        st->M = M;
-        st->capacity = (int)std::min<uint32_t>(cap, 0x7fffffff);
-        st->retries = retries;
-        st->sort_passes = passes;
+// Fixed-point scales for n contributions of |value| <= vmax and weight <= 1:
+// s = 2^(62 - ceil(log2(n * vmax))) keeps every sum below 2^62 in magnitude.
+int value_scales(const gt_ctx *c, int n, int vd, gt::ValScale &vsc) {
+    for (int k = 0; k < gt::kMaxDH; k++) { vsc.s[k] = 0.0f; vsc.inv_s[k] = 0.0f; vsc.vmax[k] = 0.0f; }
+    for (int k = 0; k < vd; k++) {
+        const double bound = std::max(1e-30, (double)n * (double)c->vmax[k] * 1.001);
+        int p = 62 - (int)std::ceil(std::log2(bound));
+        if (p > 120) p = 120;
+        if (p < -120) return fail(GT_ERANGE, "values of channel %d are too large (%g)", k, (double)c->vmax[k]);
+        vsc.s[k] = std::ldexp(1.0f, p);
+        vsc.inv_s[k] = std::ldexp(1.0f, -p);
+        vsc.vmax[k] = c->vmax[k];
     }
     return GT_OK;
 }
 
-int finish_stats(gt_ctx *c, gt_stats *st) {
-    if (!st) return GT_OK;
-    if (c->profiling) {
-        GT_HIP(hipEventSynchronize(c->ev[5]));
-        st->ms_splat = elapsed(c->ev[0], c->ev[1]);
-        st->ms_sort = elapsed(c->ev[1], c->ev[2]);
-        st->ms_nbr = elapsed(c->ev[2], c->ev[3]);
-        st->ms_blur = elapsed(c->ev[3], c->ev[4]);
-        st->ms_slice = elapsed(c->ev[4], c->ev[5]);
-        st->ms_total = elapsed(c->ev[0], c->ev[5]);
-        if (c->profiling < 2) return GT_OK;
-        gt::Counters hc;
-        GT_HIP(hipMemcpy(&hc, c->ctr.p, sizeof(hc), hipMemcpyDeviceToHost));
-        st->probes = hc.probes;
-        st->inserts = hc.inserts;
-        st->atomics = hc.atomics;
+struct SliceSpec {
+    bool enabled = false;
+    gt::PixSrc src{};
+    int n = 0;
+    gt::OutDst dst{};
+};
+
+// The whole pipeline: splat, compaction, sort, finalize, neighbour table,
+// blur, and (optionally) slice; one host synchronisation at the end.
+int run(gt_ctx *c, const gt::PixSrc &src, int n, int row_w, int d, int vd, int storage, const Tune &tu,
+        const SliceSpec &sl, bool keep_splat, gt_stats *st) {
+    hipStream_t s = c->stream;
+    const bool bilat = src.ref != nullptr;
+    const bool count = c->profiling >= 2;
+    const bool dedup = (tu.dedup == 0) ? (d <= 6 && vd <= 4) : (tu.dedup > 0);
+    gt::LatConst lc;
+    gt::plan_scale_factors(d, lc.sf);
+    for (int i = d; i < gt::kMaxDH; i++) lc.sf[i] = 0.f;
+
+    uint64_t est;
+    if (c->last_n == n && c->last_d == d && c->last_M > 0) est = (uint64_t)(c->last_M * 1.05) + 64;
+    else est = std::min<uint64_t>((uint64_t)n * (d + 1), std::max<uint64_t>(65536, (uint64_t)n));
+    uint32_t cap = next_pow2((uint64_t)(est / tu.lf));
+    const uint32_t cap_limit = next_pow2((uint64_t)((double)n * (d + 1) / tu.lf));
+    if (cap > cap_limit) cap = cap_limit;
+    if (!(c->plan_valid && c->plan_d == d && c->plan_vd == vd && c->plan_kind == (bilat ? 1 : 0)))
+        GT_TRY(make_plan(c, src, n, d, vd, bilat));
+
+    int retries = 0, passes = 0;
+    for (;;) {
+        const gt::KeyPlan kp = c->plan;
+        gt::ValScale vsc;
+        GT_TRY(value_scales(c, n, vd, vsc));
+        const bool packed = kp.packed != 0;
+        const int W = packed ? 1 : (d + 3) / 4;
+        const int m_max = (int)std::min<uint64_t>(cap, (uint64_t)n * (d + 1));
+        const int pw = packed ? 1 : std::max(1, (kp.bits + 63) / 64);
+        const int vs = value_stride(vd);
+        const unsigned ncount = (cap + 2047) / 2048;
+        GT_TRY(ensure(c->tkeys, (size_t)cap * W * 8));
+        GT_TRY(ensure(c->tacc, (size_t)cap * vd * 8));
+        GT_TRY(ensure(c->ctr, sizeof(gt::Counters)));
+        GT_TRY(ensure(c->counts, (size_t)ncount * 4));
+        GT_TRY(ensure(c->slot_list, (size_t)m_max * 4));
+        GT_TRY(ensure(c->slot_id, (size_t)cap * 4));
+        GT_TRY(ensure(c->pk_a, (size_t)pw * m_max * 8));
+        GT_TRY(ensure(c->pk_b, (size_t)pw * m_max * 8));
+        GT_TRY(ensure(c->pay_b, (size_t)m_max * 4));
+        const size_t he = gt::radix_hist_entries(m_max);
+        GT_TRY(ensure(c->hist, he * 4));
+        GT_TRY(ensure(c->vkeys, (size_t)m_max * W * 8));
+        const size_t vbytes = (size_t)m_max * vs * store_bytes(storage);
+        GT_TRY(ensure(c->vals_a, vbytes));
+        GT_TRY(ensure(c->vals_b, vbytes));
+        GT_TRY(ensure(c->nbr, (size_t)(d + 1) * m_max * 8));
+        float *splat_copy = nullptr;
+        if (keep_splat) {
+            GT_TRY(ensure(c->splat_copy, (size_t)m_max * vd * 4));
+            splat_copy = (float *)c->splat_copy.p;
+        }
+        gt::TableArgs ta;
+        ta.tkeys = (uint64_t *)c->tkeys.p;
+        ta.tacc = (unsigned long long *)c->tacc.p;
+        ta.mask = cap - 1;
+        ta.slot_id = (int *)c->slot_id.p;
+        gt::Counters *ctr = (gt::Counters *)c->ctr.p;
+
+        GT_HIP(gt::launch_clear(ta.tkeys, (size_t)cap * W, ta.tacc, (size_t)cap * vd, ctr, s));
+        if (c->profiling) GT_HIP(hipEventRecord(c->ev[0], s));
+        GT_HIP(gt::launch_splat(d, packed, bilat, dedup, src, n, row_w, tu.tile_w, vd, lc, kp, vsc, ta, ctr, count,
+                                s));
+        if (c->profiling) GT_HIP(hipEventRecord(c->ev[1], s));
+        GT_HIP(gt::launch_compact(W, ta, cap, (uint32_t *)c->counts.p, (uint32_t *)c->slot_list.p,
+                                  (uint64_t *)c->pk_a.p, packed, ctr, s));
+        if (!packed)
+            GT_HIP(gt::launch_pack(d, ta, (const uint32_t *)c->slot_list.p, ctr, m_max, kp, pw, (uint64_t *)c->pk_a.p,
+                                   s));
+        gt::SortBufs sb{(uint64_t *)c->pk_a.p, (uint64_t *)c->pk_b.p, (uint32_t *)c->slot_list.p,
+                        (uint32_t *)c->pay_b.p, (uint32_t *)c->hist.p, he};
+        bool in_b = false;
+        GT_HIP(gt::radix_sort(sb, ctr, m_max, pw, kp.bits, &in_b, &passes, s));
+        const uint32_t *sorted = in_b ? sb.pay_b : sb.pay_a;
+        GT_HIP(gt::launch_finalize(W, storage, ta, vsc, sorted, ctr, m_max, vd, vs, (uint64_t *)c->vkeys.p,
+                                   c->vals_a.p, splat_copy, s));
+        if (c->profiling) GT_HIP(hipEventRecord(c->ev[2], s));
+        int2 *nbr = (int2 *)c->nbr.p;
+        GT_HIP(gt::launch_neighbors(d, packed, ta, kp, (const uint64_t *)c->vkeys.p, ctr, m_max, nbr, ctr, count, s));
+        if (c->profiling) GT_HIP(hipEventRecord(c->ev[3], s));
+        void *cur = c->vals_a.p, *nxt = c->vals_b.p;
+        for (int j = 0; j <= d; j++) {
+            GT_HIP(gt::launch_blur(storage, vs, nbr + (size_t)j * m_max, ctr, m_max, cur, nxt, s));
+            std::swap(cur, nxt);
+        }
+        if (c->profiling) GT_HIP(hipEventRecord(c->ev[4], s));
+        if (sl.enabled)
+            GT_HIP(gt::launch_slice(d, packed, sl.src.ref != nullptr, dedup, storage, sl.src, sl.n, row_w, tu.tile_w,
+                                    vd, vs, lc, kp, ta, cur, sl.dst, ctr, count, s));
+        if (c->profiling) GT_HIP(hipEventRecord(c->ev[5], s));
+        GT_HIP(hipMemcpyAsync(c->h_ctr, ctr, sizeof(gt::Counters), hipMemcpyDeviceToHost, s));
+        GT_HIP(hipStreamSynchronize(s));
+        const gt::Counters hc = *c->h_ctr;
+
+        if (hc.flags & gt::kFlagBad)
+            return fail(GT_ERANGE, "position is NaN/Inf or a lattice key leaves the int16 range");
+        if (hc.flags & gt::kFlagRange) {
+            // Keys or values outside the cached bounds: bound these inputs again.
+            if (++retries > 6) return fail(GT_EDEVICE, "inputs keep leaving the computed bounds");
+            GT_TRY(make_plan(c, src, n, d, vd, bilat));
+            continue;
+        }
+        if (hc.flags & gt::kFlagFull) {
+            if (++retries > 6 || cap >= cap_limit)
+                return fail(GT_ENOMEM, "lattice does not fit a hash table of %u slots", cap);
+            cap = std::min<uint32_t>(cap_limit, cap * 8);
+            continue;
+        }
+        const int M = (int)hc.M;
+        c->last_n = n;
+        c->last_d = d;
+        c->last_M = (unsigned)M;
+        c->have = true;
+        c->M = M;
+        c->d = d;
+        c->vd = vd;
+        c->vs = vs;
+        c->storage = storage;
+        c->packed = packed;
+        c->cap = cap;
+        c->vals_cur = cur;
+        if (st) {
+            st->M = M;
+            st->capacity = (int)std::min<uint32_t>(cap, 0x7fffffff);
+            st->retries = retries;
+            st->sort_passes = passes;
+            if (c->profiling) {
+                st->ms_splat = elapsed(c->ev[0], c->ev[1]);
+                st->ms_sort = elapsed(c->ev[1], c->ev[2]);
+                st->ms_nbr = elapsed(c->ev[2], c->ev[3]);
+                st->ms_blur = elapsed(c->ev[3], c->ev[4]);
+                st->ms_slice = elapsed(c->ev[4], c->ev[5]);
+                st->ms_total = elapsed(c->ev[0], c->ev[5]);
+            }
+            if (count) {
+                st->probes = hc.probes;
+                st->inserts = hc.inserts;
+                st->atomics = hc.atomics;
+            }
+        }
+        return GT_OK;
     }
-    return GT_OK;
 }
 
 int check_dims(int d, int vd) {
@@ -313,25 +399,13 @@ int jb_dense(gt_ctx *c, int W, int H, int F, int C, const float *im, int64_t im_
     src.val_cs = im_cs;
     src.nval = C;
     src.homog = 1;
-    gt::LatConst lc;
-    gt::TableArgs ta;
-    GT_TRY(build(c, src, n, W, d, vd, storage, tu, lc, ta, false, st));
-    gt::OutDst dst{out, out_cs, 1};
-    GT_HIP(gt::launch_slice(d, true, storage, src, n, vd, c->vs, lc, ta, c->vals_cur, dst,
-                            (gt::Counters *)c->ctr.p, c->profiling >= 2, c->stream));
-    if (c->profiling) GT_HIP(hipEventRecord(c->ev[5], c->stream));
-    GT_HIP(hipStreamSynchronize(c->stream));
-    return finish_stats(c, st);
+    SliceSpec sl;
+    sl.enabled = true;
+    sl.src = src;
+    sl.n = n;
+    sl.dst = gt::OutDst{out, out_cs, 1};
+    return run(c, src, n, W, d, vd, storage, tu, sl, false, st);
 }
-
-} // namespace
-
-namespace gt {
-hipError_t launch_view_copy(const float *src, int64_t sys, int64_t sts, int64_t scs, float *dst, int64_t dys,
-                            int64_t dts, int64_t dcs, int W, int H, int F, int C, hipStream_t st);
-}
-
-namespace {
 
 struct View {
     const float *p;
@@ -418,12 +492,13 @@ int gt_ctx_create(gt_ctx **out, int device) {
     c->stream = c->own;
     for (auto &e : c->ev) {
         if (hipEventCreate(&e) != hipSuccess) {
-            delete c;
+            gt_ctx_destroy(c);
             return fail(GT_EDEVICE, "event creation failed");
         }
     }
-    if (hipHostMalloc((void **)&c->h_ctr, sizeof(gt::Counters), hipHostMallocDefault) != hipSuccess) {
-        delete c;
+    if (hipHostMalloc((void **)&c->h_ctr, sizeof(gt::Counters), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void **)&c->h_mm, sizeof(float) * 6 * gt::kMaxDH, hipHostMallocDefault) != hipSuccess) {
+        gt_ctx_destroy(c);
         return fail(GT_ENOMEM, "pinned allocation failed");
     }
     *out = c;
@@ -434,13 +509,15 @@ void gt_ctx_destroy(gt_ctx *c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    DevBuf *bufs[] = {&c->tkeys, &c->tvals, &c->ctr, &c->slot_list, &c->slot_id, &c->pk_a, &c->pk_b, &c->pay_b,
-                      &c->hist, &c->vkeys, &c->vals_a, &c->vals_b, &c->nbr, &c->splat_copy, &c->stage_in,
-                      &c->stage_out, &c->view_a, &c->view_b, &c->view_o};
+    DevBuf *bufs[] = {&c->tkeys, &c->tacc, &c->ctr, &c->counts, &c->slot_list, &c->slot_id, &c->pk_a, &c->pk_b,
+                      &c->pay_b, &c->hist, &c->vkeys, &c->vals_a, &c->vals_b, &c->nbr, &c->splat_copy, &c->mm_part,
+                      &c->mm_out, &c->stage_in, &c->stage_out, &c->view_a, &c->view_b, &c->view_o};
     for (DevBuf *b : bufs) free_buf(*b);
-    for (auto &e : c->ev) if (e) (void)hipEventDestroy(e);
+    for (auto &e : c->ev)
+        if (e) (void)hipEventDestroy(e);
     if (c->own) (void)hipStreamDestroy(c->own);
     if (c->h_ctr) (void)hipHostFree(c->h_ctr);
+    if (c->h_mm) (void)hipHostFree(c->h_mm);
     delete c;
 }
 
@@ -450,9 +527,9 @@ int gt_ctx_set_stream(gt_ctx *c, void *stream) {
     return GT_OK;
 }
 
-int gt_ctx_set_profiling(gt_ctx *c, int enable) {
+int gt_ctx_set_profiling(gt_ctx *c, int level) {
     if (!c) return fail(GT_EINVAL, "null context");
-    c->profiling = enable < 0 ? 0 : enable > 2 ? 2 : enable;
+    c->profiling = level < 0 ? 0 : level > 2 ? 2 : level;
     return GT_OK;
 }
 
@@ -484,27 +561,17 @@ int gt_gauss_transform(gt_ctx *c, int n, int row_w, int d, int vd, const float *
     src.val = values;
     src.val_cs = n;
     src.nval = vd;
-    src.homog = 0;
-    gt::LatConst lc;
-    gt::TableArgs ta;
-    GT_TRY(build(c, src, n, row_w, d, vd, storage, tu, lc, ta, false, st));
-    gt::PixSrc ssrc = src;
+    SliceSpec sl;
+    sl.enabled = ns > 0;
+    sl.src = src;
     if (slice) {
-        ssrc.pos = slice;
-        ssrc.pos_cs = ns;
+        sl.src.pos = slice;
+        sl.src.pos_cs = ns;
     }
-    gt::OutDst dst{out, (int64_t)ns, 0};
-    GT_HIP(gt::launch_slice(d, false, storage, ssrc, ns, vd, c->vs, lc, ta, c->vals_cur, dst,
-                            (gt::Counters *)c->ctr.p, c->profiling >= 2, c->stream));
-    if (c->profiling) GT_HIP(hipEventRecord(c->ev[5], c->stream));
-    GT_HIP(hipStreamSynchronize(c->stream));
-    GT_TRY(finish_stats(c, st));
-    // A slice at new positions may fail the range checks too.
-    gt::Counters hc;
-    GT_HIP(hipMemcpy(&hc, c->ctr.p, sizeof(hc), hipMemcpyDeviceToHost));
-    if (hc.flags & gt::kFlagBad)
-        return fail(GT_ERANGE, "slice position is NaN/Inf or a lattice key leaves the int16 range");
-    return GT_OK;
+    sl.n = ns;
+    sl.dst = gt::OutDst{out, (int64_t)ns, 0};
+    // The slice positions' tiling uses the splat row width only when they are the splat positions.
+    return run(c, src, n, slice ? 0 : row_w, d, vd, storage, tu, sl, false, st);
 }
 
 int gt_gauss_transform_host(gt_ctx *c, int n, int row_w, int d, int vd, const float *splat, const float *values,
@@ -545,8 +612,8 @@ int gt_joint_bilateral(gt_ctx *c, int W, int H, int F, int C, const float *im, i
     if (st) memset(st, 0, sizeof(*st));
     const Tune tu = resolve(tune);
     const int64_t n = (int64_t)W * H * F;
-    View a{im, W, H, F, W, (int64_t)W * H, n}, b{ref, W, H, F, W, (int64_t)W * H, n}, o{out, W, H, F, W,
-                                                                                       (int64_t)W * H, n};
+    View a{im, W, H, F, W, (int64_t)W * H, n}, b{ref, W, H, F, W, (int64_t)W * H, n},
+        o{out, W, H, F, W, (int64_t)W * H, n};
     return jb_view(c, a, C, b, Cr, o, fw, fh, ff, color_sigma, storage, tu, st);
 }
 
@@ -585,11 +652,9 @@ int gt_build_lattice(gt_ctx *c, int n, int row_w, int d, int vd, const float *sp
     src.val = values;
     src.val_cs = n;
     src.nval = vd;
-    gt::LatConst lc;
-    gt::TableArgs ta;
     c->have = false;
-    GT_TRY(build(c, src, n, row_w, d, vd, storage, tu, lc, ta, true, nullptr));
-    GT_HIP(hipStreamSynchronize(c->stream));
+    SliceSpec none;
+    GT_TRY(run(c, src, n, row_w, d, vd, storage, tu, none, true, nullptr));
     if (M_out) *M_out = c->M;
     return GT_OK;
 }
@@ -600,13 +665,18 @@ int gt_lattice_export(gt_ctx *c, int16_t *keys, int32_t *nbr, float *splat_vals,
     GT_HIP(hipSetDevice(c->device));
     const int M = c->M, d = c->d, vd = c->vd;
     if (M == 0) return GT_OK;
+    const int m_max = (int)std::min<uint64_t>(c->cap, (uint64_t)c->last_n * (d + 1));
     GT_TRY(ensure(c->stage_out, std::max((size_t)M * d * 2, (size_t)M * vd * 4)));
     if (keys) {
-        GT_HIP(gt::launch_unpack_keys(d, (const uint64_t *)c->vkeys.p, M, (int16_t *)c->stage_out.p, c->stream));
+        GT_HIP(gt::launch_unpack_keys(d, c->packed, c->plan, (const uint64_t *)c->vkeys.p, M,
+                                      (int16_t *)c->stage_out.p, c->stream));
         GT_HIP(hipMemcpyAsync(keys, c->stage_out.p, (size_t)M * d * 2, hipMemcpyDeviceToHost, c->stream));
         GT_HIP(hipStreamSynchronize(c->stream));
     }
-    if (nbr) GT_HIP(hipMemcpy(nbr, c->nbr.p, (size_t)(d + 1) * M * 8, hipMemcpyDeviceToHost));
+    if (nbr)
+        for (int j = 0; j <= d; j++)
+            GT_HIP(hipMemcpy(nbr + (size_t)j * M * 2, (const int2 *)c->nbr.p + (size_t)j * m_max, (size_t)M * 8,
+                             hipMemcpyDeviceToHost));
     if (splat_vals) GT_HIP(hipMemcpy(splat_vals, c->splat_copy.p, (size_t)M * vd * 4, hipMemcpyDeviceToHost));
     if (blur_vals) {
         GT_HIP(gt::launch_store_to_f32(c->storage, c->vals_cur, M, vd, c->vs, (float *)c->stage_out.p, c->stream));

@@ -10,15 +10,15 @@
 #include <hip/hip_fp16.h>
 #include <stdint.h>
 
+#include "gt_internal.h"
+
 namespace gt {
 
 constexpr int kMaxD = 16;
 constexpr int kMaxVD = 16;
-constexpr uint64_t kEmpty = ~0ull;     // an unused table word (no valid key packs to it)
+constexpr uint64_t kEmpty = ~0ull;     // an unused table word (no valid key word equals it)
 constexpr int kWave = 64;
-
-// Number of 64-bit words in a packed key of D int16 coordinates.
-template <int D> struct KW { static constexpr int value = (D + 3) / 4; };
+constexpr uint32_t kMaxProbes = 256;   // an insert that probes this far reports a full table
 
 // Enclosing simplex of one point: fixed-up remainder-0 point g, ranks rk and
 // barycentric weights b[0..D].  `bad` flags NaN/Inf or keys beyond int16.
@@ -117,37 +117,77 @@ __device__ __forceinline__ int vertex_coord(const Simplex<D> &s, int r, int i) {
     return s.g[i] + ((s.rk[i] <= D - r) ? r : r - (D + 1));
 }
 
-// Packed key: coordinate i biased to uint16 (k + 32768) in word i/4, bits
-// 48-16*(i%4).  Coordinate 0 is the most significant, so comparing words in
-// order as unsigned integers is the lexicographic order of the signed keys.
-// Keys are kept within +-(32767 - 2(D+1)), so no word equals kEmpty.
+// ---------------------------------------------------------------------------
+// Key formats.  Both keep coordinate 0 most significant, so comparing the
+// words from the highest index down as unsigned integers is the
+// lexicographic order of the signed keys, and neither lets a word equal
+// kEmpty.
+//
+// Packed: one word; coordinate i stored as (c_i - lo_i) in `bits_i` bits at
+// `shift_i` (host-computed KeyPlan, bit 63 always clear).  Used whenever the
+// plan fits 63 bits.
+//
+// Wide: ceil(D/4) words; coordinate i biased to uint16 (c + 32768) at bits
+// 48-16*(i%4) of word i/4.  A key coordinate stays within +-(32767 - 2(D+1)),
+// so the first field of a word is never 0xffff.
+template <int D, bool PACKED> struct KeyOps;
+
 template <int D>
-__device__ __forceinline__ void pack_vertex(const Simplex<D> &s, int r, uint64_t (&k)[KW<D>::value]) {
+struct KeyOps<D, true> {
+    static constexpr int W = 1;
+    __device__ static __forceinline__ bool from_coords(const int (&c)[D], const KeyPlan &kp, uint64_t (&k)[W]) {
+        uint64_t v = 0;
+        bool ok = true;
 #pragma unroll
-    for (int w = 0; w < KW<D>::value; w++) k[w] = 0;
-#pragma unroll
-    for (int i = 0; i < D; i++) {
-        const uint64_t u = (uint64_t)(uint16_t)(vertex_coord(s, r, i) + 32768);
-        k[i / 4] |= u << (48 - 16 * (i % 4));
+        for (int i = 0; i < D; i++) {
+            const uint32_t u = (uint32_t)(c[i] - kp.lo[i]);
+            ok &= u <= kp.span[i];
+            v |= (uint64_t)(u & kp.span[i]) << kp.shift[i];
+        }
+        k[0] = v;
+        return ok;
     }
-}
+    __device__ static __forceinline__ bool make(const Simplex<D> &s, int r, const KeyPlan &kp, uint64_t (&k)[W]) {
+        int c[D];
+#pragma unroll
+        for (int i = 0; i < D; i++) c[i] = vertex_coord(s, r, i);
+        return from_coords(c, kp, k);
+    }
+    __device__ static __forceinline__ int coord(const uint64_t (&k)[W], int i, const KeyPlan &kp) {
+        return (int)((k[0] >> kp.shift[i]) & kp.span[i]) + kp.lo[i];
+    }
+};
 
 template <int D>
-__device__ __forceinline__ int key_coord(const uint64_t (&k)[KW<D>::value], int i) {
-    return (int)((k[i / 4] >> (48 - 16 * (i % 4))) & 0xffff) - 32768;
-}
+struct KeyOps<D, false> {
+    static constexpr int W = (D + 3) / 4;
+    __device__ static __forceinline__ bool from_coords(const int (&c)[D], const KeyPlan &kp, uint64_t (&k)[W]) {
+        bool ok = true;
+#pragma unroll
+        for (int w = 0; w < W; w++) k[w] = 0;
+#pragma unroll
+        for (int i = 0; i < D; i++) {
+            ok &= (c[i] >= -32767 + 2 * (D + 1)) && (c[i] <= 32767 - 2 * (D + 1));
+            k[i / 4] |= (uint64_t)(uint16_t)(c[i] + 32768) << (48 - 16 * (i % 4));
+        }
+        return ok;
+    }
+    __device__ static __forceinline__ bool make(const Simplex<D> &s, int r, const KeyPlan &kp, uint64_t (&k)[W]) {
+        int c[D];
+#pragma unroll
+        for (int i = 0; i < D; i++) c[i] = vertex_coord(s, r, i);
+        return from_coords(c, kp, k);
+    }
+    __device__ static __forceinline__ int coord(const uint64_t (&k)[W], int i, const KeyPlan &) {
+        return (int)((k[i / 4] >> (48 - 16 * (i % 4))) & 0xffff) - 32768;
+    }
+};
 
-template <int D>
-__device__ __forceinline__ void set_key_coord(uint64_t (&k)[KW<D>::value], int i, int c) {
-    const int sh = 48 - 16 * (i % 4);
-    k[i / 4] = (k[i / 4] & ~(0xffffull << sh)) | ((uint64_t)(uint16_t)(c + 32768) << sh);
-}
-
-template <int D>
-__device__ __forceinline__ uint64_t hash_key(const uint64_t (&k)[KW<D>::value]) {
+template <int W>
+__device__ __forceinline__ uint64_t hash_key(const uint64_t (&k)[W]) {
     uint64_t h = 0x9E3779B97F4A7C15ull;
 #pragma unroll
-    for (int w = 0; w < KW<D>::value; w++) {
+    for (int w = 0; w < W; w++) {
         h ^= k[w];
         h *= 0xff51afd7ed558ccdull;
         h ^= h >> 33;
@@ -156,35 +196,35 @@ __device__ __forceinline__ uint64_t hash_key(const uint64_t (&k)[KW<D>::value]) 
     return h ^ (h >> 29);
 }
 
-// Insert-or-find in the global open-addressing table (linear probing, slot
-// s owns words tkeys[s*KW .. s*KW+KW-1]).  Each word is claimed by its own
-// 64-bit CAS from kEmpty; a thread whose word is already there moves on to
-// the next word, and a mismatch on any word sends it to the next slot.
-// Words are written once, so every thread with a given key agrees on the
-// slot that holds it, without locks or spinning (no intra-wave deadlock).
-// The thread whose CAS completes the last word of a slot is its creator.
-// Returns the slot, or -1 if max_probe slots were visited.
-template <int D>
-__device__ __forceinline__ int table_insert(uint64_t *__restrict__ tkeys, uint32_t mask,
-                                            const uint64_t (&k)[KW<D>::value], uint64_t h,
-                                            bool &created, uint32_t &probes) {
-    constexpr int W = KW<D>::value;
-    uint32_t s = (uint32_t)h & mask;
-    created = false;
-    for (uint32_t it = 0; it <= mask; it++) {
+// Insert-or-find in the global open-addressing table (linear probing; slot s
+// owns words tkeys[s*W .. s*W+W-1]).  Each word is claimed by its own 64-bit
+// CAS from kEmpty; a thread whose word is already there moves to the next
+// word, and a mismatch on any word sends it to the next slot.  Words are
+// written once, so every thread with a given key agrees on its slot, with no
+// locks or spinning (no intra-wave deadlock).  The CAS that completes the
+// last word of a slot creates the vertex.  Returns the slot, or -1 after
+// kMaxProbes slots (table full).
+template <int W>
+__device__ __forceinline__ int table_insert(uint64_t *__restrict__ tkeys, uint32_t mask, const uint64_t (&k)[W],
+                                            uint32_t &probes) {
+    uint32_t s = (uint32_t)hash_key<W>(k) & mask;
+    for (uint32_t it = 0; it < kMaxProbes; it++) {
         probes++;
         bool match = true;
 #pragma unroll
         for (int w = 0; w < W; w++) {
             if (match) {
                 uint64_t *a = tkeys + (size_t)s * W + w;
-                uint64_t cur = __hip_atomic_load(a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                // A workgroup-scope (non-coherent, cache-friendly) read first:
+                // words are write-once, so a value other than kEmpty is final;
+                // only an apparently empty word needs the device-scope CAS.
+                uint64_t cur = __hip_atomic_load(a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                 if (cur == kEmpty) {
                     uint64_t expect = kEmpty;
-                    const bool ok = __hip_atomic_compare_exchange_strong(
-                        a, &expect, k[w], __ATOMIC_RELAXED, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    cur = ok ? k[w] : expect;
-                    if (ok && w == W - 1) created = true;
+                    __hip_atomic_compare_exchange_strong(a, &expect, k[w], __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                         __HIP_MEMORY_SCOPE_AGENT);
+                    // expect now holds the previous word: kEmpty if our CAS wrote it.
+                    cur = (expect == kEmpty) ? k[w] : expect;
                 }
                 match = (cur == k[w]);
             }
@@ -195,15 +235,12 @@ __device__ __forceinline__ int table_insert(uint64_t *__restrict__ tkeys, uint32
     return -1;
 }
 
-// Read-only lookup after the table is complete (a later kernel): plain
-// loads, returns the slot or -1.
-template <int D>
-__device__ __forceinline__ int table_find(const uint64_t *__restrict__ tkeys, uint32_t mask,
-                                          const uint64_t (&k)[KW<D>::value], uint64_t h,
+// Read-only lookup in a complete table (a later kernel): plain loads.
+template <int W>
+__device__ __forceinline__ int table_find(const uint64_t *__restrict__ tkeys, uint32_t mask, const uint64_t (&k)[W],
                                           uint32_t &probes) {
-    constexpr int W = KW<D>::value;
-    uint32_t s = (uint32_t)h & mask;
-    for (uint32_t it = 0; it <= mask; it++) {
+    uint32_t s = (uint32_t)hash_key<W>(k) & mask;
+    for (uint32_t it = 0; it < kMaxProbes; it++) {
         probes++;
         const uint64_t *a = tkeys + (size_t)s * W;
         const uint64_t w0 = a[0];
@@ -217,20 +254,34 @@ __device__ __forceinline__ int table_find(const uint64_t *__restrict__ tkeys, ui
     return -1;
 }
 
-// Lattice value storage.
-template <typename T> struct Store;
-template <> struct Store<float> {
-    static __device__ __forceinline__ float load(const float *p) { return *p; }
-    static __device__ __forceinline__ void store(float *p, float v) { *p = v; }
-};
-template <> struct Store<__hip_bfloat16> {
-    static __device__ __forceinline__ float load(const __hip_bfloat16 *p) { return __bfloat162float(*p); }
-    static __device__ __forceinline__ void store(__hip_bfloat16 *p, float v) { *p = __float2bfloat16(v); }
-};
-template <> struct Store<__half> {
-    static __device__ __forceinline__ float load(const __half *p) { return __half2float(*p); }
-    static __device__ __forceinline__ void store(__half *p, float v) { *p = __float2half(v); }
-};
+// Workgroup-local key table in LDS: the distinct vertices of one pixel tile.
+// Same word-claim protocol as the global table.  Returns the LDS slot, or -1
+// after MAXP probes (a tile with more distinct vertices than the table holds
+// then sends the rest straight to the global table).
+template <int W, int LT, int MAXP>
+__device__ __forceinline__ int lds_insert(uint64_t *lk, const uint64_t (&k)[W]) {
+    uint32_t ls = (uint32_t)(hash_key<W>(k) >> 40) & (LT - 1);
+    for (int it = 0; it < MAXP; it++) {
+        bool match = true;
+#pragma unroll
+        for (int w = 0; w < W; w++) {
+            if (match) {
+                uint64_t expect = kEmpty;
+                __hip_atomic_compare_exchange_strong(lk + ls * W + w, &expect, k[w], __ATOMIC_RELAXED,
+                                                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                match = (expect == kEmpty) || (expect == k[w]);
+            }
+        }
+        if (match) return (int)ls;
+        ls = (ls + 1) & (LT - 1);
+    }
+    return -1;
+}
+
+// x * s rounded to the nearest integer, as a two's-complement 64-bit word.
+__device__ __forceinline__ unsigned long long to_fixed(float x, float s) {
+    return (unsigned long long)__float2ll_rn(x * s);
+}
 
 __device__ __forceinline__ uint64_t lanemask_lt() {
     const int lane = __lane_id();
@@ -242,6 +293,59 @@ __device__ __forceinline__ T wave_sum(T v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
     return v;
+}
+
+// Storage conversions of lattice values.
+template <typename T> __device__ __forceinline__ float to_f32(T v);
+template <> __device__ __forceinline__ float to_f32<float>(float v) { return v; }
+template <> __device__ __forceinline__ float to_f32<__hip_bfloat16>(__hip_bfloat16 v) { return __bfloat162float(v); }
+template <> __device__ __forceinline__ float to_f32<__half>(__half v) { return __half2float(v); }
+template <typename T> __device__ __forceinline__ T from_f32(float v);
+template <> __device__ __forceinline__ float from_f32<float>(float v) { return v; }
+template <> __device__ __forceinline__ __hip_bfloat16 from_f32<__hip_bfloat16>(float v) { return __float2bfloat16(v); }
+template <> __device__ __forceinline__ __half from_f32<__half>(float v) { return __float2half(v); }
+
+// Vertex value rows: vertex-major, VS (a power of two >= vd) values each.
+template <typename T, int VS>
+struct alignas((sizeof(T) * VS) < 16 ? (sizeof(T) * VS) : 16) Row {
+    T v[VS];
+};
+
+template <typename T, int VS>
+__device__ __forceinline__ void load_row(const T *base, int64_t i, float (&o)[kMaxVD]) {
+    const Row<T, VS> r = reinterpret_cast<const Row<T, VS> *>(base)[i];
+#pragma unroll
+    for (int c = 0; c < VS; c++) o[c] = to_f32<T>(r.v[c]);
+}
+
+template <typename T, int VS>
+__device__ __forceinline__ void store_row(T *base, int64_t i, const float (&o)[kMaxVD]) {
+    Row<T, VS> r;
+#pragma unroll
+    for (int c = 0; c < VS; c++) r.v[c] = from_f32<T>(o[c]);
+    reinterpret_cast<Row<T, VS> *>(base)[i] = r;
+}
+
+template <typename T>
+__device__ __forceinline__ void load_row_rt(const T *base, int64_t i, int vs, float (&o)[kMaxVD]) {
+    switch (vs) {
+    case 1: load_row<T, 1>(base, i, o); break;
+    case 2: load_row<T, 2>(base, i, o); break;
+    case 4: load_row<T, 4>(base, i, o); break;
+    case 8: load_row<T, 8>(base, i, o); break;
+    default: load_row<T, 16>(base, i, o); break;
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void store_row_rt(T *base, int64_t i, int vs, const float (&o)[kMaxVD]) {
+    switch (vs) {
+    case 1: store_row<T, 1>(base, i, o); break;
+    case 2: store_row<T, 2>(base, i, o); break;
+    case 4: store_row<T, 4>(base, i, o); break;
+    case 8: store_row<T, 8>(base, i, o); break;
+    default: store_row<T, 16>(base, i, o); break;
+    }
 }
 
 } // namespace gt

@@ -1,29 +1,26 @@
-// Host-side declarations shared by the kernel translation units and the
+// Host/device declarations shared by the kernel translation units and the
 // C-ABI implementation.  Not part of the public interface (include/gtlattice.h).
 #pragma once
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "gt_keyplan.h"
+
 namespace gt {
 
-constexpr int kMaxDH = 16;
-
-// Device counters written by the splat/compaction kernels and read back by
-// the host once per transform (one small D2H copy).
+// Device counters, read back by the host once per transform.
 struct Counters {
-    unsigned int created;      // vertices created in the table
+    unsigned int M;            // lattice vertices (written by the compaction scan)
     unsigned int flags;        // kFlag* bits
-    unsigned int listed;       // slots appended by compaction (== M)
-    unsigned int done_pixels;  // pixels fully splatted (used to extrapolate M on regrowth)
-    int kmin[kMaxDH];          // per-coordinate min / max of the keys (biased uint16)
-    int kmax[kMaxDH];
-    unsigned long long probes;
+    unsigned int pad[2];
+    unsigned long long probes;   // profiling level 2 only
     unsigned long long inserts;
     unsigned long long atomics;
 };
-constexpr unsigned kFlagFull = 1u;   // table reached its fill limit: regrow and redo
-constexpr unsigned kFlagBad = 2u;    // NaN/Inf position or key outside int16
+constexpr unsigned kFlagFull = 1u;    // an insert exceeded the probe limit: regrow and redo
+constexpr unsigned kFlagBad = 2u;     // NaN/Inf position or key outside int16
+constexpr unsigned kFlagRange = 4u;   // a key outside the packing plan: replan and redo
 
 // Where pixel positions and values come from.
 struct PixSrc {
@@ -58,49 +55,67 @@ struct OutDst {
 };
 
 struct TableArgs {
-    uint64_t *tkeys;   // capacity * KW words
-    float *tvals;      // capacity * vd FP32 accumulators
-    uint32_t mask;     // capacity - 1
-    uint32_t max_fill; // regrow threshold on created vertices
-    int *slot_id;      // capacity: slot -> sorted vertex id
+    uint64_t *tkeys;               // capacity * W words
+    unsigned long long *tacc;      // capacity * vd fixed-point accumulators (two's complement)
+    uint32_t mask;                 // capacity - 1
+    int *slot_id;                  // capacity: slot -> sorted vertex id
 };
 
-// Launchers (gt_kernels.hip).  All are asynchronous on `st`.
-hipError_t launch_clear(uint64_t *tkeys, size_t nwords, float *tvals, size_t nvals,
-                        Counters *ctr, hipStream_t st);
-hipError_t launch_splat(int d, bool bilat, bool dedup, const PixSrc &src, int n, int row_w,
-                        int tile_w, int block, int vd, const LatConst &lc, const TableArgs &ta,
-                        Counters *ctr, bool count, hipStream_t st);
-hipError_t launch_compact(int d, const TableArgs &ta, uint32_t capacity, uint32_t *slot_list,
-                          Counters *ctr, hipStream_t st);
-hipError_t launch_pack(int d, const TableArgs &ta, const uint32_t *slot_list, int M,
-                       const int *umin, const int *bits, int pw, uint64_t *pk, hipStream_t st);
-hipError_t launch_finalize(int d, int storage, const TableArgs &ta, const uint32_t *sorted_slots,
-                           int M, int vd, int vs, uint64_t *vkeys, void *vals, float *splat_copy,
-                           hipStream_t st);
-hipError_t launch_neighbors(int d, const TableArgs &ta, const uint64_t *vkeys, int M,
-                            int2 *nbr, Counters *ctr, bool count, hipStream_t st);
-hipError_t launch_blur(int storage, int vs, const int2 *nbr, int M, const void *in, void *out,
-                       hipStream_t st);
-hipError_t launch_slice(int d, bool bilat, int storage, const PixSrc &src, int n, int vd, int vs,
-                        const LatConst &lc, const TableArgs &ta, const void *vals,
-                        const OutDst &dst, Counters *ctr, bool count, hipStream_t st);
-hipError_t launch_unpack_keys(int d, const uint64_t *vkeys, int M, int16_t *keys, hipStream_t st);
-hipError_t launch_store_to_f32(int storage, const void *vals, int M, int vd, int vs, float *out,
-                               hipStream_t st);
+// Fixed-point splat accumulation: channel c of a weighted value is added as
+// round(x * s[c]) to 64-bit integers, s[c] a power of two chosen so that no
+// vertex sum can exceed 2^62 in magnitude (|value| <= vmax[c], weights <= 1,
+// at most n contributions).  Integer sums are exact and independent of the
+// order of the atomics; finalize converts them back with inv_s[c].
+struct ValScale {
+    float s[kMaxDH];
+    float inv_s[kMaxDH];
+    float vmax[kMaxDH];
+};
 
-// Radix sort (gt_sort.hip): sorts M items with pw 64-bit key words stored
-// word-major (pk[w*M + i], word 0 least significant) and a uint32 payload, by
-// the low `bits` bits.  Stable LSD passes of 8 bits.  Buffers a/b are
-// ping-pong; returns in *result_in_b whether the sorted data ended in b.
+// Launchers (gt_kernels.hip).  All are asynchronous on `st`.  Kernels after
+// the compaction read the vertex count M from ctr->M on the device; `m_max`
+// only sizes their grids.
+// Per-plane min, max and a non-finite indicator (1.0 if any NaN/Inf) -> out[3*plane..].
+hipError_t launch_minmax(const float *planes, int64_t cs, int nplanes, int n, float *partial, int nblk,
+                         float *out, hipStream_t st);
+hipError_t launch_clear(uint64_t *tkeys, size_t nwords, unsigned long long *tacc, size_t nacc, Counters *ctr,
+                        hipStream_t st);
+hipError_t launch_splat(int d, bool packed, bool bilat, bool dedup, const PixSrc &src, int n, int row_w, int tile_w,
+                        int vd, const LatConst &lc, const KeyPlan &kp, const ValScale &vsc, const TableArgs &ta,
+                        Counters *ctr, bool count, hipStream_t st);
+hipError_t launch_compact(int words, const TableArgs &ta, uint32_t capacity, uint32_t *counts, uint32_t *slot_list,
+                          uint64_t *sort_keys, bool packed, Counters *ctr, hipStream_t st);
+hipError_t launch_pack(int d, const TableArgs &ta, const uint32_t *slot_list, const Counters *ctr, int m_max,
+                       const KeyPlan &kp, int pw, uint64_t *pk, hipStream_t st);
+hipError_t launch_finalize(int words, int storage, const TableArgs &ta, const ValScale &vsc, const uint32_t *sorted,
+                           const Counters *ctr, int m_max, int vd, int vs, uint64_t *vkeys, void *vals,
+                           float *splat_copy, hipStream_t st);
+hipError_t launch_neighbors(int d, bool packed, const TableArgs &ta, const KeyPlan &kp, const uint64_t *vkeys,
+                            const Counters *ctr, int m_max, int2 *nbr, Counters *wctr, bool count, hipStream_t st);
+hipError_t launch_blur(int storage, int vs, const int2 *nbr, const Counters *ctr, int m_max, const void *in, void *out,
+                       hipStream_t st);
+hipError_t launch_slice(int d, bool packed, bool bilat, bool tiled, int storage, const PixSrc &src, int n, int row_w,
+                        int tile_w, int vd, int vs, const LatConst &lc, const KeyPlan &kp, const TableArgs &ta,
+                        const void *vals, const OutDst &dst, Counters *ctr, bool count, hipStream_t st);
+hipError_t launch_unpack_keys(int d, bool packed, const KeyPlan &kp, const uint64_t *vkeys, int M, int16_t *keys,
+                              hipStream_t st);
+hipError_t launch_store_to_f32(int storage, const void *vals, int M, int vd, int vs, float *out, hipStream_t st);
+hipError_t launch_view_copy(const float *src, int64_t sys, int64_t sts, int64_t scs, float *dst, int64_t dys,
+                            int64_t dts, int64_t dcs, int W, int H, int F, int C, hipStream_t st);
+
+// Radix sort (gt_sort.hip): stable LSD sort of up to m_max items (the live
+// count is ctr->M on the device) with `pw` key words stored word-major at
+// stride m_max (pk[w*m_max + i], word 0 least significant) and a uint32
+// payload, on the low `bits` bits, 8 bits per pass.  a/b are ping-pong
+// buffers; *result_in_b tells where the sorted data ended.
 struct SortBufs {
     uint64_t *pk_a, *pk_b;
     uint32_t *pay_a, *pay_b;
-    uint32_t *hist;        // 256 * nblocks
-    size_t hist_cap;       // entries available in hist
+    uint32_t *hist;        // 256 * blocks(m_max)
+    size_t hist_cap;
 };
-size_t radix_hist_entries(int M);
-hipError_t radix_sort(SortBufs &b, int M, int pw, int bits, bool *result_in_b, int *passes,
+size_t radix_hist_entries(int m_max);
+hipError_t radix_sort(SortBufs &b, const Counters *ctr, int m_max, int pw, int bits, bool *result_in_b, int *passes,
                       hipStream_t st);
 
 } // namespace gt

@@ -1,0 +1,657 @@
+// Lattice kernel templates for gfx950: position bounds, splat (elevate + hash insert +
+// accumulate), compaction, key packing for the radix sort, finalize (sorted
+// ids), neighbour table, blur passes and slice.  The launchers (gt_launch_*.hip)
+// map the runtime lattice dimension d and key format to template parameters.
+#pragma once
+#include "gt_device.h"
+#include "gt_internal.h"
+
+namespace gt {
+
+constexpr int kBlock = 256;          // threads per workgroup of every kernel here
+constexpr int kTileKeys = 512;       // LDS tile table slots of the splat and slice tiles
+constexpr int kCompactSlots = 2048;  // table slots per compaction workgroup
+
+// ---------------------------------------------------------------------------
+// pixel sources
+
+// Pixel index of thread `tid` of workgroup `blk`: plain linear order, or a
+// tile_w x (kBlock/tile_w) tile of an image with rows of row_w pixels.
+__device__ __forceinline__ int pixel_index(int blk, int tid, int n, int row_w, int tile_w) {
+    if (tile_w <= 0 || row_w <= 0) {
+        const int64_t p = (int64_t)blk * kBlock + tid;
+        return p < n ? (int)p : -1;
+    }
+    const int tiles_x = (row_w + tile_w - 1) / tile_w;
+    const int th = kBlock / tile_w;
+    const int bx = blk % tiles_x, by = blk / tiles_x;
+    const int x = bx * tile_w + tid % tile_w;
+    const int y = by * th + tid / tile_w;
+    const int rows = n / row_w;
+    return (x < row_w && y < rows) ? y * row_w + x : -1;
+}
+
+template <int D, bool BILAT>
+__device__ __forceinline__ void load_position(const PixSrc &s, int p, float (&pos)[D]) {
+    if constexpr (BILAT) {
+        // (ref_c * (1.0f/sigma_r), x*(1.0f/sigma_x), y*(1.0f/sigma_y),
+        //  t*(1.0f/sigma_t)) for the filtered axes, in that order
+        // (src/GaussTransform.cpp:654-666).
+        const int x = p % s.W;
+        const int q = p / s.W;
+        const int y = q % s.H;
+        const int t = q / s.H;
+        const float xs = (float)x * s.ix, ys = (float)y * s.iy, ts = (float)t * s.it;
+        const float a0 = s.fx ? xs : (s.fy ? ys : ts);
+        const float a1 = s.fx ? (s.fy ? ys : ts) : ts;
+#pragma unroll
+        for (int c = 0; c < D; c++) {
+            float v;
+            if (c < s.cr) {
+                v = s.ref[(int64_t)c * s.ref_cs + p] * s.ic;
+            } else {
+                const int k = c - s.cr;
+                v = (k == 0) ? a0 : (k == 1 ? a1 : ts);
+            }
+            pos[c] = v;
+        }
+    } else {
+        // pos[c] = splat(x,y,t,c) * invSigma[c] (src/GaussTransform.cpp:176-178)
+#pragma unroll
+        for (int c = 0; c < D; c++) pos[c] = s.pos[(int64_t)c * s.pos_cs + p] * s.inv[c];
+    }
+}
+
+template <int NV>
+__device__ __forceinline__ void load_values(const PixSrc &s, int p, float (&v)[kMaxVD]) {
+#pragma unroll
+    for (int c = 0; c < NV; c++) {
+        float x = 0.0f;
+        if (c < s.nval) x = s.val[(int64_t)c * s.val_cs + p];
+        else if (c == s.nval && s.homog) x = 1.0f;
+        v[c] = x;
+    }
+}
+
+// OR of per-lane flags, one atomic per wave that has any (rare paths only).
+__device__ __forceinline__ void report_flags(Counters *ctr, unsigned f) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) f |= __shfl_xor(f, off, kWave);
+    if (__lane_id() == 0 && f) atomicOr(&ctr->flags, f);
+}
+
+__device__ __forceinline__ void report_counts(Counters *ctr, uint32_t probes, uint32_t inserts, uint32_t atomics) {
+    const unsigned long long pr = wave_sum((unsigned long long)probes);
+    const unsigned long long in = wave_sum((unsigned long long)inserts);
+    const unsigned long long at = wave_sum((unsigned long long)atomics);
+    if (__lane_id() == 0) {
+        if (pr) atomicAdd(&ctr->probes, pr);
+        if (in) atomicAdd(&ctr->inserts, in);
+        if (at) atomicAdd(&ctr->atomics, at);
+    }
+}
+
+// Lists the occupied slots of an LDS tile table (order of slot index within a
+// wave, waves appended through one LDS counter).
+template <int W, int LT>
+__device__ __forceinline__ void list_tile(const uint64_t *lk, uint16_t *llist, unsigned *lcount) {
+    static_assert(LT % kBlock == 0, "tile table size must be a multiple of the workgroup size");
+    for (int base = 0; base < LT; base += kBlock) {
+        const int ls = base + threadIdx.x;
+        const bool occ = lk[ls * W] != kEmpty;
+        const uint64_t m = __ballot(occ);
+        unsigned wbase = 0;
+        if (__lane_id() == 0 && m) wbase = atomicAdd(lcount, (unsigned)__popcll(m));
+        wbase = __shfl(wbase, 0, kWave);
+        if (occ) llist[wbase + __popcll(m & lanemask_lt())] = (uint16_t)ls;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// splat, tiled form: the workgroup inserts its 256 pixels' D+1 vertex keys
+// into an LDS tile table and pre-reduces weight*value there (LDS FP32
+// atomics); then each distinct key is inserted into the global table once and
+// its sums added with one FP32 atomic per channel.  Pixels of a tile share
+// most simplex vertices, so global probes and atomics drop by that factor.
+// (src/Permutohedral.h:365-392)
+//
+// LDS accumulators are 64-bit fixed point (ValScale), channel-major
+// (lv[c*LT + slot]): LDS integer atomics run over ten times faster than LDS
+// FP32 atomics on gfx950, and the sums are exact.  LT = 512 slots keep the
+// workgroup's LDS small enough for high occupancy; keys that do not find an
+// LDS slot within kLdsProbes go to the global table directly.
+constexpr int kLdsProbes = 32;
+
+// Rotates a[0..N-1] left by rot (0 <= rot < N) with log2(N) stages of
+// compile-time-indexed selects (no dynamic register indexing).
+template <int N, typename T>
+__device__ __forceinline__ void rotate_left(T (&a)[N], int rot) {
+#pragma unroll
+    for (int sh = 1; sh < N; sh <<= 1) {
+        const bool on = (rot & sh) != 0;
+        T t[N];
+#pragma unroll
+        for (int i = 0; i < N; i++) t[i] = a[(i + sh) % N];
+#pragma unroll
+        for (int i = 0; i < N; i++) a[i] = on ? t[i] : a[i];
+    }
+}
+
+template <int D, bool PACKED, bool BILAT, int LVD, int LT>
+__global__ __launch_bounds__(kBlock) void k_splat_tile(PixSrc src, int n, int row_w, int tile_w, int vd, LatConst lc,
+                                                       KeyPlan kp, ValScale vsc, TableArgs ta, Counters *ctr,
+                                                       int count) {
+    using K = KeyOps<D, PACKED>;
+    constexpr int W = K::W;
+    __shared__ uint64_t lk[LT * W];
+    __shared__ unsigned long long lv[LVD * LT];
+    __shared__ uint16_t llist[LT];
+    __shared__ unsigned lcount;
+    for (int i = threadIdx.x; i < LT * W; i += kBlock) lk[i] = kEmpty;
+    for (int i = threadIdx.x; i < LVD * LT; i += kBlock) lv[i] = 0ull;
+    if (threadIdx.x == 0) lcount = 0;
+    const int p = pixel_index(blockIdx.x, threadIdx.x, n, row_w, tile_w);
+    const bool full = *(volatile const unsigned *)&ctr->flags & kFlagFull;
+    __syncthreads();
+
+    unsigned flags = 0;
+    uint32_t probes = 0, inserts = 0, atomics = 0;
+    if (p >= 0) {
+        float pos[D], v[kMaxVD];
+        load_position<D, BILAT>(src, p, pos);
+        load_values<LVD>(src, p, v);
+#pragma unroll
+        for (int c = 0; c < LVD; c++)
+            if (c < vd && !(fabsf(v[c]) <= vsc.vmax[c])) flags |= kFlagRange;
+        Simplex<D> s;
+        simplex<D>(pos, lc.sf, s);
+        if (s.bad) {
+            flags |= kFlagBad;
+        } else {
+            // Lane i visits the vertices starting at rank i mod (D+1): lanes of a
+            // wave that share a key (neighbouring pixels, same rank) update it in
+            // different instructions, which removes most LDS atomic conflicts.
+            uint64_t key[D + 1][W];
+            float wt[D + 1];
+            int okr[D + 1];
+#pragma unroll
+            for (int r = 0; r <= D; r++) {
+                okr[r] = K::make(s, r, kp, key[r]) ? 1 : 0;
+                wt[r] = s.b[r];
+            }
+            const int rot = (int)(threadIdx.x % (D + 1));
+            rotate_left<D + 1>(okr, rot);
+            rotate_left<D + 1>(wt, rot);
+#pragma unroll
+            for (int w = 0; w < W; w++) {
+                uint64_t col[D + 1];
+#pragma unroll
+                for (int r = 0; r <= D; r++) col[r] = key[r][w];
+                rotate_left<D + 1>(col, rot);
+#pragma unroll
+                for (int r = 0; r <= D; r++) key[r][w] = col[r];
+            }
+#pragma unroll
+            for (int r = 0; r <= D; r++) {
+                const uint64_t(&k)[W] = key[r];
+                if (!okr[r]) { flags |= kFlagRange; continue; }
+                const int ls = lds_insert<W, LT, kLdsProbes>(lk, k);
+                if (ls >= 0) {
+#pragma unroll
+                    for (int c = 0; c < LVD; c++)
+                        if (c < vd) atomicAdd(&lv[c * LT + ls], to_fixed(wt[r] * v[c], vsc.s[c]));
+                } else if (!full) {
+                    const int slot = table_insert<W>(ta.tkeys, ta.mask, k, probes);
+                    inserts++;
+                    if (slot < 0) { flags |= kFlagFull; continue; }
+                    unsigned long long *dst = ta.tacc + (size_t)slot * vd;
+#pragma unroll
+                    for (int c = 0; c < LVD; c++)
+                        if (c < vd) atomicAdd(dst + c, to_fixed(wt[r] * v[c], vsc.s[c]));
+                    atomics += vd;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    list_tile<W, LT>(lk, llist, &lcount);
+    __syncthreads();
+
+    // Flush: LVD lanes per distinct key, lane c adding channel c, so one
+    // instruction's atomics cover whole contiguous vertex rows.  Once any
+    // insert has found the table full the call is redone with a larger table:
+    // skip the remaining global work (a stale read only costs time).
+    const unsigned cnt = full ? 0u : lcount;
+    const int c = (int)(threadIdx.x % LVD);
+    for (unsigned base = 0; base < cnt; base += kBlock / LVD) {
+        const unsigned j = base + threadIdx.x / LVD;
+        const bool act = j < cnt;
+        const int ls = act ? (int)llist[j] : 0;
+        int slot = -1;
+        if (act && c == 0) {
+            uint64_t k[W];
+#pragma unroll
+            for (int w = 0; w < W; w++) k[w] = lk[ls * W + w];
+            slot = table_insert<W>(ta.tkeys, ta.mask, k, probes);
+            inserts++;
+            if (slot < 0) flags |= kFlagFull;
+        }
+        slot = __shfl(slot, (int)(__lane_id() & ~(LVD - 1)), kWave);
+        if (act && slot >= 0 && c < vd) {
+            atomicAdd(ta.tacc + (size_t)slot * vd + c, lv[c * LT + ls]);
+            atomics++;
+        }
+    }
+    report_flags(ctr, flags);
+    if (count) report_counts(ctr, probes, inserts, atomics);
+}
+
+// splat, direct form (D > 6 or vd > 4): every (pixel, vertex) goes to the
+// global table with FP32 atomics.
+template <int D, bool PACKED, bool BILAT>
+__global__ __launch_bounds__(kBlock) void k_splat_direct(PixSrc src, int n, int row_w, int tile_w, int vd,
+                                                         LatConst lc, KeyPlan kp, ValScale vsc, TableArgs ta,
+                                                         Counters *ctr, int count) {
+    using K = KeyOps<D, PACKED>;
+    constexpr int W = K::W;
+    const int p = pixel_index(blockIdx.x, threadIdx.x, n, row_w, tile_w);
+    unsigned flags = 0;
+    uint32_t probes = 0, inserts = 0, atomics = 0;
+    const bool full = *(volatile const unsigned *)&ctr->flags & kFlagFull;
+    if (p >= 0 && !full) {
+        float pos[D], v[kMaxVD];
+        load_position<D, BILAT>(src, p, pos);
+        load_values<kMaxVD>(src, p, v);
+#pragma unroll
+        for (int c = 0; c < kMaxVD; c++)
+            if (c < vd && !(fabsf(v[c]) <= vsc.vmax[c])) flags |= kFlagRange;
+        Simplex<D> s;
+        simplex<D>(pos, lc.sf, s);
+        if (s.bad) {
+            flags |= kFlagBad;
+        } else {
+#pragma unroll
+            for (int r = 0; r <= D; r++) {
+                uint64_t k[W];
+                if (!K::make(s, r, kp, k)) { flags |= kFlagRange; continue; }
+                const int slot = table_insert<W>(ta.tkeys, ta.mask, k, probes);
+                inserts++;
+                if (slot < 0) { flags |= kFlagFull; continue; }
+                unsigned long long *dst = ta.tacc + (size_t)slot * vd;
+#pragma unroll
+                for (int c = 0; c < kMaxVD; c++)
+                    if (c < vd) atomicAdd(dst + c, to_fixed(s.b[r] * v[c], vsc.s[c]));
+                atomics += vd;
+            }
+        }
+    }
+    report_flags(ctr, flags);
+    if (count) report_counts(ctr, probes, inserts, atomics);
+}
+
+// ---------------------------------------------------------------------------
+// compaction: occupied slots -> list, in slot order, by count / scan / write.
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_count(const uint64_t *__restrict__ tkeys, uint32_t cap,
+                                                  uint32_t *__restrict__ counts) {
+    __shared__ unsigned ws[kBlock / kWave];
+    const size_t base = (size_t)blockIdx.x * kCompactSlots;
+    unsigned c = 0;
+#pragma unroll
+    for (int k = 0; k < kCompactSlots / kBlock; k++) {
+        const size_t s = base + (size_t)k * kBlock + threadIdx.x;
+        c += (s < cap && tkeys[s * W] != kEmpty) ? 1u : 0u;
+    }
+    c = wave_sum(c);
+    if (__lane_id() == 0) ws[threadIdx.x / kWave] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned t = 0;
+        for (int w = 0; w < kBlock / kWave; w++) t += ws[w];
+        counts[blockIdx.x] = t;
+    }
+}
+
+template <int W, bool PACKED>
+__global__ __launch_bounds__(kBlock) void k_write_list(const uint64_t *__restrict__ tkeys, uint32_t cap,
+                                                       const uint32_t *__restrict__ offsets,
+                                                       uint32_t *__restrict__ list, uint64_t *__restrict__ sort_keys) {
+    constexpr int NW = kBlock / kWave;
+    __shared__ unsigned ws[NW];
+    const size_t base = (size_t)blockIdx.x * kCompactSlots;
+    unsigned run = offsets[blockIdx.x];
+    const int wave = threadIdx.x / kWave;
+    for (int k = 0; k < kCompactSlots / kBlock; k++) {
+        const size_t s = base + (size_t)k * kBlock + threadIdx.x;
+        uint64_t w0 = kEmpty;
+        if (s < cap) w0 = tkeys[s * W];
+        const bool occ = w0 != kEmpty;
+        const uint64_t m = __ballot(occ);
+        if (__lane_id() == 0) ws[wave] = (unsigned)__popcll(m);
+        __syncthreads();
+        unsigned off = run, tot = 0;
+#pragma unroll
+        for (int w = 0; w < NW; w++) {
+            off += (w < wave) ? ws[w] : 0u;
+            tot += ws[w];
+        }
+        if (occ) {
+            const unsigned pos = off + (unsigned)__popcll(m & lanemask_lt());
+            list[pos] = (uint32_t)s;
+            if (PACKED) sort_keys[pos] = w0;
+        }
+        run += tot;
+        __syncthreads();
+    }
+}
+
+// Wide keys -> packed sort key (big integer of pw words, word 0 least
+// significant) by the plan's offsets; coordinates outside the plan flag a
+// replan.
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_pack(TableArgs ta, const uint32_t *__restrict__ list, const Counters *ctr,
+                                                 int stride, KeyPlan kp, int pw, uint64_t *__restrict__ pk,
+                                                 Counters *wctr) {
+    using K = KeyOps<D, false>;
+    constexpr int W = K::W;
+    const unsigned M = ctr->M;
+    const unsigned i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= M) return;
+    const uint32_t s = list[i];
+    uint64_t k[W];
+#pragma unroll
+    for (int w = 0; w < W; w++) k[w] = ta.tkeys[(size_t)s * W + w];
+    uint64_t out[4] = {0, 0, 0, 0};
+    bool ok = true;
+#pragma unroll
+    for (int c = 0; c < D; c++) {
+        const uint32_t rel = (uint32_t)(K::coord(k, c, kp) - kp.lo[c]);
+        ok &= rel <= kp.span[c];
+        const uint64_t v = rel & kp.span[c];
+        const int off = kp.shift[c];
+        const int w0 = off >> 6, sh = off & 63;
+#pragma unroll
+        for (int w = 0; w < 4; w++) {
+            if (w == w0) out[w] |= v << sh;
+            if (w == w0 + 1 && sh != 0) out[w] |= v >> (64 - sh);
+        }
+    }
+    if (!ok) atomicOr(&wctr->flags, kFlagRange);
+#pragma unroll
+    for (int w = 0; w < 4; w++)
+        if (w < pw) pk[(size_t)w * stride + i] = out[w];
+}
+
+// finalize: sorted position i -> vertex id i.  Records slot -> id, copies
+// the key, and converts the fixed-point splat sums to the storage type (one
+// rounding: int64 -> double is exact below 2^53, then one FP32 rounding).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_finalize(TableArgs ta, ValScale vsc, int kw,
+                                                     const uint32_t *__restrict__ sorted, const Counters *ctr, int vd,
+                                                     int vs, uint64_t *__restrict__ vkeys, T *__restrict__ vals,
+                                                     float *__restrict__ splat_copy) {
+    const unsigned M = ctr->M;
+    const unsigned i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= M) return;
+    const uint32_t s = sorted[i];
+    ta.slot_id[s] = (int)i;
+    for (int w = 0; w < kw; w++) vkeys[(size_t)i * kw + w] = ta.tkeys[(size_t)s * kw + w];
+    float o[kMaxVD];
+#pragma unroll
+    for (int c = 0; c < kMaxVD; c++)
+        o[c] = (c < vd) ? (float)((double)(long long)ta.tacc[(size_t)s * vd + c] * (double)vsc.inv_s[c]) : 0.0f;
+    store_row_rt<T>(vals, i, vs, o);
+    if (splat_copy)
+        for (int c = 0; c < vd; c++) splat_copy[(size_t)i * vd + c] = o[c];
+}
+
+// Neighbour table (src/Permutohedral.h:446-451): along axis j < D the two
+// neighbours are k + 1 - (D+1) e_j and k - 1 + (D+1) e_j on the first D
+// coordinates; axis D moves the implicit coordinate, i.e. k + 1 and k - 1.
+// A neighbour outside the packing plan cannot be in the lattice.  One thread
+// per (vertex, axis): blockIdx.y is the axis.
+template <int D, bool PACKED>
+__global__ __launch_bounds__(kBlock) void k_neighbors(TableArgs ta, KeyPlan kp, const uint64_t *__restrict__ vkeys,
+                                                      const Counters *ctr, int stride, int2 *__restrict__ nbr,
+                                                      Counters *wctr, int count) {
+    using K = KeyOps<D, PACKED>;
+    constexpr int W = K::W;
+    const unsigned M = ctr->M;
+    const unsigned i = blockIdx.x * kBlock + threadIdx.x;
+    const int j = blockIdx.y;
+    uint32_t probes = 0;
+    if (i < M) {
+        uint64_t k[W];
+#pragma unroll
+        for (int w = 0; w < W; w++) k[w] = vkeys[(size_t)i * W + w];
+        int ca[D], cb[D];
+#pragma unroll
+        for (int q = 0; q < D; q++) {
+            const int c = K::coord(k, q, kp);
+            ca[q] = c + ((q == j) ? -D : 1);
+            cb[q] = c + ((q == j) ? D : -1);
+        }
+        uint64_t a[W], b[W];
+        const bool oka = K::from_coords(ca, kp, a);
+        const bool okb = K::from_coords(cb, kp, b);
+        const int sa = oka ? table_find<W>(ta.tkeys, ta.mask, a, probes) : -1;
+        const int sb = okb ? table_find<W>(ta.tkeys, ta.mask, b, probes) : -1;
+        nbr[(size_t)j * stride + i] = make_int2(sa >= 0 ? ta.slot_id[sa] : -1, sb >= 0 ? ta.slot_id[sb] : -1);
+    }
+    if (count) {
+        const unsigned long long pr = wave_sum((unsigned long long)probes);
+        if (__lane_id() == 0 && pr) atomicAdd(&wctr->probes, pr);
+    }
+}
+
+// One blur pass along one axis (src/Permutohedral.h:442-475): Jacobi, new =
+// 0.25*a + 0.5*self + 0.25*b evaluated left to right, missing neighbour = 0.
+template <typename T, int VS>
+__global__ __launch_bounds__(kBlock) void k_blur(const int2 *__restrict__ nbr, const Counters *ctr,
+                                                 const T *__restrict__ in, T *__restrict__ out) {
+    const unsigned M = ctr->M;
+    const unsigned i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= M) return;
+    const int2 nb = nbr[i];
+    float a[kMaxVD], o[kMaxVD], b[kMaxVD], r[kMaxVD];
+#pragma unroll
+    for (int c = 0; c < VS; c++) { a[c] = 0.0f; b[c] = 0.0f; }
+    if (nb.x >= 0) load_row<T, VS>(in, nb.x, a);
+    load_row<T, VS>(in, i, o);
+    if (nb.y >= 0) load_row<T, VS>(in, nb.y, b);
+#pragma unroll
+    for (int c = 0; c < VS; c++) r[c] = (0.25f * a[c] + 0.5f * o[c]) + 0.25f * b[c];
+    store_row<T, VS>(out, i, r);
+}
+
+// Final write of one pixel: raw vd channels, or the first vd-1 divided by the
+// homogeneous last one (true division, src/GaussTransform.cpp:673-676).
+template <int NV>
+__device__ __forceinline__ void write_pixel(const OutDst &dst, int p, int vd, const float (&acc)[kMaxVD]) {
+    if (dst.normalize) {
+        float h = acc[0];
+#pragma unroll
+        for (int c = 1; c < NV; c++) h = (c == vd - 1) ? acc[c] : h;
+#pragma unroll
+        for (int c = 0; c < NV; c++)
+            if (c < vd - 1) dst.out[(int64_t)c * dst.out_cs + p] = acc[c] / h;
+    } else {
+#pragma unroll
+        for (int c = 0; c < NV; c++)
+            if (c < vd) dst.out[(int64_t)c * dst.out_cs + p] = acc[c];
+    }
+}
+
+// Slice, tiled form (src/Permutohedral.h:405-427): the tile's distinct
+// vertices are looked up once and their blurred values staged in LDS
+// (channel-major); each pixel then sums weight * value over its D+1 vertices
+// in vertex order.  A vertex absent from the lattice (slice positions differ
+// from the splat positions) contributes nothing; a key that found no LDS
+// slot is looked up in the global table directly.
+template <int D, bool PACKED, bool BILAT, typename T, int LVD, int LT>
+__global__ __launch_bounds__(kBlock) void k_slice_tile(PixSrc src, int n, int row_w, int tile_w, int vd,
+                                                       LatConst lc, KeyPlan kp, TableArgs ta,
+                                                       const T *__restrict__ vals, OutDst dst, Counters *ctr,
+                                                       int count) {
+    using K = KeyOps<D, PACKED>;
+    constexpr int W = K::W;
+    __shared__ uint64_t lk[LT * W];
+    __shared__ float lv[LT * LVD];
+    __shared__ uint16_t llist[LT];
+    __shared__ unsigned lcount;
+    for (int i = threadIdx.x; i < LT * W; i += kBlock) lk[i] = kEmpty;
+    if (threadIdx.x == 0) lcount = 0;
+    const int p = pixel_index(blockIdx.x, threadIdx.x, n, row_w, tile_w);
+    __syncthreads();
+
+    int lsl[D + 1];   // LDS slot, -1 absent (outside the plan), -2 look up globally
+    Simplex<D> s;
+    unsigned flags = 0;
+    bool ok = false;
+#pragma unroll
+    for (int r = 0; r <= D; r++) lsl[r] = -1;
+    if (p >= 0) {
+        float pos[D];
+        load_position<D, BILAT>(src, p, pos);
+        simplex<D>(pos, lc.sf, s);
+        ok = !s.bad;
+        if (!ok) {
+            flags |= kFlagBad;
+        } else {
+            // Insert in a per-lane rotated vertex order (fewer LDS conflicts,
+            // as in the splat), then restore vertex order for the sums.
+            uint64_t key[D + 1][W];
+            int okr[D + 1];
+#pragma unroll
+            for (int r = 0; r <= D; r++) okr[r] = K::make(s, r, kp, key[r]) ? 1 : 0;
+            const int rot = (int)(threadIdx.x % (D + 1));
+            rotate_left<D + 1>(okr, rot);
+#pragma unroll
+            for (int w = 0; w < W; w++) {
+                uint64_t col[D + 1];
+#pragma unroll
+                for (int r = 0; r <= D; r++) col[r] = key[r][w];
+                rotate_left<D + 1>(col, rot);
+#pragma unroll
+                for (int r = 0; r <= D; r++) key[r][w] = col[r];
+            }
+#pragma unroll
+            for (int r = 0; r <= D; r++) {
+                if (okr[r]) {
+                    const int ls = lds_insert<W, LT, kLdsProbes>(lk, key[r]);
+                    lsl[r] = ls >= 0 ? ls : -2;
+                }
+            }
+            rotate_left<D + 1>(lsl, (D + 1 - rot) % (D + 1));
+        }
+    }
+    __syncthreads();
+    list_tile<W, LT>(lk, llist, &lcount);
+    __syncthreads();
+
+    const unsigned cnt = lcount;
+    uint32_t probes = 0;
+    for (unsigned j = threadIdx.x; j < cnt; j += kBlock) {
+        const int ls = llist[j];
+        uint64_t k[W];
+#pragma unroll
+        for (int q = 0; q < W; q++) k[q] = lk[ls * W + q];
+        const int slot = table_find<W>(ta.tkeys, ta.mask, k, probes);
+        float row[kMaxVD];
+#pragma unroll
+        for (int c = 0; c < LVD; c++) row[c] = 0.0f;
+        if (slot >= 0) load_row<T, LVD>(vals, ta.slot_id[slot], row);
+#pragma unroll
+        for (int c = 0; c < LVD; c++) lv[c * LT + ls] = row[c];
+    }
+    __syncthreads();
+
+    if (ok) {
+        float acc[kMaxVD];
+#pragma unroll
+        for (int c = 0; c < LVD; c++) acc[c] = 0.0f;
+#pragma unroll
+        for (int r = 0; r <= D; r++) {
+            if (lsl[r] >= 0) {
+#pragma unroll
+                for (int c = 0; c < LVD; c++) acc[c] = acc[c] + s.b[r] * lv[c * LT + lsl[r]];
+            } else if (lsl[r] == -2) {
+                uint64_t k[W];
+                K::make(s, r, kp, k);
+                const int slot = table_find<W>(ta.tkeys, ta.mask, k, probes);
+                if (slot >= 0) {
+                    float row[kMaxVD];
+                    load_row<T, LVD>(vals, ta.slot_id[slot], row);
+#pragma unroll
+                    for (int c = 0; c < LVD; c++) acc[c] = acc[c] + s.b[r] * row[c];
+                }
+            }
+        }
+        write_pixel<LVD>(dst, p, vd, acc);
+    }
+    report_flags(ctr, flags);
+    if (count) report_counts(ctr, probes, 0, 0);
+}
+
+// Slice, direct form (D > 6 or vd > 4): per-pixel global lookups.
+template <int D, bool PACKED, bool BILAT, typename T>
+__global__ __launch_bounds__(kBlock) void k_slice_direct(PixSrc src, int n, int vd, int vs, LatConst lc, KeyPlan kp,
+                                                         TableArgs ta, const T *__restrict__ vals, OutDst dst,
+                                                         Counters *ctr, int count) {
+    using K = KeyOps<D, PACKED>;
+    constexpr int W = K::W;
+    const int p = blockIdx.x * kBlock + threadIdx.x;
+    uint32_t probes = 0;
+    unsigned flags = 0;
+    if (p < n) {
+        float pos[D];
+        load_position<D, BILAT>(src, p, pos);
+        Simplex<D> s;
+        simplex<D>(pos, lc.sf, s);
+        float acc[kMaxVD];
+#pragma unroll
+        for (int c = 0; c < kMaxVD; c++) acc[c] = 0.0f;
+        if (s.bad) {
+            flags |= kFlagBad;
+        } else {
+#pragma unroll
+            for (int r = 0; r <= D; r++) {
+                uint64_t k[W];
+                if (!K::make(s, r, kp, k)) continue;
+                const int slot = table_find<W>(ta.tkeys, ta.mask, k, probes);
+                if (slot < 0) continue;
+                float v[kMaxVD];
+                load_row_rt<T>(vals, ta.slot_id[slot], vs, v);
+#pragma unroll
+                for (int c = 0; c < kMaxVD; c++) acc[c] = acc[c] + s.b[r] * v[c];
+            }
+            write_pixel<kMaxVD>(dst, p, vd, acc);
+        }
+    }
+    report_flags(ctr, flags);
+    if (count) report_counts(ctr, probes, 0, 0);
+}
+
+template <int D, bool PACKED>
+__global__ __launch_bounds__(kBlock) void k_unpack_keys(KeyPlan kp, const uint64_t *__restrict__ vkeys, int M,
+                                                        int16_t *__restrict__ keys) {
+    using K = KeyOps<D, PACKED>;
+    constexpr int W = K::W;
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= M) return;
+    uint64_t k[W];
+#pragma unroll
+    for (int w = 0; w < W; w++) k[w] = vkeys[(size_t)i * W + w];
+#pragma unroll
+    for (int q = 0; q < D; q++) keys[(size_t)i * D + q] = (int16_t)K::coord(k, q, kp);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_store_to_f32(const T *__restrict__ vals, int M, int vd, int vs,
+                                                         float *__restrict__ out) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= M) return;
+    for (int c = 0; c < vd; c++) out[(size_t)i * vd + c] = to_f32<T>(vals[(size_t)i * vs + c]);
+}
+
+} // namespace gt

@@ -1,0 +1,47 @@
+// Shared helpers of the kernel launchers (gt_launch_*.hip).
+#pragma once
+#include "gt_kernels.h"
+
+namespace gt {
+
+inline unsigned blocks_for(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+#define GT_DISPATCH_D(d, MACRO)                                                      \
+    switch (d) {                                                                     \
+    case 1: MACRO(1); break;  case 2: MACRO(2); break;  case 3: MACRO(3); break;     \
+    case 4: MACRO(4); break;  case 5: MACRO(5); break;  case 6: MACRO(6); break;     \
+    case 7: MACRO(7); break;  case 8: MACRO(8); break;  case 9: MACRO(9); break;     \
+    case 10: MACRO(10); break; case 11: MACRO(11); break; case 12: MACRO(12); break; \
+    case 13: MACRO(13); break; case 14: MACRO(14); break; case 15: MACRO(15); break; \
+    case 16: MACRO(16); break;                                                       \
+    default: return hipErrorInvalidValue;                                            \
+    }
+
+#define GT_DISPATCH_DT(d, MACRO)                                                     \
+    switch (d) {                                                                     \
+    case 1: MACRO(1); break;  case 2: MACRO(2); break;  case 3: MACRO(3); break;     \
+    case 4: MACRO(4); break;  case 5: MACRO(5); break;  case 6: MACRO(6); break;     \
+    default: return hipErrorInvalidValue;                                            \
+    }
+
+inline int64_t pixel_blocks(int n, int row_w, int &tile_w) {
+    if (tile_w > 0 && row_w > 0 && (kBlock % tile_w) == 0 && tile_w <= kBlock) {
+        const int tiles_x = (row_w + tile_w - 1) / tile_w;
+        const int th = kBlock / tile_w;
+        const int rows = n / row_w;
+        return (int64_t)tiles_x * ((rows + th - 1) / th);
+    }
+    tile_w = 0;
+    return blocks_for(n, kBlock);
+}
+
+
+// Launchers for the lattice dimensions without an LDS tile table (d > 6).
+hipError_t launch_splat_direct_hi(int d, bool packed, bool bilat, const PixSrc &src, int n, int row_w, int tile_w,
+                                  int vd, const LatConst &lc, const KeyPlan &kp, const ValScale &vsc,
+                                  const TableArgs &ta, Counters *ctr, bool count, hipStream_t st);
+hipError_t launch_slice_direct_hi(int d, bool packed, bool bilat, int storage, const PixSrc &src, int n, int vd,
+                                  int vs, const LatConst &lc, const KeyPlan &kp, const TableArgs &ta, const void *vals,
+                                  const OutDst &dst, Counters *ctr, bool count, hipStream_t st);
+
+} // namespace gt

@@ -1,0 +1,281 @@
+// Launchers of the non-pixel lattice kernels: bounds, clear, compaction,
+// key packing, finalize, neighbour table, blur, export helpers.
+#include "gt_launch.h"
+
+namespace gt {
+
+// ---------------------------------------------------------------------------
+// Per-plane min/max of the finite values and a non-finite indicator (for the
+// key packing plan and the fixed-point value scales).
+__global__ __launch_bounds__(kBlock) void k_minmax(const float *__restrict__ planes, int64_t cs, int nplanes, int n,
+                                                   float *__restrict__ partial) {
+    __shared__ float smn[kBlock / kWave], smx[kBlock / kWave], snf[kBlock / kWave];
+    for (int c = 0; c < nplanes; c++) {
+        float mn = INFINITY, mx = -INFINITY, nf = 0.0f;
+        const float *pl = planes + (int64_t)c * cs;
+        for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+            const float v = pl[i];
+            if (isfinite(v)) {
+                mn = fminf(mn, v);
+                mx = fmaxf(mx, v);
+            } else {
+                nf = 1.0f;
+            }
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            mn = fminf(mn, __shfl_xor(mn, off, kWave));
+            mx = fmaxf(mx, __shfl_xor(mx, off, kWave));
+            nf = fmaxf(nf, __shfl_xor(nf, off, kWave));
+        }
+        if (__lane_id() == 0) {
+            smn[threadIdx.x / kWave] = mn;
+            smx[threadIdx.x / kWave] = mx;
+            snf[threadIdx.x / kWave] = nf;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int w = 1; w < kBlock / kWave; w++) {
+                mn = fminf(mn, smn[w]);
+                mx = fmaxf(mx, smx[w]);
+                nf = fmaxf(nf, snf[w]);
+            }
+            float *o = partial + ((size_t)blockIdx.x * nplanes + c) * 3;
+            o[0] = mn;
+            o[1] = mx;
+            o[2] = nf;
+        }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_minmax_final(const float *__restrict__ partial, int nblk, int nplanes,
+                                                         float *__restrict__ out) {
+    const int c = threadIdx.x;
+    if (c >= nplanes) return;
+    float mn = INFINITY, mx = -INFINITY, nf = 0.0f;
+    for (int b = 0; b < nblk; b++) {
+        const float *o = partial + ((size_t)b * nplanes + c) * 3;
+        mn = fminf(mn, o[0]);
+        mx = fmaxf(mx, o[1]);
+        nf = fmaxf(nf, o[2]);
+    }
+    out[c * 3 + 0] = mn;
+    out[c * 3 + 1] = mx;
+    out[c * 3 + 2] = nf;
+}
+
+// ---------------------------------------------------------------------------
+// clear: table words to kEmpty, accumulators and counters to zero (a fill
+// kernel rather than hipMemsetAsync, so the sequence can be graph-captured).
+__global__ __launch_bounds__(kBlock) void k_clear(uint64_t *__restrict__ tkeys, size_t nwords,
+                                                  unsigned long long *__restrict__ tacc, size_t nacc, Counters *ctr) {
+    const size_t stride = (size_t)gridDim.x * kBlock;
+    const size_t t0 = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    for (size_t i = t0; i < nwords; i += stride) tkeys[i] = kEmpty;
+    for (size_t i = t0; i < nacc; i += stride) tacc[i] = 0ull;
+    if (t0 == 0) {
+        ctr->M = 0;
+        ctr->flags = 0;
+        ctr->probes = 0;
+        ctr->inserts = 0;
+        ctr->atomics = 0;
+    }
+}
+
+// Exclusive scan of n uint32 in one workgroup of 1024; the total goes to *total.
+__global__ __launch_bounds__(1024) void k_scan_1wg(uint32_t *__restrict__ a, int n, unsigned *total) {
+    __shared__ uint32_t part[1024];
+    const int chunk = (n + 1023) / 1024;
+    const int lo = threadIdx.x * chunk;
+    const int hi = min(n, lo + chunk);
+    uint32_t s = 0;
+    for (int i = lo; i < hi; i++) s += a[i];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const uint32_t v = threadIdx.x >= off ? part[threadIdx.x - off] : 0;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint32_t run = part[threadIdx.x] - s;
+    for (int i = lo; i < hi; i++) {
+        const uint32_t v = a[i];
+        a[i] = run;
+        run += v;
+    }
+    if (threadIdx.x == 1023 && total) *total = part[1023];
+}
+
+// Strided view -> strided view copy, used to gather the non-dense views of
+// JointBilateral's zero-sigma recursion into dense scratch images and back.
+__global__ __launch_bounds__(kBlock) void k_view_copy(const float *__restrict__ src, int64_t sys, int64_t sts,
+                                                      int64_t scs, float *__restrict__ dst, int64_t dys, int64_t dts,
+                                                      int64_t dcs, int W, int H, int F, int64_t total) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= total) return;
+    const int x = (int)(i % W);
+    int64_t q = i / W;
+    const int y = (int)(q % H);
+    q /= H;
+    const int t = (int)(q % F);
+    const int64_t c = q / F;
+    dst[c * dcs + t * dts + y * dys + x] = src[c * scs + t * sts + y * sys + x];
+}
+
+
+hipError_t launch_minmax(const float *planes, int64_t cs, int nplanes, int n, float *partial, int nblk, float *out,
+                         hipStream_t st) {
+    if (nplanes <= 0 || n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_minmax, dim3(nblk), dim3(kBlock), 0, st, planes, cs, nplanes, n, partial);
+    hipLaunchKernelGGL(k_minmax_final, dim3(1), dim3(kBlock), 0, st, partial, nblk, nplanes, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_clear(uint64_t *tkeys, size_t nwords, unsigned long long *tacc, size_t nacc, Counters *ctr,
+                        hipStream_t st) {
+    const size_t m = nwords > nacc ? nwords : nacc;
+    unsigned g = blocks_for((int64_t)m, kBlock);
+    if (g > 8192) g = 8192;
+    if (g == 0) g = 1;
+    hipLaunchKernelGGL(k_clear, dim3(g), dim3(kBlock), 0, st, tkeys, nwords, tacc, nacc, ctr);
+    return hipGetLastError();
+}
+
+hipError_t launch_compact(int words, const TableArgs &ta, uint32_t capacity, uint32_t *counts, uint32_t *slot_list,
+                          uint64_t *sort_keys, bool packed, Counters *ctr, hipStream_t st) {
+    const unsigned nblk = blocks_for(capacity, kCompactSlots);
+    switch (words) {
+    case 1: hipLaunchKernelGGL(k_count<1>, dim3(nblk), dim3(kBlock), 0, st, ta.tkeys, capacity, counts); break;
+    case 2: hipLaunchKernelGGL(k_count<2>, dim3(nblk), dim3(kBlock), 0, st, ta.tkeys, capacity, counts); break;
+    case 3: hipLaunchKernelGGL(k_count<3>, dim3(nblk), dim3(kBlock), 0, st, ta.tkeys, capacity, counts); break;
+    case 4: hipLaunchKernelGGL(k_count<4>, dim3(nblk), dim3(kBlock), 0, st, ta.tkeys, capacity, counts); break;
+    default: return hipErrorInvalidValue;
+    }
+    hipLaunchKernelGGL(k_scan_1wg, dim3(1), dim3(1024), 0, st, counts, (int)nblk, &ctr->M);
+    switch (words) {
+    case 1:
+        if (packed) hipLaunchKernelGGL((k_write_list<1, true>), dim3(nblk), dim3(kBlock), 0, st, ta.tkeys, capacity,
+                                       counts, slot_list, sort_keys);
+        else hipLaunchKernelGGL((k_write_list<1, false>), dim3(nblk), dim3(kBlock), 0, st, ta.tkeys, capacity,
+                                counts, slot_list, sort_keys);
+        break;
+    case 2:
+        hipLaunchKernelGGL((k_write_list<2, false>), dim3(nblk), dim3(kBlock), 0, st, ta.tkeys, capacity, counts,
+                           slot_list, sort_keys);
+        break;
+    case 3:
+        hipLaunchKernelGGL((k_write_list<3, false>), dim3(nblk), dim3(kBlock), 0, st, ta.tkeys, capacity, counts,
+                           slot_list, sort_keys);
+        break;
+    case 4:
+        hipLaunchKernelGGL((k_write_list<4, false>), dim3(nblk), dim3(kBlock), 0, st, ta.tkeys, capacity, counts,
+                           slot_list, sort_keys);
+        break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_pack(int d, const TableArgs &ta, const uint32_t *slot_list, const Counters *ctr, int m_max,
+                       const KeyPlan &kp, int pw, uint64_t *pk, hipStream_t st) {
+    if (m_max <= 0) return hipSuccess;
+#define GT_PACK(D) hipLaunchKernelGGL(k_pack<D>, dim3(blocks_for(m_max, kBlock)), dim3(kBlock), 0, st, ta, slot_list, \
+                                      ctr, m_max, kp, pw, pk, (Counters *)ctr)
+    GT_DISPATCH_D(d, GT_PACK)
+#undef GT_PACK
+    return hipGetLastError();
+}
+
+hipError_t launch_finalize(int words, int storage, const TableArgs &ta, const ValScale &vsc, const uint32_t *sorted,
+                           const Counters *ctr, int m_max, int vd, int vs, uint64_t *vkeys, void *vals,
+                           float *splat_copy, hipStream_t st) {
+    if (m_max <= 0) return hipSuccess;
+    const dim3 g(blocks_for(m_max, kBlock)), b(kBlock);
+    if (storage == 0)
+        hipLaunchKernelGGL(k_finalize<float>, g, b, 0, st, ta, vsc, words, sorted, ctr, vd, vs, vkeys, (float *)vals,
+                           splat_copy);
+    else if (storage == 1)
+        hipLaunchKernelGGL(k_finalize<__hip_bfloat16>, g, b, 0, st, ta, vsc, words, sorted, ctr, vd, vs, vkeys,
+                           (__hip_bfloat16 *)vals, splat_copy);
+    else
+        hipLaunchKernelGGL(k_finalize<__half>, g, b, 0, st, ta, vsc, words, sorted, ctr, vd, vs, vkeys,
+                           (__half *)vals, splat_copy);
+    return hipGetLastError();
+}
+
+hipError_t launch_neighbors(int d, bool packed, const TableArgs &ta, const KeyPlan &kp, const uint64_t *vkeys,
+                            const Counters *ctr, int m_max, int2 *nbr, Counters *wctr, bool count, hipStream_t st) {
+    if (m_max <= 0) return hipSuccess;
+    const int cnt = count ? 1 : 0;
+    const dim3 g(blocks_for(m_max, kBlock), d + 1), b(kBlock);
+#define GT_NBR(D)                                                                                          \
+    {                                                                                                      \
+        if (packed) hipLaunchKernelGGL((k_neighbors<D, true>), g, b, 0, st, ta, kp, vkeys, ctr, m_max, nbr, \
+                                       wctr, cnt);                                                          \
+        else hipLaunchKernelGGL((k_neighbors<D, false>), g, b, 0, st, ta, kp, vkeys, ctr, m_max, nbr, wctr, \
+                                cnt);                                                                       \
+    }
+    GT_DISPATCH_D(d, GT_NBR)
+#undef GT_NBR
+    return hipGetLastError();
+}
+
+template <typename T>
+static hipError_t launch_blur_t(int vs, const int2 *nbr, const Counters *ctr, int m_max, const void *in, void *out,
+                                hipStream_t st) {
+    const dim3 g(blocks_for(m_max, kBlock)), b(kBlock);
+    const T *i = (const T *)in;
+    T *o = (T *)out;
+    switch (vs) {
+    case 2: hipLaunchKernelGGL((k_blur<T, 2>), g, b, 0, st, nbr, ctr, i, o); break;
+    case 4: hipLaunchKernelGGL((k_blur<T, 4>), g, b, 0, st, nbr, ctr, i, o); break;
+    case 8: hipLaunchKernelGGL((k_blur<T, 8>), g, b, 0, st, nbr, ctr, i, o); break;
+    case 16: hipLaunchKernelGGL((k_blur<T, 16>), g, b, 0, st, nbr, ctr, i, o); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_blur(int storage, int vs, const int2 *nbr, const Counters *ctr, int m_max, const void *in, void *out,
+                       hipStream_t st) {
+    if (m_max <= 0) return hipSuccess;
+    if (storage == 0) return launch_blur_t<float>(vs, nbr, ctr, m_max, in, out, st);
+    if (storage == 1) return launch_blur_t<__hip_bfloat16>(vs, nbr, ctr, m_max, in, out, st);
+    return launch_blur_t<__half>(vs, nbr, ctr, m_max, in, out, st);
+}
+
+hipError_t launch_unpack_keys(int d, bool packed, const KeyPlan &kp, const uint64_t *vkeys, int M, int16_t *keys,
+                              hipStream_t st) {
+    if (M <= 0) return hipSuccess;
+    const dim3 g(blocks_for(M, kBlock)), b(kBlock);
+#define GT_UNPACK(D)                                                                                       \
+    {                                                                                                      \
+        if (packed) hipLaunchKernelGGL((k_unpack_keys<D, true>), g, b, 0, st, kp, vkeys, M, keys);         \
+        else hipLaunchKernelGGL((k_unpack_keys<D, false>), g, b, 0, st, kp, vkeys, M, keys);               \
+    }
+    GT_DISPATCH_D(d, GT_UNPACK)
+#undef GT_UNPACK
+    return hipGetLastError();
+}
+
+hipError_t launch_store_to_f32(int storage, const void *vals, int M, int vd, int vs, float *out, hipStream_t st) {
+    if (M <= 0) return hipSuccess;
+    const dim3 g(blocks_for(M, kBlock)), b(kBlock);
+    if (storage == 0) hipLaunchKernelGGL(k_store_to_f32<float>, g, b, 0, st, (const float *)vals, M, vd, vs, out);
+    else if (storage == 1)
+        hipLaunchKernelGGL(k_store_to_f32<__hip_bfloat16>, g, b, 0, st, (const __hip_bfloat16 *)vals, M, vd, vs, out);
+    else hipLaunchKernelGGL(k_store_to_f32<__half>, g, b, 0, st, (const __half *)vals, M, vd, vs, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_view_copy(const float *src, int64_t sys, int64_t sts, int64_t scs, float *dst, int64_t dys,
+                            int64_t dts, int64_t dcs, int W, int H, int F, int C, hipStream_t st) {
+    const int64_t total = (int64_t)W * H * F * C;
+    if (total <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_view_copy, dim3(blocks_for(total, kBlock)), dim3(kBlock), 0, st, src, sys, sts, scs, dst, dys,
+                       dts, dcs, W, H, F, total);
+    return hipGetLastError();
+}
+
+} // namespace gt

@@ -1,0 +1,62 @@
+// Slice launcher: LDS-tiled slice for d <= 6 and vd <= 4, direct slice
+// otherwise (d > 6 in gt_launch_slice_hi.hip).
+#include "gt_launch.h"
+
+namespace gt {
+
+hipError_t launch_slice(int d, bool packed, bool bilat, bool tiled, int storage, const PixSrc &src, int n, int row_w,
+                        int tile_w, int vd, int vs, const LatConst &lc, const KeyPlan &kp, const TableArgs &ta,
+                        const void *vals, const OutDst &dst, Counters *ctr, bool count, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    if (d > 6) return launch_slice_direct_hi(d, packed, bilat, storage, src, n, vd, vs, lc, kp, ta, vals, dst, ctr,
+                                             count, st);
+    const int cnt = count ? 1 : 0;
+    tiled = tiled && vs <= 4;
+    if (tiled) {
+        const int64_t nblk = pixel_blocks(n, row_w, tile_w);
+#define GT_SLT(D, P, B, T)                                                                                      \
+    {                                                                                                           \
+        if (vs <= 2) hipLaunchKernelGGL((k_slice_tile<D, P, B, T, 2, kTileKeys>), dim3(nblk), dim3(kBlock), 0, st, src, n,  \
+                                        row_w, tile_w, vd, lc, kp, ta, (const T *)vals, dst, ctr, cnt);          \
+        else hipLaunchKernelGGL((k_slice_tile<D, P, B, T, 4, kTileKeys>), dim3(nblk), dim3(kBlock), 0, st, src, n, row_w,   \
+                                tile_w, vd, lc, kp, ta, (const T *)vals, dst, ctr, cnt);                         \
+    }
+#define GT_SLT_S(D, P, B)                                              \
+    {                                                                  \
+        if (storage == 0) GT_SLT(D, P, B, float)                       \
+        else if (storage == 1) GT_SLT(D, P, B, __hip_bfloat16)         \
+        else GT_SLT(D, P, B, __half)                                   \
+    }
+#define GT_SLICE_TILE(D)                                                                 \
+    {                                                                                    \
+        if (packed) { if (bilat) GT_SLT_S(D, true, true) else GT_SLT_S(D, true, false) } \
+        else { if (bilat) GT_SLT_S(D, false, true) else GT_SLT_S(D, false, false) }      \
+    }
+        GT_DISPATCH_DT(d, GT_SLICE_TILE)
+#undef GT_SLT
+#undef GT_SLT_S
+#undef GT_SLICE_TILE
+    } else {
+        const dim3 g(blocks_for(n, kBlock)), b(kBlock);
+#define GT_SLD(D, P, B, T) hipLaunchKernelGGL((k_slice_direct<D, P, B, T>), g, b, 0, st, src, n, vd, vs, lc, kp, ta, \
+                                              (const T *)vals, dst, ctr, cnt)
+#define GT_SLD_S(D, P, B)                                              \
+    {                                                                  \
+        if (storage == 0) GT_SLD(D, P, B, float);                      \
+        else if (storage == 1) GT_SLD(D, P, B, __hip_bfloat16);        \
+        else GT_SLD(D, P, B, __half);                                  \
+    }
+#define GT_SLICE_DIRECT(D)                                                               \
+    {                                                                                    \
+        if (packed) { if (bilat) GT_SLD_S(D, true, true) else GT_SLD_S(D, true, false) } \
+        else { if (bilat) GT_SLD_S(D, false, true) else GT_SLD_S(D, false, false) }      \
+    }
+        GT_DISPATCH_DT(d, GT_SLICE_DIRECT)
+#undef GT_SLD
+#undef GT_SLD_S
+#undef GT_SLICE_DIRECT
+    }
+    return hipGetLastError();
+}
+
+} // namespace gt

@@ -3,8 +3,13 @@
 // histogram -> exclusive scan -> stable scatter.  Stability inside a
 // workgroup comes from processing items in index order, 256 at a time, and
 // ranking equal digits inside each wave with eight ballots (no LDS sort).
+// The item count lives on the device (Counters::M): grids are sized for the
+// host's upper bound and idle workgroups exit at once, so no host round trip
+// sits between the compaction and the sort.
 #include "gt_device.h"
 #include "gt_internal.h"
+
+#include <utility>
 
 namespace gt {
 
@@ -12,29 +17,42 @@ constexpr int kSortThreads = 256;
 constexpr int kSortTiles = 16;                        // 256-item tiles per workgroup
 constexpr int kSortItems = kSortThreads * kSortTiles; // items per workgroup
 
-__global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint64_t *__restrict__ kw, int M, int shift,
-                                                            int nblk, uint32_t *__restrict__ hist) {
+__device__ __forceinline__ int sort_blocks(unsigned M) { return (int)((M + kSortItems - 1) / kSortItems); }
+
+__global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint64_t *__restrict__ kw, const Counters *ctr,
+                                                            int shift, uint32_t *__restrict__ hist) {
+    const unsigned M = ctr->M;
+    const int nblk = sort_blocks(M);
+    if ((int)blockIdx.x >= nblk) return;
     __shared__ uint32_t h[256];
     h[threadIdx.x] = 0;
     __syncthreads();
-    const int base = blockIdx.x * kSortItems;
+    const unsigned base = blockIdx.x * kSortItems;
 #pragma unroll 4
     for (int t = 0; t < kSortTiles; t++) {
-        const int i = base + t * kSortThreads + threadIdx.x;
+        const unsigned i = base + t * kSortThreads + threadIdx.x;
         if (i < M) atomicAdd(&h[(kw[i] >> shift) & 255], 1u);
     }
     __syncthreads();
     hist[(size_t)threadIdx.x * nblk + blockIdx.x] = h[threadIdx.x];
 }
 
-// Exclusive scan of E entries in one workgroup of 1024 threads.
-__global__ __launch_bounds__(1024) void k_radix_scan(uint32_t *__restrict__ hist, int E) {
+// Exclusive scan of the digit-major histogram (256 * nblk entries, a multiple
+// of 4) in one workgroup of 1024 threads: each thread owns a contiguous run of
+// uint4s, loaded four at a time so the loads overlap.
+__global__ __launch_bounds__(1024) void k_radix_scan(uint32_t *__restrict__ hist, const Counters *ctr) {
     __shared__ uint32_t part[1024];
-    const int chunk = (E + 1023) / 1024;
+    const int E4 = 64 * sort_blocks(ctr->M);
+    const int chunk = (E4 + 1023) / 1024;
     const int lo = threadIdx.x * chunk;
-    const int hi = min(E, lo + chunk);
+    const int hi = min(E4, lo + chunk);
+    uint4 *h4 = reinterpret_cast<uint4 *>(hist);
     uint32_t s = 0;
-    for (int i = lo; i < hi; i++) s += hist[i];
+#pragma unroll 4
+    for (int i = lo; i < hi; i++) {
+        const uint4 v = h4[i];
+        s += v.x + v.y + v.z + v.w;
+    }
     part[threadIdx.x] = s;
     __syncthreads();
     for (int off = 1; off < 1024; off <<= 1) {
@@ -44,33 +62,41 @@ __global__ __launch_bounds__(1024) void k_radix_scan(uint32_t *__restrict__ hist
         __syncthreads();
     }
     uint32_t run = part[threadIdx.x] - s;
+#pragma unroll 4
     for (int i = lo; i < hi; i++) {
-        const uint32_t v = hist[i];
-        hist[i] = run;
-        run += v;
+        const uint4 v = h4[i];
+        uint4 o;
+        o.x = run; run += v.x;
+        o.y = run; run += v.y;
+        o.z = run; run += v.z;
+        o.w = run; run += v.w;
+        h4[i] = o;
     }
 }
 
 __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint64_t *__restrict__ pk_in,
                                                                const uint32_t *__restrict__ pay_in,
                                                                uint64_t *__restrict__ pk_out,
-                                                               uint32_t *__restrict__ pay_out, int M, int pw,
-                                                               int word, int shift, int nblk,
+                                                               uint32_t *__restrict__ pay_out, const Counters *ctr,
+                                                               int stride, int pw, int word, int shift,
                                                                const uint32_t *__restrict__ hist) {
     constexpr int NW = kSortThreads / kWave;
+    const unsigned M = ctr->M;
+    const int nblk = sort_blocks(M);
+    if ((int)blockIdx.x >= nblk) return;
     __shared__ uint32_t base[256];
     __shared__ uint32_t wcnt[NW][256];
     const int tid = threadIdx.x;
     const int wave = tid / kWave;
     base[tid] = hist[(size_t)tid * nblk + blockIdx.x];
-    const int b0 = blockIdx.x * kSortItems;
+    const unsigned b0 = blockIdx.x * kSortItems;
     for (int t = 0; t < kSortTiles; t++) {
 #pragma unroll
         for (int w = 0; w < NW; w++) wcnt[w][tid] = 0;
         __syncthreads();
-        const int i = b0 + t * kSortThreads + tid;
+        const unsigned i = b0 + t * kSortThreads + tid;
         const bool valid = i < M;
-        const uint32_t dg = valid ? (uint32_t)((pk_in[(size_t)word * M + i] >> shift) & 255) : 0u;
+        const uint32_t dg = valid ? (uint32_t)((pk_in[(size_t)word * stride + i] >> shift) & 255) : 0u;
         uint64_t peers = __ballot(valid);
 #pragma unroll
         for (int b = 0; b < 8; b++) {
@@ -85,7 +111,7 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint64_t *
             uint32_t off = base[dg] + (uint32_t)rank;
 #pragma unroll
             for (int w = 0; w < NW; w++) off += (w < wave) ? wcnt[w][dg] : 0u;
-            for (int w = 0; w < pw; w++) pk_out[(size_t)w * M + off] = pk_in[(size_t)w * M + i];
+            for (int w = 0; w < pw; w++) pk_out[(size_t)w * stride + off] = pk_in[(size_t)w * stride + i];
             pay_out[off] = pay_in[i];
         }
         __syncthreads();
@@ -97,28 +123,28 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint64_t *
     }
 }
 
-size_t radix_hist_entries(int M) {
-    const size_t nblk = ((size_t)M + kSortItems - 1) / kSortItems;
+size_t radix_hist_entries(int m_max) {
+    const size_t nblk = ((size_t)m_max + kSortItems - 1) / kSortItems;
     return 256 * (nblk ? nblk : 1);
 }
 
-hipError_t radix_sort(SortBufs &b, int M, int pw, int bits, bool *result_in_b, int *passes, hipStream_t st) {
+hipError_t radix_sort(SortBufs &b, const Counters *ctr, int m_max, int pw, int bits, bool *result_in_b, int *passes,
+                      hipStream_t st) {
     *result_in_b = false;
     *passes = 0;
-    if (M <= 1 || bits <= 0) return hipSuccess;
-    const int nblk = (M + kSortItems - 1) / kSortItems;
-    const size_t E = (size_t)256 * nblk;
-    if (E > b.hist_cap || E > 0x7fffffff) return hipErrorInvalidValue;
+    if (m_max <= 1 || bits <= 0) return hipSuccess;
+    const int nblk = (m_max + kSortItems - 1) / kSortItems;
+    if ((size_t)256 * nblk > b.hist_cap) return hipErrorInvalidValue;
     const int np = (bits + 7) / 8;
     uint64_t *pin = b.pk_a, *pout = b.pk_b;
     uint32_t *yin = b.pay_a, *yout = b.pay_b;
     for (int p = 0; p < np; p++) {
         const int word = p / 8, shift = (p % 8) * 8;
-        hipLaunchKernelGGL(k_radix_hist, dim3(nblk), dim3(kSortThreads), 0, st, pin + (size_t)word * M, M, shift,
-                           nblk, b.hist);
-        hipLaunchKernelGGL(k_radix_scan, dim3(1), dim3(1024), 0, st, b.hist, (int)E);
-        hipLaunchKernelGGL(k_radix_scatter, dim3(nblk), dim3(kSortThreads), 0, st, pin, yin, pout, yout, M, pw,
-                           word, shift, nblk, b.hist);
+        hipLaunchKernelGGL(k_radix_hist, dim3(nblk), dim3(kSortThreads), 0, st, pin + (size_t)word * m_max, ctr, shift,
+                           b.hist);
+        hipLaunchKernelGGL(k_radix_scan, dim3(1), dim3(1024), 0, st, b.hist, ctr);
+        hipLaunchKernelGGL(k_radix_scatter, dim3(nblk), dim3(kSortThreads), 0, st, pin, yin, pout, yout, ctr, m_max,
+                           pw, word, shift, b.hist);
         std::swap(pin, pout);
         std::swap(yin, yout);
     }

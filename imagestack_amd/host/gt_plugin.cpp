@@ -1,0 +1,199 @@
+// ImageStack plugin: GPU permutohedral lattice behind the reference's own
+// operations.  Load it into an unmodified ImageStack binary with
+//
+//     ImageStack -plugin libgt_plugin.so -load im.tmp -bilateral 0.1 8 8 0 permutohedral -save out.tmp
+//
+// init_imagestack_plugin (src/Plugin.cpp:28-35, examples/plugin.cpp:73-77)
+// replaces -gausstransform, -jointbilateral and -bilateral with subclasses
+// that keep the reference's argument grammar (src/GaussTransform.cpp:79-113,
+// :473-502, :699-728) and its AUTO method choice (:540-564).  Whenever the
+// method resolves to PERMUTOHEDRAL the lattice runs in libgtlattice.so; the
+// other methods (exact, grid, gkdtree) run the reference's own code.
+//
+// Built against the reference's headers (main.h is force-included, as the
+// reference's own build does), so Image and Operation match the host binary.
+#include "Operation.h"
+#include "GaussTransform.h"
+#include "Statistics.h"
+#include "gt_imagestack.h"
+
+namespace ImageStack {
+namespace GPU {
+
+GaussTransform::Method methodByName(const string &s) {
+    if (s == "exact") return GaussTransform::EXACT;
+    if (s == "grid") return GaussTransform::GRID;
+    if (s == "permutohedral") return GaussTransform::PERMUTOHEDRAL;
+    if (s == "gkdtree") return GaussTransform::GKDTREE;
+    panic("Unknown method %s\n", s.c_str());
+    return GaussTransform::AUTO;
+}
+
+// The method JointBilateral::apply would use for this call, or AUTO when the
+// zero-sigma recursion (:512-538) splits the image first (each piece then
+// chooses on its own, so the reference path is used for it).
+GaussTransform::Method resolveMethod(Image im, Image ref, float fw, float fh, float ff, GaussTransform::Method m) {
+    if (im.width == 1) fw = 0;
+    if (im.height == 1) fh = 0;
+    if (im.frames == 1) ff = 0;
+    if ((im.width > 1 && fw == 0) || (im.height > 1 && fh == 0) || (im.frames > 1 && ff == 0))
+        return m == GaussTransform::AUTO ? GaussTransform::AUTO : m;
+    if (m != GaussTransform::AUTO) return m;
+    int posChannels = ref.channels;
+    bool fx = im.width > 1 && fw < 10 * im.width;
+    bool fy = im.height > 1 && fh < 10 * im.height;
+    bool ft = im.frames > 1 && ff < 10 * im.frames;
+    posChannels += (fx ? 1 : 0) + (fy ? 1 : 0) + (ft ? 1 : 0);
+    int sx = fx ? ((int)(fw * 6 + 1) | 1) : 1;
+    int sy = fy ? ((int)(fh * 6 + 1) | 1) : 1;
+    int st = ft ? ((int)(ff * 6 + 1) | 1) : 1;
+    if (st * sx * sy < 16) return GaussTransform::EXACT;
+    if (posChannels <= 4) return GaussTransform::GRID;
+    if (posChannels <= 10) return GaussTransform::PERMUTOHEDRAL;
+    return GaussTransform::GKDTREE;
+}
+
+void jointBilateralDispatch(Image im, Image ref, float fw, float fh, float ff, float cs, GaussTransform::Method m) {
+    if (resolveMethod(im, ref, fw, fh, ff, m) == GaussTransform::PERMUTOHEDRAL)
+        jointBilateral(im, ref, fw, fh, ff, cs);
+    else
+        JointBilateral::apply(im, ref, fw, fh, ff, cs, m);
+}
+
+// Relative difference below tol everywhere (floor of 1e-3 of the largest
+// magnitude, so near-zero homogeneous weights do not dominate).
+bool closeTo(Image a, Image b, float tol) {
+    float mx = 0;
+    for (int c = 0; c < b.channels; c++)
+        for (int t = 0; t < b.frames; t++)
+            for (int y = 0; y < b.height; y++)
+                for (int x = 0; x < b.width; x++) mx = std::max(mx, fabsf(b(x, y, t, c)));
+    for (int c = 0; c < b.channels; c++)
+        for (int t = 0; t < b.frames; t++)
+            for (int y = 0; y < b.height; y++)
+                for (int x = 0; x < b.width; x++) {
+                    float den = std::max(fabsf(b(x, y, t, c)), 1e-3f * mx);
+                    if (fabsf(a(x, y, t, c) - b(x, y, t, c)) > tol * den) return false;
+                }
+    return true;
+}
+
+class GaussTransformGPU : public GaussTransform {
+public:
+    void help() {
+        GaussTransform::help();
+        pprintf("(GPU plugin: the permutohedral method runs on the GPU.)\n");
+    }
+
+    // GaussTransform::test (src/GaussTransform.cpp:39-77) with the GPU lattice:
+    // agreement with the exact transform in the homogeneous sense
+    // (nearlyEqual), and with the reference's own CPU lattice to 1e-4.
+    bool test() {
+        Image splat(56, 34, 2, 3);
+        Image slice(56, 34, 2, 3);
+        Image values(56, 34, 2, 2);
+        Noise::apply(splat, 0, 1);
+        Noise::apply(slice, 0, 1);
+        Noise::apply(values, 0, 1);
+        values.channel(1).set(1.0f);
+        vector<float> sigma(3);
+        sigma[0] = 0.2f;
+        sigma[1] = 0.3f;
+        sigma[2] = 0.4f;
+        Image correct = GaussTransform::apply(slice, splat, values, sigma, EXACT);
+        correct = correct.channel(0) / correct.channel(1);
+        Image cpu = GaussTransform::apply(slice, splat, values, sigma, PERMUTOHEDRAL);
+        Image gpu = gaussTransform(slice, splat, values, sigma);
+        if (!closeTo(gpu, cpu, 1e-4f)) return false;
+        Image out = gpu.channel(0) / gpu.channel(1);
+        return nearlyEqual(out, correct);
+    }
+
+    void parse(vector<string> args) {
+        Method m = EXACT;
+        assert(args.size() > 0, "-gausstransform takes at least one argument");
+        m = methodByName(args[0]);
+        vector<float> sigmas;
+        if (args.size() == 1) {
+            sigmas = vector<float>(stack(0).channels, 1.0);
+        } else if (args.size() == 2) {
+            sigmas = vector<float>(stack(0).channels, readFloat(args[1]));
+        } else if ((int)args.size() == 1 + stack(1).channels) {
+            for (int i = 0; i < stack(0).channels; i++) sigmas.push_back(readFloat(args[i + 1]));
+        } else {
+            panic("-gausstransform takes one argument, two arguments, or one plus the"
+                  " number of channels in the second image on the stack arguments\n");
+        }
+        Image im = (m == PERMUTOHEDRAL) ? gaussTransform(stack(0), stack(1), stack(2), sigmas)
+                                        : GaussTransform::apply(stack(0), stack(1), stack(2), sigmas, m);
+        pop();
+        push(im);
+    }
+};
+
+static void parseBilateralArgs(const vector<string> &args, const char *name, float &cs, float &fw, float &fh,
+                               float &ff, GaussTransform::Method &m) {
+    if (args.size() < 2 || args.size() > 5) panic("%s takes from two to five arguments\n", name);
+    ff = 0;
+    m = GaussTransform::AUTO;
+    cs = readFloat(args[0]);
+    fw = fh = readFloat(args[1]);
+    if (args.size() > 2) fh = readFloat(args[2]);
+    if (args.size() > 3) ff = readFloat(args[3]);
+    if (args.size() > 4) m = methodByName(args[4]);
+}
+
+class JointBilateralGPU : public JointBilateral {
+public:
+    void help() {
+        JointBilateral::help();
+        pprintf("(GPU plugin: the permutohedral method runs on the GPU.)\n");
+    }
+
+    // JointBilateral::test (src/GaussTransform.cpp:442-471) with the GPU
+    // lattice: nearlyEqual to EXACT, and within 1e-4 of the CPU lattice.
+    bool test() {
+        Image im(60, 50, 3, 3);
+        Image ref(im.width, im.height, im.frames, im.channels);
+        Noise::apply(im, 0, 1);
+        Noise::apply(ref, 0, 1);
+        Image correct = im.copy();
+        JointBilateral::apply(correct, ref, 3, 4, 1, 0.25, GaussTransform::EXACT);
+        Image cpu = im.copy();
+        JointBilateral::apply(cpu, ref, 3, 4, 1, 0.25, GaussTransform::PERMUTOHEDRAL);
+        Image out = im.copy();
+        jointBilateral(out, ref, 3, 4, 1, 0.25);
+        return closeTo(out, cpu, 1e-4f) && nearlyEqual(out, correct);
+    }
+
+    void parse(vector<string> args) {
+        float cs, fw, fh, ff;
+        GaussTransform::Method m;
+        parseBilateralArgs(args, "-jointbilateral", cs, fw, fh, ff, m);
+        jointBilateralDispatch(stack(0), stack(1), fw, fh, ff, cs, m);
+    }
+};
+
+class BilateralGPU : public Bilateral {
+public:
+    void help() {
+        Bilateral::help();
+        pprintf("(GPU plugin: the permutohedral method runs on the GPU.)\n");
+    }
+    bool test() { return true; }  // as the reference: it calls joint bilateral
+    void parse(vector<string> args) {
+        float cs, fw, fh, ff;
+        GaussTransform::Method m;
+        parseBilateralArgs(args, "-bilateral", cs, fw, fh, ff, m);
+        jointBilateralDispatch(stack(0), stack(0), fw, fh, ff, cs, m);
+    }
+};
+
+} // namespace GPU
+} // namespace ImageStack
+
+extern "C" void init_imagestack_plugin(map<string, ImageStack::Operation *> &operationMap) {
+    operationMap["-gausstransform"] = new ImageStack::GPU::GaussTransformGPU();
+    operationMap["-jointbilateral"] = new ImageStack::GPU::JointBilateralGPU();
+    operationMap["-bilateral"] = new ImageStack::GPU::BilateralGPU();
+}

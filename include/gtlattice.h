@@ -58,7 +58,7 @@ typedef struct {
     int block;           /* threads per workgroup of the pixel kernels: 64..1024   */
     float load_factor;   /* target hash-table load, 0.05..0.7 (default 0.35)       */
     int waves_per_cu;    /* occupancy of the persistent batch kernels (default 8)  */
-    int dedup;           /* workgroup LDS key dedup / pre-reduction: -1 auto, 0, 1 */
+    int dedup;           /* LDS tile dedup / pre-reduction: 0 auto, 1 on, -1 off     */
     int tile_w;          /* pixel-tile width of a workgroup (default: whole rows)  */
 } gt_tuning;
 

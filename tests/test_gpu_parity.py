@@ -105,16 +105,21 @@ def test_cfg1_key_set(gpu_ctx, oracle_lib):
     assert np.array_equal(g["nbr"], o["nbr"])
 
 
-@pytest.mark.parametrize("storage,tol", [("bf16", 2e-2), ("f16", 3e-3)])
-def test_narrow_storage(gpu_ctx, oracle_lib, storage, tol):
+@pytest.mark.parametrize("storage,l2_tol,range_tol", [("bf16", 5e-3, 2e-2), ("f16", 1e-3, 3e-3)])
+def test_narrow_storage(gpu_ctx, oracle_lib, storage, l2_tol, range_tol):
+    """BF16/FP16 lattice storage with FP32 accumulation.  The bars are on
+    relative L2 and on the largest error relative to the output range: an
+    8-bit (BF16) or 11-bit (FP16) significand bounds accuracy per stored value,
+    so near-zero outputs cannot meet a per-element relative bar."""
     O = oracle_lib
     case(f"storage {storage}")
     im = S.rect_image(128, 128, 3, K=16, noise=0.05, seed=9)
     out = gpu_ctx.joint_bilateral(dev(im), dev(im), 8.0, 8.0, 0.0, 0.1, storage=storage).cpu().numpy()
     o64 = O.joint_bilateral(im, im, 8.0, 8.0, 0.0, 0.1, mode=O.F64_LAT)
-    e = rel_err(out, o64)
-    print(storage, "err64", e, "relL2", rel_l2(out, o64))
-    assert e < tol
+    l2 = rel_l2(out, o64)
+    rng = float(np.abs(out - o64).max() / np.abs(o64).max())
+    print(storage, "relL2", l2, "max/range", rng, "max rel", rel_err(out, o64))
+    assert l2 < l2_tol and rng < range_tol
 
 
 def test_highdim_d16_small(gpu_ctx, oracle_lib):

@@ -1,0 +1,98 @@
+"""Drop-in through the reference's own plugin interface: the unmodified
+ImageStack CLI (built from the reference sources into oracle/_ref) loads
+imagestack_amd/libgt_plugin.so with -plugin and runs the GPU lattice behind
+its own -gausstransform / -jointbilateral / -bilateral operations."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from imagestack_amd import synth as S
+from tests.util import FP32_REL_TOL, rel_err
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CLI = os.path.join(ROOT, "oracle", "_ref", "ImageStack")
+PLUGIN = os.path.join(ROOT, "imagestack_amd", "libgt_plugin.so")
+
+
+def need_cli():
+    if not (os.path.exists(CLI) and os.path.exists(PLUGIN)):
+        pytest.skip("reference CLI or plugin not built (reference sources absent at build time)")
+
+
+def write_tmp(path, im):
+    """ImageStack .tmp: int32 width, height, frames, channels, type 0, then
+    planar float32 data by channel, frame, row (src/FileTMP.cpp)."""
+    C, F, H, W = im.shape
+    with open(path, "wb") as f:
+        np.array([W, H, F, C, 0], np.int32).tofile(f)
+        np.ascontiguousarray(im, np.float32).tofile(f)
+
+
+def read_tmp(path):
+    with open(path, "rb") as f:
+        W, H, F, C, t = np.fromfile(f, np.int32, 5)
+        assert t == 0
+        return np.fromfile(f, np.float32, W * H * F * C).reshape(C, F, H, W)
+
+
+def run(args, timeout=120):
+    print("[cli]", " ".join(os.path.basename(a) if a.startswith("/") else a for a in args), flush=True)
+    r = subprocess.run([CLI] + args, capture_output=True, text=True, timeout=timeout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    return r.stdout
+
+
+def test_plugin_runs_reference_style_tests():
+    need_cli()
+    out = run(["-plugin", PLUGIN, "-test", "gausstransform", "-test", "jointbilateral"])
+    assert out.count("Passed") == 2, out
+    assert "Failed" not in out
+
+
+def test_plugin_bilateral_matches_reference_cli(tmp_path, oracle_lib):
+    need_cli()
+    im = S.rect_image(256, 192, 3, K=24, noise=0.05, seed=31)
+    src = str(tmp_path / "in.tmp")
+    write_tmp(src, im)
+    cpu_path, gpu_path = str(tmp_path / "cpu.tmp"), str(tmp_path / "gpu.tmp")
+    run(["-load", src, "-bilateral", "0.1", "8", "8", "0", "permutohedral", "-save", cpu_path])
+    run(["-plugin", PLUGIN, "-load", src, "-bilateral", "0.1", "8", "8", "0", "permutohedral", "-save", gpu_path])
+    cpu, gpu = read_tmp(cpu_path), read_tmp(gpu_path)
+    o64 = oracle_lib.joint_bilateral(im, im, 8.0, 8.0, 0.0, 0.1, mode=oracle_lib.F64_LAT)
+    assert rel_err(gpu, o64) < FP32_REL_TOL
+    assert rel_err(gpu, cpu) < 2 * FP32_REL_TOL
+    # AUTO picks the lattice for d = 5 (src/GaussTransform.cpp:553-564)
+    auto_path = str(tmp_path / "auto.tmp")
+    run(["-plugin", PLUGIN, "-load", src, "-bilateral", "0.1", "8", "-save", auto_path])
+    assert rel_err(read_tmp(auto_path), o64) < FP32_REL_TOL
+
+
+def test_plugin_gausstransform_recipe(tmp_path, oracle_lib):
+    """The -gausstransform help recipe (src/GaussTransform.cpp:31-35): values
+    [r,g,b,1], positions [x,y,r,g,b] pushed with -dup so slice and splat are
+    distinct images (non-replay slice)."""
+    need_cli()
+    im = S.rect_image(96, 80, 3, K=10, noise=0.05, seed=32)
+    W, H = 96, 80
+    vals = S.homogeneous_values(im)
+    xs = np.broadcast_to(np.arange(W, dtype=np.float32), (1, H, W))[None]
+    ys = np.broadcast_to(np.arange(H, dtype=np.float32)[:, None], (1, H, W))[None]
+    pos = np.concatenate([xs, ys, im]).astype(np.float32)
+    vp, pp = str(tmp_path / "v.tmp"), str(tmp_path / "p.tmp")
+    write_tmp(vp, vals)
+    write_tmp(pp, pos)
+    outs = {}
+    for tag, plug in (("cpu", []), ("gpu", ["-plugin", PLUGIN])):
+        o = str(tmp_path / f"{tag}.tmp")
+        run(plug + ["-load", vp, "-load", pp, "-dup", "-gausstransform", "permutohedral", "4", "4", "0.1", "0.1",
+                    "0.1", "-save", o])
+        outs[tag] = read_tmp(o)
+    sig = np.array([4, 4, 0.1, 0.1, 0.1], np.float32)
+    inv = (np.float32(1) / sig).astype(np.float32)
+    o64, _ = oracle_lib.gauss_transform(pos, vals, inv, slice_pos=pos, mode=oracle_lib.F64_LAT)
+    assert rel_err(outs["gpu"].reshape(4, -1), o64) < FP32_REL_TOL
+    assert rel_err(outs["gpu"], outs["cpu"]) < 2 * FP32_REL_TOL
