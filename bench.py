@@ -106,20 +106,34 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
-    ctx = gtlattice.Context(0, profiling=1)
+    # Timed steps: the production configuration (the pipeline replayed as
+    # one cached hipGraph per call, no stage events).
+    ctx = gtlattice.Context(0, profiling=0)
     for _ in range(a.warmup):
         ctx.joint_bilateral(d_im, d_guide, ss, ss, 0.0, sr, out=d_out, storage=a.storage)
     torch.cuda.synchronize()
-    stats = []
     t0 = time.perf_counter()
     for _ in range(a.steps):
-        _, st = ctx.joint_bilateral(d_im, d_guide, ss, ss, 0.0, sr, out=d_out, storage=a.storage, stats=True)
-        stats.append(st)
+        ctx.joint_bilateral(d_im, d_guide, ss, ss, 0.0, sr, out=d_out, storage=a.storage)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     ms_step = (t1 - t0) * 1e3 / a.steps
     value = N * a.steps / (t1 - t0) / 1e6
     out_gpu = d_out.cpu().numpy()
+
+    # Kernel durations: the same steps launched eagerly with HIP events
+    # around every stage on the context's stream (the stream every kernel
+    # of the pipeline is launched on).
+    ctx.set_profiling(1)
+    ctx.joint_bilateral(d_im, d_guide, ss, ss, 0.0, sr, out=d_out, storage=a.storage)
+    torch.cuda.synchronize()
+    stats = []
+    t2 = time.perf_counter()
+    for _ in range(a.steps):
+        _, st = ctx.joint_bilateral(d_im, d_guide, ss, ss, 0.0, sr, out=d_out, storage=a.storage, stats=True)
+        stats.append(st)
+    torch.cuda.synchronize()
+    ms_step_events = (time.perf_counter() - t2) * 1e3 / a.steps
 
     # Counters (probes / inserts / atomics) in a separate, untimed call.
     ctx.set_profiling(2)
@@ -142,11 +156,13 @@ def main():
     achieved = nbytes / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-                "algorithmic_bytes": nbytes, "avg_launch_ms": round(ms, 5)}
+                "algorithmic_bytes": nbytes, "avg_launch_ms": round(ms, 5),
+                "timing": f"HIP events around the kernel on its stream, mean of {a.steps} eager steps"}
 
     line = {
         "metric": METRIC, "value": round(value, 3), "unit": "Mpix/s", "n_gpus": 1, "steps": a.steps,
-        "warmup": a.warmup, "ms_per_step": round(ms_step, 4), "higher_is_better": True, "scaling": "weak",
+        "warmup": a.warmup, "ms_per_step": round(ms_step, 4), "ms_per_step_eager_with_events": round(ms_step_events, 4),
+        "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32" if a.storage == "f32" else f"f32 accumulate, {a.storage} storage",
         "data": "synthetic",
         "config": {"workload": WORKLOAD, "pixels": N, "d": d, "vd": 4, "storage": a.storage},

@@ -83,6 +83,15 @@ struct gt_ctx {
     int plan_d = 0, plan_kind = -1, plan_vd = 0;
     gt::KeyPlan plan{};
     float vmax[gt::kMaxDH] = {};   // value bounds behind the fixed-point scales
+    // cached pipeline graph
+    hipGraphExec_t gexec = nullptr;
+    std::string gsig;
+    int gpasses = 0;
+    void *gvals = nullptr;
+    // batch descriptors
+    DevBuf batch_desc;
+    gt::BatchImage *h_batch = nullptr;   // pinned
+    int h_batch_cap = 0;
     // lattice kept for export
     bool have = false;
     int M = 0, d = 0, vd = 0, vs = 0, storage = 0;
@@ -117,6 +126,7 @@ struct Tune {
     float lf = 0.35f;
     int dedup = 0;      // 0 auto (LDS tiles when d <= 6 and vd <= 4), 1 on, -1 off
     int tile_w = 32;
+    bool graph = true;  // replay the pipeline as a cached hipGraph
 };
 
 Tune resolve(const gt_tuning *t) {
@@ -125,6 +135,7 @@ Tune resolve(const gt_tuning *t) {
     if (t->load_factor > 0) r.lf = std::min(0.7f, std::max(0.05f, t->load_factor));
     r.dedup = t->dedup;
     if (t->tile_w != 0) r.tile_w = t->tile_w;
+    if (t->graph < 0) r.graph = false;
     return r;
 }
 
@@ -212,17 +223,158 @@ struct SliceSpec {
     gt::OutDst dst{};
 };
 
+// A batch of bilateral images (gt_joint_bilateral_batch): device descriptors
+// and the tile count; the PixSrc/OutDst templates carry the shared fields.
+struct BatchSpec {
+    gt::BatchArgs ba{};
+    int ntiles = 0;
+    int Cr = 0, C = 0;
+    gt::PixSrc tmpl{};
+    gt::OutDst tmpl_dst{};
+};
+
+template <class T>
+void zero(T &x) { memset(&x, 0, sizeof(T)); }
+
+// Appends raw bytes to a graph signature.
+template <class T>
+void sig_add(std::string &k, const T &v) { k.append((const char *)&v, sizeof(T)); }
+
 // The whole pipeline: splat, compaction, sort, finalize, neighbour table,
-// blur, and (optionally) slice; one host synchronisation at the end.
-int run(gt_ctx *c, const gt::PixSrc &src, int n, int row_w, int d, int vd, int storage, const Tune &tu,
-        const SliceSpec &sl, bool keep_splat, gt_stats *st) {
-    hipStream_t s = c->stream;
-    const bool bilat = src.ref != nullptr;
-    const bool count = c->profiling >= 2;
-    const bool dedup = (tu.dedup == 0) ? (d <= 6 && vd <= 4) : (tu.dedup > 0);
+// blur, and (optionally) slice, enqueued on the context's stream.
+struct Launch {
+    int d, vd, storage, W, pw, m_max, vs, n, row_w, tile_w;
+    bool packed, bilat, dedup, count;
+    uint32_t cap;
+    size_t he;
     gt::LatConst lc;
-    gt::plan_scale_factors(d, lc.sf);
-    for (int i = d; i < gt::kMaxDH; i++) lc.sf[i] = 0.f;
+    gt::KeyPlan kp;
+    gt::ValScale vsc;
+    gt::TableArgs ta;
+    float *splat_copy;
+};
+
+// Stage boundary event (eager runs only: a captured event record is a graph
+// dependency, not a timestamp, so profiled calls are never captured).
+hipError_t mark(gt_ctx *c, int i, hipStream_t s) { return hipEventRecord(c->ev[i], s); }
+
+int enqueue(gt_ctx *c, const Launch &L, const gt::PixSrc &src, const SliceSpec &sl, const BatchSpec *batch,
+            int *passes, void **vals_out) {
+    hipStream_t s = c->stream;
+    gt::Counters *ctr = (gt::Counters *)c->ctr.p;
+    const gt::TableArgs &ta = L.ta;
+    GT_HIP(gt::launch_clear(ta.tkeys, (size_t)L.cap * L.W, ta.tacc, (size_t)L.cap * L.vd, ctr, s));
+    if (c->profiling) GT_HIP(mark(c, 0, s));
+    if (batch)
+        GT_HIP(gt::launch_splat_batch(L.d, batch->tmpl, batch->ba, batch->ntiles, L.tile_w, L.vd, L.lc, L.kp, L.vsc,
+                                      ta, ctr, L.count, s));
+    else
+        GT_HIP(gt::launch_splat(L.d, L.packed, L.bilat, L.dedup, src, L.n, L.row_w, L.tile_w, L.vd, L.lc, L.kp, L.vsc,
+                                ta, ctr, L.count, s));
+    if (c->profiling) GT_HIP(mark(c, 1, s));
+    GT_HIP(gt::launch_compact(L.W, ta, L.cap, (uint32_t *)c->counts.p, (uint32_t *)c->slot_list.p,
+                              (uint64_t *)c->pk_a.p, L.packed, ctr, s));
+    if (!L.packed)
+        GT_HIP(gt::launch_pack(L.d, ta, (const uint32_t *)c->slot_list.p, ctr, L.m_max, L.kp, L.pw,
+                               (uint64_t *)c->pk_a.p, s));
+    gt::SortBufs sb{(uint64_t *)c->pk_a.p, (uint64_t *)c->pk_b.p, (uint32_t *)c->slot_list.p,
+                    (uint32_t *)c->pay_b.p, (uint32_t *)c->hist.p, L.he};
+    bool in_b = false;
+    GT_HIP(gt::radix_sort(sb, ctr, L.m_max, L.pw, L.kp.bits, &in_b, passes, s));
+    const uint32_t *sorted = in_b ? sb.pay_b : sb.pay_a;
+    GT_HIP(gt::launch_finalize(L.W, L.storage, ta, L.vsc, sorted, ctr, L.m_max, L.vd, L.vs, (uint64_t *)c->vkeys.p,
+                               c->vals_a.p, L.splat_copy, s));
+    if (c->profiling) GT_HIP(mark(c, 2, s));
+    int2 *nbr = (int2 *)c->nbr.p;
+    GT_HIP(gt::launch_neighbors(L.d, L.packed, ta, L.kp, (const uint64_t *)c->vkeys.p, ctr, L.m_max, nbr, ctr,
+                                L.count, s));
+    if (c->profiling) GT_HIP(mark(c, 3, s));
+    void *cur = c->vals_a.p, *nxt = c->vals_b.p;
+    for (int j = 0; j <= L.d; j++) {
+        GT_HIP(gt::launch_blur(L.storage, L.vs, nbr + (size_t)j * L.m_max, ctr, L.m_max, cur, nxt, s));
+        std::swap(cur, nxt);
+    }
+    if (c->profiling) GT_HIP(mark(c, 4, s));
+    if (batch)
+        GT_HIP(gt::launch_slice_batch(L.d, L.storage, batch->tmpl, batch->ba, batch->ntiles, L.tile_w, L.vd, L.lc,
+                                      L.kp, ta, cur, batch->tmpl_dst, ctr, L.count, s));
+    else if (sl.enabled)
+        GT_HIP(gt::launch_slice(L.d, L.packed, sl.src.ref != nullptr, L.dedup, L.storage, sl.src, sl.n, L.row_w,
+                                L.tile_w, L.vd, L.vs, L.lc, L.kp, ta, cur, sl.dst, ctr, L.count, s));
+    if (c->profiling) GT_HIP(mark(c, 5, s));
+    GT_HIP(hipMemcpyAsync(c->h_ctr, ctr, sizeof(gt::Counters), hipMemcpyDeviceToHost, s));
+    *vals_out = cur;
+    return GT_OK;
+}
+
+void drop_graph(gt_ctx *c) {
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    c->gexec = nullptr;
+    c->gsig.clear();
+}
+
+// Runs the enqueue sequence, through a cached hipGraph when `graph` is set:
+// the graph is captured once per signature (every value baked into a launch)
+// and replayed afterwards, one launch for the whole pipeline.
+int execute(gt_ctx *c, const Launch &L, const gt::PixSrc &src, const SliceSpec &sl, const BatchSpec *batch,
+            bool graph, int *passes, void **vals_out) {
+    if (!graph) return enqueue(c, L, src, sl, batch, passes, vals_out);
+    std::string sig;
+    sig_add(sig, L);
+    sig_add(sig, src);
+    sig_add(sig, sl.enabled);
+    sig_add(sig, sl.src);
+    sig_add(sig, sl.n);
+    sig_add(sig, sl.dst);
+    sig_add(sig, c->profiling);
+    sig_add(sig, c->stream);
+    const void *bufs[] = {c->tkeys.p, c->tacc.p, c->ctr.p, c->counts.p, c->slot_list.p, c->slot_id.p, c->pk_a.p,
+                          c->pk_b.p, c->pay_b.p, c->hist.p, c->vkeys.p, c->vals_a.p, c->vals_b.p, c->nbr.p};
+    sig_add(sig, bufs);
+    if (batch) {
+        sig_add(sig, batch->ba);
+        sig_add(sig, batch->ntiles);
+        sig_add(sig, batch->tmpl);
+        sig_add(sig, batch->tmpl_dst);
+    }
+    if (!(c->gexec && c->gsig == sig)) {
+        drop_graph(c);
+        hipGraph_t g = nullptr;
+        GT_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        const int rc = enqueue(c, L, src, sl, batch, &c->gpasses, &c->gvals);
+        hipGraph_t captured = nullptr;
+        const hipError_t e = hipStreamEndCapture(c->stream, &captured);
+        if (rc != GT_OK) {
+            if (captured) (void)hipGraphDestroy(captured);
+            return rc;
+        }
+        if (e != hipSuccess) return fail(GT_EDEVICE, "graph capture failed: %s", hipGetErrorString(e));
+        g = captured;
+        const hipError_t ei = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ei != hipSuccess) {
+            c->gexec = nullptr;
+            return fail(GT_EDEVICE, "graph instantiation failed: %s", hipGetErrorString(ei));
+        }
+        c->gsig = sig;
+    }
+    GT_HIP(hipGraphLaunch(c->gexec, c->stream));
+    *passes = c->gpasses;
+    *vals_out = c->gvals;
+    return GT_OK;
+}
+
+int make_plan_batch(gt_ctx *c, const BatchSpec &b, const std::vector<gt::BatchImage> &imgs, int d, int vd,
+                    int tile_w);
+
+int run(gt_ctx *c, const gt::PixSrc &src, int n, int row_w, int d, int vd, int storage, const Tune &tu,
+        const SliceSpec &sl, bool keep_splat, gt_stats *st, const BatchSpec *batch = nullptr,
+        const std::vector<gt::BatchImage> *bimgs = nullptr) {
+    hipStream_t s = c->stream;
+    const bool bilat = batch ? true : src.ref != nullptr;
+    const bool count = c->profiling >= 2;
+    const bool dedup = batch ? true : (tu.dedup == 0) ? (d <= 6 && vd <= 4) : (tu.dedup > 0);
+    const int kind = batch ? 2 : bilat ? 1 : 0;
 
     uint64_t est;
     if (c->last_n == n && c->last_d == d && c->last_M > 0) est = (uint64_t)(c->last_M * 1.05) + 64;
@@ -230,80 +382,67 @@ int run(gt_ctx *c, const gt::PixSrc &src, int n, int row_w, int d, int vd, int s
     uint32_t cap = next_pow2((uint64_t)(est / tu.lf));
     const uint32_t cap_limit = next_pow2((uint64_t)((double)n * (d + 1) / tu.lf));
     if (cap > cap_limit) cap = cap_limit;
-    if (!(c->plan_valid && c->plan_d == d && c->plan_vd == vd && c->plan_kind == (bilat ? 1 : 0)))
-        GT_TRY(make_plan(c, src, n, d, vd, bilat));
+    auto replan = [&]() -> int {
+        if (batch) return make_plan_batch(c, *batch, *bimgs, d, vd, tu.tile_w);
+        return make_plan(c, src, n, d, vd, bilat);
+    };
+    const int images = batch ? batch->ba.count : 1;
+    if (!(c->plan_valid && c->plan_d == d && c->plan_vd == vd && c->plan_kind == kind &&
+          (images <= 1 || (images - 1) >> c->plan.img_bits == 0)))
+        GT_TRY(replan());
 
     int retries = 0, passes = 0;
     for (;;) {
-        const gt::KeyPlan kp = c->plan;
-        gt::ValScale vsc;
-        GT_TRY(value_scales(c, n, vd, vsc));
-        const bool packed = kp.packed != 0;
-        const int W = packed ? 1 : (d + 3) / 4;
-        const int m_max = (int)std::min<uint64_t>(cap, (uint64_t)n * (d + 1));
-        const int pw = packed ? 1 : std::max(1, (kp.bits + 63) / 64);
-        const int vs = value_stride(vd);
+        Launch L;
+        zero(L);
+        L.d = d;
+        L.vd = vd;
+        L.storage = storage;
+        L.n = n;
+        L.row_w = row_w;
+        L.tile_w = tu.tile_w;
+        L.bilat = bilat;
+        L.dedup = dedup;
+        L.count = count;
+        L.cap = cap;
+        gt::plan_scale_factors(d, L.lc.sf);
+        L.kp = c->plan;
+        GT_TRY(value_scales(c, n, vd, L.vsc));
+        L.packed = L.kp.packed != 0;
+        if (batch && !L.packed) return fail(GT_EINVAL, "batched lattice keys do not fit 63 bits");
+        L.W = L.packed ? 1 : (d + 3) / 4;
+        L.m_max = (int)std::min<uint64_t>(cap, (uint64_t)n * (d + 1));
+        L.pw = L.packed ? 1 : std::max(1, (L.kp.bits + 63) / 64);
+        L.vs = value_stride(vd);
         const unsigned ncount = (cap + 2047) / 2048;
-        GT_TRY(ensure(c->tkeys, (size_t)cap * W * 8));
+        GT_TRY(ensure(c->tkeys, (size_t)cap * L.W * 8));
         GT_TRY(ensure(c->tacc, (size_t)cap * vd * 8));
         GT_TRY(ensure(c->ctr, sizeof(gt::Counters)));
         GT_TRY(ensure(c->counts, (size_t)ncount * 4));
-        GT_TRY(ensure(c->slot_list, (size_t)m_max * 4));
+        GT_TRY(ensure(c->slot_list, (size_t)L.m_max * 4));
         GT_TRY(ensure(c->slot_id, (size_t)cap * 4));
-        GT_TRY(ensure(c->pk_a, (size_t)pw * m_max * 8));
-        GT_TRY(ensure(c->pk_b, (size_t)pw * m_max * 8));
-        GT_TRY(ensure(c->pay_b, (size_t)m_max * 4));
-        const size_t he = gt::radix_hist_entries(m_max);
-        GT_TRY(ensure(c->hist, he * 4));
-        GT_TRY(ensure(c->vkeys, (size_t)m_max * W * 8));
-        const size_t vbytes = (size_t)m_max * vs * store_bytes(storage);
+        GT_TRY(ensure(c->pk_a, (size_t)L.pw * L.m_max * 8));
+        GT_TRY(ensure(c->pk_b, (size_t)L.pw * L.m_max * 8));
+        GT_TRY(ensure(c->pay_b, (size_t)L.m_max * 4));
+        L.he = gt::radix_hist_entries(L.m_max);
+        GT_TRY(ensure(c->hist, L.he * 4));
+        GT_TRY(ensure(c->vkeys, (size_t)L.m_max * L.W * 8));
+        const size_t vbytes = (size_t)L.m_max * L.vs * store_bytes(storage);
         GT_TRY(ensure(c->vals_a, vbytes));
         GT_TRY(ensure(c->vals_b, vbytes));
-        GT_TRY(ensure(c->nbr, (size_t)(d + 1) * m_max * 8));
-        float *splat_copy = nullptr;
+        GT_TRY(ensure(c->nbr, (size_t)(d + 1) * L.m_max * 8));
+        L.splat_copy = nullptr;
         if (keep_splat) {
-            GT_TRY(ensure(c->splat_copy, (size_t)m_max * vd * 4));
-            splat_copy = (float *)c->splat_copy.p;
+            GT_TRY(ensure(c->splat_copy, (size_t)L.m_max * vd * 4));
+            L.splat_copy = (float *)c->splat_copy.p;
         }
-        gt::TableArgs ta;
-        ta.tkeys = (uint64_t *)c->tkeys.p;
-        ta.tacc = (unsigned long long *)c->tacc.p;
-        ta.mask = cap - 1;
-        ta.slot_id = (int *)c->slot_id.p;
-        gt::Counters *ctr = (gt::Counters *)c->ctr.p;
+        L.ta.tkeys = (uint64_t *)c->tkeys.p;
+        L.ta.tacc = (unsigned long long *)c->tacc.p;
+        L.ta.mask = cap - 1;
+        L.ta.slot_id = (int *)c->slot_id.p;
 
-        GT_HIP(gt::launch_clear(ta.tkeys, (size_t)cap * W, ta.tacc, (size_t)cap * vd, ctr, s));
-        if (c->profiling) GT_HIP(hipEventRecord(c->ev[0], s));
-        GT_HIP(gt::launch_splat(d, packed, bilat, dedup, src, n, row_w, tu.tile_w, vd, lc, kp, vsc, ta, ctr, count,
-                                s));
-        if (c->profiling) GT_HIP(hipEventRecord(c->ev[1], s));
-        GT_HIP(gt::launch_compact(W, ta, cap, (uint32_t *)c->counts.p, (uint32_t *)c->slot_list.p,
-                                  (uint64_t *)c->pk_a.p, packed, ctr, s));
-        if (!packed)
-            GT_HIP(gt::launch_pack(d, ta, (const uint32_t *)c->slot_list.p, ctr, m_max, kp, pw, (uint64_t *)c->pk_a.p,
-                                   s));
-        gt::SortBufs sb{(uint64_t *)c->pk_a.p, (uint64_t *)c->pk_b.p, (uint32_t *)c->slot_list.p,
-                        (uint32_t *)c->pay_b.p, (uint32_t *)c->hist.p, he};
-        bool in_b = false;
-        GT_HIP(gt::radix_sort(sb, ctr, m_max, pw, kp.bits, &in_b, &passes, s));
-        const uint32_t *sorted = in_b ? sb.pay_b : sb.pay_a;
-        GT_HIP(gt::launch_finalize(W, storage, ta, vsc, sorted, ctr, m_max, vd, vs, (uint64_t *)c->vkeys.p,
-                                   c->vals_a.p, splat_copy, s));
-        if (c->profiling) GT_HIP(hipEventRecord(c->ev[2], s));
-        int2 *nbr = (int2 *)c->nbr.p;
-        GT_HIP(gt::launch_neighbors(d, packed, ta, kp, (const uint64_t *)c->vkeys.p, ctr, m_max, nbr, ctr, count, s));
-        if (c->profiling) GT_HIP(hipEventRecord(c->ev[3], s));
-        void *cur = c->vals_a.p, *nxt = c->vals_b.p;
-        for (int j = 0; j <= d; j++) {
-            GT_HIP(gt::launch_blur(storage, vs, nbr + (size_t)j * m_max, ctr, m_max, cur, nxt, s));
-            std::swap(cur, nxt);
-        }
-        if (c->profiling) GT_HIP(hipEventRecord(c->ev[4], s));
-        if (sl.enabled)
-            GT_HIP(gt::launch_slice(d, packed, sl.src.ref != nullptr, dedup, storage, sl.src, sl.n, row_w, tu.tile_w,
-                                    vd, vs, lc, kp, ta, cur, sl.dst, ctr, count, s));
-        if (c->profiling) GT_HIP(hipEventRecord(c->ev[5], s));
-        GT_HIP(hipMemcpyAsync(c->h_ctr, ctr, sizeof(gt::Counters), hipMemcpyDeviceToHost, s));
+        void *cur = nullptr;
+        GT_TRY(execute(c, L, src, sl, batch, tu.graph && !keep_splat && c->profiling == 0, &passes, &cur));
         GT_HIP(hipStreamSynchronize(s));
         const gt::Counters hc = *c->h_ctr;
 
@@ -312,7 +451,7 @@ int run(gt_ctx *c, const gt::PixSrc &src, int n, int row_w, int d, int vd, int s
         if (hc.flags & gt::kFlagRange) {
             // Keys or values outside the cached bounds: bound these inputs again.
             if (++retries > 6) return fail(GT_EDEVICE, "inputs keep leaving the computed bounds");
-            GT_TRY(make_plan(c, src, n, d, vd, bilat));
+            GT_TRY(replan());
             continue;
         }
         if (hc.flags & gt::kFlagFull) {
@@ -329,9 +468,9 @@ int run(gt_ctx *c, const gt::PixSrc &src, int n, int row_w, int d, int vd, int s
         c->M = M;
         c->d = d;
         c->vd = vd;
-        c->vs = vs;
+        c->vs = L.vs;
         c->storage = storage;
-        c->packed = packed;
+        c->packed = L.packed;
         c->cap = cap;
         c->vals_cur = cur;
         if (st) {
@@ -355,6 +494,48 @@ int run(gt_ctx *c, const gt::PixSrc &src, int n, int row_w, int d, int vd, int s
         }
         return GT_OK;
     }
+}
+
+// Bounds of a batch: one device pass over every tile of every image.
+int make_plan_batch(gt_ctx *c, const BatchSpec &b, const std::vector<gt::BatchImage> &imgs, int d, int vd,
+                    int tile_w) {
+    const int np = b.Cr + b.C;
+    const int nblk = std::max(1, std::min(1024, b.ntiles));
+    GT_TRY(ensure(c->mm_part, (size_t)nblk * np * 3 * 4));
+    GT_TRY(ensure(c->mm_out, (size_t)np * 3 * 4));
+    GT_HIP(gt::launch_bounds_batch(b.ba, b.ntiles, tile_w, b.Cr, b.C, (float *)c->mm_part.p, nblk,
+                                   (float *)c->mm_out.p, c->stream));
+    GT_HIP(hipMemcpyAsync(c->h_mm, c->mm_out.p, (size_t)np * 3 * 4, hipMemcpyDeviceToHost, c->stream));
+    GT_HIP(hipStreamSynchronize(c->stream));
+    const float *mm = c->h_mm, *vm = c->h_mm + (size_t)b.Cr * 3;
+    for (int k = 0; k < b.C; k++)
+        if (vm[3 * k + 2] != 0.0f) return fail(GT_ERANGE, "value channel %d holds NaN or Inf", k);
+    float lo[gt::kMaxDH], hi[gt::kMaxDH];
+    for (int i = 0; i < b.Cr; i++) {
+        lo[i] = mm[3 * i];
+        hi[i] = mm[3 * i + 1];
+        if (mm[3 * i + 2] != 0.0f || !(lo[i] <= hi[i])) lo[i] = hi[i] = INFINITY;
+    }
+    float xmax = 0.0f, ymax = 0.0f;
+    for (const gt::BatchImage &bi : imgs) {
+        xmax = std::max(xmax, (float)(bi.W - 1) * bi.ix);
+        ymax = std::max(ymax, (float)(bi.H - 1) * bi.iy);
+    }
+    lo[b.Cr] = 0.0f;
+    hi[b.Cr] = xmax;
+    lo[b.Cr + 1] = 0.0f;
+    hi[b.Cr + 1] = ymax;
+    float sf[gt::kMaxDH];
+    gt::plan_scale_factors(d, sf);
+    gt::plan_from_bounds(d, sf, lo, hi, c->plan, (int)imgs.size());
+    for (int k = 0; k < gt::kMaxDH; k++) c->vmax[k] = 0.0f;
+    for (int k = 0; k < b.C; k++) c->vmax[k] = std::max(std::fabs(vm[3 * k]), std::fabs(vm[3 * k + 1]));
+    c->vmax[b.C] = 1.0f;
+    c->plan_valid = true;
+    c->plan_d = d;
+    c->plan_vd = vd;
+    c->plan_kind = 2;
+    return GT_OK;
 }
 
 int check_dims(int d, int vd) {
@@ -509,6 +690,9 @@ void gt_ctx_destroy(gt_ctx *c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    drop_graph(c);
+    if (c->h_batch) (void)hipHostFree(c->h_batch);
+    free_buf(c->batch_desc);
     DevBuf *bufs[] = {&c->tkeys, &c->tacc, &c->ctr, &c->counts, &c->slot_list, &c->slot_id, &c->pk_a, &c->pk_b,
                       &c->pay_b, &c->hist, &c->vkeys, &c->vals_a, &c->vals_b, &c->nbr, &c->splat_copy, &c->mm_part,
                       &c->mm_out, &c->stage_in, &c->stage_out, &c->view_a, &c->view_b, &c->view_o};
@@ -633,6 +817,91 @@ int gt_joint_bilateral_host(gt_ctx *c, int W, int H, int F, int C, const float *
     GT_TRY(gt_joint_bilateral(c, W, H, F, C, di, Cr, dr, fw, fh, ff, color_sigma, (float *)c->stage_out.p, storage,
                               tune, st));
     GT_HIP(hipMemcpy(out, c->stage_out.p, n * C * 4, hipMemcpyDeviceToHost));
+    return GT_OK;
+}
+
+int gt_joint_bilateral_batch(gt_ctx *c, int count, const gt_bilateral_image *images, int C, int Cr,
+                             gt_storage storage, const gt_tuning *tune, gt_stats *st) {
+    if (!c) return fail(GT_EINVAL, "null context");
+    if (count < 0 || (count > 0 && !images)) return fail(GT_EINVAL, "bad image list");
+    if (C < 1 || Cr < 1 || C + 1 > GT_MAX_VD) return fail(GT_EINVAL, "bad channel counts %d, %d", C, Cr);
+    GT_TRY(check_storage(storage));
+    GT_HIP(hipSetDevice(c->device));
+    if (st) memset(st, 0, sizeof(*st));
+    const Tune tu = resolve(tune);
+    const int tw = (tu.tile_w == 16 || tu.tile_w == 64) ? tu.tile_w : 32;
+    const int th = 256 / tw;
+    const int d = Cr + 2, vd = C + 1;
+    const bool tiled = d <= 6 && vd <= 4;
+    std::vector<gt::BatchImage> imgs;
+    std::vector<int> single;
+    int64_t n = 0, ntiles = 0;
+    for (int i = 0; i < count; i++) {
+        const gt_bilateral_image &b = images[i];
+        if (b.W < 1 || b.H < 1 || !b.im || !b.ref || !b.out) return fail(GT_EINVAL, "bad image %d", i);
+        // The batch holds images whose lattice is (colour, x, y): both axes
+        // filtered (src/GaussTransform.cpp:512-546); others go one by one.
+        const bool plain = b.W > 1 && b.H > 1 && b.sigma_s > 0 && b.sigma_s < 10 * b.W && b.sigma_s < 10 * b.H;
+        if (!tiled || !plain) {
+            single.push_back(i);
+            continue;
+        }
+        gt::BatchImage bi;
+        zero(bi);
+        bi.ref = b.ref;
+        bi.im = b.im;
+        bi.out = b.out;
+        bi.W = b.W;
+        bi.H = b.H;
+        bi.ic = 1.0f / b.sigma_r;
+        bi.ix = 1.0f / b.sigma_s;
+        bi.iy = 1.0f / b.sigma_s;
+        bi.tile0 = (int)ntiles;
+        ntiles += (int64_t)((b.W + tw - 1) / tw) * ((b.H + th - 1) / th);
+        n += (int64_t)b.W * b.H;
+        imgs.push_back(bi);
+    }
+    if (n >= (1ll << 31) || ntiles >= (1ll << 31)) return fail(GT_EINVAL, "batch of %lld pixels exceeds 2^31", (long long)n);
+    if (!imgs.empty()) {
+        const int nb = (int)imgs.size();
+        if (c->h_batch_cap < nb) {
+            if (c->h_batch) (void)hipHostFree(c->h_batch);
+            c->h_batch = nullptr;
+            c->h_batch_cap = 0;
+            if (hipHostMalloc((void **)&c->h_batch, sizeof(gt::BatchImage) * nb, hipHostMallocDefault) != hipSuccess)
+                return fail(GT_ENOMEM, "pinned allocation failed");
+            c->h_batch_cap = nb;
+        }
+        memcpy(c->h_batch, imgs.data(), sizeof(gt::BatchImage) * nb);
+        GT_TRY(ensure(c->batch_desc, sizeof(gt::BatchImage) * nb));
+        GT_HIP(hipMemcpyAsync(c->batch_desc.p, c->h_batch, sizeof(gt::BatchImage) * nb, hipMemcpyHostToDevice,
+                              c->stream));
+        BatchSpec b;
+        b.ba.img = (const gt::BatchImage *)c->batch_desc.p;
+        b.ba.count = nb;
+        b.ntiles = (int)ntiles;
+        b.Cr = Cr;
+        b.C = C;
+        zero(b.tmpl);
+        b.tmpl.cr = Cr;
+        b.tmpl.fx = 1;
+        b.tmpl.fy = 1;
+        b.tmpl.ft = 0;
+        b.tmpl.nval = C;
+        b.tmpl.homog = 1;
+        b.tmpl.ref = imgs[0].ref;   // marks the bilateral form; per-image fields come from the descriptors
+        zero(b.tmpl_dst);
+        b.tmpl_dst.normalize = 1;
+        Tune t2 = tu;
+        t2.tile_w = tw;
+        SliceSpec none;
+        GT_TRY(run(c, b.tmpl, (int)n, 0, d, vd, storage, t2, none, false, st, &b, &imgs));
+    }
+    for (int i : single) {
+        const gt_bilateral_image &b = images[i];
+        GT_TRY(gt_joint_bilateral(c, b.W, b.H, 1, C, b.im, Cr, b.ref, b.sigma_s, b.sigma_s, 0.0f, b.sigma_r, b.out,
+                                  storage, tune, nullptr));
+    }
     return GT_OK;
 }
 

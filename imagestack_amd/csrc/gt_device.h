@@ -132,11 +132,14 @@ __device__ __forceinline__ int vertex_coord(const Simplex<D> &s, int r, int i) {
 // so the first field of a word is never 0xffff.
 template <int D, bool PACKED> struct KeyOps;
 
+// `tag` holds fields above the coordinates: the image index of a batched
+// lattice (packed format only; 0 otherwise).
 template <int D>
 struct KeyOps<D, true> {
     static constexpr int W = 1;
-    __device__ static __forceinline__ bool from_coords(const int (&c)[D], const KeyPlan &kp, uint64_t (&k)[W]) {
-        uint64_t v = 0;
+    __device__ static __forceinline__ bool from_coords(const int (&c)[D], const KeyPlan &kp, uint64_t tag,
+                                                       uint64_t (&k)[W]) {
+        uint64_t v = tag;
         bool ok = true;
 #pragma unroll
         for (int i = 0; i < D; i++) {
@@ -147,21 +150,26 @@ struct KeyOps<D, true> {
         k[0] = v;
         return ok;
     }
-    __device__ static __forceinline__ bool make(const Simplex<D> &s, int r, const KeyPlan &kp, uint64_t (&k)[W]) {
+    __device__ static __forceinline__ bool make(const Simplex<D> &s, int r, const KeyPlan &kp, uint64_t tag,
+                                                uint64_t (&k)[W]) {
         int c[D];
 #pragma unroll
         for (int i = 0; i < D; i++) c[i] = vertex_coord(s, r, i);
-        return from_coords(c, kp, k);
+        return from_coords(c, kp, tag, k);
     }
     __device__ static __forceinline__ int coord(const uint64_t (&k)[W], int i, const KeyPlan &kp) {
         return (int)((k[0] >> kp.shift[i]) & kp.span[i]) + kp.lo[i];
+    }
+    __device__ static __forceinline__ uint64_t tag_of(const uint64_t (&k)[W], const KeyPlan &kp) {
+        return kp.coord_bits >= 64 ? 0ull : (k[0] >> kp.coord_bits) << kp.coord_bits;
     }
 };
 
 template <int D>
 struct KeyOps<D, false> {
     static constexpr int W = (D + 3) / 4;
-    __device__ static __forceinline__ bool from_coords(const int (&c)[D], const KeyPlan &kp, uint64_t (&k)[W]) {
+    __device__ static __forceinline__ bool from_coords(const int (&c)[D], const KeyPlan &kp, uint64_t,
+                                                       uint64_t (&k)[W]) {
         bool ok = true;
 #pragma unroll
         for (int w = 0; w < W; w++) k[w] = 0;
@@ -172,15 +180,17 @@ struct KeyOps<D, false> {
         }
         return ok;
     }
-    __device__ static __forceinline__ bool make(const Simplex<D> &s, int r, const KeyPlan &kp, uint64_t (&k)[W]) {
+    __device__ static __forceinline__ bool make(const Simplex<D> &s, int r, const KeyPlan &kp, uint64_t tag,
+                                                uint64_t (&k)[W]) {
         int c[D];
 #pragma unroll
         for (int i = 0; i < D; i++) c[i] = vertex_coord(s, r, i);
-        return from_coords(c, kp, k);
+        return from_coords(c, kp, tag, k);
     }
     __device__ static __forceinline__ int coord(const uint64_t (&k)[W], int i, const KeyPlan &) {
         return (int)((k[i / 4] >> (48 - 16 * (i % 4))) & 0xffff) - 32768;
     }
+    __device__ static __forceinline__ uint64_t tag_of(const uint64_t (&)[W], const KeyPlan &) { return 0ull; }
 };
 
 template <int W>

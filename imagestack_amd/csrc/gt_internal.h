@@ -47,6 +47,23 @@ struct LatConst {
     float sf[kMaxDH];
 };
 
+// A batch of independent joint-bilateral images filtered through one lattice
+// (the image index is the key's tag field).  Frames are 1; all images share
+// the channel counts and storage.
+struct BatchImage {
+    const float *ref;   // Cr guide planes of W*H pixels
+    const float *im;    // C image planes
+    float *out;         // C output planes
+    int W, H;
+    float ic, ix, iy;   // 1/sigma_r, 1/sigma_s, 1/sigma_s
+    int tile0;          // first tile of this image in the batch's tile order
+};
+
+struct BatchArgs {
+    const BatchImage *img;  // device array of `count`
+    int count;
+};
+
 // Output of the slice.
 struct OutDst {
     float *out;
@@ -78,11 +95,21 @@ struct ValScale {
 // Per-plane min, max and a non-finite indicator (1.0 if any NaN/Inf) -> out[3*plane..].
 hipError_t launch_minmax(const float *planes, int64_t cs, int nplanes, int n, float *partial, int nblk,
                          float *out, hipStream_t st);
+hipError_t launch_bounds_batch(const BatchArgs &ba, int ntiles, int tile_w, int Cr, int C, float *partial, int nblk,
+                               float *out, hipStream_t st);
 hipError_t launch_clear(uint64_t *tkeys, size_t nwords, unsigned long long *tacc, size_t nacc, Counters *ctr,
                         hipStream_t st);
 hipError_t launch_splat(int d, bool packed, bool bilat, bool dedup, const PixSrc &src, int n, int row_w, int tile_w,
                         int vd, const LatConst &lc, const KeyPlan &kp, const ValScale &vsc, const TableArgs &ta,
                         Counters *ctr, bool count, hipStream_t st);
+// Batched forms (bilateral, packed keys, d <= 6, vd <= 4): one workgroup per
+// tile, ntiles tiles over all images.
+hipError_t launch_splat_batch(int d, const PixSrc &tmpl, const BatchArgs &ba, int ntiles, int tile_w, int vd,
+                              const LatConst &lc, const KeyPlan &kp, const ValScale &vsc, const TableArgs &ta,
+                              Counters *ctr, bool count, hipStream_t st);
+hipError_t launch_slice_batch(int d, int storage, const PixSrc &tmpl, const BatchArgs &ba, int ntiles, int tile_w,
+                              int vd, const LatConst &lc, const KeyPlan &kp, const TableArgs &ta, const void *vals,
+                              const OutDst &tmpl_dst, Counters *ctr, bool count, hipStream_t st);
 hipError_t launch_compact(int words, const TableArgs &ta, uint32_t capacity, uint32_t *counts, uint32_t *slot_list,
                           uint64_t *sort_keys, bool packed, Counters *ctr, hipStream_t st);
 hipError_t launch_pack(int d, const TableArgs &ta, const uint32_t *slot_list, const Counters *ctr, int m_max,

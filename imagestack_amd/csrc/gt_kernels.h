@@ -108,6 +108,56 @@ __device__ __forceinline__ void list_tile(const uint64_t *lk, uint16_t *llist, u
 }
 
 // ---------------------------------------------------------------------------
+// Tiles.  A tiled kernel runs one workgroup per tile (a loop over tiles keeps
+// the kernel arguments live across iterations: 100 VGPRs and SGPR spills
+// instead of 67, occupancy 4 instead of 7).  A batch maps a global tile to its
+// image by binary search over the images' first tiles; the image index
+// becomes the key tag, so the images' lattices never mix.
+struct TileView {
+    PixSrc src;
+    OutDst dst;
+    int n, row_w, local;
+    uint64_t tag;
+};
+
+template <bool BATCH>
+__device__ __forceinline__ void tile_view(int tile, const PixSrc &src0, const OutDst &dst0, int n0, int row_w0,
+                                          const BatchArgs &ba, const KeyPlan &kp, TileView &tv) {
+    tv.src = src0;
+    tv.dst = dst0;
+    if constexpr (!BATCH) {
+        tv.n = n0;
+        tv.row_w = row_w0;
+        tv.local = tile;
+        tv.tag = 0;
+    } else {
+        int lo = 0, hi = ba.count - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (ba.img[mid].tile0 <= tile) lo = mid;
+            else hi = mid - 1;
+        }
+        const BatchImage &bi = ba.img[lo];
+        const int64_t plane = (int64_t)bi.W * bi.H;
+        tv.src.ref = bi.ref;
+        tv.src.ref_cs = plane;
+        tv.src.W = bi.W;
+        tv.src.H = bi.H;
+        tv.src.ic = bi.ic;
+        tv.src.ix = bi.ix;
+        tv.src.iy = bi.iy;
+        tv.src.val = bi.im;
+        tv.src.val_cs = plane;
+        tv.dst.out = bi.out;
+        tv.dst.out_cs = plane;
+        tv.n = (int)plane;
+        tv.row_w = bi.W;
+        tv.local = tile - bi.tile0;
+        tv.tag = (uint64_t)lo << kp.coord_bits;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // splat, tiled form: the workgroup inserts its 256 pixels' D+1 vertex keys
 // into an LDS tile table and pre-reduces weight*value there (LDS FP32
 // atomics); then each distinct key is inserted into the global table once and
@@ -137,25 +187,29 @@ __device__ __forceinline__ void rotate_left(T (&a)[N], int rot) {
     }
 }
 
-template <int D, bool PACKED, bool BILAT, int LVD, int LT>
-__global__ __launch_bounds__(kBlock) void k_splat_tile(PixSrc src, int n, int row_w, int tile_w, int vd, LatConst lc,
-                                                       KeyPlan kp, ValScale vsc, TableArgs ta, Counters *ctr,
-                                                       int count) {
+template <int D, bool PACKED, bool BILAT, int LVD, int LT, bool BATCH>
+__global__ __launch_bounds__(kBlock) void k_splat_tile(PixSrc src0, int n0, int row_w0, int tile_w, BatchArgs ba,
+                                                       int vd, LatConst lc, KeyPlan kp, ValScale vsc, TableArgs ta,
+                                                       Counters *ctr, int count) {
     using K = KeyOps<D, PACKED>;
     constexpr int W = K::W;
     __shared__ uint64_t lk[LT * W];
     __shared__ unsigned long long lv[LVD * LT];
     __shared__ uint16_t llist[LT];
     __shared__ unsigned lcount;
+    unsigned flags = 0;
+    uint32_t probes = 0, inserts = 0, atomics = 0;
+    const OutDst no_dst{nullptr, 0, 0};
+    TileView tv;
+    tile_view<BATCH>(blockIdx.x, src0, no_dst, n0, row_w0, ba, kp, tv);
+    const PixSrc &src = tv.src;
     for (int i = threadIdx.x; i < LT * W; i += kBlock) lk[i] = kEmpty;
     for (int i = threadIdx.x; i < LVD * LT; i += kBlock) lv[i] = 0ull;
     if (threadIdx.x == 0) lcount = 0;
-    const int p = pixel_index(blockIdx.x, threadIdx.x, n, row_w, tile_w);
+    const int p = pixel_index(tv.local, threadIdx.x, tv.n, tv.row_w, tile_w);
     const bool full = *(volatile const unsigned *)&ctr->flags & kFlagFull;
     __syncthreads();
 
-    unsigned flags = 0;
-    uint32_t probes = 0, inserts = 0, atomics = 0;
     if (p >= 0) {
         float pos[D], v[kMaxVD];
         load_position<D, BILAT>(src, p, pos);
@@ -176,7 +230,7 @@ __global__ __launch_bounds__(kBlock) void k_splat_tile(PixSrc src, int n, int ro
             int okr[D + 1];
 #pragma unroll
             for (int r = 0; r <= D; r++) {
-                okr[r] = K::make(s, r, kp, key[r]) ? 1 : 0;
+                okr[r] = K::make(s, r, kp, tv.tag, key[r]) ? 1 : 0;
                 wt[r] = s.b[r];
             }
             const int rot = (int)(threadIdx.x % (D + 1));
@@ -273,7 +327,7 @@ __global__ __launch_bounds__(kBlock) void k_splat_direct(PixSrc src, int n, int 
 #pragma unroll
             for (int r = 0; r <= D; r++) {
                 uint64_t k[W];
-                if (!K::make(s, r, kp, k)) { flags |= kFlagRange; continue; }
+                if (!K::make(s, r, kp, 0ull, k)) { flags |= kFlagRange; continue; }
                 const int slot = table_insert<W>(ta.tkeys, ta.mask, k, probes);
                 inserts++;
                 if (slot < 0) { flags |= kFlagFull; continue; }
@@ -432,8 +486,9 @@ __global__ __launch_bounds__(kBlock) void k_neighbors(TableArgs ta, KeyPlan kp, 
             cb[q] = c + ((q == j) ? D : -1);
         }
         uint64_t a[W], b[W];
-        const bool oka = K::from_coords(ca, kp, a);
-        const bool okb = K::from_coords(cb, kp, b);
+        const uint64_t tag = K::tag_of(k, kp);
+        const bool oka = K::from_coords(ca, kp, tag, a);
+        const bool okb = K::from_coords(cb, kp, tag, b);
         const int sa = oka ? table_find<W>(ta.tkeys, ta.mask, a, probes) : -1;
         const int sb = okb ? table_find<W>(ta.tkeys, ta.mask, b, probes) : -1;
         nbr[(size_t)j * stride + i] = make_int2(sa >= 0 ? ta.slot_id[sa] : -1, sb >= 0 ? ta.slot_id[sb] : -1);
@@ -488,10 +543,10 @@ __device__ __forceinline__ void write_pixel(const OutDst &dst, int p, int vd, co
 // in vertex order.  A vertex absent from the lattice (slice positions differ
 // from the splat positions) contributes nothing; a key that found no LDS
 // slot is looked up in the global table directly.
-template <int D, bool PACKED, bool BILAT, typename T, int LVD, int LT>
-__global__ __launch_bounds__(kBlock) void k_slice_tile(PixSrc src, int n, int row_w, int tile_w, int vd,
-                                                       LatConst lc, KeyPlan kp, TableArgs ta,
-                                                       const T *__restrict__ vals, OutDst dst, Counters *ctr,
+template <int D, bool PACKED, bool BILAT, typename T, int LVD, int LT, bool BATCH>
+__global__ __launch_bounds__(kBlock) void k_slice_tile(PixSrc src0, int n0, int row_w0, int tile_w, BatchArgs ba,
+                                                       int vd, LatConst lc, KeyPlan kp, TableArgs ta,
+                                                       const T *__restrict__ vals, OutDst dst0, Counters *ctr,
                                                        int count) {
     using K = KeyOps<D, PACKED>;
     constexpr int W = K::W;
@@ -499,14 +554,18 @@ __global__ __launch_bounds__(kBlock) void k_slice_tile(PixSrc src, int n, int ro
     __shared__ float lv[LT * LVD];
     __shared__ uint16_t llist[LT];
     __shared__ unsigned lcount;
+    unsigned flags = 0;
+    uint32_t probes = 0;
+    TileView tv;
+    tile_view<BATCH>(blockIdx.x, src0, dst0, n0, row_w0, ba, kp, tv);
+    const PixSrc &src = tv.src;
     for (int i = threadIdx.x; i < LT * W; i += kBlock) lk[i] = kEmpty;
     if (threadIdx.x == 0) lcount = 0;
-    const int p = pixel_index(blockIdx.x, threadIdx.x, n, row_w, tile_w);
+    const int p = pixel_index(tv.local, threadIdx.x, tv.n, tv.row_w, tile_w);
     __syncthreads();
 
     int lsl[D + 1];   // LDS slot, -1 absent (outside the plan), -2 look up globally
     Simplex<D> s;
-    unsigned flags = 0;
     bool ok = false;
 #pragma unroll
     for (int r = 0; r <= D; r++) lsl[r] = -1;
@@ -523,7 +582,7 @@ __global__ __launch_bounds__(kBlock) void k_slice_tile(PixSrc src, int n, int ro
             uint64_t key[D + 1][W];
             int okr[D + 1];
 #pragma unroll
-            for (int r = 0; r <= D; r++) okr[r] = K::make(s, r, kp, key[r]) ? 1 : 0;
+            for (int r = 0; r <= D; r++) okr[r] = K::make(s, r, kp, tv.tag, key[r]) ? 1 : 0;
             const int rot = (int)(threadIdx.x % (D + 1));
             rotate_left<D + 1>(okr, rot);
 #pragma unroll
@@ -550,7 +609,6 @@ __global__ __launch_bounds__(kBlock) void k_slice_tile(PixSrc src, int n, int ro
     __syncthreads();
 
     const unsigned cnt = lcount;
-    uint32_t probes = 0;
     for (unsigned j = threadIdx.x; j < cnt; j += kBlock) {
         const int ls = llist[j];
         uint64_t k[W];
@@ -577,7 +635,7 @@ __global__ __launch_bounds__(kBlock) void k_slice_tile(PixSrc src, int n, int ro
                 for (int c = 0; c < LVD; c++) acc[c] = acc[c] + s.b[r] * lv[c * LT + lsl[r]];
             } else if (lsl[r] == -2) {
                 uint64_t k[W];
-                K::make(s, r, kp, k);
+                K::make(s, r, kp, tv.tag, k);
                 const int slot = table_find<W>(ta.tkeys, ta.mask, k, probes);
                 if (slot >= 0) {
                     float row[kMaxVD];
@@ -587,7 +645,7 @@ __global__ __launch_bounds__(kBlock) void k_slice_tile(PixSrc src, int n, int ro
                 }
             }
         }
-        write_pixel<LVD>(dst, p, vd, acc);
+        write_pixel<LVD>(tv.dst, p, vd, acc);
     }
     report_flags(ctr, flags);
     if (count) report_counts(ctr, probes, 0, 0);
@@ -617,7 +675,7 @@ __global__ __launch_bounds__(kBlock) void k_slice_direct(PixSrc src, int n, int 
 #pragma unroll
             for (int r = 0; r <= D; r++) {
                 uint64_t k[W];
-                if (!K::make(s, r, kp, k)) continue;
+                if (!K::make(s, r, kp, 0ull, k)) continue;
                 const int slot = table_find<W>(ta.tkeys, ta.mask, k, probes);
                 if (slot < 0) continue;
                 float v[kMaxVD];

@@ -18,7 +18,9 @@ struct KeyPlan {
     uint32_t span[kMaxDH];
     int shift[kMaxDH];     // bit offset in the packed big integer (word = shift/64 for the sort)
     int packed;            // 1: table keys are the packed word
-    int bits;              // total packed bits
+    int bits;              // total packed bits (coordinates + image index)
+    int coord_bits;        // bits of the coordinate fields (bits [0, coord_bits))
+    int img_bits;          // batched lattices: image index in bits [coord_bits, bits)
 };
 
 } // namespace gt

@@ -65,6 +65,70 @@ __global__ __launch_bounds__(kBlock) void k_minmax_final(const float *__restrict
     out[c * 3 + 2] = nf;
 }
 
+// Bounds of a batch, one tile per workgroup iteration: per guide channel the
+// min/max of ref * ic (the scaled colour position, ic per image) and per image
+// channel the min/max of the value, each with a non-finite indicator, as
+// partial[blk][plane][3] for k_minmax_final.
+__global__ __launch_bounds__(kBlock) void k_bounds_batch(BatchArgs ba, int ntiles, int tile_w, int Cr, int C,
+                                                         float *__restrict__ partial) {
+    __shared__ float red[3][kBlock / kWave];
+    const int np = Cr + C;
+    for (int c = 0; c < np; c++) {
+        float mn = INFINITY, mx = -INFINITY, nf = 0.0f;
+        for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+            int lo = 0, hi = ba.count - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (ba.img[mid].tile0 <= tile) lo = mid;
+                else hi = mid - 1;
+            }
+            const BatchImage &bi = ba.img[lo];
+            const int n = bi.W * bi.H;
+            const int p = pixel_index(tile - bi.tile0, threadIdx.x, n, bi.W, tile_w);
+            if (p >= 0) {
+                const float v = c < Cr ? bi.ref[(int64_t)c * n + p] * bi.ic : bi.im[(int64_t)(c - Cr) * n + p];
+                if (isfinite(v)) {
+                    mn = fminf(mn, v);
+                    mx = fmaxf(mx, v);
+                } else {
+                    nf = 1.0f;
+                }
+            }
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            mn = fminf(mn, __shfl_xor(mn, off, kWave));
+            mx = fmaxf(mx, __shfl_xor(mx, off, kWave));
+            nf = fmaxf(nf, __shfl_xor(nf, off, kWave));
+        }
+        if (__lane_id() == 0) {
+            red[0][threadIdx.x / kWave] = mn;
+            red[1][threadIdx.x / kWave] = mx;
+            red[2][threadIdx.x / kWave] = nf;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int w = 1; w < kBlock / kWave; w++) {
+                mn = fminf(mn, red[0][w]);
+                mx = fmaxf(mx, red[1][w]);
+                nf = fmaxf(nf, red[2][w]);
+            }
+            float *o = partial + ((size_t)blockIdx.x * np + c) * 3;
+            o[0] = mn;
+            o[1] = mx;
+            o[2] = nf;
+        }
+        __syncthreads();
+    }
+}
+
+hipError_t launch_bounds_batch(const BatchArgs &ba, int ntiles, int tile_w, int Cr, int C, float *partial, int nblk,
+                               float *out, hipStream_t st) {
+    hipLaunchKernelGGL(k_bounds_batch, dim3(nblk), dim3(kBlock), 0, st, ba, ntiles, tile_w, Cr, C, partial);
+    hipLaunchKernelGGL(k_minmax_final, dim3(1), dim3(kBlock), 0, st, partial, nblk, Cr + C, out);
+    return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------
 // clear: table words to kEmpty, accumulators and counters to zero (a fill
 // kernel rather than hipMemsetAsync, so the sequence can be graph-captured).

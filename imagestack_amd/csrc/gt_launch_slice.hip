@@ -1,5 +1,5 @@
-// Slice launcher: LDS-tiled slice for d <= 6 and vd <= 4, direct slice
-// otherwise (d > 6 in gt_launch_slice_hi.hip).
+// Slice launchers: LDS-tiled slice for d <= 6 and vd <= 4 (single image or a
+// batch), direct slice otherwise (d > 6 in gt_launch_slice_hi.hip).
 #include "gt_launch.h"
 
 namespace gt {
@@ -13,13 +13,16 @@ hipError_t launch_slice(int d, bool packed, bool bilat, bool tiled, int storage,
     const int cnt = count ? 1 : 0;
     tiled = tiled && vs <= 4;
     if (tiled) {
-        const int64_t nblk = pixel_blocks(n, row_w, tile_w);
+        const int64_t ntiles = pixel_blocks(n, row_w, tile_w);
+        const BatchArgs none{nullptr, 0};
 #define GT_SLT(D, P, B, T)                                                                                      \
     {                                                                                                           \
-        if (vs <= 2) hipLaunchKernelGGL((k_slice_tile<D, P, B, T, 2, kTileKeys>), dim3(nblk), dim3(kBlock), 0, st, src, n,  \
-                                        row_w, tile_w, vd, lc, kp, ta, (const T *)vals, dst, ctr, cnt);          \
-        else hipLaunchKernelGGL((k_slice_tile<D, P, B, T, 4, kTileKeys>), dim3(nblk), dim3(kBlock), 0, st, src, n, row_w,   \
-                                tile_w, vd, lc, kp, ta, (const T *)vals, dst, ctr, cnt);                         \
+        if (vs <= 2) hipLaunchKernelGGL((k_slice_tile<D, P, B, T, 2, kTileKeys, false>), dim3(ntiles),           \
+                                        dim3(kBlock), 0, st, src, n, row_w, tile_w, none, vd, lc, kp, ta,        \
+                                        (const T *)vals, dst, ctr, cnt);                                         \
+        else hipLaunchKernelGGL((k_slice_tile<D, P, B, T, 4, kTileKeys, false>), dim3(ntiles), dim3(kBlock), 0,  \
+                                st, src, n, row_w, tile_w, none, vd, lc, kp, ta, (const T *)vals, dst, ctr,      \
+                                cnt);                                                                            \
     }
 #define GT_SLT_S(D, P, B)                                              \
     {                                                                  \
@@ -56,6 +59,33 @@ hipError_t launch_slice(int d, bool packed, bool bilat, bool tiled, int storage,
 #undef GT_SLD_S
 #undef GT_SLICE_DIRECT
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_slice_batch(int d, int storage, const PixSrc &tmpl, const BatchArgs &ba, int ntiles, int tile_w,
+                              int vd, const LatConst &lc, const KeyPlan &kp, const TableArgs &ta, const void *vals,
+                              const OutDst &tmpl_dst, Counters *ctr, bool count, hipStream_t st) {
+    if (ntiles <= 0) return hipSuccess;
+    if (vd > 4 || !kp.packed) return hipErrorInvalidValue;
+    const int cnt = count ? 1 : 0;
+#define GT_SLB(D, T)                                                                                            \
+    {                                                                                                           \
+        if (vd <= 2) hipLaunchKernelGGL((k_slice_tile<D, true, true, T, 2, kTileKeys, true>), dim3(ntiles),      \
+                                        dim3(kBlock), 0, st, tmpl, 0, 0, tile_w, ba, vd, lc, kp, ta,             \
+                                        (const T *)vals, tmpl_dst, ctr, cnt);                                    \
+        else hipLaunchKernelGGL((k_slice_tile<D, true, true, T, 4, kTileKeys, true>), dim3(ntiles),              \
+                                dim3(kBlock), 0, st, tmpl, 0, 0, tile_w, ba, vd, lc, kp, ta, (const T *)vals,    \
+                                tmpl_dst, ctr, cnt);                                                             \
+    }
+#define GT_SLB_S(D)                                          \
+    {                                                        \
+        if (storage == 0) GT_SLB(D, float)                   \
+        else if (storage == 1) GT_SLB(D, __hip_bfloat16)     \
+        else GT_SLB(D, __half)                               \
+    }
+    GT_DISPATCH_DT(d, GT_SLB_S)
+#undef GT_SLB
+#undef GT_SLB_S
     return hipGetLastError();
 }
 

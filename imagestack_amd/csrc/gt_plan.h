@@ -35,7 +35,10 @@ inline int plan_bitlen(uint32_t v) {
 }
 
 // lo/hi: per-channel bounds of the scaled positions (d values each).
-inline void plan_from_bounds(int d, const float *sf, const float *lo, const float *hi, KeyPlan &kp) {
+// images > 1: a batch of independent lattices, the image index stored above
+// the coordinates (most significant, so a batch sorts image by image).
+inline void plan_from_bounds(int d, const float *sf, const float *lo, const float *hi, KeyPlan &kp,
+                             int images = 1) {
     double A[kMaxDH + 1][kMaxDH];
     for (int i = 0; i <= d; i++)
         for (int j = 0; j < d; j++) A[i][j] = 0.0;
@@ -78,8 +81,10 @@ inline void plan_from_bounds(int d, const float *sf, const float *lo, const floa
         total += bits[i];
     }
     for (int i = d; i < kMaxDH; i++) { kp.lo[i] = 0; kp.span[i] = 0; kp.shift[i] = 0; }
-    kp.bits = total;
-    kp.packed = total <= 63 ? 1 : 0;
+    kp.coord_bits = total;
+    kp.img_bits = images > 1 ? plan_bitlen((uint32_t)(images - 1)) : 0;
+    kp.bits = total + kp.img_bits;
+    kp.packed = kp.bits <= 63 ? 1 : 0;
 }
 
 } // namespace gt

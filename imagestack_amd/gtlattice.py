@@ -22,7 +22,7 @@ ERRORS = {1: "GT_EINVAL", 2: "GT_ENOMEM", 3: "GT_ERANGE", 4: "GT_EDEVICE", 5: "G
 EXPORTS = [
     "gt_ctx_create", "gt_ctx_destroy", "gt_ctx_set_stream", "gt_ctx_set_profiling", "gt_last_error",
     "gt_gauss_transform", "gt_gauss_transform_host", "gt_joint_bilateral", "gt_joint_bilateral_host",
-    "gt_build_lattice", "gt_lattice_export",
+    "gt_build_lattice", "gt_lattice_export", "gt_joint_bilateral_batch",
 ]
 
 
@@ -33,8 +33,12 @@ class GTError(RuntimeError):
 
 
 class Tuning(C.Structure):
-    _fields_ = [("block", C.c_int), ("load_factor", C.c_float), ("waves_per_cu", C.c_int),
-                ("dedup", C.c_int), ("tile_w", C.c_int)]
+    _fields_ = [("load_factor", C.c_float), ("dedup", C.c_int), ("tile_w", C.c_int), ("graph", C.c_int)]
+
+
+class BilateralImage(C.Structure):
+    _fields_ = [("W", C.c_int), ("H", C.c_int), ("im", C.c_void_p), ("ref", C.c_void_p), ("out", C.c_void_p),
+                ("sigma_s", C.c_float), ("sigma_r", C.c_float)]
 
 
 class Stats(C.Structure):
@@ -76,6 +80,8 @@ def lib():
         L.gt_build_lattice.argtypes = [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp, _fp, C.c_int,
                                        C.POINTER(Tuning), C.POINTER(C.c_int)]
         L.gt_lattice_export.argtypes = [_vp, _vp, _vp, _vp, _vp]
+        L.gt_joint_bilateral_batch.argtypes = [_vp, C.c_int, C.POINTER(BilateralImage), C.c_int, C.c_int, C.c_int,
+                                               C.POINTER(Tuning), C.POINTER(Stats)]
         _lib = L
     return _lib
 
@@ -160,6 +166,21 @@ class Context:
                                         ff, color_sigma, _dptr(out), STORAGE[storage], _tuning(tune),
                                         C.byref(st)))
         return (out, st.as_dict()) if stats else out
+
+    def joint_bilateral_batch(self, items, storage: str = "f32", tune=None, stats: bool = False):
+        """items: list of (im (C,1,H,W), ref (Cr,1,H,W), out, sigma_s, sigma_r)
+        device tensors; filters every image in one batched lattice."""
+        arr = (BilateralImage * len(items))()
+        C_ = Cr = None
+        for k, (im, ref, out, ss, sr) in enumerate(items):
+            Cc, F, H, W = im.shape
+            assert F == 1 and out.shape == im.shape
+            C_, Cr = Cc, ref.shape[0]
+            arr[k] = BilateralImage(W, H, _dptr(im), _dptr(ref), _dptr(out), ss, sr)
+        st = Stats()
+        _check(lib().gt_joint_bilateral_batch(self._h, len(items), arr, C_ or 1, Cr or 1, STORAGE[storage],
+                                              _tuning(tune), C.byref(st)))
+        return st.as_dict() if stats else None
 
     def build_lattice(self, splat, values, inv_sigma, row_w: int = 0, storage: str = "f32", tune=None):
         """Splat + blur and export: dict(keys (M,d) int16, nbr (d+1,M,2) int32,

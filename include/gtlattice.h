@@ -55,11 +55,10 @@ typedef enum { GT_STORE_F32 = 0, GT_STORE_BF16 = 1, GT_STORE_F16 = 2 } gt_storag
 
 /* Tuning knobs.  Zero fields select defaults (or the autotuner's choice). */
 typedef struct {
-    int block;           /* threads per workgroup of the pixel kernels: 64..1024   */
     float load_factor;   /* target hash-table load, 0.05..0.7 (default 0.35)       */
-    int waves_per_cu;    /* occupancy of the persistent batch kernels (default 8)  */
     int dedup;           /* LDS tile dedup / pre-reduction: 0 auto, 1 on, -1 off     */
-    int tile_w;          /* pixel-tile width of a workgroup (default: whole rows)  */
+    int tile_w;          /* pixel-tile width of a workgroup: 16, 32 (default), 64    */
+    int graph;           /* 0: replay the pipeline as a cached hipGraph; -1: don't  */
 } gt_tuning;
 
 /* Per-call statistics (all optional). */
@@ -70,14 +69,14 @@ typedef struct {
     int sort_passes;     /* radix passes run                                       */
     /* HIP-event times on the context's stream (profiling on):                   */
     float ms_splat;      /* the splat kernel alone                                 */
-    float ms_sort;       /* compaction, host read-back of M, radix sort, finalize  */
+    float ms_sort;       /* compaction, radix sort, finalize                       */
     float ms_nbr;        /* the neighbour-table kernel                             */
     float ms_blur;       /* the d+1 blur-pass kernels                              */
     float ms_slice;      /* the slice kernel alone                                 */
     float ms_total;      /* first to last                                          */
     unsigned long long probes;    /* hash probes (slot visits), splat+slice+nbr    */
     unsigned long long inserts;   /* global insert attempts (after LDS dedup)      */
-    unsigned long long atomics;   /* global FP32 atomic adds issued by splat       */
+    unsigned long long atomics;   /* global atomic adds issued by splat            */
 } gt_stats;
 
 typedef struct gt_ctx gt_ctx;
@@ -88,7 +87,8 @@ void gt_ctx_destroy(gt_ctx *ctx);
 /* Use an existing hipStream_t (NULL = the context's own stream). */
 int gt_ctx_set_stream(gt_ctx *ctx, void *hip_stream);
 /* 0: off; 1: per-stage HIP-event times in gt_stats; 2: also count hash
- * probes, global inserts and FP32 atomics (slower). */
+ * probes, global inserts and atomics (slower).  Profiled calls run the
+ * pipeline eagerly, launch by launch (a captured graph has no stage events). */
 int gt_ctx_set_profiling(gt_ctx *ctx, int level);
 const char *gt_last_error(void);
 
@@ -135,6 +135,27 @@ int gt_joint_bilateral_host(gt_ctx *ctx, int W, int H, int F, int C, const float
                             int Cr, const float *ref, float fw, float fh, float ff,
                             float color_sigma, float *out, gt_storage storage,
                             const gt_tuning *tune, gt_stats *stats);
+
+/*
+ * A batch of joint-bilateral filters: many images of uneven sizes through one
+ * lattice (the image index is part of every vertex key, so the images never
+ * mix), one graph launch for the whole pipeline.  Each image is filtered as
+ * gt_joint_bilateral(ctx, W, H, 1, C, im, Cr, ref, sigma_s, sigma_s, 0,
+ * sigma_r, out) would filter it; images that need the zero-sigma recursion
+ * or a lattice beyond d = 6 / vd = 4 are filtered one by one.
+ */
+typedef struct {
+    int W, H;
+    const float *im;     /* C planes of W*H pixels (device)        */
+    const float *ref;    /* Cr planes (device)                     */
+    float *out;          /* C planes (device); may alias im        */
+    float sigma_s;       /* spatial standard deviation (x and y)   */
+    float sigma_r;       /* colour standard deviation              */
+} gt_bilateral_image;
+
+int gt_joint_bilateral_batch(gt_ctx *ctx, int count, const gt_bilateral_image *images,
+                             int C, int Cr, gt_storage storage, const gt_tuning *tune,
+                             gt_stats *stats);
 
 /*
  * Lattice inspection for parity checks: splat (and blur) the given points and

@@ -1,0 +1,94 @@
+"""Batched lattices (gt_joint_bilateral_batch) and graph replays.
+
+Each image of a batch is filtered as it would be alone; the splat
+accumulates in fixed point, so a replayed graph reproduces its output bit for
+bit and a batch matches the single-image runs to fixed-point rounding."""
+import numpy as np
+import pytest
+
+from imagestack_amd import synth as S
+from tests.util import FP32_REL_TOL, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def dev(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32)).cuda()
+
+
+def make_batch(seed=0, count=7, C=3):
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        W = int(rng.integers(33, 200))
+        H = int(rng.integers(17, 160))
+        ref = S.rect_image(W, H, C, K=6, noise=0.05, seed=1000 + seed * 100 + k)
+        im = S.uniform_image(W, H, C, seed=2000 + seed * 100 + k) if k % 2 else ref.copy()
+        ss = float(rng.choice([4.0, 8.0, 16.0]))
+        sr = float(rng.choice([0.05, 0.1, 0.2]))
+        out.append((im, ref, ss, sr))
+    return out
+
+
+def test_batch_matches_single_and_oracle(gpu_ctx, oracle_lib):
+    import torch
+    print("\n[case] batch of 7 RGB images vs single runs and FP64 oracle", flush=True)
+    batch = make_batch(1)
+    items, dev_in = [], []
+    for im, ref, ss, sr in batch:
+        a, b = dev(im), dev(ref)
+        o = torch.empty_like(a)
+        items.append((a, b, o, ss, sr))
+    st = gpu_ctx.joint_bilateral_batch(items, stats=True)
+    print("batch M", st["M"], "sort passes", st["sort_passes"])
+    for (im, ref, ss, sr), (a, b, o, _, _) in zip(batch, items):
+        single = gpu_ctx.joint_bilateral(a, b, ss, ss, 0.0, sr).cpu().numpy()
+        got = o.cpu().numpy()
+        assert rel_err(got, single) < 1e-6
+        o64 = oracle_lib.joint_bilateral(im, ref, ss, ss, 0.0, sr, mode=oracle_lib.F64_LAT)
+        assert rel_err(got, o64) < FP32_REL_TOL
+
+
+def test_batch_grayscale_narrow_tiles(gpu_ctx, oracle_lib):
+    import torch
+    print("\n[case] grayscale batch (d=3, vd=2), 16-pixel-wide tiles", flush=True)
+    batch = make_batch(2, count=5, C=1)
+    items = [(dev(im), dev(ref), torch.empty((1, 1) + im.shape[2:], device="cuda"), ss, sr)
+             for im, ref, ss, sr in batch]
+    gpu_ctx.joint_bilateral_batch(items, tune={"tile_w": 16})
+    for (im, ref, ss, sr), (_, _, o, _, _) in zip(batch, items):
+        o64 = oracle_lib.joint_bilateral(im, ref, ss, ss, 0.0, sr, mode=oracle_lib.F64_LAT)
+        assert rel_err(o.cpu().numpy(), o64) < FP32_REL_TOL
+
+
+def test_graph_replays_are_identical(gpu_ctx):
+    """The captured pipeline graph replays three times, with eager single-image
+    work and synchronisation between replays; every replay is bit-identical."""
+    import torch
+    print("\n[case] graph replays", flush=True)
+    batch = make_batch(3, count=4)
+    items = [(dev(im), dev(ref), None, ss, sr) for im, ref, ss, sr in batch]
+    items = [(a, b, torch.empty_like(a), ss, sr) for a, b, _, ss, sr in items]
+    outs = []
+    for rep in range(3):
+        for it in items:
+            it[2].zero_()
+        gpu_ctx.joint_bilateral_batch(items)
+        outs.append([it[2].cpu().numpy() for it in items])
+        eager = gpu_ctx.joint_bilateral(items[0][0], items[0][1], 8.0, 8.0, 0.0, 0.1, tune={"graph": -1})
+        torch.cuda.synchronize()
+        assert np.isfinite(eager.cpu().numpy()).all()
+    for rep in range(1, 3):
+        for a, b in zip(outs[0], outs[rep]):
+            assert np.array_equal(a, b)
+
+
+def test_single_image_graph_vs_eager(gpu_ctx):
+    print("\n[case] single image graph vs eager", flush=True)
+    im = S.rect_image(160, 120, 3, K=12, noise=0.05, seed=77)
+    a = dev(im)
+    g1 = gpu_ctx.joint_bilateral(a, a, 8.0, 8.0, 0.0, 0.1).cpu().numpy()
+    g2 = gpu_ctx.joint_bilateral(a, a, 8.0, 8.0, 0.0, 0.1).cpu().numpy()
+    e = gpu_ctx.joint_bilateral(a, a, 8.0, 8.0, 0.0, 0.1, tune={"graph": -1}).cpu().numpy()
+    assert np.array_equal(g1, g2) and np.array_equal(g1, e)
