@@ -7,7 +7,8 @@
  * driving src/Permutohedral.h) with hand-written HIP kernels:
  *   splat  : elevation, enclosing simplex, barycentric weights
  *            (src/Permutohedral.h:290-393), GPU open-addressing hash insert and
- *            FP32 accumulation of weighted values;
+ *            exact 64-bit fixed-point accumulation of weighted values (integer
+ *            atomics commute, so the sums do not depend on scheduling);
  *   sort   : compaction of the vertex keys and LSD radix sort, so vertex ids
  *            are the lexicographic rank of the key (deterministic);
  *   blur   : neighbour table, then d+1 Jacobi passes of [1/4 1/2 1/4]
@@ -50,7 +51,8 @@ enum {
     GT_ESTATE = 5       /* call out of order (e.g. export before build)       */
 };
 
-/* Storage type of lattice values.  Accumulation is always FP32. */
+/* Storage type of lattice values between splat, blur and slice.  Blur and
+ * slice arithmetic is always FP32; splat sums are 64-bit fixed point. */
 typedef enum { GT_STORE_F32 = 0, GT_STORE_BF16 = 1, GT_STORE_F16 = 2 } gt_storage;
 
 /* Tuning knobs.  Zero fields select defaults (or the autotuner's choice). */
