@@ -64,6 +64,10 @@ size_t store_bytes(int storage) { return storage == GT_STORE_F32 ? 4 : 2; }
 
 } // namespace
 
+namespace gt {
+int set_error(int code, const char *msg) { return fail(code, "%s", msg); }
+} // namespace gt
+
 struct gt_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -126,6 +130,7 @@ struct Tune {
     float lf = 0.35f;
     int dedup = 0;      // 0 auto (LDS tiles when d <= 6 and vd <= 4), 1 on, -1 off
     int tile_w = 32;
+    int wpc = 0;        // waves-per-CU cap of the tiled and neighbour kernels (0: none)
     bool graph = true;  // replay the pipeline as a cached hipGraph
 };
 
@@ -135,6 +140,7 @@ Tune resolve(const gt_tuning *t) {
     if (t->load_factor > 0) r.lf = std::min(0.7f, std::max(0.05f, t->load_factor));
     r.dedup = t->dedup;
     if (t->tile_w != 0) r.tile_w = t->tile_w;
+    if (t->waves_per_cu > 0) r.wpc = std::min(64, std::max(12, t->waves_per_cu));
     if (t->graph < 0) r.graph = false;
     return r;
 }
@@ -157,9 +163,7 @@ int make_plan(gt_ctx *c, const gt::PixSrc &src, int n, int d, int vd, bool bilat
     GT_TRY(ensure(c->mm_part, (size_t)nblk * (nplanes + nval) * 3 * 4));
     GT_TRY(ensure(c->mm_out, (size_t)(nplanes + nval) * 3 * 4));
     float *part = (float *)c->mm_part.p, *mout = (float *)c->mm_out.p;
-    GT_HIP(gt::launch_minmax(planes, cs, nplanes, n, part, nblk, mout, c->stream));
-    GT_HIP(gt::launch_minmax(src.val, src.val_cs, nval, n, part + (size_t)nblk * nplanes * 3, nblk,
-                             mout + (size_t)nplanes * 3, c->stream));
+    GT_HIP(gt::launch_minmax(planes, cs, nplanes, src.val, src.val_cs, nval, n, part, nblk, mout, c->stream));
     GT_HIP(hipMemcpyAsync(c->h_mm, mout, (size_t)(nplanes + nval) * 3 * 4, hipMemcpyDeviceToHost, c->stream));
     GT_HIP(hipStreamSynchronize(c->stream));
     const float *mm = c->h_mm, *vm = c->h_mm + (size_t)nplanes * 3;
@@ -243,7 +247,7 @@ void sig_add(std::string &k, const T &v) { k.append((const char *)&v, sizeof(T))
 // The whole pipeline: splat, compaction, sort, finalize, neighbour table,
 // blur, and (optionally) slice, enqueued on the context's stream.
 struct Launch {
-    int d, vd, storage, W, pw, m_max, vs, n, row_w, tile_w;
+    int d, vd, storage, W, pw, m_max, vs, n, row_w, tile_w, wpc;
     bool packed, bilat, dedup, count;
     uint32_t cap;
     size_t he;
@@ -267,10 +271,10 @@ int enqueue(gt_ctx *c, const Launch &L, const gt::PixSrc &src, const SliceSpec &
     if (c->profiling) GT_HIP(mark(c, 0, s));
     if (batch)
         GT_HIP(gt::launch_splat_batch(L.d, batch->tmpl, batch->ba, batch->ntiles, L.tile_w, L.vd, L.lc, L.kp, L.vsc,
-                                      ta, ctr, L.count, s));
+                                      ta, ctr, L.count, L.wpc, s));
     else
         GT_HIP(gt::launch_splat(L.d, L.packed, L.bilat, L.dedup, src, L.n, L.row_w, L.tile_w, L.vd, L.lc, L.kp, L.vsc,
-                                ta, ctr, L.count, s));
+                                ta, ctr, L.count, L.wpc, s));
     if (c->profiling) GT_HIP(mark(c, 1, s));
     GT_HIP(gt::launch_compact(L.W, ta, L.cap, (uint32_t *)c->counts.p, (uint32_t *)c->slot_list.p,
                               (uint64_t *)c->pk_a.p, L.packed, ctr, s));
@@ -287,7 +291,7 @@ int enqueue(gt_ctx *c, const Launch &L, const gt::PixSrc &src, const SliceSpec &
     if (c->profiling) GT_HIP(mark(c, 2, s));
     int2 *nbr = (int2 *)c->nbr.p;
     GT_HIP(gt::launch_neighbors(L.d, L.packed, ta, L.kp, (const uint64_t *)c->vkeys.p, ctr, L.m_max, nbr, ctr,
-                                L.count, s));
+                                L.count, L.wpc, s));
     if (c->profiling) GT_HIP(mark(c, 3, s));
     void *cur = c->vals_a.p, *nxt = c->vals_b.p;
     for (int j = 0; j <= L.d; j++) {
@@ -297,10 +301,10 @@ int enqueue(gt_ctx *c, const Launch &L, const gt::PixSrc &src, const SliceSpec &
     if (c->profiling) GT_HIP(mark(c, 4, s));
     if (batch)
         GT_HIP(gt::launch_slice_batch(L.d, L.storage, batch->tmpl, batch->ba, batch->ntiles, L.tile_w, L.vd, L.lc,
-                                      L.kp, ta, cur, batch->tmpl_dst, ctr, L.count, s));
+                                      L.kp, ta, cur, batch->tmpl_dst, ctr, L.count, L.wpc, s));
     else if (sl.enabled)
         GT_HIP(gt::launch_slice(L.d, L.packed, sl.src.ref != nullptr, L.dedup, L.storage, sl.src, sl.n, L.row_w,
-                                L.tile_w, L.vd, L.vs, L.lc, L.kp, ta, cur, sl.dst, ctr, L.count, s));
+                                L.tile_w, L.vd, L.vs, L.lc, L.kp, ta, cur, sl.dst, ctr, L.count, L.wpc, s));
     if (c->profiling) GT_HIP(mark(c, 5, s));
     GT_HIP(hipMemcpyAsync(c->h_ctr, ctr, sizeof(gt::Counters), hipMemcpyDeviceToHost, s));
     *vals_out = cur;
@@ -401,6 +405,7 @@ int run(gt_ctx *c, const gt::PixSrc &src, int n, int row_w, int d, int vd, int s
         L.n = n;
         L.row_w = row_w;
         L.tile_w = tu.tile_w;
+        L.wpc = tu.wpc;
         L.bilat = bilat;
         L.dedup = dedup;
         L.count = count;
@@ -649,6 +654,10 @@ int jb_view(gt_ctx *c, View im, int C, View ref, int Cr, View out, float fw, flo
 }
 
 } // namespace
+
+namespace gt {
+void invalidate_plan(gt_ctx *c) { c->plan_valid = false; }
+} // namespace gt
 
 extern "C" {
 

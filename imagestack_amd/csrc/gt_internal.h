@@ -7,6 +7,8 @@
 
 #include "gt_keyplan.h"
 
+struct gt_ctx;   // the C ABI's opaque context (gt_api.cpp)
+
 namespace gt {
 
 // Device counters, read back by the host once per transform.
@@ -89,27 +91,29 @@ struct ValScale {
     float vmax[kMaxDH];
 };
 
-// Launchers (gt_kernels.hip).  All are asynchronous on `st`.  Kernels after
-// the compaction read the vertex count M from ctr->M on the device; `m_max`
-// only sizes their grids.
-// Per-plane min, max and a non-finite indicator (1.0 if any NaN/Inf) -> out[3*plane..].
-hipError_t launch_minmax(const float *planes, int64_t cs, int nplanes, int n, float *partial, int nblk,
-                         float *out, hipStream_t st);
+// Launchers (gt_launch_*.hip, gt_sort.hip).  All are asynchronous on `st`.
+// Kernels after the compaction read the vertex count M from ctr->M on the
+// device; `m_max` only sizes their grids.
+// Per-plane min, max and a non-finite indicator (1.0 if any NaN/Inf) of two
+// plane groups (A then B, n pixels each) -> out[3*plane..].
+hipError_t launch_minmax(const float *pa, int64_t csa, int na, const float *pb, int64_t csb, int nb, int n,
+                         float *partial, int nblk, float *out, hipStream_t st);
 hipError_t launch_bounds_batch(const BatchArgs &ba, int ntiles, int tile_w, int Cr, int C, float *partial, int nblk,
                                float *out, hipStream_t st);
 hipError_t launch_clear(uint64_t *tkeys, size_t nwords, unsigned long long *tacc, size_t nacc, Counters *ctr,
                         hipStream_t st);
 hipError_t launch_splat(int d, bool packed, bool bilat, bool dedup, const PixSrc &src, int n, int row_w, int tile_w,
                         int vd, const LatConst &lc, const KeyPlan &kp, const ValScale &vsc, const TableArgs &ta,
-                        Counters *ctr, bool count, hipStream_t st);
+                        Counters *ctr, bool count, int wpc, hipStream_t st);
 // Batched forms (bilateral, packed keys, d <= 6, vd <= 4): one workgroup per
-// tile, ntiles tiles over all images.
+// tile, ntiles tiles over all images.  wpc: the waves-per-CU cap of the tiled
+// and neighbour kernels (0: none).
 hipError_t launch_splat_batch(int d, const PixSrc &tmpl, const BatchArgs &ba, int ntiles, int tile_w, int vd,
                               const LatConst &lc, const KeyPlan &kp, const ValScale &vsc, const TableArgs &ta,
-                              Counters *ctr, bool count, hipStream_t st);
+                              Counters *ctr, bool count, int wpc, hipStream_t st);
 hipError_t launch_slice_batch(int d, int storage, const PixSrc &tmpl, const BatchArgs &ba, int ntiles, int tile_w,
                               int vd, const LatConst &lc, const KeyPlan &kp, const TableArgs &ta, const void *vals,
-                              const OutDst &tmpl_dst, Counters *ctr, bool count, hipStream_t st);
+                              const OutDst &tmpl_dst, Counters *ctr, bool count, int wpc, hipStream_t st);
 hipError_t launch_compact(int words, const TableArgs &ta, uint32_t capacity, uint32_t *counts, uint32_t *slot_list,
                           uint64_t *sort_keys, bool packed, Counters *ctr, hipStream_t st);
 hipError_t launch_pack(int d, const TableArgs &ta, const uint32_t *slot_list, const Counters *ctr, int m_max,
@@ -118,12 +122,13 @@ hipError_t launch_finalize(int words, int storage, const TableArgs &ta, const Va
                            const Counters *ctr, int m_max, int vd, int vs, uint64_t *vkeys, void *vals,
                            float *splat_copy, hipStream_t st);
 hipError_t launch_neighbors(int d, bool packed, const TableArgs &ta, const KeyPlan &kp, const uint64_t *vkeys,
-                            const Counters *ctr, int m_max, int2 *nbr, Counters *wctr, bool count, hipStream_t st);
+                            const Counters *ctr, int m_max, int2 *nbr, Counters *wctr, bool count, int wpc,
+                            hipStream_t st);
 hipError_t launch_blur(int storage, int vs, const int2 *nbr, const Counters *ctr, int m_max, const void *in, void *out,
                        hipStream_t st);
 hipError_t launch_slice(int d, bool packed, bool bilat, bool tiled, int storage, const PixSrc &src, int n, int row_w,
                         int tile_w, int vd, int vs, const LatConst &lc, const KeyPlan &kp, const TableArgs &ta,
-                        const void *vals, const OutDst &dst, Counters *ctr, bool count, hipStream_t st);
+                        const void *vals, const OutDst &dst, Counters *ctr, bool count, int wpc, hipStream_t st);
 hipError_t launch_unpack_keys(int d, bool packed, const KeyPlan &kp, const uint64_t *vkeys, int M, int16_t *keys,
                               hipStream_t st);
 hipError_t launch_store_to_f32(int storage, const void *vals, int M, int vd, int vs, float *out, hipStream_t st);
@@ -142,6 +147,11 @@ struct SortBufs {
     size_t hist_cap;
 };
 size_t radix_hist_entries(int m_max);
+// Sets the thread's gt_last_error() message; returns code.
+int set_error(int code, const char *msg);
+// Forgets the context's cached key-packing plan, so the next call bounds its
+// inputs afresh (a server's next request is an unrelated image).
+void invalidate_plan(::gt_ctx *c);
 hipError_t radix_sort(SortBufs &b, const Counters *ctr, int m_max, int pw, int bits, bool *result_in_b, int *passes,
                       hipStream_t st);
 

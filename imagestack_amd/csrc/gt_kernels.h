@@ -187,6 +187,12 @@ __device__ __forceinline__ void rotate_left(T (&a)[N], int rot) {
     }
 }
 
+// Static LDS of the tiled kernels (for the waves-per-CU padding).
+template <int W, int LVD, int LT>
+constexpr size_t splat_tile_lds() { return (size_t)LT * W * 8 + (size_t)LVD * LT * 8 + (size_t)LT * 2 + 16; }
+template <int W, int LVD, int LT>
+constexpr size_t slice_tile_lds() { return (size_t)LT * W * 8 + (size_t)LT * LVD * 4 + (size_t)LT * 2 + 16; }
+
 template <int D, bool PACKED, bool BILAT, int LVD, int LT, bool BATCH>
 __global__ __launch_bounds__(kBlock) void k_splat_tile(PixSrc src0, int n0, int row_w0, int tile_w, BatchArgs ba,
                                                        int vd, LatConst lc, KeyPlan kp, ValScale vsc, TableArgs ta,

@@ -6,6 +6,20 @@ namespace gt {
 
 inline unsigned blocks_for(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
+// Waves-per-CU knob of the pixel and neighbour kernels: dynamic LDS (unused by
+// the kernel) sized so that at most waves_per_cu / 4 workgroups of 256
+// threads fit the CU's 160 KiB of LDS.  A workgroup holds at most 64 KiB, so
+// the cap starts at 3 workgroups (12 waves); 0 leaves occupancy to registers.
+constexpr int64_t kLdsPerCU = 160 * 1024;
+inline unsigned occupancy_pad(size_t static_lds, int waves_per_cu) {
+    if (waves_per_cu <= 0) return 0;
+    int wgs = waves_per_cu / (kBlock / kWave);
+    if (wgs < 3) wgs = 3;
+    int64_t pad = kLdsPerCU / wgs - (int64_t)static_lds - 2048;
+    if (pad > 65536 - (int64_t)static_lds) pad = 65536 - (int64_t)static_lds;
+    return pad > 0 ? (unsigned)pad : 0u;
+}
+
 #define GT_DISPATCH_D(d, MACRO)                                                      \
     switch (d) {                                                                     \
     case 1: MACRO(1); break;  case 2: MACRO(2); break;  case 3: MACRO(3); break;     \

@@ -7,12 +7,17 @@ namespace gt {
 // ---------------------------------------------------------------------------
 // Per-plane min/max of the finite values and a non-finite indicator (for the
 // key packing plan and the fixed-point value scales).
-__global__ __launch_bounds__(kBlock) void k_minmax(const float *__restrict__ planes, int64_t cs, int nplanes, int n,
+// Per-plane bounds of two groups of planes (A: nA planes at stride csA, B: nB
+// planes at stride csB; n pixels each): min, max and a non-finite indicator
+// per plane and workgroup in partial[blk][plane][3].
+__global__ __launch_bounds__(kBlock) void k_minmax(const float *__restrict__ pa, int64_t csa, int na,
+                                                   const float *__restrict__ pb, int64_t csb, int nb, int n,
                                                    float *__restrict__ partial) {
     __shared__ float smn[kBlock / kWave], smx[kBlock / kWave], snf[kBlock / kWave];
-    for (int c = 0; c < nplanes; c++) {
+    const int np = na + nb;
+    for (int c = 0; c < np; c++) {
         float mn = INFINITY, mx = -INFINITY, nf = 0.0f;
-        const float *pl = planes + (int64_t)c * cs;
+        const float *pl = c < na ? pa + (int64_t)c * csa : pb + (int64_t)(c - na) * csb;
         for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
             const float v = pl[i];
             if (isfinite(v)) {
@@ -40,7 +45,7 @@ __global__ __launch_bounds__(kBlock) void k_minmax(const float *__restrict__ pla
                 mx = fmaxf(mx, smx[w]);
                 nf = fmaxf(nf, snf[w]);
             }
-            float *o = partial + ((size_t)blockIdx.x * nplanes + c) * 3;
+            float *o = partial + ((size_t)blockIdx.x * np + c) * 3;
             o[0] = mn;
             o[1] = mx;
             o[2] = nf;
@@ -49,20 +54,41 @@ __global__ __launch_bounds__(kBlock) void k_minmax(const float *__restrict__ pla
     }
 }
 
+// Reduces partial[nblk][nplanes][3] to out[nplanes][3]: one workgroup per
+// plane, the workgroup's threads striding over the blocks.
 __global__ __launch_bounds__(kBlock) void k_minmax_final(const float *__restrict__ partial, int nblk, int nplanes,
                                                          float *__restrict__ out) {
-    const int c = threadIdx.x;
-    if (c >= nplanes) return;
+    __shared__ float smn[kBlock / kWave], smx[kBlock / kWave], snf[kBlock / kWave];
+    const int c = blockIdx.x;
     float mn = INFINITY, mx = -INFINITY, nf = 0.0f;
-    for (int b = 0; b < nblk; b++) {
+    for (int b = threadIdx.x; b < nblk; b += kBlock) {
         const float *o = partial + ((size_t)b * nplanes + c) * 3;
         mn = fminf(mn, o[0]);
         mx = fmaxf(mx, o[1]);
         nf = fmaxf(nf, o[2]);
     }
-    out[c * 3 + 0] = mn;
-    out[c * 3 + 1] = mx;
-    out[c * 3 + 2] = nf;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        mn = fminf(mn, __shfl_xor(mn, off, kWave));
+        mx = fmaxf(mx, __shfl_xor(mx, off, kWave));
+        nf = fmaxf(nf, __shfl_xor(nf, off, kWave));
+    }
+    if (__lane_id() == 0) {
+        smn[threadIdx.x / kWave] = mn;
+        smx[threadIdx.x / kWave] = mx;
+        snf[threadIdx.x / kWave] = nf;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kBlock / kWave; w++) {
+            mn = fminf(mn, smn[w]);
+            mx = fmaxf(mx, smx[w]);
+            nf = fmaxf(nf, snf[w]);
+        }
+        out[c * 3 + 0] = mn;
+        out[c * 3 + 1] = mx;
+        out[c * 3 + 2] = nf;
+    }
 }
 
 // Bounds of a batch, one tile per workgroup iteration: per guide channel the
@@ -125,7 +151,7 @@ __global__ __launch_bounds__(kBlock) void k_bounds_batch(BatchArgs ba, int ntile
 hipError_t launch_bounds_batch(const BatchArgs &ba, int ntiles, int tile_w, int Cr, int C, float *partial, int nblk,
                                float *out, hipStream_t st) {
     hipLaunchKernelGGL(k_bounds_batch, dim3(nblk), dim3(kBlock), 0, st, ba, ntiles, tile_w, Cr, C, partial);
-    hipLaunchKernelGGL(k_minmax_final, dim3(1), dim3(kBlock), 0, st, partial, nblk, Cr + C, out);
+    hipLaunchKernelGGL(k_minmax_final, dim3(Cr + C), dim3(kBlock), 0, st, partial, nblk, Cr + C, out);
     return hipGetLastError();
 }
 
@@ -189,11 +215,11 @@ __global__ __launch_bounds__(kBlock) void k_view_copy(const float *__restrict__ 
 }
 
 
-hipError_t launch_minmax(const float *planes, int64_t cs, int nplanes, int n, float *partial, int nblk, float *out,
-                         hipStream_t st) {
-    if (nplanes <= 0 || n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_minmax, dim3(nblk), dim3(kBlock), 0, st, planes, cs, nplanes, n, partial);
-    hipLaunchKernelGGL(k_minmax_final, dim3(1), dim3(kBlock), 0, st, partial, nblk, nplanes, out);
+hipError_t launch_minmax(const float *pa, int64_t csa, int na, const float *pb, int64_t csb, int nb, int n,
+                         float *partial, int nblk, float *out, hipStream_t st) {
+    if (na + nb <= 0 || n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_minmax, dim3(nblk), dim3(kBlock), 0, st, pa, csa, na, pb, csb, nb, n, partial);
+    hipLaunchKernelGGL(k_minmax_final, dim3(na + nb), dim3(kBlock), 0, st, partial, nblk, na + nb, out);
     return hipGetLastError();
 }
 
@@ -269,16 +295,18 @@ hipError_t launch_finalize(int words, int storage, const TableArgs &ta, const Va
 }
 
 hipError_t launch_neighbors(int d, bool packed, const TableArgs &ta, const KeyPlan &kp, const uint64_t *vkeys,
-                            const Counters *ctr, int m_max, int2 *nbr, Counters *wctr, bool count, hipStream_t st) {
+                            const Counters *ctr, int m_max, int2 *nbr, Counters *wctr, bool count, int wpc,
+                            hipStream_t st) {
     if (m_max <= 0) return hipSuccess;
     const int cnt = count ? 1 : 0;
     const dim3 g(blocks_for(m_max, kBlock), d + 1), b(kBlock);
-#define GT_NBR(D)                                                                                          \
-    {                                                                                                      \
-        if (packed) hipLaunchKernelGGL((k_neighbors<D, true>), g, b, 0, st, ta, kp, vkeys, ctr, m_max, nbr, \
-                                       wctr, cnt);                                                          \
-        else hipLaunchKernelGGL((k_neighbors<D, false>), g, b, 0, st, ta, kp, vkeys, ctr, m_max, nbr, wctr, \
-                                cnt);                                                                       \
+    const unsigned pad = occupancy_pad(0, wpc);
+#define GT_NBR(D)                                                                                            \
+    {                                                                                                        \
+        if (packed) hipLaunchKernelGGL((k_neighbors<D, true>), g, b, pad, st, ta, kp, vkeys, ctr, m_max, nbr, \
+                                       wctr, cnt);                                                            \
+        else hipLaunchKernelGGL((k_neighbors<D, false>), g, b, pad, st, ta, kp, vkeys, ctr, m_max, nbr, wctr, \
+                                cnt);                                                                         \
     }
     GT_DISPATCH_D(d, GT_NBR)
 #undef GT_NBR

@@ -6,7 +6,7 @@ namespace gt {
 
 hipError_t launch_splat(int d, bool packed, bool bilat, bool dedup, const PixSrc &src, int n, int row_w, int tile_w,
                         int vd, const LatConst &lc, const KeyPlan &kp, const ValScale &vsc, const TableArgs &ta,
-                        Counters *ctr, bool count, hipStream_t st) {
+                        Counters *ctr, bool count, int wpc, hipStream_t st) {
     if (d > 6)
         return launch_splat_direct_hi(d, packed, bilat, src, n, row_w, tile_w, vd, lc, kp, vsc, ta, ctr, count, st);
     const int64_t ntiles = pixel_blocks(n, row_w, tile_w);
@@ -16,11 +16,13 @@ hipError_t launch_splat(int d, bool packed, bool bilat, bool dedup, const PixSrc
     const BatchArgs none{nullptr, 0};
 #define GT_SPLAT_T(D, P, B)                                                                                     \
     {                                                                                                           \
+        constexpr int W_ = KeyOps<D, P>::W;                                                                     \
         if (vd <= 2) hipLaunchKernelGGL((k_splat_tile<D, P, B, 2, kTileKeys, false>), dim3(ntiles),             \
-                                        dim3(kBlock), 0, st, src, n, row_w, tile_w, none, vd, lc, kp, vsc, ta,   \
-                                        ctr, cnt);                                                               \
-        else hipLaunchKernelGGL((k_splat_tile<D, P, B, 4, kTileKeys, false>), dim3(ntiles), dim3(kBlock), 0, st, \
-                                src, n, row_w, tile_w, none, vd, lc, kp, vsc, ta, ctr, cnt);                     \
+                                        dim3(kBlock), occupancy_pad(splat_tile_lds<W_, 2, kTileKeys>(), wpc),    \
+                                        st, src, n, row_w, tile_w, none, vd, lc, kp, vsc, ta, ctr, cnt);         \
+        else hipLaunchKernelGGL((k_splat_tile<D, P, B, 4, kTileKeys, false>), dim3(ntiles), dim3(kBlock),        \
+                                occupancy_pad(splat_tile_lds<W_, 4, kTileKeys>(), wpc), st, src, n, row_w,       \
+                                tile_w, none, vd, lc, kp, vsc, ta, ctr, cnt);                                    \
     }
 #define GT_SPLAT_TILE(D)                                                                    \
     {                                                                                       \
@@ -48,17 +50,18 @@ hipError_t launch_splat(int d, bool packed, bool bilat, bool dedup, const PixSrc
 
 hipError_t launch_splat_batch(int d, const PixSrc &tmpl, const BatchArgs &ba, int ntiles, int tile_w, int vd,
                               const LatConst &lc, const KeyPlan &kp, const ValScale &vsc, const TableArgs &ta,
-                              Counters *ctr, bool count, hipStream_t st) {
+                              Counters *ctr, bool count, int wpc, hipStream_t st) {
     if (ntiles <= 0) return hipSuccess;
     if (vd > 4 || !kp.packed) return hipErrorInvalidValue;
     const int cnt = count ? 1 : 0;
 #define GT_SPLAT_B(D)                                                                                           \
     {                                                                                                           \
         if (vd <= 2) hipLaunchKernelGGL((k_splat_tile<D, true, true, 2, kTileKeys, true>), dim3(ntiles),         \
-                                        dim3(kBlock), 0, st, tmpl, 0, 0, tile_w, ba, vd, lc, kp, vsc, ta, ctr,   \
-                                        cnt);                                                                    \
+                                        dim3(kBlock), occupancy_pad(splat_tile_lds<1, 2, kTileKeys>(), wpc), st, \
+                                        tmpl, 0, 0, tile_w, ba, vd, lc, kp, vsc, ta, ctr, cnt);                  \
         else hipLaunchKernelGGL((k_splat_tile<D, true, true, 4, kTileKeys, true>), dim3(ntiles), dim3(kBlock),   \
-                                0, st, tmpl, 0, 0, tile_w, ba, vd, lc, kp, vsc, ta, ctr, cnt);                   \
+                                occupancy_pad(splat_tile_lds<1, 4, kTileKeys>(), wpc), st, tmpl, 0, 0, tile_w,   \
+                                ba, vd, lc, kp, vsc, ta, ctr, cnt);                                              \
     }
     GT_DISPATCH_DT(d, GT_SPLAT_B)
 #undef GT_SPLAT_B

@@ -23,6 +23,8 @@ EXPORTS = [
     "gt_ctx_create", "gt_ctx_destroy", "gt_ctx_set_stream", "gt_ctx_set_profiling", "gt_last_error",
     "gt_gauss_transform", "gt_gauss_transform_host", "gt_joint_bilateral", "gt_joint_bilateral_host",
     "gt_build_lattice", "gt_lattice_export", "gt_joint_bilateral_batch",
+    "gt_autotune_joint_bilateral", "gt_autotune_joint_bilateral_batch",
+    "gt_server_create", "gt_server_destroy", "gt_server_submit", "gt_server_wait", "gt_server_replay",
 ]
 
 
@@ -33,7 +35,31 @@ class GTError(RuntimeError):
 
 
 class Tuning(C.Structure):
-    _fields_ = [("load_factor", C.c_float), ("dedup", C.c_int), ("tile_w", C.c_int), ("graph", C.c_int)]
+    _fields_ = [("load_factor", C.c_float), ("waves_per_cu", C.c_int), ("dedup", C.c_int), ("tile_w", C.c_int),
+                ("graph", C.c_int)]
+
+
+class TuneReport(C.Structure):
+    _fields_ = [("trials", C.c_int), ("best_ms", C.c_float), ("default_ms", C.c_float)]
+
+
+def tuning_dict(t: Tuning) -> dict:
+    return {k: getattr(t, k) for k, _ in Tuning._fields_}
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("slots", C.c_int), ("max_batch", C.c_int), ("batch_pixels", C.c_int), ("hbm_budget", C.c_size_t)]
+
+
+class Request(C.Structure):
+    _fields_ = [("W", C.c_int), ("H", C.c_int), ("C", C.c_int), ("im", C.c_void_p), ("row_pitch", C.c_longlong),
+                ("plane_pitch", C.c_longlong), ("out", C.c_void_p), ("sigma_s", C.c_float), ("sigma_r", C.c_float),
+                ("storage", C.c_int)]
+
+
+class RequestResult(C.Structure):
+    _fields_ = [("status", C.c_int), ("batch", C.c_int), ("latency_ms", C.c_double), ("queue_ms", C.c_double),
+                ("footprint", C.c_size_t)]
 
 
 class BilateralImage(C.Structure):
@@ -82,6 +108,18 @@ def lib():
         L.gt_lattice_export.argtypes = [_vp, _vp, _vp, _vp, _vp]
         L.gt_joint_bilateral_batch.argtypes = [_vp, C.c_int, C.POINTER(BilateralImage), C.c_int, C.c_int, C.c_int,
                                                C.POINTER(Tuning), C.POINTER(Stats)]
+        L.gt_autotune_joint_bilateral.argtypes = [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp, C.c_int, _vp,
+                                                  C.c_float, C.c_float, C.c_float, C.c_float, C.c_int, C.c_int,
+                                                  C.POINTER(Tuning), C.POINTER(TuneReport)]
+        L.gt_autotune_joint_bilateral_batch.argtypes = [_vp, C.c_int, C.POINTER(BilateralImage), C.c_int, C.c_int,
+                                                        C.c_int, C.c_int, C.POINTER(Tuning), C.POINTER(TuneReport)]
+        L.gt_server_create.argtypes = [C.POINTER(_vp), C.c_int, C.POINTER(ServerConfig)]
+        L.gt_server_destroy.argtypes = [_vp]
+        L.gt_server_destroy.restype = None
+        L.gt_server_submit.argtypes = [_vp, C.POINTER(Request), C.POINTER(C.c_longlong)]
+        L.gt_server_wait.argtypes = [_vp, C.c_longlong, C.POINTER(RequestResult)]
+        L.gt_server_replay.argtypes = [_vp, C.c_int, C.POINTER(Request), C.POINTER(C.c_double),
+                                       C.POINTER(RequestResult)]
         _lib = L
     return _lib
 
@@ -170,6 +208,14 @@ class Context:
     def joint_bilateral_batch(self, items, storage: str = "f32", tune=None, stats: bool = False):
         """items: list of (im (C,1,H,W), ref (Cr,1,H,W), out, sigma_s, sigma_r)
         device tensors; filters every image in one batched lattice."""
+        arr, C_, Cr = self._batch_array(items)
+        st = Stats()
+        _check(lib().gt_joint_bilateral_batch(self._h, len(items), arr, C_, Cr, STORAGE[storage], _tuning(tune),
+                                              C.byref(st)))
+        return st.as_dict() if stats else None
+
+    @staticmethod
+    def _batch_array(items):
         arr = (BilateralImage * len(items))()
         C_ = Cr = None
         for k, (im, ref, out, ss, sr) in enumerate(items):
@@ -177,10 +223,24 @@ class Context:
             assert F == 1 and out.shape == im.shape
             C_, Cr = Cc, ref.shape[0]
             arr[k] = BilateralImage(W, H, _dptr(im), _dptr(ref), _dptr(out), ss, sr)
-        st = Stats()
-        _check(lib().gt_joint_bilateral_batch(self._h, len(items), arr, C_ or 1, Cr or 1, STORAGE[storage],
-                                              _tuning(tune), C.byref(st)))
-        return st.as_dict() if stats else None
+        return arr, C_ or 1, Cr or 1
+
+    def autotune_joint_bilateral(self, im, ref, fw, fh, ff, color_sigma, storage: str = "f32", reps: int = 3):
+        """Search tile width x load factor x waves per CU on these inputs;
+        returns (tuning dict, report dict).  `im` is only read."""
+        Cc, F, H, W = im.shape
+        best, rep = Tuning(), TuneReport()
+        _check(lib().gt_autotune_joint_bilateral(self._h, W, H, F, Cc, _dptr(im), ref.shape[0], _dptr(ref), fw, fh,
+                                                 ff, color_sigma, STORAGE[storage], reps, C.byref(best),
+                                                 C.byref(rep)))
+        return tuning_dict(best), {k: getattr(rep, k) for k, _ in TuneReport._fields_}
+
+    def autotune_joint_bilateral_batch(self, items, storage: str = "f32", reps: int = 3):
+        arr, C_, Cr = self._batch_array(items)
+        best, rep = Tuning(), TuneReport()
+        _check(lib().gt_autotune_joint_bilateral_batch(self._h, len(items), arr, C_, Cr, STORAGE[storage], reps,
+                                                       C.byref(best), C.byref(rep)))
+        return tuning_dict(best), {k: getattr(rep, k) for k, _ in TuneReport._fields_}
 
     def build_lattice(self, splat, values, inv_sigma, row_w: int = 0, storage: str = "f32", tune=None):
         """Splat + blur and export: dict(keys (M,d) int16, nbr (d+1,M,2) int32,
@@ -227,4 +287,73 @@ class Context:
         _check(lib().gt_joint_bilateral_host(self._h, W, H, F, Cc, im.ctypes.data, ref.shape[0],
                                              ref.ctypes.data, fw, fh, ff, color_sigma, out.ctypes.data,
                                              STORAGE[storage], _tuning(tune), C.byref(st)))
+        return out
+
+
+def _host_ptr(a) -> int:
+    """Address of a float32 host buffer: a numpy array or a CPU torch tensor
+    (pinned or not)."""
+    if isinstance(a, np.ndarray):
+        if a.dtype != np.float32:
+            raise ValueError("expected float32")
+        return a.ctypes.data
+    if getattr(a, "is_cuda", False) or a.dtype.itemsize != 4:
+        raise ValueError("expected a float32 host tensor")
+    return a.data_ptr()
+
+
+def make_request(im, out, sigma_s: float, sigma_r: float, storage: str = "f32", row_pitch: int | None = None,
+                 plane_pitch: int | None = None, W: int | None = None, H: int | None = None) -> Request:
+    """A bilateral request on host memory.  `im` is (C, H, W) dense float32, or
+    (with W, H and the pitches given) a view into a larger planar buffer whose
+    first pixel `im` points at; `out` is dense (C, H, W)."""
+    Cc = out.shape[0]
+    Hh = H if H is not None else out.shape[-2]
+    Ww = W if W is not None else out.shape[-1]
+    rp = row_pitch if row_pitch is not None else Ww
+    pp = plane_pitch if plane_pitch is not None else Ww * Hh
+    addr = im if isinstance(im, int) else _host_ptr(im)
+    return Request(Ww, Hh, Cc, addr, rp, pp, _host_ptr(out), sigma_s, sigma_r, STORAGE[storage])
+
+
+class Server:
+    """gt_server: a request loop with admission by HBM footprint, continuous
+    batching and `slots` concurrent workers."""
+
+    def __init__(self, device: int = 0, slots: int = 4, max_batch: int = 16, batch_pixels: int = 1 << 22,
+                 hbm_budget: int = 0):
+        self._h = _vp()
+        cfg = ServerConfig(slots, max_batch, batch_pixels, hbm_budget)
+        _check(lib().gt_server_create(C.byref(self._h), device, C.byref(cfg)))
+
+    def close(self) -> None:
+        if self._h:
+            lib().gt_server_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, req: Request) -> int:
+        t = C.c_longlong()
+        _check(lib().gt_server_submit(self._h, C.byref(req), C.byref(t)))
+        return t.value
+
+    def wait(self, ticket: int) -> dict:
+        r = RequestResult()
+        _check(lib().gt_server_wait(self._h, ticket, C.byref(r)))
+        return {k: getattr(r, k) for k, _ in RequestResult._fields_}
+
+    def replay(self, reqs, arrival_ms) -> list:
+        n = len(reqs)
+        arr = (Request * n)(*reqs)
+        t = (C.c_double * n)(*[float(x) for x in arrival_ms])
+        res = (RequestResult * n)()
+        rc = lib().gt_server_replay(self._h, n, arr, t, res)
+        out = [{k: getattr(r, k) for k, _ in RequestResult._fields_} for r in res]
+        if rc != 0:
+            raise GTError(rc, lib().gt_last_error().decode())
         return out

@@ -33,6 +33,7 @@
 #ifndef GTLATTICE_H
 #define GTLATTICE_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -58,6 +59,8 @@ typedef enum { GT_STORE_F32 = 0, GT_STORE_BF16 = 1, GT_STORE_F16 = 2 } gt_storag
 /* Tuning knobs.  Zero fields select defaults (or the autotuner's choice). */
 typedef struct {
     float load_factor;   /* target hash-table load, 0.05..0.7 (default 0.35)       */
+    int waves_per_cu;    /* resident waves per CU of the tiled and neighbour kernels:
+                            12..64 caps occupancy through LDS; 0 = no cap         */
     int dedup;           /* LDS tile dedup / pre-reduction: 0 auto, 1 on, -1 off     */
     int tile_w;          /* pixel-tile width of a workgroup: 16, 32 (default), 64    */
     int graph;           /* 0: replay the pipeline as a cached hipGraph; -1: don't  */
@@ -158,6 +161,83 @@ typedef struct {
 int gt_joint_bilateral_batch(gt_ctx *ctx, int count, const gt_bilateral_image *images,
                              int C, int Cr, gt_storage storage, const gt_tuning *tune,
                              gt_stats *stats);
+
+/*
+ * Autotuner: times the pipeline on these inputs over a grid of tunings
+ * (tile_w 16/32/64 x load_factor 0.25/0.35/0.5 x waves_per_cu none/12/20,
+ * plus the defaults), `reps` timed calls each after one untimed call, and
+ * returns the tuning with the lowest median time.  Trial outputs go to a
+ * scratch buffer; the inputs are only read.  No reference counterpart: the
+ * reference has no tunable parameters.
+ */
+typedef struct {
+    int trials;          /* tunings timed (including the defaults)               */
+    float best_ms;       /* median ms per call of the returned tuning            */
+    float default_ms;    /* median ms per call of the default tuning             */
+} gt_tune_report;
+
+int gt_autotune_joint_bilateral(gt_ctx *ctx, int W, int H, int F, int C, const float *im,
+                                int Cr, const float *ref, float fw, float fh, float ff,
+                                float color_sigma, gt_storage storage, int reps,
+                                gt_tuning *best, gt_tune_report *report);
+int gt_autotune_joint_bilateral_batch(gt_ctx *ctx, int count, const gt_bilateral_image *images,
+                                      int C, int Cr, gt_storage storage, int reps,
+                                      gt_tuning *best, gt_tune_report *report);
+
+/*
+ * Request loop (serving).  A server owns `slots` worker threads, each with
+ * its own context (lattice buffers stay resident between requests) and its
+ * own stream.  A request is a bilateral filter of a host image by itself
+ * (Bilateral::apply, src/GaussTransform.cpp:730-734, PERMUTOHEDRAL) with
+ * sigma_s in x and y.  Requests are admitted in arrival order when their HBM
+ * footprint estimate fits the budget next to the running groups (or nothing
+ * runs); a worker that frees up takes the oldest request plus every waiting
+ * request of the same channel count and storage (up to max_batch requests
+ * and batch_pixels pixels) as one batched lattice.  Workers run
+ * concurrently, so groups overlap on the device.  Results are the same as
+ * one gt_joint_bilateral call per request up to the batch's fixed-point
+ * scale (relative differences below 1e-6).  No reference counterpart: the
+ * reference filters one image per process.
+ */
+typedef struct {
+    int slots;           /* worker threads / streams (default 4, at most 16)     */
+    int max_batch;       /* requests per batched lattice (default 16)            */
+    int batch_pixels;    /* pixels per batched lattice (default 4 Mi)            */
+    size_t hbm_budget;   /* bytes of admitted footprints (default: half the free memory) */
+} gt_server_config;
+
+typedef struct {
+    int W, H, C;                 /* C = 1 (grey) or 3 (RGB)                      */
+    const float *im;             /* host: pixel (x,y) of channel c at
+                                    im[c * plane_pitch + y * row_pitch + x]      */
+    long long row_pitch;         /* floats between rows   (>= W)                 */
+    long long plane_pitch;       /* floats between planes                        */
+    float *out;                  /* host: dense C x H x W                        */
+    float sigma_s, sigma_r;      /* spatial and colour standard deviations       */
+    gt_storage storage;
+} gt_request;
+
+typedef struct {
+    int status;                  /* GT_OK or the request's error code            */
+    int batch;                   /* requests in the lattice it ran in            */
+    double latency_ms;           /* submit -> result in host memory              */
+    double queue_ms;             /* submit -> taken by a worker                  */
+    size_t footprint;            /* admitted HBM footprint estimate (bytes)      */
+} gt_request_result;
+
+typedef struct gt_server gt_server;
+
+int gt_server_create(gt_server **server, int device, const gt_server_config *config);
+void gt_server_destroy(gt_server *server);
+/* Queues a request (the pixels are read when a worker takes it: keep `im` and
+ * `out` alive until gt_server_wait returns).  Never blocks. */
+int gt_server_submit(gt_server *server, const gt_request *request, long long *ticket);
+/* Blocks until the request is done; returns its status. */
+int gt_server_wait(gt_server *server, long long ticket, gt_request_result *result);
+/* Open-loop replay: submits request i at arrival_ms[i] after the call starts
+ * (one submitting thread), then waits for all of them. */
+int gt_server_replay(gt_server *server, int count, const gt_request *requests,
+                     const double *arrival_ms, gt_request_result *results);
 
 /*
  * Lattice inspection for parity checks: splat (and blur) the given points and
