@@ -85,6 +85,95 @@ def cpu_baseline_leg(out_gpu, im, guide, ss, sr):
     return cpu, err
 
 
+def progress(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def batch_leg(gtlattice, synth, torch, steps):
+    """cfg4: 64 RGB images of uneven sizes as one batched lattice (one graph
+    launch per call), tuned by the autotuner; also the same images one call
+    each, for comparison."""
+    imgs = synth.cfg4()
+    dev = [torch.from_numpy(im[:, None]).cuda() for im, _, _ in imgs]
+    outs = [torch.empty_like(d) for d in dev]
+    items = [(d, d, o, ss, sr) for d, o, (_, ss, sr) in zip(dev, outs, imgs)]
+    npx = sum(im.shape[1] * im.shape[2] for im, _, _ in imgs)
+    ctx = gtlattice.Context(0)
+    t0 = time.perf_counter()
+    best, rep = ctx.autotune_joint_bilateral_batch(items, reps=2)
+    tune_s = time.perf_counter() - t0
+    ctx.joint_bilateral_batch(items, tune=best)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.joint_bilateral_batch(items, tune=best)
+    torch.cuda.synchronize()
+    ms_batch = (time.perf_counter() - t0) * 1e3 / steps
+    t0 = time.perf_counter()
+    for _ in range(max(1, steps // 4)):
+        for d, o, (_, ss, sr) in zip(dev, outs, imgs):
+            ctx.joint_bilateral(d, d, ss, ss, 0.0, sr, out=o)
+    torch.cuda.synchronize()
+    ms_single = (time.perf_counter() - t0) * 1e3 / max(1, steps // 4)
+    ctx.close()
+    return {"workload": "64 RGB images, sides log-uniform [256,1536] per axis, 64 rectangles + N(0,0.05), "
+                        "sigma_s in {4,8,16}, sigma_r in {0.05,0.1,0.2} per image; FP32",
+            "pixels": int(npx), "ms_per_batch": round(ms_batch, 4), "value": round(npx / ms_batch / 1e3, 2),
+            "unit": "Mpix/s", "one_call_per_image_ms": round(ms_single, 4),
+            "one_call_per_image_mpix_s": round(npx / ms_single / 1e3, 2),
+            "autotune": {"best": best, "trials": rep["trials"], "best_ms": round(rep["best_ms"], 4),
+                         "default_ms": round(rep["default_ms"], 4), "search_s": round(tune_s, 2)}}
+
+
+def serve_leg(gtlattice, synth, torch, n_req, qps, slots, storages=("f32", "bf16")):
+    """cfg5: open-loop Poisson replay through the request server; latency is
+    submit -> result in (pinned) host memory, PCIe copies included."""
+    import numpy as np
+    pool_side = 4096
+    pool = torch.from_numpy(synth.rect_image(pool_side, pool_side, 3, K=256, noise=0.05, seed=9)[:, 0]).pin_memory()
+    ring = [torch.empty((3, pool_side, pool_side), dtype=torch.float32).pin_memory() for _ in range(8)]
+    base = pool.data_ptr()
+    plane = pool_side * pool_side
+    srv = gtlattice.Server(0, slots=slots)
+
+    def requests(sched, storage):
+        reqs = []
+        for i in range(len(sched["side"])):
+            side, Cc = int(sched["side"][i]), int(sched["C"][i])
+            addr = base + 4 * (int(sched["y0"][i]) * pool_side + int(sched["x0"][i]))
+            out = ring[i % len(ring)][:Cc].reshape(-1)[:Cc * side * side].view(Cc, side, side)
+            reqs.append(gtlattice.make_request(addr, out, float(sched["sigma_s"][i]), float(sched["sigma_r"][i]),
+                                               storage=storage, row_pitch=pool_side, plane_pitch=plane, W=side,
+                                               H=side))
+        return reqs
+
+    out = {"workload": f"Poisson arrivals at {qps:g} QPS; square sides log-uniform [128,4096]; 60% grey (d=3, vd=2), "
+                       "40% RGB (d=5, vd=4); sigma_s in {4,16,32}, sigma_r in {0.05,0.2}; crops of a pinned "
+                       "4096^2 host image; latency = submit -> result in pinned host memory (PCIe included)",
+           "qps": qps, "slots": slots, "requests": n_req}
+    warm = synth.cfg5_schedule(100, qps, seed=50)
+    srv.replay(requests(warm, "f32"), warm["arrival_ms"])
+    for storage in storages:
+        n = n_req if storage == "f32" else max(200, n_req // 2)
+        progress(f"serving {n} {storage} requests")
+        sched = synth.cfg5_schedule(n, qps, seed=5 if storage == "f32" else 6)
+        reqs = requests(sched, storage)
+        t0 = time.perf_counter()
+        res = srv.replay(reqs, sched["arrival_ms"])
+        wall = time.perf_counter() - t0
+        lat = np.array([r["latency_ms"] for r in res])
+        q = np.array([r["queue_ms"] for r in res])
+        px = float(np.sum(sched["side"].astype(np.float64) ** 2))
+        out[storage] = {"requests": n, "p50_ms": round(float(np.percentile(lat, 50)), 3),
+                        "p99_ms": round(float(np.percentile(lat, 99)), 3), "mean_ms": round(float(lat.mean()), 3),
+                        "max_ms": round(float(lat.max()), 3), "queue_p99_ms": round(float(np.percentile(q, 99)), 3),
+                        "mean_batch": round(float(np.mean([r["batch"] for r in res])), 2),
+                        "achieved_qps": round(n / wall, 2), "mpix_s": round(px / wall / 1e6, 2)}
+    out["hbm_budget_note"] = "default budget: half the free device memory at server start"
+    srv.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -93,6 +182,11 @@ def main():
     ap.add_argument("--side", type=int, default=2048)
     ap.add_argument("--storage", default="f32", choices=["f32", "bf16", "f16"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-lattice leg (64 images)")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
+    ap.add_argument("--serve-requests", type=int, default=2000)
+    ap.add_argument("--qps", type=float, default=100.0)
+    ap.add_argument("--slots", type=int, default=4)
     ap.add_argument("--no-counters", action="store_true",
                     help="skip the extra call that counts hash probes and atomics (it runs slower kernels)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -198,7 +292,19 @@ def main():
         "lattice": {"M": M, "capacity": stats[-1]["capacity"], "sort_passes": stats[-1]["sort_passes"],
                     "probes": cst["probes"], "global_inserts": cst["inserts"], "atomics": cst["atomics"]},
     }
+    progress(f"2048^2 joint bilateral: {value:.1f} Mpix/s")
+    if not a.no_batch:
+        progress("batched lattice leg (64 images)")
+        line["batch"] = batch_leg(gtlattice, synth, torch, max(4, a.steps // 2))
+    if not a.no_serve:
+        progress(f"serving leg: {a.serve_requests} requests at {a.qps:g} QPS")
+        sv = serve_leg(gtlattice, synth, torch, a.serve_requests, a.qps, a.slots)
+        line["served"] = sv
+        line["served_p50_ms"] = sv["f32"]["p50_ms"]
+        line["served_p99_ms"] = sv["f32"]["p99_ms"]
+        line["served_qps"] = a.qps
     if not a.no_cpu:
+        progress("CPU baseline leg")
         cpu, err = cpu_baseline_leg(out_gpu, im, guide, ss, sr)
         line["cpu_baseline"] = cpu
         line["gpu_over_cpu"] = round(value / cpu["value"], 1)

@@ -74,8 +74,8 @@ struct gt_ctx {
     hipStream_t own = nullptr;
     int profiling = 0;   // 0 off, 1 HIP events, 2 events + probe/atomic counters
     hipEvent_t ev[8] = {};
-    DevBuf tkeys, tacc, ctr, counts, slot_list, slot_id, pk_a, pk_b, pay_b, hist, vkeys, vals_a, vals_b, nbr,
-        splat_copy, mm_part, mm_out;
+    DevBuf tkeys, tacc, ctr, counts, slot_list, slot_id, pk_a, pk_b, pay_b, hist, sort_status, vkeys, vals_a,
+        vals_b, nbr, splat_copy, mm_part, mm_out;
     DevBuf stage_in, stage_out, view_a, view_b, view_o;
     gt::Counters *h_ctr = nullptr;   // pinned
     float *h_mm = nullptr;           // pinned, 3 * 2 * kMaxDH
@@ -117,6 +117,16 @@ int ensure(DevBuf &b, size_t bytes) {
         return fail(GT_ENOMEM, "device allocation of %zu bytes failed", bytes);
     }
     b.cap = bytes;
+    return GT_OK;
+}
+
+// As ensure(), and a newly allocated buffer is zeroed on `st` (never inside
+// a graph capture: ensure() runs before the pipeline is enqueued).
+int ensure_zeroed(DevBuf &b, size_t bytes, hipStream_t st) {
+    const void *old = b.p;
+    const size_t old_cap = b.cap;
+    GT_TRY(ensure(b, bytes));
+    if (b.p != old || b.cap != old_cap) GT_HIP(hipMemsetAsync(b.p, 0, b.cap, st));
     return GT_OK;
 }
 
@@ -250,7 +260,6 @@ struct Launch {
     int d, vd, storage, W, pw, m_max, vs, n, row_w, tile_w, wpc;
     bool packed, bilat, dedup, count;
     uint32_t cap;
-    size_t he;
     gt::LatConst lc;
     gt::KeyPlan kp;
     gt::ValScale vsc;
@@ -282,7 +291,7 @@ int enqueue(gt_ctx *c, const Launch &L, const gt::PixSrc &src, const SliceSpec &
         GT_HIP(gt::launch_pack(L.d, ta, (const uint32_t *)c->slot_list.p, ctr, L.m_max, L.kp, L.pw,
                                (uint64_t *)c->pk_a.p, s));
     gt::SortBufs sb{(uint64_t *)c->pk_a.p, (uint64_t *)c->pk_b.p, (uint32_t *)c->slot_list.p,
-                    (uint32_t *)c->pay_b.p, (uint32_t *)c->hist.p, L.he};
+                    (uint32_t *)c->pay_b.p, (uint32_t *)c->hist.p, (uint64_t *)c->sort_status.p};
     bool in_b = false;
     GT_HIP(gt::radix_sort(sb, ctr, L.m_max, L.pw, L.kp.bits, &in_b, passes, s));
     const uint32_t *sorted = in_b ? sb.pay_b : sb.pay_a;
@@ -333,7 +342,8 @@ int execute(gt_ctx *c, const Launch &L, const gt::PixSrc &src, const SliceSpec &
     sig_add(sig, c->profiling);
     sig_add(sig, c->stream);
     const void *bufs[] = {c->tkeys.p, c->tacc.p, c->ctr.p, c->counts.p, c->slot_list.p, c->slot_id.p, c->pk_a.p,
-                          c->pk_b.p, c->pay_b.p, c->hist.p, c->vkeys.p, c->vals_a.p, c->vals_b.p, c->nbr.p};
+                          c->pk_b.p, c->pay_b.p, c->hist.p, c->sort_status.p, c->vkeys.p, c->vals_a.p,
+                          c->vals_b.p, c->nbr.p};
     sig_add(sig, bufs);
     if (batch) {
         sig_add(sig, batch->ba);
@@ -429,8 +439,8 @@ int run(gt_ctx *c, const gt::PixSrc &src, int n, int row_w, int d, int vd, int s
         GT_TRY(ensure(c->pk_a, (size_t)L.pw * L.m_max * 8));
         GT_TRY(ensure(c->pk_b, (size_t)L.pw * L.m_max * 8));
         GT_TRY(ensure(c->pay_b, (size_t)L.m_max * 4));
-        L.he = gt::radix_hist_entries(L.m_max);
-        GT_TRY(ensure(c->hist, L.he * 4));
+        GT_TRY(ensure_zeroed(c->hist, gt::sort_state_words() * 4, s));
+        GT_TRY(ensure_zeroed(c->sort_status, gt::sort_status_entries(L.m_max, L.pw) * 8, s));
         GT_TRY(ensure(c->vkeys, (size_t)L.m_max * L.W * 8));
         const size_t vbytes = (size_t)L.m_max * L.vs * store_bytes(storage);
         GT_TRY(ensure(c->vals_a, vbytes));
@@ -453,6 +463,7 @@ int run(gt_ctx *c, const gt::PixSrc &src, int n, int row_w, int d, int vd, int s
 
         if (hc.flags & gt::kFlagBad)
             return fail(GT_ERANGE, "position is NaN/Inf or a lattice key leaves the int16 range");
+        if (hc.flags & gt::kFlagSort) return fail(GT_EDEVICE, "radix sort look-back timed out");
         if (hc.flags & gt::kFlagRange) {
             // Keys or values outside the cached bounds: bound these inputs again.
             if (++retries > 6) return fail(GT_EDEVICE, "inputs keep leaving the computed bounds");
@@ -703,7 +714,7 @@ void gt_ctx_destroy(gt_ctx *c) {
     if (c->h_batch) (void)hipHostFree(c->h_batch);
     free_buf(c->batch_desc);
     DevBuf *bufs[] = {&c->tkeys, &c->tacc, &c->ctr, &c->counts, &c->slot_list, &c->slot_id, &c->pk_a, &c->pk_b,
-                      &c->pay_b, &c->hist, &c->vkeys, &c->vals_a, &c->vals_b, &c->nbr, &c->splat_copy, &c->mm_part,
+                      &c->pay_b, &c->hist, &c->sort_status, &c->vkeys, &c->vals_a, &c->vals_b, &c->nbr, &c->splat_copy, &c->mm_part,
                       &c->mm_out, &c->stage_in, &c->stage_out, &c->view_a, &c->view_b, &c->view_o};
     for (DevBuf *b : bufs) free_buf(*b);
     for (auto &e : c->ev)

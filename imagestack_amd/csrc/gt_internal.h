@@ -23,6 +23,7 @@ struct Counters {
 constexpr unsigned kFlagFull = 1u;    // an insert exceeded the probe limit: regrow and redo
 constexpr unsigned kFlagBad = 2u;     // NaN/Inf position or key outside int16
 constexpr unsigned kFlagRange = 4u;   // a key outside the packing plan: replan and redo
+constexpr unsigned kFlagSort = 8u;    // a radix-sort look-back gave up waiting (a bug, never expected)
 
 // Where pixel positions and values come from.
 struct PixSrc {
@@ -141,18 +142,21 @@ hipError_t launch_view_copy(const float *src, int64_t sys, int64_t sts, int64_t 
 // payload, on the low `bits` bits, 8 bits per pass.  a/b are ping-pong
 // buffers; *result_in_b tells where the sorted data ended.
 struct SortBufs {
-    uint64_t *pk_a, *pk_b;
-    uint32_t *pay_a, *pay_b;
-    uint32_t *hist;        // 256 * blocks(m_max)
-    size_t hist_cap;
+    uint64_t *pk_a, *pk_b;      // keys, word-major (word w of key i at w * m_max + i)
+    uint32_t *pay_a, *pay_b;    // payloads
+    uint32_t *state;            // sort_state_words(): epoch, tile counters, digit counts (zeroed once)
+    uint64_t *status;           // sort_status_entries(): look-back words (zeroed once)
 };
-size_t radix_hist_entries(int m_max);
+size_t sort_state_words();
+size_t sort_status_entries(int m_max, int pw);
 // Sets the thread's gt_last_error() message; returns code.
 int set_error(int code, const char *msg);
 // Forgets the context's cached key-packing plan, so the next call bounds its
 // inputs afresh (a server's next request is an unrelated image).
 void invalidate_plan(::gt_ctx *c);
-hipError_t radix_sort(SortBufs &b, const Counters *ctr, int m_max, int pw, int bits, bool *result_in_b, int *passes,
+// Sorts the first ctr->M (key, payload) pairs by key (bits significant bits,
+// pw words); the result is in the b buffers when *result_in_b.
+hipError_t radix_sort(SortBufs &b, Counters *ctr, int m_max, int pw, int bits, bool *result_in_b, int *passes,
                       hipStream_t st);
 
 } // namespace gt

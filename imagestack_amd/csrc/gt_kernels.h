@@ -505,6 +505,132 @@ __global__ __launch_bounds__(kBlock) void k_neighbors(TableArgs ta, KeyPlan kp, 
     }
 }
 
+// Neighbour table from the sorted packed keys, without the hash table.
+// Moving a key by +-(1,...,1) -+ (D+1) e_j moves every field of its packed
+// word by a constant, so when no field leaves its range the neighbour's word
+// is the vertex's word plus a constant delta: the targets of a run of sorted
+// vertices are sorted too, and finding them is a merge with the sorted keys.
+// Workgroup = 2048 consecutive vertices x one direction (blockIdx.y = 2 axis
+// + sign): wave 0 finds the first target's lower bound by a 64-ary search,
+// then the workgroup stages windows of 4096 sorted keys in LDS and each
+// thread binary-searches its targets there.  Ids are sorted ranks, so the
+// table equals the hash-built one entry for entry.
+constexpr int kNbrItems = 8;
+constexpr int kNbrBlock = kBlock * kNbrItems;
+constexpr int kNbrWindow = 4096;
+
+// First index in s[0, n) whose value is >= x (n if none), by wave 0 of the
+// workgroup: 64 probes per round narrow the range 64-fold.
+__device__ __forceinline__ unsigned wave_lower_bound(const uint64_t *__restrict__ s, unsigned n, uint64_t x) {
+    const int lane = __lane_id();
+    unsigned lo = 0, hi = n;   // answer in [lo, hi]
+    while (hi - lo > kWave) {
+        const unsigned step = (hi - lo + kWave - 1) / kWave;
+        const unsigned p = lo + (unsigned)(lane + 1) * step - 1;
+        const bool in = p < hi;
+        const bool below = in && s[p] < x;
+        const int c = __popcll(__ballot(below));   // probes below x form a prefix
+        const unsigned nlo = c == 0 ? lo : lo + (unsigned)c * step;
+        const unsigned cp = lo + (unsigned)(c + 1) * step - 1;
+        hi = (c < kWave && cp < hi) ? cp : hi;
+        lo = nlo;
+    }
+    const unsigned p = lo + (unsigned)lane;
+    const bool below = p < hi && s[p] < x;
+    return lo + (unsigned)__popcll(__ballot(below));
+}
+
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_nbr_merge(const uint64_t *__restrict__ sk, KeyPlan kp,
+                                                      const Counters *ctr, int stride, int *__restrict__ nbr) {
+    __shared__ uint64_t win[kNbrWindow];
+    __shared__ uint64_t red[2][kBlock / kWave];
+    __shared__ unsigned s_cur;
+    const unsigned M = ctr->M;
+    const unsigned b0 = blockIdx.x * kNbrBlock;
+    if (b0 >= M) return;
+    const int j = blockIdx.y >> 1, sgn = blockIdx.y & 1;   // sgn 0: k + 1 - (D+1)e_j, 1: k - 1 + (D+1)e_j
+    uint64_t one = 0;
+#pragma unroll
+    for (int q = 0; q < D; q++) one += 1ull << kp.shift[q];
+    const uint64_t d1 = j < D ? one - ((uint64_t)(D + 1) << kp.shift[j < D ? j : 0]) : one;
+    const uint64_t delta = sgn ? (0ull - d1) : d1;
+
+    uint64_t tgt[kNbrItems];
+    bool pend[kNbrItems];
+    int res[kNbrItems];
+    uint64_t tmin = ~0ull;
+#pragma unroll
+    for (int r = 0; r < kNbrItems; r++) {
+        const unsigned i = b0 + r * kBlock + threadIdx.x;
+        bool ok = i < M;
+        const uint64_t k = ok ? sk[i] : 0ull;
+#pragma unroll
+        for (int q = 0; q < D; q++) {
+            const uint32_t f = (uint32_t)((k >> kp.shift[q]) & kp.span[q]);
+            if (q == j) ok = ok && (sgn ? f + D <= kp.span[q] : f >= (uint32_t)D);
+            else ok = ok && (sgn ? f >= 1u : f + 1u <= kp.span[q]);
+        }
+        tgt[r] = k + delta;   // exact when every field stays in range
+        pend[r] = ok;
+        res[r] = -1;
+        if (ok) tmin = tgt[r] < tmin ? tgt[r] : tmin;
+    }
+    // workgroup minimum of the valid targets
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+        const uint64_t o = __shfl_xor(tmin, off, kWave);
+        tmin = o < tmin ? o : tmin;
+    }
+    if (__lane_id() == 0) red[0][threadIdx.x / kWave] = tmin;
+    __syncthreads();
+    tmin = red[0][0];
+#pragma unroll
+    for (int w = 1; w < kBlock / kWave; w++) tmin = red[0][w] < tmin ? red[0][w] : tmin;
+    if (tmin != ~0ull) {
+        if (threadIdx.x < kWave) {
+            const unsigned lb = wave_lower_bound(sk, M, tmin);
+            if (threadIdx.x == 0) s_cur = lb;
+        }
+        __syncthreads();
+        // cur is the same in every thread; each round either resolves every
+        // pending target (the last window) or advances cur by a full window.
+        // cur == M (every target beyond the largest key): nothing to find.
+        unsigned cur = s_cur;
+        while (cur < M) {
+            const unsigned wlen = min((unsigned)kNbrWindow, M - cur);
+            for (unsigned t = threadIdx.x; t < wlen; t += kBlock) win[t] = sk[cur + t];
+            __syncthreads();
+            const bool last_window = cur + wlen >= M;
+            const uint64_t last = win[wlen - 1];
+            bool more = false;
+#pragma unroll
+            for (int r = 0; r < kNbrItems; r++) {
+                if (!pend[r]) continue;
+                if (!last_window && tgt[r] > last) {
+                    more = true;
+                    continue;
+                }
+                unsigned a = 0, b = wlen;   // lower bound in win[0, wlen)
+                while (a < b) {
+                    const unsigned m = (a + b) >> 1;
+                    if (win[m] < tgt[r]) a = m + 1;
+                    else b = m;
+                }
+                if (a < wlen && win[a] == tgt[r]) res[r] = (int)(cur + a);
+                pend[r] = false;
+            }
+            if (!__syncthreads_or(more ? 1 : 0)) break;
+            cur += wlen;   // every pending target exceeds this window's last key
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < kNbrItems; r++) {
+        const unsigned i = b0 + r * kBlock + threadIdx.x;
+        if (i < M) nbr[2 * ((size_t)j * stride + i) + sgn] = res[r];
+    }
+}
+
 // One blur pass along one axis (src/Permutohedral.h:442-475): Jacobi, new =
 // 0.25*a + 0.5*self + 0.25*b evaluated left to right, missing neighbour = 0.
 template <typename T, int VS>

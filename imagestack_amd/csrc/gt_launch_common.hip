@@ -301,10 +301,13 @@ hipError_t launch_neighbors(int d, bool packed, const TableArgs &ta, const KeyPl
     const int cnt = count ? 1 : 0;
     const dim3 g(blocks_for(m_max, kBlock), d + 1), b(kBlock);
     const unsigned pad = occupancy_pad(0, wpc);
+    // Packed keys: merge with the sorted keys (no hash lookups); wide keys:
+    // hash lookups.
+    const dim3 gm(blocks_for(m_max, kNbrBlock), 2 * (d + 1));
+    const unsigned padm = occupancy_pad(kNbrWindow * 8 + 64, wpc);
 #define GT_NBR(D)                                                                                            \
     {                                                                                                        \
-        if (packed) hipLaunchKernelGGL((k_neighbors<D, true>), g, b, pad, st, ta, kp, vkeys, ctr, m_max, nbr, \
-                                       wctr, cnt);                                                            \
+        if (packed) hipLaunchKernelGGL((k_nbr_merge<D>), gm, b, padm, st, vkeys, kp, ctr, m_max, (int *)nbr); \
         else hipLaunchKernelGGL((k_neighbors<D, false>), g, b, pad, st, ta, kp, vkeys, ctr, m_max, nbr, wctr, \
                                 cnt);                                                                         \
     }

@@ -1,11 +1,24 @@
 // LSD radix sort of (multi-word key, uint32 payload) pairs for the lattice
-// vertex keys, hand-written for gfx950 (wave64).  Each 8-bit pass is
-// histogram -> exclusive scan -> stable scatter.  Stability inside a
-// workgroup comes from processing items in index order, 256 at a time, and
-// ranking equal digits inside each wave with eight ballots (no LDS sort).
+// vertex keys, hand-written for gfx950 (wave64): one pass per 8-bit digit
+// with decoupled look-back (a "one-sweep" radix sort).
+//
+//   k_sort_init   : new epoch, tile counters and digit counts to zero.
+//   k_sort_counts : the digit counts of every pass in one read of the keys
+//                   (an LSD pass permutes the keys but not their digits, so
+//                   every pass's global digit offsets are known up front).
+//   k_sort_pass   : per 8-bit digit.  Workgroups take tiles in order from an
+//                   atomic counter; each ranks its tile stably (each wave
+//                   owns a run of consecutive keys and ranks 64 at a time
+//                   with eight ballots), publishes its per-digit counts,
+//                   looks back over earlier tiles for their running totals,
+//                   orders the tile by digit in LDS and writes it out in
+//                   digit runs.
+//
 // The item count lives on the device (Counters::M): grids are sized for the
-// host's upper bound and idle workgroups exit at once, so no host round trip
-// sits between the compaction and the sort.
+// host's upper bound and surplus workgroups exit at once.  The look-back
+// status words carry a tag (epoch * 32 + pass), so they never need clearing;
+// a look-back that waits too long (it cannot, short of a bug) raises
+// kFlagSort instead of spinning forever.
 #include "gt_device.h"
 #include "gt_internal.h"
 
@@ -14,139 +27,233 @@
 namespace gt {
 
 constexpr int kSortThreads = 256;
-constexpr int kSortTiles = 16;                        // 256-item tiles per workgroup
-constexpr int kSortItems = kSortThreads * kSortTiles; // items per workgroup
+constexpr int kSortWaves = kSortThreads / kWave;
+constexpr int kMaxPasses = 8 * 4;                 // 4 key words of 64 bits
+constexpr int kStateEpoch = 0;                    // state words
+constexpr int kStateTiles = 1;                    // [1, 1 + kMaxPasses)
+constexpr int kStateCounts = 64;                  // [64, 64 + 256 * kMaxPasses)
+constexpr unsigned kLookbackSpins = 1u << 20;
 
-__device__ __forceinline__ int sort_blocks(unsigned M) { return (int)((M + kSortItems - 1) / kSortItems); }
+template <int PW> struct SortTile {
+    static constexpr int kItems = PW <= 2 ? 8 : 4;                // keys per thread
+    static constexpr int kTile = kSortThreads * kItems;          // keys per tile
+    static constexpr int kWaveRun = kTile / kSortWaves;          // consecutive keys per wave
+};
 
-__global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint64_t *__restrict__ kw, const Counters *ctr,
-                                                            int shift, uint32_t *__restrict__ hist) {
-    const unsigned M = ctr->M;
-    const int nblk = sort_blocks(M);
-    if ((int)blockIdx.x >= nblk) return;
-    __shared__ uint32_t h[256];
-    h[threadIdx.x] = 0;
-    __syncthreads();
-    const unsigned base = blockIdx.x * kSortItems;
-#pragma unroll 4
-    for (int t = 0; t < kSortTiles; t++) {
-        const unsigned i = base + t * kSortThreads + threadIdx.x;
-        if (i < M) atomicAdd(&h[(kw[i] >> shift) & 255], 1u);
-    }
-    __syncthreads();
-    hist[(size_t)threadIdx.x * nblk + blockIdx.x] = h[threadIdx.x];
+__device__ __forceinline__ unsigned status_tag(uint64_t v) { return (unsigned)(v >> 32); }
+__device__ __forceinline__ bool status_incl(uint64_t v) { return (v >> 31) & 1u; }
+__device__ __forceinline__ unsigned status_count(uint64_t v) { return (unsigned)v & 0x7fffffffu; }
+
+__global__ __launch_bounds__(kSortThreads) void k_sort_init(uint32_t *__restrict__ state, int npass) {
+    if (threadIdx.x == 0) state[kStateEpoch] = state[kStateEpoch] + 1;
+    if (threadIdx.x < kMaxPasses) state[kStateTiles + threadIdx.x] = 0;
+    for (int i = threadIdx.x; i < 256 * npass; i += kSortThreads) state[kStateCounts + i] = 0;
 }
 
-// Exclusive scan of the digit-major histogram (256 * nblk entries, a multiple
-// of 4) in one workgroup of 1024 threads: each thread owns a contiguous run of
-// uint4s, loaded four at a time so the loads overlap.
-__global__ __launch_bounds__(1024) void k_radix_scan(uint32_t *__restrict__ hist, const Counters *ctr) {
-    __shared__ uint32_t part[1024];
-    const int E4 = 64 * sort_blocks(ctr->M);
-    const int chunk = (E4 + 1023) / 1024;
-    const int lo = threadIdx.x * chunk;
-    const int hi = min(E4, lo + chunk);
-    uint4 *h4 = reinterpret_cast<uint4 *>(hist);
-    uint32_t s = 0;
-#pragma unroll 4
-    for (int i = lo; i < hi; i++) {
-        const uint4 v = h4[i];
-        s += v.x + v.y + v.z + v.w;
-    }
-    part[threadIdx.x] = s;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        const uint32_t v = threadIdx.x >= off ? part[threadIdx.x - off] : 0;
-        __syncthreads();
-        part[threadIdx.x] += v;
-        __syncthreads();
-    }
-    uint32_t run = part[threadIdx.x] - s;
-#pragma unroll 4
-    for (int i = lo; i < hi; i++) {
-        const uint4 v = h4[i];
-        uint4 o;
-        o.x = run; run += v.x;
-        o.y = run; run += v.y;
-        o.z = run; run += v.z;
-        o.w = run; run += v.w;
-        h4[i] = o;
-    }
-}
-
-__global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint64_t *__restrict__ pk_in,
-                                                               const uint32_t *__restrict__ pay_in,
-                                                               uint64_t *__restrict__ pk_out,
-                                                               uint32_t *__restrict__ pay_out, const Counters *ctr,
-                                                               int stride, int pw, int word, int shift,
-                                                               const uint32_t *__restrict__ hist) {
-    constexpr int NW = kSortThreads / kWave;
+// Digit counts of all passes.  keys: word-major, PW words, stride m_max.
+template <int PW>
+__global__ __launch_bounds__(kSortThreads) void k_sort_counts(const uint64_t *__restrict__ keys, int stride,
+                                                             const Counters *ctr, int npass,
+                                                             uint32_t *__restrict__ state) {
+    __shared__ uint32_t h[kMaxPasses * 256];
     const unsigned M = ctr->M;
-    const int nblk = sort_blocks(M);
-    if ((int)blockIdx.x >= nblk) return;
-    __shared__ uint32_t base[256];
-    __shared__ uint32_t wcnt[NW][256];
-    const int tid = threadIdx.x;
-    const int wave = tid / kWave;
-    base[tid] = hist[(size_t)tid * nblk + blockIdx.x];
-    const unsigned b0 = blockIdx.x * kSortItems;
-    for (int t = 0; t < kSortTiles; t++) {
+    for (int i = threadIdx.x; i < 256 * npass; i += kSortThreads) h[i] = 0;
+    __syncthreads();
+    for (unsigned i = blockIdx.x * kSortThreads + threadIdx.x; i < M; i += gridDim.x * kSortThreads) {
 #pragma unroll
-        for (int w = 0; w < NW; w++) wcnt[w][tid] = 0;
-        __syncthreads();
-        const unsigned i = b0 + t * kSortThreads + tid;
+        for (int w = 0; w < PW; w++) {
+            const uint64_t k = keys[(size_t)w * stride + i];
+#pragma unroll
+            for (int b = 0; b < 8; b++) {
+                const int p = w * 8 + b;
+                if (p < npass) atomicAdd(&h[p * 256 + ((k >> (8 * b)) & 255)], 1u);
+            }
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < 256 * npass; i += kSortThreads)
+        if (h[i]) atomicAdd(&state[kStateCounts + i], h[i]);
+}
+
+// Exclusive scan of one value per thread over the workgroup's 256 threads.
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *tmp /* kSortWaves */) {
+    const int lane = __lane_id(), wave = threadIdx.x / kWave;
+    uint32_t x = v;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+        const uint32_t y = __shfl_up(x, off, kWave);
+        if (lane >= off) x += y;
+    }
+    if (lane == kWave - 1) tmp[wave] = x;
+    __syncthreads();
+    uint32_t before = 0;
+#pragma unroll
+    for (int w = 0; w < kSortWaves; w++) before += (w < wave) ? tmp[w] : 0u;
+    __syncthreads();
+    return before + x - v;
+}
+
+template <int PW>
+__global__ __launch_bounds__(kSortThreads) void k_sort_pass(const uint64_t *__restrict__ kin,
+                                                           const uint32_t *__restrict__ pin,
+                                                           uint64_t *__restrict__ kout, uint32_t *__restrict__ pout,
+                                                           int stride, Counters *ctr, int pass,
+                                                           uint32_t *__restrict__ state,
+                                                           uint64_t *__restrict__ status) {
+    using T = SortTile<PW>;
+    constexpr int ITEMS = T::kItems, TILE = T::kTile, RUN = T::kWaveRun;
+    __shared__ uint64_t skey[TILE * PW];
+    __shared__ uint32_t spay[TILE];
+    __shared__ uint32_t whist[kSortWaves][256];
+    __shared__ uint32_t gbase[256];      // output index of the tile's first key of each digit
+    __shared__ uint32_t dstart[256];     // first LDS slot of each digit in the ordered tile
+    __shared__ uint32_t tmp[kSortWaves];
+    __shared__ int s_tile;
+
+    const unsigned M = ctr->M;
+    const int ntiles = (int)((M + TILE - 1) / TILE);
+    if (threadIdx.x == 0) s_tile = (int)atomicAdd(&state[kStateTiles + pass], 1u);
+    const int tid = threadIdx.x, lane = __lane_id(), wave = tid / kWave;
+    for (int w = 0; w < kSortWaves; w++) whist[w][tid] = 0;
+    __syncthreads();
+    const int tile = s_tile;
+    if (tile >= ntiles) return;
+    const int word = pass / 8, shift = (pass % 8) * 8;
+    const unsigned tag = state[kStateEpoch] * (unsigned)kMaxPasses + (unsigned)pass;
+
+    // Rank: wave `wave` owns keys [tile*TILE + wave*RUN, +RUN), 64 per round.
+    uint64_t key[ITEMS][PW];
+    uint32_t pay[ITEMS];
+    int dg[ITEMS], rk[ITEMS];
+    const unsigned base = (unsigned)tile * TILE + wave * RUN;
+#pragma unroll
+    for (int r = 0; r < ITEMS; r++) {
+        const unsigned i = base + r * kWave + lane;
         const bool valid = i < M;
-        const uint32_t dg = valid ? (uint32_t)((pk_in[(size_t)word * stride + i] >> shift) & 255) : 0u;
+#pragma unroll
+        for (int w = 0; w < PW; w++) key[r][w] = valid ? kin[(size_t)w * stride + i] : 0ull;
+        pay[r] = valid ? pin[i] : 0u;
+        const uint32_t d = valid ? (uint32_t)((key[r][word] >> shift) & 255) : 0u;
         uint64_t peers = __ballot(valid);
 #pragma unroll
         for (int b = 0; b < 8; b++) {
-            const bool bit = (dg >> b) & 1u;
+            const bool bit = (d >> b) & 1u;
             const uint64_t bal = __ballot(bit);
             peers &= bit ? bal : ~bal;
         }
-        const int rank = __popcll(peers & lanemask_lt());
-        if (valid && rank == 0) wcnt[wave][dg] = (uint32_t)__popcll(peers);
-        __syncthreads();
-        if (valid) {
-            uint32_t off = base[dg] + (uint32_t)rank;
+        const uint32_t before = valid ? whist[wave][d] : 0u;
+        const int below = __popcll(peers & lanemask_lt());
+        if (valid && below == 0) whist[wave][d] = before + (uint32_t)__popcll(peers);
+        dg[r] = valid ? (int)d : -1;
+        rk[r] = (int)before + below;
+    }
+    __syncthreads();
+
+    // Per digit (thread tid = digit): wave offsets, tile count, look-back.
+    uint32_t cnt = 0;
 #pragma unroll
-            for (int w = 0; w < NW; w++) off += (w < wave) ? wcnt[w][dg] : 0u;
-            for (int w = 0; w < pw; w++) pk_out[(size_t)w * stride + off] = pk_in[(size_t)w * stride + i];
-            pay_out[off] = pay_in[i];
+    for (int w = 0; w < kSortWaves; w++) {
+        const uint32_t c = whist[w][tid];
+        whist[w][tid] = cnt;   // now the wave's offset inside the digit
+        cnt += c;
+    }
+    uint64_t *my = status + (size_t)tile * 256 + tid;
+    const uint64_t hi = (uint64_t)tag << 32;
+    uint32_t excl = 0;
+    if (tile == 0) {
+        __hip_atomic_store(my, hi | (1ull << 31) | cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+        __hip_atomic_store(my, hi | cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        bool timed_out = false;
+        for (int j = tile - 1; j >= 0; j--) {
+            const uint64_t *p = status + (size_t)j * 256 + tid;
+            uint64_t v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            unsigned spins = 0;
+            while (status_tag(v) != tag && ++spins < kLookbackSpins) {
+                __builtin_amdgcn_s_sleep(1);
+                v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                // after one look-back has given up, the others stop waiting too
+                if ((spins & 1023u) == 0 &&
+                    (__hip_atomic_load(&ctr->flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & kFlagSort))
+                    spins = kLookbackSpins;
+            }
+            if (status_tag(v) != tag) {
+                timed_out = true;
+                break;
+            }
+            excl += status_count(v);
+            if (status_incl(v)) break;
         }
-        __syncthreads();
-        uint32_t add = 0;
+        if (timed_out) atomicOr(&ctr->flags, kFlagSort);
+        __hip_atomic_store(my, hi | (1ull << 31) | (excl + cnt), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    // Global offset of the digit: digits before it over all keys, plus this
+    // digit's keys in earlier tiles.
+    const uint32_t goff = block_excl_scan(state[kStateCounts + pass * 256 + tid], tmp);
+    const uint32_t ds = block_excl_scan(cnt, tmp);
+    gbase[tid] = goff + excl;
+    dstart[tid] = ds;
+    __syncthreads();
+
+    // Order the tile by digit in LDS, then write digit runs.
 #pragma unroll
-        for (int w = 0; w < NW; w++) add += wcnt[w][tid];
-        base[tid] += add;
-        __syncthreads();
+    for (int r = 0; r < ITEMS; r++) {
+        if (dg[r] < 0) continue;
+        const uint32_t l = dstart[dg[r]] + whist[wave][dg[r]] + (uint32_t)rk[r];
+#pragma unroll
+        for (int w = 0; w < PW; w++) skey[w * TILE + l] = key[r][w];
+        spay[l] = pay[r];
+    }
+    __syncthreads();
+    const int len = (int)min((unsigned)TILE, M - (unsigned)tile * TILE);
+    for (int l = tid; l < len; l += kSortThreads) {
+        const uint64_t kw = skey[word * TILE + l];
+        const int d = (int)((kw >> shift) & 255);
+        const uint32_t o = gbase[d] + (uint32_t)l - dstart[d];
+#pragma unroll
+        for (int w = 0; w < PW; w++) kout[(size_t)w * stride + o] = skey[w * TILE + l];
+        pout[o] = spay[l];
     }
 }
 
-size_t radix_hist_entries(int m_max) {
-    const size_t nblk = ((size_t)m_max + kSortItems - 1) / kSortItems;
-    return 256 * (nblk ? nblk : 1);
+size_t sort_state_words() { return kStateCounts + 256 * kMaxPasses; }
+
+size_t sort_status_entries(int m_max, int pw) {
+    const int tile = pw <= 2 ? SortTile<1>::kTile : SortTile<3>::kTile;
+    const size_t tiles = ((size_t)(m_max > 0 ? m_max : 1) + tile - 1) / tile;
+    return tiles * 256;
 }
 
-hipError_t radix_sort(SortBufs &b, const Counters *ctr, int m_max, int pw, int bits, bool *result_in_b, int *passes,
+template <int PW>
+static void sort_launch(SortBufs &b, Counters *ctr, int m_max, int np, hipStream_t st) {
+    using T = SortTile<PW>;
+    const int tiles = (m_max + T::kTile - 1) / T::kTile;
+    const int cblk = std::max(1, std::min(256, (m_max + kSortThreads * 16 - 1) / (kSortThreads * 16)));
+    hipLaunchKernelGGL(k_sort_init, dim3(1), dim3(kSortThreads), 0, st, b.state, np);
+    hipLaunchKernelGGL((k_sort_counts<PW>), dim3(cblk), dim3(kSortThreads), 0, st, b.pk_a, m_max, ctr, np,
+                       b.state);
+    uint64_t *kin = b.pk_a, *kout = b.pk_b;
+    uint32_t *pin = b.pay_a, *pout = b.pay_b;
+    for (int p = 0; p < np; p++) {
+        hipLaunchKernelGGL((k_sort_pass<PW>), dim3(tiles), dim3(kSortThreads), 0, st, kin, pin, kout, pout, m_max,
+                           ctr, p, b.state, b.status);
+        std::swap(kin, kout);
+        std::swap(pin, pout);
+    }
+}
+
+hipError_t radix_sort(SortBufs &b, Counters *ctr, int m_max, int pw, int bits, bool *result_in_b, int *passes,
                       hipStream_t st) {
     *result_in_b = false;
     *passes = 0;
     if (m_max <= 1 || bits <= 0) return hipSuccess;
-    const int nblk = (m_max + kSortItems - 1) / kSortItems;
-    if ((size_t)256 * nblk > b.hist_cap) return hipErrorInvalidValue;
     const int np = (bits + 7) / 8;
-    uint64_t *pin = b.pk_a, *pout = b.pk_b;
-    uint32_t *yin = b.pay_a, *yout = b.pay_b;
-    for (int p = 0; p < np; p++) {
-        const int word = p / 8, shift = (p % 8) * 8;
-        hipLaunchKernelGGL(k_radix_hist, dim3(nblk), dim3(kSortThreads), 0, st, pin + (size_t)word * m_max, ctr, shift,
-                           b.hist);
-        hipLaunchKernelGGL(k_radix_scan, dim3(1), dim3(1024), 0, st, b.hist, ctr);
-        hipLaunchKernelGGL(k_radix_scatter, dim3(nblk), dim3(kSortThreads), 0, st, pin, yin, pout, yout, ctr, m_max,
-                           pw, word, shift, b.hist);
-        std::swap(pin, pout);
-        std::swap(yin, yout);
+    if (pw < 1 || pw > 4 || np > kMaxPasses || np > 8 * pw) return hipErrorInvalidValue;
+    switch (pw) {
+    case 1: sort_launch<1>(b, ctr, m_max, np, st); break;
+    case 2: sort_launch<2>(b, ctr, m_max, np, st); break;
+    case 3: sort_launch<3>(b, ctr, m_max, np, st); break;
+    default: sort_launch<4>(b, ctr, m_max, np, st); break;
     }
     *result_in_b = (np % 2) == 1;
     *passes = np;

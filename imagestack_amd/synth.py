@@ -133,3 +133,40 @@ def cfg2(seed: int = 1, side: int = 2048):
     guide = rect_image(side, side, 3, K=256, noise=0.03, seed=seed)
     im = uniform_image(side, side, 3, seed=seed + 100)
     return im, guide, 16.0, 0.1
+
+
+def cfg4(seed: int = 4, count: int = 64):
+    """Batch of `count` RGB bilateral images: sides log-uniform in [256, 1536],
+    drawn independently per axis; content as cfg1 (64 rectangles +
+    N(0, 0.05)); sigma_s from {4, 8, 16} and sigma_r from {0.05, 0.1, 0.2}
+    per image.  Returns [(im (3, H, W), sigma_s, sigma_r)]."""
+    s = Stream(seed, 11)
+    lo, hi = np.log(256.0), np.log(1536.0)
+    sides = np.rint(np.exp(lo + s.uniform(2 * count) * (hi - lo))).astype(np.int64).reshape(count, 2)
+    ss = np.array([4.0, 8.0, 16.0])[s.integers(0, 3, count)]
+    sr = np.array([0.05, 0.1, 0.2])[s.integers(0, 3, count)]
+    out = []
+    for i in range(count):
+        W, H = int(sides[i, 0]), int(sides[i, 1])
+        im = rect_image(W, H, 3, K=64, noise=0.05, seed=seed * 1000 + i)[:, 0]
+        out.append((np.ascontiguousarray(im), float(ss[i]), float(sr[i])))
+    return out
+
+
+def cfg5_schedule(n: int, qps: float, seed: int = 5, pool_side: int = 4096):
+    """Serving workload: Poisson arrivals at `qps`; square sides log-uniform
+    in [128, 4096]; 60% grey (C=1, d=3, vd=2) and 40% RGB (C=3, d=5, vd=4);
+    sigma_s from {4, 16, 32}, sigma_r from {0.05, 0.2}.  Each request is a crop
+    at a uniform offset of a pool image.  Returns dict of arrays."""
+    s = Stream(seed, 13)
+    arrival = np.cumsum(-np.log1p(-s.uniform(n)) / qps * 1000.0)
+    arrival -= arrival[0]
+    lo, hi = np.log(128.0), np.log(4096.0)
+    side = np.minimum(np.rint(np.exp(lo + s.uniform(n) * (hi - lo))).astype(np.int64), pool_side)
+    grey = s.uniform(n) < 0.6
+    ss = np.array([4.0, 16.0, 32.0])[s.integers(0, 3, n)]
+    sr = np.array([0.05, 0.2])[s.integers(0, 2, n)]
+    x0 = np.floor(s.uniform(n) * (pool_side - side + 1)).astype(np.int64)
+    y0 = np.floor(s.uniform(n) * (pool_side - side + 1)).astype(np.int64)
+    return {"arrival_ms": arrival, "side": side, "C": np.where(grey, 1, 3), "sigma_s": ss, "sigma_r": sr,
+            "x0": x0, "y0": y0}
