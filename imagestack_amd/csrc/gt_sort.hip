@@ -6,8 +6,11 @@
 //   k_sort_counts : the digit counts of every pass in one read of the keys
 //                   (an LSD pass permutes the keys but not their digits, so
 //                   every pass's global digit offsets are known up front).
-//   k_sort_pass   : per 8-bit digit.  Workgroups take tiles in order from an
-//                   atomic counter; each ranks its tile stably (each wave
+//   k_sort_pass   : per 8-bit digit, one workgroup per tile (tile = workgroup
+//                   index: workgroups are dispatched in index order, so every
+//                   tile a look-back waits for is running or done; a shared
+//                   atomic tile counter instead costs ~50 ns per workgroup,
+//                   serialised).  Each ranks its tile stably (each wave
 //                   owns a run of consecutive keys and ranks 64 at a time
 //                   with eight ballots), publishes its per-digit counts,
 //                   looks back over earlier tiles for their running totals,
@@ -33,6 +36,7 @@ constexpr int kStateEpoch = 0;                    // state words
 constexpr int kStateTiles = 1;                    // [1, 1 + kMaxPasses)
 constexpr int kStateCounts = 64;                  // [64, 64 + 256 * kMaxPasses)
 constexpr unsigned kLookbackSpins = 1u << 20;
+constexpr int kLookWidth = 16;                     // predecessor tiles read per look-back round
 
 template <int PW> struct SortTile {
     static constexpr int kItems = PW <= 2 ? 8 : 4;                // keys per thread
@@ -108,16 +112,14 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_pass(const uint64_t *__re
     __shared__ uint32_t gbase[256];      // output index of the tile's first key of each digit
     __shared__ uint32_t dstart[256];     // first LDS slot of each digit in the ordered tile
     __shared__ uint32_t tmp[kSortWaves];
-    __shared__ int s_tile;
 
     const unsigned M = ctr->M;
     const int ntiles = (int)((M + TILE - 1) / TILE);
-    if (threadIdx.x == 0) s_tile = (int)atomicAdd(&state[kStateTiles + pass], 1u);
     const int tid = threadIdx.x, lane = __lane_id(), wave = tid / kWave;
+    const int tile = (int)blockIdx.x;
+    if (tile >= ntiles) return;
     for (int w = 0; w < kSortWaves; w++) whist[w][tid] = 0;
     __syncthreads();
-    const int tile = s_tile;
-    if (tile >= ntiles) return;
     const int word = pass / 8, shift = (pass % 8) * 8;
     const unsigned tag = state[kStateEpoch] * (unsigned)kMaxPasses + (unsigned)pass;
 
@@ -164,25 +166,47 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_pass(const uint64_t *__re
         __hip_atomic_store(my, hi | (1ull << 31) | cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     } else {
         __hip_atomic_store(my, hi | cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        // Look back kLookWidth tiles per round (independent loads in flight),
+        // taking counts in order until a tile with its running total; stop at
+        // the first tile not yet published and poll again from there.
         bool timed_out = false;
-        for (int j = tile - 1; j >= 0; j--) {
-            const uint64_t *p = status + (size_t)j * 256 + tid;
-            uint64_t v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            unsigned spins = 0;
-            while (status_tag(v) != tag && ++spins < kLookbackSpins) {
-                __builtin_amdgcn_s_sleep(1);
-                v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                // after one look-back has given up, the others stop waiting too
-                if ((spins & 1023u) == 0 &&
-                    (__hip_atomic_load(&ctr->flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & kFlagSort))
-                    spins = kLookbackSpins;
+        int j = tile - 1;
+        unsigned spins = 0;
+        while (j >= 0) {
+            uint64_t v[kLookWidth];
+#pragma unroll
+            for (int q = 0; q < kLookWidth; q++)
+                v[q] = j - q >= 0 ? __hip_atomic_load(status + (size_t)(j - q) * 256 + tid, __ATOMIC_RELAXED,
+                                                      __HIP_MEMORY_SCOPE_AGENT)
+                                  : 0ull;
+            bool done = false, stalled = false;
+#pragma unroll
+            for (int q = 0; q < kLookWidth; q++) {
+                if (done || stalled || j - q < 0) continue;
+                if (status_tag(v[q]) != tag) {
+                    stalled = true;
+                    j -= q;   // resume at the unpublished tile
+                    continue;
+                }
+                excl += status_count(v[q]);
+                if (status_incl(v[q])) done = true;
             }
-            if (status_tag(v) != tag) {
+            if (done) break;
+            if (!stalled) {
+                j -= kLookWidth;
+                continue;
+            }
+            if (++spins >= kLookbackSpins) {
                 timed_out = true;
                 break;
             }
-            excl += status_count(v);
-            if (status_incl(v)) break;
+            __builtin_amdgcn_s_sleep(1);
+            // after one look-back has given up, the others stop waiting too
+            if ((spins & 1023u) == 0 &&
+                (__hip_atomic_load(&ctr->flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & kFlagSort)) {
+                timed_out = true;
+                break;
+            }
         }
         if (timed_out) atomicOr(&ctr->flags, kFlagSort);
         __hip_atomic_store(my, hi | (1ull << 31) | (excl + cnt), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
