@@ -59,6 +59,11 @@ uint32_t next_pow2(uint64_t v) {
     return c > (1ull << 31) ? (1u << 31) : (uint32_t)c;
 }
 
+// Radix sort: decoupled look-back while the tiles fit the GPU at once, the
+// reduce-then-scan form beyond (the look-back's running totals advance a few
+// tiles per status round trip, which bounds it once tiles queue up).
+constexpr int kLookbackTiles = 256;
+
 int value_stride(int vd) { return vd <= 2 ? 2 : vd <= 4 ? 4 : vd <= 8 ? 8 : 16; }
 size_t store_bytes(int storage) { return storage == GT_STORE_F32 ? 4 : 2; }
 
@@ -74,8 +79,8 @@ struct gt_ctx {
     hipStream_t own = nullptr;
     int profiling = 0;   // 0 off, 1 HIP events, 2 events + probe/atomic counters
     hipEvent_t ev[8] = {};
-    DevBuf tkeys, tacc, ctr, counts, slot_list, slot_id, pk_a, pk_b, pay_b, hist, sort_status, vkeys, vals_a,
-        vals_b, nbr, splat_copy, mm_part, mm_out;
+    DevBuf tkeys, tacc, ctr, counts, slot_list, slot_id, pk_a, pk_b, pay_b, hist, sort_status, sort_hist, vkeys,
+        vals_a, vals_b, nbr, splat_copy, mm_part, mm_out;
     DevBuf stage_in, stage_out, view_a, view_b, view_o;
     gt::Counters *h_ctr = nullptr;   // pinned
     float *h_mm = nullptr;           // pinned, 3 * 2 * kMaxDH
@@ -290,13 +295,15 @@ int enqueue(gt_ctx *c, const Launch &L, const gt::PixSrc &src, const SliceSpec &
     if (!L.packed)
         GT_HIP(gt::launch_pack(L.d, ta, (const uint32_t *)c->slot_list.p, ctr, L.m_max, L.kp, L.pw,
                                (uint64_t *)c->pk_a.p, s));
-    gt::SortBufs sb{(uint64_t *)c->pk_a.p, (uint64_t *)c->pk_b.p, (uint32_t *)c->slot_list.p,
-                    (uint32_t *)c->pay_b.p, (uint32_t *)c->hist.p, (uint64_t *)c->sort_status.p};
+    gt::SortBufs sb{(uint64_t *)c->pk_a.p,         (uint64_t *)c->pk_b.p,    (uint32_t *)c->slot_list.p,
+                    (uint32_t *)c->pay_b.p,        (uint32_t *)c->hist.p,    (uint64_t *)c->sort_status.p,
+                    (uint32_t *)c->sort_hist.p, kLookbackTiles};
     bool in_b = false;
     GT_HIP(gt::radix_sort(sb, ctr, L.m_max, L.pw, L.kp.bits, &in_b, passes, s));
     const uint32_t *sorted = in_b ? sb.pay_b : sb.pay_a;
-    GT_HIP(gt::launch_finalize(L.W, L.storage, ta, L.vsc, sorted, ctr, L.m_max, L.vd, L.vs, (uint64_t *)c->vkeys.p,
-                               c->vals_a.p, L.splat_copy, s));
+    const uint64_t *sorted_keys = L.packed ? (in_b ? sb.pk_b : sb.pk_a) : nullptr;
+    GT_HIP(gt::launch_finalize(L.W, L.storage, ta, L.vsc, sorted, sorted_keys, ctr, L.m_max, L.vd, L.vs,
+                               (uint64_t *)c->vkeys.p, c->vals_a.p, L.splat_copy, s));
     if (c->profiling) GT_HIP(mark(c, 2, s));
     int2 *nbr = (int2 *)c->nbr.p;
     GT_HIP(gt::launch_neighbors(L.d, L.packed, ta, L.kp, (const uint64_t *)c->vkeys.p, ctr, L.m_max, nbr, ctr,
@@ -341,9 +348,10 @@ int execute(gt_ctx *c, const Launch &L, const gt::PixSrc &src, const SliceSpec &
     sig_add(sig, sl.dst);
     sig_add(sig, c->profiling);
     sig_add(sig, c->stream);
-    const void *bufs[] = {c->tkeys.p, c->tacc.p, c->ctr.p, c->counts.p, c->slot_list.p, c->slot_id.p, c->pk_a.p,
-                          c->pk_b.p, c->pay_b.p, c->hist.p, c->sort_status.p, c->vkeys.p, c->vals_a.p,
-                          c->vals_b.p, c->nbr.p};
+    const void *bufs[] = {c->tkeys.p,  c->tacc.p,        c->ctr.p,         c->counts.p,    c->slot_list.p,
+                          c->slot_id.p, c->pk_a.p,       c->pk_b.p,        c->pay_b.p,     c->hist.p,
+                          c->sort_status.p, c->sort_hist.p, c->vkeys.p,    c->vals_a.p,    c->vals_b.p,
+                          c->nbr.p};
     sig_add(sig, bufs);
     if (batch) {
         sig_add(sig, batch->ba);
@@ -441,6 +449,7 @@ int run(gt_ctx *c, const gt::PixSrc &src, int n, int row_w, int d, int vd, int s
         GT_TRY(ensure(c->pay_b, (size_t)L.m_max * 4));
         GT_TRY(ensure_zeroed(c->hist, gt::sort_state_words() * 4, s));
         GT_TRY(ensure_zeroed(c->sort_status, gt::sort_status_entries(L.m_max, L.pw) * 8, s));
+        GT_TRY(ensure(c->sort_hist, gt::sort_hist_words(L.m_max, L.pw) * 4));
         GT_TRY(ensure(c->vkeys, (size_t)L.m_max * L.W * 8));
         const size_t vbytes = (size_t)L.m_max * L.vs * store_bytes(storage);
         GT_TRY(ensure(c->vals_a, vbytes));
@@ -713,9 +722,11 @@ void gt_ctx_destroy(gt_ctx *c) {
     drop_graph(c);
     if (c->h_batch) (void)hipHostFree(c->h_batch);
     free_buf(c->batch_desc);
-    DevBuf *bufs[] = {&c->tkeys, &c->tacc, &c->ctr, &c->counts, &c->slot_list, &c->slot_id, &c->pk_a, &c->pk_b,
-                      &c->pay_b, &c->hist, &c->sort_status, &c->vkeys, &c->vals_a, &c->vals_b, &c->nbr, &c->splat_copy, &c->mm_part,
-                      &c->mm_out, &c->stage_in, &c->stage_out, &c->view_a, &c->view_b, &c->view_o};
+    DevBuf *bufs[] = {&c->tkeys,       &c->tacc,      &c->ctr,      &c->counts,    &c->slot_list, &c->slot_id,
+                      &c->pk_a,        &c->pk_b,      &c->pay_b,    &c->hist,      &c->sort_status,
+                      &c->sort_hist,   &c->vkeys,     &c->vals_a,   &c->vals_b,    &c->nbr,       &c->splat_copy,
+                      &c->mm_part,     &c->mm_out,    &c->stage_in, &c->stage_out, &c->view_a,    &c->view_b,
+                      &c->view_o};
     for (DevBuf *b : bufs) free_buf(*b);
     for (auto &e : c->ev)
         if (e) (void)hipEventDestroy(e);

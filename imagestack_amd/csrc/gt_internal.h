@@ -119,9 +119,11 @@ hipError_t launch_compact(int words, const TableArgs &ta, uint32_t capacity, uin
                           uint64_t *sort_keys, bool packed, Counters *ctr, hipStream_t st);
 hipError_t launch_pack(int d, const TableArgs &ta, const uint32_t *slot_list, const Counters *ctr, int m_max,
                        const KeyPlan &kp, int pw, uint64_t *pk, hipStream_t st);
+// sorted_keys: the sorted packed keys (one word each), or null to gather the
+// table's key words by slot.
 hipError_t launch_finalize(int words, int storage, const TableArgs &ta, const ValScale &vsc, const uint32_t *sorted,
-                           const Counters *ctr, int m_max, int vd, int vs, uint64_t *vkeys, void *vals,
-                           float *splat_copy, hipStream_t st);
+                           const uint64_t *sorted_keys, const Counters *ctr, int m_max, int vd, int vs,
+                           uint64_t *vkeys, void *vals, float *splat_copy, hipStream_t st);
 hipError_t launch_neighbors(int d, bool packed, const TableArgs &ta, const KeyPlan &kp, const uint64_t *vkeys,
                             const Counters *ctr, int m_max, int2 *nbr, Counters *wctr, bool count, int wpc,
                             hipStream_t st);
@@ -146,9 +148,12 @@ struct SortBufs {
     uint32_t *pay_a, *pay_b;    // payloads
     uint32_t *state;            // sort_state_words(): epoch, tile counters, digit counts (zeroed once)
     uint64_t *status;           // sort_status_entries(): look-back words (zeroed once)
+    uint32_t *hist;             // sort_hist_words(): tile counts and chunk sums of the reduce-then-scan form
+    int lookback_tiles;         // decoupled look-back up to this many tiles, reduce-then-scan above (-1: never)
 };
 size_t sort_state_words();
 size_t sort_status_entries(int m_max, int pw);
+size_t sort_hist_words(int m_max, int pw);
 // Sets the thread's gt_last_error() message; returns code.
 int set_error(int code, const char *msg);
 // Forgets the context's cached key-packing plan, so the next call bounds its

@@ -447,15 +447,19 @@ __global__ __launch_bounds__(kBlock) void k_pack(TableArgs ta, const uint32_t *_
 // rounding: int64 -> double is exact below 2^53, then one FP32 rounding).
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_finalize(TableArgs ta, ValScale vsc, int kw,
-                                                     const uint32_t *__restrict__ sorted, const Counters *ctr, int vd,
-                                                     int vs, uint64_t *__restrict__ vkeys, T *__restrict__ vals,
-                                                     float *__restrict__ splat_copy) {
+                                                     const uint32_t *__restrict__ sorted,
+                                                     const uint64_t *__restrict__ sorted_keys, const Counters *ctr,
+                                                     int vd, int vs, uint64_t *__restrict__ vkeys,
+                                                     T *__restrict__ vals, float *__restrict__ splat_copy) {
     const unsigned M = ctr->M;
     const unsigned i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= M) return;
     const uint32_t s = sorted[i];
     ta.slot_id[s] = (int)i;
-    for (int w = 0; w < kw; w++) vkeys[(size_t)i * kw + w] = ta.tkeys[(size_t)s * kw + w];
+    if (sorted_keys)   // packed keys: the sort's own output, read in order
+        vkeys[i] = sorted_keys[i];
+    else
+        for (int w = 0; w < kw; w++) vkeys[(size_t)i * kw + w] = ta.tkeys[(size_t)s * kw + w];
     float o[kMaxVD];
 #pragma unroll
     for (int c = 0; c < kMaxVD; c++)
@@ -686,6 +690,8 @@ __global__ __launch_bounds__(kBlock) void k_slice_tile(PixSrc src0, int n0, int 
     __shared__ float lv[LT * LVD];
     __shared__ uint16_t llist[LT];
     __shared__ unsigned lcount;
+    // a full table means the call is redone with a larger one: skip the slice
+    if (*(volatile const unsigned *)&ctr->flags & kFlagFull) return;
     unsigned flags = 0;
     uint32_t probes = 0;
     TileView tv;
@@ -790,6 +796,7 @@ __global__ __launch_bounds__(kBlock) void k_slice_direct(PixSrc src, int n, int 
                                                          Counters *ctr, int count) {
     using K = KeyOps<D, PACKED>;
     constexpr int W = K::W;
+    if (*(volatile const unsigned *)&ctr->flags & kFlagFull) return;   // redone with a larger table
     const int p = blockIdx.x * kBlock + threadIdx.x;
     uint32_t probes = 0;
     unsigned flags = 0;

@@ -174,13 +174,23 @@ __global__ __launch_bounds__(kBlock) void k_clear(uint64_t *__restrict__ tkeys, 
 }
 
 // Exclusive scan of n uint32 in one workgroup of 1024; the total goes to *total.
-__global__ __launch_bounds__(1024) void k_scan_1wg(uint32_t *__restrict__ a, int n, unsigned *total) {
+// In-place exclusive scan of n counts in one workgroup; the total goes to
+// *total, or 0 when `flags` carries kFlagFull (the call is redone with a
+// larger table, so everything after the compaction sees no vertices).
+__global__ __launch_bounds__(1024) void k_scan_1wg(uint32_t *__restrict__ a, int n, unsigned *total,
+                                                   const unsigned *flags) {
     __shared__ uint32_t part[1024];
     const int chunk = (n + 1023) / 1024;
     const int lo = threadIdx.x * chunk;
     const int hi = min(n, lo + chunk);
     uint32_t s = 0;
-    for (int i = lo; i < hi; i++) s += a[i];
+    for (int i = lo; i < hi; i += 8) {
+        uint32_t v[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) v[q] = i + q < hi ? a[i + q] : 0u;
+#pragma unroll
+        for (int q = 0; q < 8; q++) s += v[q];
+    }
     part[threadIdx.x] = s;
     __syncthreads();
     for (int off = 1; off < 1024; off <<= 1) {
@@ -190,12 +200,17 @@ __global__ __launch_bounds__(1024) void k_scan_1wg(uint32_t *__restrict__ a, int
         __syncthreads();
     }
     uint32_t run = part[threadIdx.x] - s;
-    for (int i = lo; i < hi; i++) {
-        const uint32_t v = a[i];
-        a[i] = run;
-        run += v;
+    for (int i = lo; i < hi; i += 8) {
+        uint32_t v[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) v[q] = i + q < hi ? a[i + q] : 0u;
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            if (i + q < hi) a[i + q] = run;
+            run += v[q];
+        }
     }
-    if (threadIdx.x == 1023 && total) *total = part[1023];
+    if (threadIdx.x == 1023 && total) *total = (flags && (*flags & kFlagFull)) ? 0u : part[1023];
 }
 
 // Strided view -> strided view copy, used to gather the non-dense views of
@@ -243,7 +258,7 @@ hipError_t launch_compact(int words, const TableArgs &ta, uint32_t capacity, uin
     case 4: hipLaunchKernelGGL(k_count<4>, dim3(nblk), dim3(kBlock), 0, st, ta.tkeys, capacity, counts); break;
     default: return hipErrorInvalidValue;
     }
-    hipLaunchKernelGGL(k_scan_1wg, dim3(1), dim3(1024), 0, st, counts, (int)nblk, &ctr->M);
+    hipLaunchKernelGGL(k_scan_1wg, dim3(1), dim3(1024), 0, st, counts, (int)nblk, &ctr->M, &ctr->flags);
     switch (words) {
     case 1:
         if (packed) hipLaunchKernelGGL((k_write_list<1, true>), dim3(nblk), dim3(kBlock), 0, st, ta.tkeys, capacity,
@@ -278,18 +293,18 @@ hipError_t launch_pack(int d, const TableArgs &ta, const uint32_t *slot_list, co
 }
 
 hipError_t launch_finalize(int words, int storage, const TableArgs &ta, const ValScale &vsc, const uint32_t *sorted,
-                           const Counters *ctr, int m_max, int vd, int vs, uint64_t *vkeys, void *vals,
-                           float *splat_copy, hipStream_t st) {
+                           const uint64_t *sorted_keys, const Counters *ctr, int m_max, int vd, int vs,
+                           uint64_t *vkeys, void *vals, float *splat_copy, hipStream_t st) {
     if (m_max <= 0) return hipSuccess;
     const dim3 g(blocks_for(m_max, kBlock)), b(kBlock);
     if (storage == 0)
-        hipLaunchKernelGGL(k_finalize<float>, g, b, 0, st, ta, vsc, words, sorted, ctr, vd, vs, vkeys, (float *)vals,
-                           splat_copy);
+        hipLaunchKernelGGL(k_finalize<float>, g, b, 0, st, ta, vsc, words, sorted, sorted_keys, ctr, vd, vs, vkeys,
+                           (float *)vals, splat_copy);
     else if (storage == 1)
-        hipLaunchKernelGGL(k_finalize<__hip_bfloat16>, g, b, 0, st, ta, vsc, words, sorted, ctr, vd, vs, vkeys,
-                           (__hip_bfloat16 *)vals, splat_copy);
+        hipLaunchKernelGGL(k_finalize<__hip_bfloat16>, g, b, 0, st, ta, vsc, words, sorted, sorted_keys, ctr, vd, vs,
+                           vkeys, (__hip_bfloat16 *)vals, splat_copy);
     else
-        hipLaunchKernelGGL(k_finalize<__half>, g, b, 0, st, ta, vsc, words, sorted, ctr, vd, vs, vkeys,
+        hipLaunchKernelGGL(k_finalize<__half>, g, b, 0, st, ta, vsc, words, sorted, sorted_keys, ctr, vd, vs, vkeys,
                            (__half *)vals, splat_copy);
     return hipGetLastError();
 }

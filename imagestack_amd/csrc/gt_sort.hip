@@ -97,13 +97,119 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *tmp /*
     return before + x - v;
 }
 
+// Reduce-then-scan form of a pass (many tiles): per-tile digit counts
+// (tile-major hist[tile * 256 + digit]), then a column scan of that matrix in
+// three launches over groups of `gt` tiles: group sums, a scan of the group
+// sums (one workgroup per digit), and the running totals written back in
+// place, so hist[tile][digit] becomes the output index of the tile's first
+// key with that digit.  Group sums follow the tile counts in `hist`.
 template <int PW>
+__global__ __launch_bounds__(kSortThreads) void k_sort_hist(const uint64_t *__restrict__ kin, int stride,
+                                                           const Counters *ctr, int pass,
+                                                           uint32_t *__restrict__ hist) {
+    using T = SortTile<PW>;
+    __shared__ uint32_t h[256];
+    const unsigned M = ctr->M;
+    const int tile = (int)blockIdx.x;
+    if ((unsigned)tile * T::kTile >= M) return;
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const int word = pass / 8, shift = (pass % 8) * 8;
+#pragma unroll
+    for (int r = 0; r < T::kItems; r++) {
+        const unsigned i = (unsigned)tile * T::kTile + r * kSortThreads + threadIdx.x;
+        if (i < M) atomicAdd(&h[(kin[(size_t)word * stride + i] >> shift) & 255], 1u);
+    }
+    __syncthreads();
+    hist[(size_t)tile * 256 + threadIdx.x] = h[threadIdx.x];
+}
+
+template <int PW>
+__global__ __launch_bounds__(kSortThreads) void k_sort_group_sums(const Counters *ctr, uint32_t *__restrict__ hist,
+                                                                 int max_tiles, int gt) {
+    using T = SortTile<PW>;
+    const int ntiles = (int)((ctr->M + T::kTile - 1) / T::kTile);
+    const int g = (int)blockIdx.x, t0 = g * gt;
+    if (t0 >= ntiles) return;
+    const int t1 = min(ntiles, t0 + gt);
+    uint32_t s = 0;
+    for (int t = t0; t < t1; t += 8) {
+        uint32_t v[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) v[q] = t + q < t1 ? hist[(size_t)(t + q) * 256 + threadIdx.x] : 0u;
+#pragma unroll
+        for (int q = 0; q < 8; q++) s += v[q];
+    }
+    hist[(size_t)max_tiles * 256 + (size_t)g * 256 + threadIdx.x] = s;
+}
+
+// One workgroup per digit: exclusive scan of that digit's group sums (at most
+// kSortThreads * 4 groups), offset by the digit's global start.
+template <int PW>
+__global__ __launch_bounds__(kSortThreads) void k_sort_group_scan(const Counters *ctr, int pass,
+                                                                 const uint32_t *__restrict__ state,
+                                                                 uint32_t *__restrict__ hist, int max_tiles,
+                                                                 int gt) {
+    using T = SortTile<PW>;
+    __shared__ uint32_t tmp[kSortWaves];
+    __shared__ uint32_t s_goff;
+    const int d = (int)blockIdx.x;
+    const int ntiles = (int)((ctr->M + T::kTile - 1) / T::kTile);
+    const int ng = (ntiles + gt - 1) / gt;
+    // the digit's global start: digits below it over all keys
+    uint32_t below = 0;
+    for (int e = threadIdx.x; e < d; e += kSortThreads) below += state[kStateCounts + pass * 256 + e];
+    below = block_excl_scan(below, tmp) + below;   // inclusive: the last thread holds the total
+    if (threadIdx.x == kSortThreads - 1) s_goff = below;
+    __syncthreads();
+    uint32_t *gs = hist + (size_t)max_tiles * 256 + d;
+    uint32_t v[4], s = 0;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int g = threadIdx.x * 4 + q;
+        v[q] = g < ng ? gs[(size_t)g * 256] : 0u;
+        s += v[q];
+    }
+    uint32_t run = s_goff + block_excl_scan(s, tmp);
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int g = threadIdx.x * 4 + q;
+        if (g < ng) gs[(size_t)g * 256] = run;
+        run += v[q];
+    }
+}
+
+template <int PW>
+__global__ __launch_bounds__(kSortThreads) void k_sort_tile_offsets(const Counters *ctr, uint32_t *__restrict__ hist,
+                                                                   int max_tiles, int gt) {
+    using T = SortTile<PW>;
+    const int ntiles = (int)((ctr->M + T::kTile - 1) / T::kTile);
+    const int g = (int)blockIdx.x, t0 = g * gt;
+    if (t0 >= ntiles) return;
+    const int t1 = min(ntiles, t0 + gt);
+    uint32_t run = hist[(size_t)max_tiles * 256 + (size_t)g * 256 + threadIdx.x];
+    for (int t = t0; t < t1; t += 8) {
+        uint32_t v[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) v[q] = t + q < t1 ? hist[(size_t)(t + q) * 256 + threadIdx.x] : 0u;
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            if (t + q < t1) hist[(size_t)(t + q) * 256 + threadIdx.x] = run;
+            run += v[q];
+        }
+    }
+}
+
+// LOOKBACK: decoupled look-back over the tiles' status words (few tiles);
+// otherwise the chunk totals and tile counts of the reduce-then-scan form.
+template <int PW, bool LOOKBACK>
 __global__ __launch_bounds__(kSortThreads) void k_sort_pass(const uint64_t *__restrict__ kin,
                                                            const uint32_t *__restrict__ pin,
                                                            uint64_t *__restrict__ kout, uint32_t *__restrict__ pout,
                                                            int stride, Counters *ctr, int pass,
                                                            uint32_t *__restrict__ state,
-                                                           uint64_t *__restrict__ status) {
+                                                           uint64_t *__restrict__ status,
+                                                           const uint32_t *__restrict__ hist, int max_tiles) {
     using T = SortTile<PW>;
     constexpr int ITEMS = T::kItems, TILE = T::kTile, RUN = T::kWaveRun;
     __shared__ uint64_t skey[TILE * PW];
@@ -162,7 +268,9 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_pass(const uint64_t *__re
     uint64_t *my = status + (size_t)tile * 256 + tid;
     const uint64_t hi = (uint64_t)tag << 32;
     uint32_t excl = 0;
-    if (tile == 0) {
+    if constexpr (!LOOKBACK) {
+        excl = hist[(size_t)tile * 256 + tid];   // output index of the digit's first key (global start included)
+    } else if (tile == 0) {
         __hip_atomic_store(my, hi | (1ull << 31) | cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     } else {
         __hip_atomic_store(my, hi | cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -212,8 +320,9 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_pass(const uint64_t *__re
         __hip_atomic_store(my, hi | (1ull << 31) | (excl + cnt), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     // Global offset of the digit: digits before it over all keys, plus this
-    // digit's keys in earlier tiles.
-    const uint32_t goff = block_excl_scan(state[kStateCounts + pass * 256 + tid], tmp);
+    // digit's keys in earlier tiles (the reduce-then-scan totals include the
+    // former already).
+    const uint32_t goff = LOOKBACK ? block_excl_scan(state[kStateCounts + pass * 256 + tid], tmp) : 0u;
     const uint32_t ds = block_excl_scan(cnt, tmp);
     gbase[tid] = goff + excl;
     dstart[tid] = ds;
@@ -248,19 +357,40 @@ size_t sort_status_entries(int m_max, int pw) {
     return tiles * 256;
 }
 
+size_t sort_hist_words(int m_max, int pw) {
+    const size_t tiles = sort_status_entries(m_max, pw) / 256;
+    return (tiles + 4 * kSortThreads) * 256;   // tile counts, then up to 1024 group sums
+}
+
 template <int PW>
 static void sort_launch(SortBufs &b, Counters *ctr, int m_max, int np, hipStream_t st) {
     using T = SortTile<PW>;
     const int tiles = (m_max + T::kTile - 1) / T::kTile;
+    // groups of gt tiles (a multiple of 8), at most 4 * 256 groups
+    const int gt = std::max(8, ((tiles + 4 * kSortThreads - 1) / (4 * kSortThreads) + 7) / 8 * 8);
+    const int groups = (tiles + gt - 1) / gt;
     const int cblk = std::max(1, std::min(256, (m_max + kSortThreads * 16 - 1) / (kSortThreads * 16)));
+    const bool lookback = b.lookback_tiles < 0 ? false : tiles <= b.lookback_tiles;
     hipLaunchKernelGGL(k_sort_init, dim3(1), dim3(kSortThreads), 0, st, b.state, np);
     hipLaunchKernelGGL((k_sort_counts<PW>), dim3(cblk), dim3(kSortThreads), 0, st, b.pk_a, m_max, ctr, np,
                        b.state);
     uint64_t *kin = b.pk_a, *kout = b.pk_b;
     uint32_t *pin = b.pay_a, *pout = b.pay_b;
     for (int p = 0; p < np; p++) {
-        hipLaunchKernelGGL((k_sort_pass<PW>), dim3(tiles), dim3(kSortThreads), 0, st, kin, pin, kout, pout, m_max,
-                           ctr, p, b.state, b.status);
+        if (lookback) {
+            hipLaunchKernelGGL((k_sort_pass<PW, true>), dim3(tiles), dim3(kSortThreads), 0, st, kin, pin, kout, pout,
+                               m_max, ctr, p, b.state, b.status, b.hist, tiles);
+        } else {
+            hipLaunchKernelGGL((k_sort_hist<PW>), dim3(tiles), dim3(kSortThreads), 0, st, kin, m_max, ctr, p, b.hist);
+            hipLaunchKernelGGL((k_sort_group_sums<PW>), dim3(groups), dim3(kSortThreads), 0, st, ctr, b.hist, tiles,
+                               gt);
+            hipLaunchKernelGGL((k_sort_group_scan<PW>), dim3(256), dim3(kSortThreads), 0, st, ctr, p, b.state, b.hist,
+                               tiles, gt);
+            hipLaunchKernelGGL((k_sort_tile_offsets<PW>), dim3(groups), dim3(kSortThreads), 0, st, ctr, b.hist, tiles,
+                               gt);
+            hipLaunchKernelGGL((k_sort_pass<PW, false>), dim3(tiles), dim3(kSortThreads), 0, st, kin, pin, kout,
+                               pout, m_max, ctr, p, b.state, b.status, b.hist, tiles);
+        }
         std::swap(kin, kout);
         std::swap(pin, pout);
     }
