@@ -125,6 +125,67 @@ def batch_leg(gtlattice, synth, torch, steps):
                          "default_ms": round(rep["default_ms"], 4), "search_s": round(tune_s, 2)}}
 
 
+def highdim_leg(gtlattice, synth, torch, steps):
+    """cfg3: d=16 Gauss transform at 1024x1024 (14 N(0,1) features + x/12,
+    y/12; 4 values U[0,1]; unnormalised output), lattice values stored in
+    FP32, BF16 and FP16 with FP32 arithmetic.  At full size each narrow
+    storage is compared with the FP32-stored result; the GPU outputs on a 96x96
+    crop of the same workload are returned for the CPU leg, which checks them
+    against the FP64 oracle (the full frame takes the oracle minutes).
+    Returns (report, crop)."""
+    import numpy as np
+    side = 1024
+    pos = synth.highdim_positions(side, side, nfeat=14, sigma_s=12.0, seed=3)
+    d = pos.shape[0]
+    vals = synth.uniform_image(side, side, 4, seed=33)
+    dp = torch.from_numpy(pos.reshape(d, -1)).cuda()
+    dv = torch.from_numpy(vals.reshape(4, -1)).cuda()
+    inv = np.ones(d, np.float32)
+    ctx = gtlattice.Context(0, profiling=1)
+    out = {"workload": "d=16 Gauss transform 1024x1024: 14 N(0,1) features + x/12, y/12; values 4 x U[0,1]; "
+                       "unnormalised output; FP32 arithmetic, lattice values stored as named"}
+    full = {}
+    for storage in ("f32", "bf16", "f16"):
+        r, st = ctx.gauss_transform(dp, dv, inv, row_w=side, storage=storage, stats=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            r, st = ctx.gauss_transform(dp, dv, inv, row_w=side, storage=storage, stats=True)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        full[storage] = r.cpu().numpy().astype(np.float64)
+        out[storage] = {"ms": round(ms, 3), "mpix_s": round(side * side / ms / 1e3, 2), "M": st["M"],
+                        "stages_ms": {k: round(st[k], 3) for k in ("ms_splat", "ms_sort", "ms_nbr", "ms_blur",
+                                                                    "ms_slice")}}
+    ref = full["f32"]
+    for storage in ("bf16", "f16"):
+        e = full[storage] - ref
+        out[storage]["rel_l2_vs_f32_storage"] = float(np.linalg.norm(e) / np.linalg.norm(ref))
+        out[storage]["max_err_over_range_vs_f32_storage"] = float(np.abs(e).max() / (ref.max() - ref.min()))
+    c = 96
+    pc = np.ascontiguousarray(pos[:, :, :c, :c])
+    vc = np.ascontiguousarray(vals[:, :, :c, :c])
+    crop = {"pos": pc, "vals": vc, "inv": inv}
+    for storage in ("f32", "bf16", "f16"):
+        crop[storage] = ctx.gauss_transform(torch.from_numpy(pc.reshape(d, -1)).cuda(),
+                                            torch.from_numpy(vc.reshape(4, -1)).cuda(), inv, row_w=c,
+                                            storage=storage).cpu().numpy().astype(np.float64)
+    ctx.close()
+    return out, crop
+
+
+def highdim_oracle_check(report, crop):
+    """CPU leg: the d=16 crop outputs of each storage against the FP64 oracle
+    of the same lattice."""
+    import numpy as np
+    from oracle import oracle as O
+    o64, _ = O.gauss_transform(crop["pos"], crop["vals"], crop["inv"], mode=O.F64_LAT)
+    for storage in ("f32", "bf16", "f16"):
+        e = crop[storage] - o64
+        report[storage]["crop96_rel_l2_vs_fp64"] = float(np.linalg.norm(e) / np.linalg.norm(o64))
+        report[storage]["crop96_max_err_over_range_vs_fp64"] = float(np.abs(e).max() / (o64.max() - o64.min()))
+
+
 def serve_leg(gtlattice, synth, torch, n_req, qps, slots, storages=("f32", "bf16")):
     """cfg5: open-loop Poisson replay through the request server; latency is
     submit -> result in (pinned) host memory, PCIe copies included."""
@@ -134,7 +195,7 @@ def serve_leg(gtlattice, synth, torch, n_req, qps, slots, storages=("f32", "bf16
     ring = [torch.empty((3, pool_side, pool_side), dtype=torch.float32).pin_memory() for _ in range(8)]
     base = pool.data_ptr()
     plane = pool_side * pool_side
-    srv = gtlattice.Server(0, slots=slots)
+    srv = gtlattice.Server(0, slots=slots, warm_side=pool_side)
 
     def requests(sched, storage):
         reqs = []
@@ -169,7 +230,8 @@ def serve_leg(gtlattice, synth, torch, n_req, qps, slots, storages=("f32", "bf16
                         "max_ms": round(float(lat.max()), 3), "queue_p99_ms": round(float(np.percentile(q, 99)), 3),
                         "mean_batch": round(float(np.mean([r["batch"] for r in res])), 2),
                         "achieved_qps": round(n / wall, 2), "mpix_s": round(px / wall / 1e6, 2)}
-    out["hbm_budget_note"] = "default budget: half the free device memory at server start"
+    out["hbm_budget_note"] = ("default budget: half the free device memory at server start; each worker "
+                              "warmed with one dense 4096^2 RGB filter before the replay")
     srv.close()
     return out
 
@@ -184,6 +246,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-lattice leg (64 images)")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
+    ap.add_argument("--no-highdim", action="store_true", help="skip the d=16 precision leg")
     ap.add_argument("--serve-requests", type=int, default=2000)
     ap.add_argument("--qps", type=float, default=100.0)
     ap.add_argument("--slots", type=int, default=4)
@@ -296,6 +359,9 @@ def main():
     if not a.no_batch:
         progress("batched lattice leg (64 images)")
         line["batch"] = batch_leg(gtlattice, synth, torch, max(4, a.steps // 2))
+    if not a.no_highdim:
+        progress("d=16 precision leg (1024^2, FP32/BF16/FP16 storage)")
+        line["highdim"], hd_crop = highdim_leg(gtlattice, synth, torch, 3)
     if not a.no_serve:
         progress(f"serving leg: {a.serve_requests} requests at {a.qps:g} QPS")
         sv = serve_leg(gtlattice, synth, torch, a.serve_requests, a.qps, a.slots)
@@ -306,6 +372,8 @@ def main():
     if not a.no_cpu:
         progress("CPU baseline leg")
         cpu, err = cpu_baseline_leg(out_gpu, im, guide, ss, sr)
+        if not a.no_highdim:
+            highdim_oracle_check(line["highdim"], hd_crop)
         line["cpu_baseline"] = cpu
         line["gpu_over_cpu"] = round(value / cpu["value"], 1)
         line["accuracy"] = err

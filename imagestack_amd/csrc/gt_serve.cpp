@@ -76,6 +76,7 @@ struct gt_server {
     std::vector<std::thread> workers;
     std::string init_err;
     int init_rc = GT_OK;
+    std::vector<float> warm;   // host warm-up image (3 planes of warm_side^2), shared read-only
 };
 
 namespace {
@@ -205,6 +206,26 @@ void worker_main(gt_server *s, std::mutex *init_mu, int *init_left, std::conditi
     if (rc == GT_OK) rc = gt_ctx_create(&ctx, s->device);
     if (rc == GT_OK && hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) rc = GT_EDEVICE;
     if (rc == GT_OK) rc = gt_ctx_set_stream(ctx, st);
+    DevArena arena;
+    // Warm-up: one dense RGB filter of the configured size grows the context's
+    // lattice buffers and the arena to their serving size.
+    if (rc == GT_OK && s->cfg.warm_side > 0) {
+        const int side = s->cfg.warm_side;
+        const size_t n = (size_t)side * side;
+        if (arena.ensure(6 * n) != GT_OK) {
+            rc = GT_ENOMEM;
+        } else if (hipMemcpyAsync(arena.p, s->warm.data(), 3 * n * sizeof(float), hipMemcpyHostToDevice, st) !=
+                   hipSuccess) {
+            (void)hipGetLastError();
+            rc = GT_EDEVICE;
+        } else {
+            gt_tuning tu{};
+            tu.graph = -1;
+            rc = gt_joint_bilateral(ctx, side, side, 1, 3, arena.p, 3, arena.p, 4.0f, 4.0f, 0.0f, 0.1f,
+                                    arena.p + 3 * n, GT_STORE_F32, &tu, nullptr);
+            gt::invalidate_plan(ctx);
+        }
+    }
     {
         std::lock_guard<std::mutex> lk(*init_mu);
         if (rc != GT_OK && s->init_rc == GT_OK) {
@@ -214,7 +235,6 @@ void worker_main(gt_server *s, std::mutex *init_mu, int *init_left, std::conditi
         (*init_left)--;
     }
     init_cv->notify_all();
-    DevArena arena;
     for (;;) {
         std::vector<Req *> g;
         {
@@ -277,6 +297,18 @@ int gt_server_create(gt_server **out, int device, const gt_server_config *cfg) {
         }
         s->cfg.hbm_budget = fr / 2;
     }
+    if (s->cfg.warm_side > 0) {
+        // dense content: independent uniform noise per channel (xorshift)
+        const size_t n = (size_t)s->cfg.warm_side * s->cfg.warm_side;
+        s->warm.resize(3 * n);
+        uint64_t x = 0x9E3779B97F4A7C15ull;
+        for (size_t i = 0; i < 3 * n; i++) {
+            x ^= x << 13;
+            x ^= x >> 7;
+            x ^= x << 17;
+            s->warm[i] = (float)(x >> 40) * (1.0f / 16777216.0f);
+        }
+    }
     std::mutex init_mu;
     std::condition_variable init_cv;
     int left = s->cfg.slots;
@@ -285,6 +317,8 @@ int gt_server_create(gt_server **out, int device, const gt_server_config *cfg) {
         std::unique_lock<std::mutex> lk(init_mu);
         init_cv.wait(lk, [&] { return left == 0; });
     }
+    s->warm.clear();
+    s->warm.shrink_to_fit();
     if (s->init_rc != GT_OK) {
         const int rc = s->init_rc;
         const std::string m = s->init_err;

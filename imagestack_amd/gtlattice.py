@@ -48,7 +48,8 @@ def tuning_dict(t: Tuning) -> dict:
 
 
 class ServerConfig(C.Structure):
-    _fields_ = [("slots", C.c_int), ("max_batch", C.c_int), ("batch_pixels", C.c_int), ("hbm_budget", C.c_size_t)]
+    _fields_ = [("slots", C.c_int), ("max_batch", C.c_int), ("batch_pixels", C.c_int), ("hbm_budget", C.c_size_t),
+                ("warm_side", C.c_int)]
 
 
 class Request(C.Structure):
@@ -321,9 +322,9 @@ class Server:
     batching and `slots` concurrent workers."""
 
     def __init__(self, device: int = 0, slots: int = 4, max_batch: int = 16, batch_pixels: int = 1 << 22,
-                 hbm_budget: int = 0):
+                 hbm_budget: int = 0, warm_side: int = 0):
         self._h = _vp()
-        cfg = ServerConfig(slots, max_batch, batch_pixels, hbm_budget)
+        cfg = ServerConfig(slots, max_batch, batch_pixels, hbm_budget, warm_side)
         _check(lib().gt_server_create(C.byref(self._h), device, C.byref(cfg)))
 
     def close(self) -> None:

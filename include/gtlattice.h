@@ -204,6 +204,8 @@ typedef struct {
     int max_batch;       /* requests per batched lattice (default 16)            */
     int batch_pixels;    /* pixels per batched lattice (default 4 Mi)            */
     size_t hbm_budget;   /* bytes of admitted footprints (default: half the free memory) */
+    int warm_side;       /* >0: at start each worker filters a dense warm_side^2 RGB image, so its
+                            buffers already fit requests up to that size (no growth while serving) */
 } gt_server_config;
 
 typedef struct {
