@@ -306,8 +306,7 @@ int enqueue(gt_ctx *c, const Launch &L, const gt::PixSrc &src, const SliceSpec &
                                (uint64_t *)c->vkeys.p, c->vals_a.p, L.splat_copy, s));
     if (c->profiling) GT_HIP(mark(c, 2, s));
     int2 *nbr = (int2 *)c->nbr.p;
-    GT_HIP(gt::launch_neighbors(L.d, L.packed, ta, L.kp, (const uint64_t *)c->vkeys.p, ctr, L.m_max, nbr, ctr,
-                                L.count, L.wpc, s));
+    GT_HIP(gt::launch_neighbors(L.d, L.packed, L.kp, (const uint64_t *)c->vkeys.p, ctr, L.m_max, nbr, L.wpc, s));
     if (c->profiling) GT_HIP(mark(c, 3, s));
     void *cur = c->vals_a.p, *nxt = c->vals_b.p;
     for (int j = 0; j <= L.d; j++) {

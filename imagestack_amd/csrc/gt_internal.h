@@ -124,9 +124,8 @@ hipError_t launch_pack(int d, const TableArgs &ta, const uint32_t *slot_list, co
 hipError_t launch_finalize(int words, int storage, const TableArgs &ta, const ValScale &vsc, const uint32_t *sorted,
                            const uint64_t *sorted_keys, const Counters *ctr, int m_max, int vd, int vs,
                            uint64_t *vkeys, void *vals, float *splat_copy, hipStream_t st);
-hipError_t launch_neighbors(int d, bool packed, const TableArgs &ta, const KeyPlan &kp, const uint64_t *vkeys,
-                            const Counters *ctr, int m_max, int2 *nbr, Counters *wctr, bool count, int wpc,
-                            hipStream_t st);
+hipError_t launch_neighbors(int d, bool packed, const KeyPlan &kp, const uint64_t *vkeys, const Counters *ctr,
+                            int m_max, int2 *nbr, int wpc, hipStream_t st);
 hipError_t launch_blur(int storage, int vs, const int2 *nbr, const Counters *ctr, int m_max, const void *in, void *out,
                        hipStream_t st);
 hipError_t launch_slice(int d, bool packed, bool bilat, bool tiled, int storage, const PixSrc &src, int n, int row_w,

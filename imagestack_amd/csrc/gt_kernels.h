@@ -469,70 +469,49 @@ __global__ __launch_bounds__(kBlock) void k_finalize(TableArgs ta, ValScale vsc,
         for (int c = 0; c < vd; c++) splat_copy[(size_t)i * vd + c] = o[c];
 }
 
-// Neighbour table (src/Permutohedral.h:446-451): along axis j < D the two
-// neighbours are k + 1 - (D+1) e_j and k - 1 + (D+1) e_j on the first D
-// coordinates; axis D moves the implicit coordinate, i.e. k + 1 and k - 1.
-// A neighbour outside the packing plan cannot be in the lattice.  One thread
-// per (vertex, axis): blockIdx.y is the axis.
-template <int D, bool PACKED>
-__global__ __launch_bounds__(kBlock) void k_neighbors(TableArgs ta, KeyPlan kp, const uint64_t *__restrict__ vkeys,
-                                                      const Counters *ctr, int stride, int2 *__restrict__ nbr,
-                                                      Counters *wctr, int count) {
-    using K = KeyOps<D, PACKED>;
-    constexpr int W = K::W;
-    const unsigned M = ctr->M;
-    const unsigned i = blockIdx.x * kBlock + threadIdx.x;
-    const int j = blockIdx.y;
-    uint32_t probes = 0;
-    if (i < M) {
-        uint64_t k[W];
-#pragma unroll
-        for (int w = 0; w < W; w++) k[w] = vkeys[(size_t)i * W + w];
-        int ca[D], cb[D];
-#pragma unroll
-        for (int q = 0; q < D; q++) {
-            const int c = K::coord(k, q, kp);
-            ca[q] = c + ((q == j) ? -D : 1);
-            cb[q] = c + ((q == j) ? D : -1);
-        }
-        uint64_t a[W], b[W];
-        const uint64_t tag = K::tag_of(k, kp);
-        const bool oka = K::from_coords(ca, kp, tag, a);
-        const bool okb = K::from_coords(cb, kp, tag, b);
-        const int sa = oka ? table_find<W>(ta.tkeys, ta.mask, a, probes) : -1;
-        const int sb = okb ? table_find<W>(ta.tkeys, ta.mask, b, probes) : -1;
-        nbr[(size_t)j * stride + i] = make_int2(sa >= 0 ? ta.slot_id[sa] : -1, sb >= 0 ? ta.slot_id[sb] : -1);
-    }
-    if (count) {
-        const unsigned long long pr = wave_sum((unsigned long long)probes);
-        if (__lane_id() == 0 && pr) atomicAdd(&wctr->probes, pr);
-    }
-}
-
-// Neighbour table from the sorted packed keys, without the hash table.
-// Moving a key by +-(1,...,1) -+ (D+1) e_j moves every field of its packed
-// word by a constant, so when no field leaves its range the neighbour's word
-// is the vertex's word plus a constant delta: the targets of a run of sorted
-// vertices are sorted too, and finding them is a merge with the sorted keys.
-// Workgroup = 2048 consecutive vertices x one direction (blockIdx.y = 2 axis
-// + sign): wave 0 finds the first target's lower bound by a 64-ary search,
-// then the workgroup stages windows of 4096 sorted keys in LDS and each
-// thread binary-searches its targets there.  Ids are sorted ranks, so the
-// table equals the hash-built one entry for entry.
+// Neighbour table from the sorted keys, without the hash table.  Moving a
+// key by +-(1,...,1) -+ (D+1) e_j moves every field of its words by a
+// constant, so when no field leaves its range the neighbour's words are the
+// vertex's words plus constant deltas (exact: no carry or borrow crosses a
+// field).  The targets of a run of sorted vertices are then sorted too, and
+// finding them is a merge with the sorted keys.  Workgroup = 2048 consecutive
+// vertices x one direction (blockIdx.y = 2 axis + sign): wave 0 finds the
+// first target's lower bound by a 64-ary search, then the workgroup stages
+// windows of sorted keys in LDS and each thread binary-searches its targets
+// there.  Ids are sorted ranks, so the table equals a hash-built one entry
+// for entry.  Packed keys use the plan's fields; wide keys (W words of four
+// int16+32768 fields, coordinate 0 most significant) their 16-bit fields.
 constexpr int kNbrItems = 8;
 constexpr int kNbrBlock = kBlock * kNbrItems;
-constexpr int kNbrWindow = 4096;
+constexpr int kNbrWindowWords = 4096;
 
-// First index in s[0, n) whose value is >= x (n if none), by wave 0 of the
-// workgroup: 64 probes per round narrow the range 64-fold.
-__device__ __forceinline__ unsigned wave_lower_bound(const uint64_t *__restrict__ s, unsigned n, uint64_t x) {
+template <int W>
+__device__ __forceinline__ bool key_less(const uint64_t *a, const uint64_t (&b)[W]) {
+#pragma unroll
+    for (int w = 0; w < W; w++)
+        if (a[w] != b[w]) return a[w] < b[w];
+    return false;
+}
+
+template <int W>
+__device__ __forceinline__ bool key_eq(const uint64_t *a, const uint64_t (&b)[W]) {
+    bool e = true;
+#pragma unroll
+    for (int w = 0; w < W; w++) e = e && a[w] == b[w];
+    return e;
+}
+
+// First index in s[0, n) (W words per key) whose key is >= x (n if none), by
+// one wave: 64 probes per round narrow the range 64-fold.
+template <int W>
+__device__ __forceinline__ unsigned wave_lower_bound(const uint64_t *__restrict__ s, unsigned n,
+                                                     const uint64_t (&x)[W]) {
     const int lane = __lane_id();
     unsigned lo = 0, hi = n;   // answer in [lo, hi]
     while (hi - lo > kWave) {
         const unsigned step = (hi - lo + kWave - 1) / kWave;
         const unsigned p = lo + (unsigned)(lane + 1) * step - 1;
-        const bool in = p < hi;
-        const bool below = in && s[p] < x;
+        const bool below = p < hi && key_less<W>(s + (size_t)p * W, x);
         const int c = __popcll(__ballot(below));   // probes below x form a prefix
         const unsigned nlo = c == 0 ? lo : lo + (unsigned)c * step;
         const unsigned cp = lo + (unsigned)(c + 1) * step - 1;
@@ -540,60 +519,102 @@ __device__ __forceinline__ unsigned wave_lower_bound(const uint64_t *__restrict_
         lo = nlo;
     }
     const unsigned p = lo + (unsigned)lane;
-    const bool below = p < hi && s[p] < x;
+    const bool below = p < hi && key_less<W>(s + (size_t)p * W, x);
     return lo + (unsigned)__popcll(__ballot(below));
 }
 
-template <int D>
+template <int D, bool PACKED>
 __global__ __launch_bounds__(kBlock) void k_nbr_merge(const uint64_t *__restrict__ sk, KeyPlan kp,
                                                       const Counters *ctr, int stride, int *__restrict__ nbr) {
-    __shared__ uint64_t win[kNbrWindow];
-    __shared__ uint64_t red[2][kBlock / kWave];
+    constexpr int W = KeyOps<D, PACKED>::W;
+    constexpr int WIN = kNbrWindowWords / W;   // keys per LDS window
+    __shared__ uint64_t win[WIN * W];
+    __shared__ uint64_t red[kBlock / kWave][W];
     __shared__ unsigned s_cur;
     const unsigned M = ctr->M;
     const unsigned b0 = blockIdx.x * kNbrBlock;
     if (b0 >= M) return;
     const int j = blockIdx.y >> 1, sgn = blockIdx.y & 1;   // sgn 0: k + 1 - (D+1)e_j, 1: k - 1 + (D+1)e_j
-    uint64_t one = 0;
-#pragma unroll
-    for (int q = 0; q < D; q++) one += 1ull << kp.shift[q];
-    const uint64_t d1 = j < D ? one - ((uint64_t)(D + 1) << kp.shift[j < D ? j : 0]) : one;
-    const uint64_t delta = sgn ? (0ull - d1) : d1;
 
-    uint64_t tgt[kNbrItems];
+    // per-word deltas of the move
+    uint64_t delta[W];
+#pragma unroll
+    for (int w = 0; w < W; w++) delta[w] = 0;
+#pragma unroll
+    for (int q = 0; q < D; q++) {
+        const int64_t dq = (q == j) ? -(int64_t)D : 1;
+        const int64_t v = sgn ? -dq : dq;
+        if (PACKED) delta[0] += (uint64_t)v << kp.shift[q];
+        else delta[q / 4] += (uint64_t)v << (48 - 16 * (q % 4));
+    }
+
+    uint64_t tgt[kNbrItems][W];
     bool pend[kNbrItems];
     int res[kNbrItems];
-    uint64_t tmin = ~0ull;
+    uint64_t tmin[W];
+#pragma unroll
+    for (int w = 0; w < W; w++) tmin[w] = ~0ull;
 #pragma unroll
     for (int r = 0; r < kNbrItems; r++) {
         const unsigned i = b0 + r * kBlock + threadIdx.x;
         bool ok = i < M;
-        const uint64_t k = ok ? sk[i] : 0ull;
+        uint64_t k[W];
+#pragma unroll
+        for (int w = 0; w < W; w++) k[w] = ok ? sk[(size_t)i * W + w] : 0ull;
 #pragma unroll
         for (int q = 0; q < D; q++) {
-            const uint32_t f = (uint32_t)((k >> kp.shift[q]) & kp.span[q]);
-            if (q == j) ok = ok && (sgn ? f + D <= kp.span[q] : f >= (uint32_t)D);
-            else ok = ok && (sgn ? f >= 1u : f + 1u <= kp.span[q]);
+            uint32_t f, lo, hi;
+            if (PACKED) {
+                f = (uint32_t)((k[0] >> kp.shift[q]) & kp.span[q]);
+                lo = 0;
+                hi = kp.span[q];
+            } else {
+                f = (uint32_t)((k[q / 4] >> (48 - 16 * (q % 4))) & 0xffff);
+                lo = 1 + 2 * (D + 1);            // the int16 key range of from_coords
+                hi = 65535 - 2 * (D + 1);
+            }
+            const bool up = (q == j) == (sgn != 0);   // this field grows
+            const uint32_t step = (q == j) ? (uint32_t)D : 1u;
+            ok = ok && (up ? f + step <= hi : f >= lo + step);
         }
-        tgt[r] = k + delta;   // exact when every field stays in range
+#pragma unroll
+        for (int w = 0; w < W; w++) tgt[r][w] = k[w] + delta[w];   // exact when every field stays in range
         pend[r] = ok;
         res[r] = -1;
-        if (ok) tmin = tgt[r] < tmin ? tgt[r] : tmin;
+        if (ok && key_less<W>(tgt[r], tmin)) {
+#pragma unroll
+            for (int w = 0; w < W; w++) tmin[w] = tgt[r][w];
+        }
     }
     // workgroup minimum of the valid targets
 #pragma unroll
     for (int off = kWave / 2; off > 0; off >>= 1) {
-        const uint64_t o = __shfl_xor(tmin, off, kWave);
-        tmin = o < tmin ? o : tmin;
-    }
-    if (__lane_id() == 0) red[0][threadIdx.x / kWave] = tmin;
-    __syncthreads();
-    tmin = red[0][0];
+        uint64_t o[W];
 #pragma unroll
-    for (int w = 1; w < kBlock / kWave; w++) tmin = red[0][w] < tmin ? red[0][w] : tmin;
-    if (tmin != ~0ull) {
+        for (int w = 0; w < W; w++) o[w] = __shfl_xor(tmin[w], off, kWave);
+        if (key_less<W>(o, tmin)) {
+#pragma unroll
+            for (int w = 0; w < W; w++) tmin[w] = o[w];
+        }
+    }
+    if (__lane_id() == 0) {
+#pragma unroll
+        for (int w = 0; w < W; w++) red[threadIdx.x / kWave][w] = tmin[w];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int w = 0; w < W; w++) tmin[w] = red[0][w];
+    for (int v = 1; v < kBlock / kWave; v++)
+        if (key_less<W>(red[v], tmin)) {
+#pragma unroll
+            for (int w = 0; w < W; w++) tmin[w] = red[v][w];
+        }
+    bool any = false;
+#pragma unroll
+    for (int w = 0; w < W; w++) any = any || tmin[w] != ~0ull;
+    if (any) {
         if (threadIdx.x < kWave) {
-            const unsigned lb = wave_lower_bound(sk, M, tmin);
+            const unsigned lb = wave_lower_bound<W>(sk, M, tmin);
             if (threadIdx.x == 0) s_cur = lb;
         }
         __syncthreads();
@@ -602,26 +623,25 @@ __global__ __launch_bounds__(kBlock) void k_nbr_merge(const uint64_t *__restrict
         // cur == M (every target beyond the largest key): nothing to find.
         unsigned cur = s_cur;
         while (cur < M) {
-            const unsigned wlen = min((unsigned)kNbrWindow, M - cur);
-            for (unsigned t = threadIdx.x; t < wlen; t += kBlock) win[t] = sk[cur + t];
+            const unsigned wlen = min((unsigned)WIN, M - cur);
+            for (unsigned t = threadIdx.x; t < wlen * W; t += kBlock) win[t] = sk[(size_t)cur * W + t];
             __syncthreads();
             const bool last_window = cur + wlen >= M;
-            const uint64_t last = win[wlen - 1];
             bool more = false;
 #pragma unroll
             for (int r = 0; r < kNbrItems; r++) {
                 if (!pend[r]) continue;
-                if (!last_window && tgt[r] > last) {
-                    more = true;
+                if (!last_window && key_less<W>(win + (size_t)(wlen - 1) * W, tgt[r]) ) {
+                    more = true;   // beyond this window's last key
                     continue;
                 }
                 unsigned a = 0, b = wlen;   // lower bound in win[0, wlen)
                 while (a < b) {
                     const unsigned m = (a + b) >> 1;
-                    if (win[m] < tgt[r]) a = m + 1;
+                    if (key_less<W>(win + (size_t)m * W, tgt[r])) a = m + 1;
                     else b = m;
                 }
-                if (a < wlen && win[a] == tgt[r]) res[r] = (int)(cur + a);
+                if (a < wlen && key_eq<W>(win + (size_t)a * W, tgt[r])) res[r] = (int)(cur + a);
                 pend[r] = false;
             }
             if (!__syncthreads_or(more ? 1 : 0)) break;

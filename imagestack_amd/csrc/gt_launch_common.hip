@@ -5,11 +5,10 @@
 namespace gt {
 
 // ---------------------------------------------------------------------------
-// Per-plane min/max of the finite values and a non-finite indicator (for the
-// key packing plan and the fixed-point value scales).
-// Per-plane bounds of two groups of planes (A: nA planes at stride csA, B: nB
-// planes at stride csB; n pixels each): min, max and a non-finite indicator
-// per plane and workgroup in partial[blk][plane][3].
+// Per-plane bounds (for the key packing plan and the fixed-point value
+// scales) of two groups of planes (A: nA planes at stride csA, B: nB planes
+// at stride csB; n pixels each): min and max of the finite values and a
+// non-finite indicator per plane and workgroup in partial[blk][plane][3].
 __global__ __launch_bounds__(kBlock) void k_minmax(const float *__restrict__ pa, int64_t csa, int na,
                                                    const float *__restrict__ pb, int64_t csb, int nb, int n,
                                                    float *__restrict__ partial) {
@@ -173,7 +172,6 @@ __global__ __launch_bounds__(kBlock) void k_clear(uint64_t *__restrict__ tkeys, 
     }
 }
 
-// Exclusive scan of n uint32 in one workgroup of 1024; the total goes to *total.
 // In-place exclusive scan of n counts in one workgroup; the total goes to
 // *total, or 0 when `flags` carries kFlagFull (the call is redone with a
 // larger table, so everything after the compaction sees no vertices).
@@ -309,22 +307,16 @@ hipError_t launch_finalize(int words, int storage, const TableArgs &ta, const Va
     return hipGetLastError();
 }
 
-hipError_t launch_neighbors(int d, bool packed, const TableArgs &ta, const KeyPlan &kp, const uint64_t *vkeys,
-                            const Counters *ctr, int m_max, int2 *nbr, Counters *wctr, bool count, int wpc,
-                            hipStream_t st) {
+// Neighbour table by merging with the sorted keys (no hash lookups).
+hipError_t launch_neighbors(int d, bool packed, const KeyPlan &kp, const uint64_t *vkeys, const Counters *ctr,
+                            int m_max, int2 *nbr, int wpc, hipStream_t st) {
     if (m_max <= 0) return hipSuccess;
-    const int cnt = count ? 1 : 0;
-    const dim3 g(blocks_for(m_max, kBlock), d + 1), b(kBlock);
-    const unsigned pad = occupancy_pad(0, wpc);
-    // Packed keys: merge with the sorted keys (no hash lookups); wide keys:
-    // hash lookups.
-    const dim3 gm(blocks_for(m_max, kNbrBlock), 2 * (d + 1));
-    const unsigned padm = occupancy_pad(kNbrWindow * 8 + 64, wpc);
-#define GT_NBR(D)                                                                                            \
-    {                                                                                                        \
-        if (packed) hipLaunchKernelGGL((k_nbr_merge<D>), gm, b, padm, st, vkeys, kp, ctr, m_max, (int *)nbr); \
-        else hipLaunchKernelGGL((k_neighbors<D, false>), g, b, pad, st, ta, kp, vkeys, ctr, m_max, nbr, wctr, \
-                                cnt);                                                                         \
+    const dim3 g(blocks_for(m_max, kNbrBlock), 2 * (d + 1)), b(kBlock);
+    const unsigned pad = occupancy_pad(kNbrWindowWords * 8 + 256, wpc);
+#define GT_NBR(D)                                                                                                \
+    {                                                                                                            \
+        if (packed) hipLaunchKernelGGL((k_nbr_merge<D, true>), g, b, pad, st, vkeys, kp, ctr, m_max, (int *)nbr);  \
+        else hipLaunchKernelGGL((k_nbr_merge<D, false>), g, b, pad, st, vkeys, kp, ctr, m_max, (int *)nbr);        \
     }
     GT_DISPATCH_D(d, GT_NBR)
 #undef GT_NBR
