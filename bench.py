@@ -43,7 +43,7 @@ def kernel_bytes(kind: str, N: int, M: int, d: int, Cr: int, C: int, sv: int) ->
 
 
 PMC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_jointbilateral_2048_f32.txt")
-PMC_KERNEL = {"splat": "k_splat_tile", "slice": "k_slice_tile", "blur_pass": "k_blur", "nbr": "k_neighbors"}
+PMC_KERNEL = {"splat": "k_splat_tile", "slice": "k_slice_tile", "blur_pass": "k_blur", "nbr": "k_nbr_merge"}
 
 
 def pmc_traffic():
