@@ -82,6 +82,7 @@ struct gt_ctx {
     DevBuf tkeys, tacc, ctr, counts, slot_list, slot_id, pk_a, pk_b, pay_b, hist, sort_status, sort_hist, vkeys,
         vals_a, vals_b, nbr, splat_copy, mm_part, mm_out;
     DevBuf stage_in, stage_out, view_a, view_b, view_o;
+    DevBuf grid_acc, grid_a, grid_b;   // bilateral grid: fixed-point sums, FP32 ping-pong
     gt::Counters *h_ctr = nullptr;   // pinned
     float *h_mm = nullptr;           // pinned, 3 * 2 * kMaxDH
     // capacity memory: last lattice size seen for (n, d)
@@ -147,6 +148,7 @@ struct Tune {
     int tile_w = 32;
     int wpc = 0;        // waves-per-CU cap of the tiled and neighbour kernels (0: none)
     bool graph = true;  // replay the pipeline as a cached hipGraph
+    bool grid = false;  // the bilateral grid instead of the lattice (GRID entry points only)
 };
 
 Tune resolve(const gt_tuning *t) {
@@ -575,6 +577,115 @@ int check_storage(gt_storage s) {
 
 // The lattice part of JointBilateral on a dense view: positions from the
 // guide, values [im, 1], normalised output (src/GaussTransform.cpp:540-677).
+// The bilateral grid (GaussTransform::apply, GRID case,
+// src/GaussTransform.cpp:227-297): bounds of position * sf over the splat
+// (and a distinct slice) positions, grid sizes ceil(max - min) + 1
+// (src/DenseGrid.h:159-169), fixed-point scales from the value bounds, then
+// zero / splat / finish / blur / slice on the device.  `slice` equal to
+// `src` slices at the splat positions.
+int run_grid(gt_ctx *c, const gt::PixSrc &src, int n, int d, int vd, const gt::PixSrc &slice, int n_slice,
+             const gt::OutDst &dst, bool bilat, gt_stats *st) {
+    if (d > gt::kMaxGridD)
+        return fail(GT_EINVAL, "the bilateral grid takes at most %d position channels (got %d)", gt::kMaxGridD, d);
+    const bool same = !bilat && (slice.pos == src.pos) && n_slice == n;
+    const int np = bilat ? src.cr : d;
+    const float *planes = bilat ? src.ref : src.pos;
+    const int64_t pcs = bilat ? src.ref_cs : src.pos_cs;
+    const int nval = src.nval;
+    const int nblk = (int)std::min<int64_t>(1024, std::max<int64_t>(1, ((int64_t)std::max(n, n_slice) + 4095) / 4096));
+    GT_TRY(ensure(c->mm_part, (size_t)nblk * (np + nval + d) * 3 * 4));
+    GT_TRY(ensure(c->mm_out, (size_t)(np + nval + d) * 3 * 4));
+    float *part = (float *)c->mm_part.p, *mout = (float *)c->mm_out.p;
+    if (n > 0)
+        GT_HIP(gt::launch_minmax(planes, pcs, np, src.val, src.val_cs, nval, n, part, nblk, mout, c->stream));
+    const bool sep = !bilat && !same && n_slice > 0;
+    if (sep)
+        GT_HIP(gt::launch_minmax(slice.pos, slice.pos_cs, d, nullptr, 0, 0, n_slice, part, nblk,
+                                 mout + (size_t)(np + nval) * 3, c->stream));
+    GT_HIP(hipMemcpyAsync(c->h_mm, mout, (size_t)(np + nval + (sep ? d : 0)) * 3 * 4, hipMemcpyDeviceToHost,
+                          c->stream));
+    GT_HIP(hipStreamSynchronize(c->stream));
+    const float *mm = c->h_mm, *vm = c->h_mm + (size_t)np * 3, *sm = c->h_mm + (size_t)(np + nval) * 3;
+    for (int k = 0; k < (n > 0 ? nval : 0); k++)
+        if (vm[3 * k + 2] != 0.0f) return fail(GT_ERANGE, "value channel %d holds NaN or Inf", k);
+    gt::GridArgs g{};
+    g.vd = vd;
+    g.sf = sqrtf((float)(1.0 / 3 + (5 - 1) * 0.25));   // taps = 5 (src/GaussTransform.cpp:229)
+    float mx[gt::kMaxGridD];
+    for (int i = 0; i < d; i++) {
+        float lo = INFINITY, hi = -INFINITY;
+        auto take = [&](float raw_lo, float raw_hi, float scale, bool bad) -> bool {
+            if (bad || !(raw_lo <= raw_hi)) return false;
+            const float a = (raw_lo * scale) * g.sf, b = (raw_hi * scale) * g.sf;
+            lo = std::min(lo, std::min(a, b));
+            hi = std::max(hi, std::max(a, b));
+            return true;
+        };
+        bool ok = true;
+        if (n > 0) {
+            if (bilat && i >= src.cr) {
+                // x * (1/sigma_x) etc. over the image extent
+                const int k = i - src.cr;
+                int ext[3], m = 0;
+                float inv[3];
+                if (src.fx) { ext[m] = src.W; inv[m++] = src.ix; }
+                if (src.fy) { ext[m] = src.H; inv[m++] = src.iy; }
+                if (src.ft) { ext[m] = n / (src.W * src.H); inv[m++] = src.it; }
+                ok = take(0.0f, (float)(ext[k] - 1), inv[k], false);
+            } else {
+                ok = take(mm[3 * i], mm[3 * i + 1], bilat ? src.ic : src.inv[i], mm[3 * i + 2] != 0.0f);
+            }
+        }
+        if (ok && sep) ok = take(sm[3 * i], sm[3 * i + 1], slice.inv[i], sm[3 * i + 2] != 0.0f);
+        if (!ok || !std::isfinite(lo) || !std::isfinite(hi))
+            return fail(GT_ERANGE, "position channel %d holds NaN or Inf", i);
+        g.mn[i] = lo;
+        mx[i] = hi;
+    }
+    int64_t cells = 1;
+    for (int i = 0; i < d; i++) {
+        const float ext = std::ceil(mx[i] - g.mn[i]) + 1;
+        if (!(ext <= 32767.0f))
+            return fail(GT_ERANGE, "grid dimension %d spans %g cells (more than int16 indices)", i, (double)ext);
+        g.size[i] = (int)ext;
+        g.stride[i] = (i == 0) ? vd : g.stride[i - 1] * g.size[i - 1];
+        cells *= g.size[i];
+        if (cells > ((int64_t)1 << 34)) return fail(GT_ENOMEM, "bilateral grid of more than 2^34 cells");
+    }
+    gt::ValScale vsc{};
+    for (int k = 0; k < vd; k++) {
+        const float vmax = k < nval ? std::max(std::fabs(vm[3 * k]), std::fabs(vm[3 * k + 1])) : 1.0f;
+        const double bound = std::max(1e-30, (double)std::max(n, 1) * (double)vmax * 1.001);
+        const int p = std::min(120, 62 - (int)std::ceil(std::log2(bound)));
+        if (p < -120) return fail(GT_ERANGE, "values of channel %d are too large", k);
+        vsc.s[k] = std::ldexp(1.0f, p);
+        vsc.inv_s[k] = std::ldexp(1.0f, -p);
+        vsc.vmax[k] = vmax;
+    }
+    const size_t total = (size_t)cells * vd;
+    GT_TRY(ensure(c->grid_acc, total * 8));
+    GT_TRY(ensure(c->grid_a, total * 4));
+    GT_TRY(ensure(c->grid_b, total * 4));
+    GT_HIP(gt::launch_grid(d, bilat, src, n, slice, n_slice, dst, g, cells, vsc,
+                           (unsigned long long *)c->grid_acc.p, (float *)c->grid_a.p, (float *)c->grid_b.p,
+                           c->profiling ? c->ev : nullptr, c->stream));
+    GT_HIP(hipStreamSynchronize(c->stream));
+    if (st) {
+        st->M = (int)std::min<int64_t>(cells, 0x7fffffff);
+        st->capacity = st->M;
+        if (c->profiling) {
+            st->ms_splat = elapsed(c->ev[0], c->ev[1]);
+            st->ms_sort = elapsed(c->ev[1], c->ev[2]);
+            st->ms_nbr = 0.0f;
+            st->ms_blur = elapsed(c->ev[3], c->ev[4]);
+            st->ms_slice = elapsed(c->ev[4], c->ev[5]);
+            st->ms_total = elapsed(c->ev[0], c->ev[5]);
+        }
+    }
+    c->have = false;   // no lattice to export
+    return GT_OK;
+}
+
 int jb_dense(gt_ctx *c, int W, int H, int F, int C, const float *im, int64_t im_cs, int Cr, const float *ref,
              int64_t ref_cs, float fw, float fh, float ff, float color_sigma, float *out, int64_t out_cs,
              int storage, const Tune &tu, gt_stats *st) {
@@ -609,6 +720,7 @@ int jb_dense(gt_ctx *c, int W, int H, int F, int C, const float *im, int64_t im_
     sl.src = src;
     sl.n = n;
     sl.dst = gt::OutDst{out, out_cs, 1};
+    if (tu.grid) return run_grid(c, src, n, d, vd, src, n, sl.dst, true, st);
     return run(c, src, n, W, d, vd, storage, tu, sl, false, st);
 }
 
@@ -725,7 +837,7 @@ void gt_ctx_destroy(gt_ctx *c) {
                       &c->pk_a,        &c->pk_b,      &c->pay_b,    &c->hist,      &c->sort_status,
                       &c->sort_hist,   &c->vkeys,     &c->vals_a,   &c->vals_b,    &c->nbr,       &c->splat_copy,
                       &c->mm_part,     &c->mm_out,    &c->stage_in, &c->stage_out, &c->view_a,    &c->view_b,
-                      &c->view_o};
+                      &c->view_o,      &c->grid_acc,  &c->grid_a,   &c->grid_b};
     for (DevBuf *b : bufs) free_buf(*b);
     for (auto &e : c->ev)
         if (e) (void)hipEventDestroy(e);
@@ -814,9 +926,9 @@ int gt_gauss_transform_host(gt_ctx *c, int n, int row_w, int d, int vd, const fl
     return GT_OK;
 }
 
-int gt_joint_bilateral(gt_ctx *c, int W, int H, int F, int C, const float *im, int Cr, const float *ref, float fw,
-                       float fh, float ff, float color_sigma, float *out, gt_storage storage, const gt_tuning *tune,
-                       gt_stats *st) {
+static int jb_entry(gt_ctx *c, int W, int H, int F, int C, const float *im, int Cr, const float *ref, float fw,
+                    float fh, float ff, float color_sigma, float *out, gt_storage storage, const gt_tuning *tune,
+                    gt_stats *st, bool grid) {
     if (!c) return fail(GT_EINVAL, "null context");
     if (W < 1 || H < 1 || F < 1 || C < 1 || Cr < 1) return fail(GT_EINVAL, "bad image size %dx%dx%d", W, H, F);
     if (!im || !ref || !out) return fail(GT_EINVAL, "null image pointer");
@@ -824,16 +936,28 @@ int gt_joint_bilateral(gt_ctx *c, int W, int H, int F, int C, const float *im, i
     GT_TRY(check_storage(storage));
     GT_HIP(hipSetDevice(c->device));
     if (st) memset(st, 0, sizeof(*st));
-    const Tune tu = resolve(tune);
+    Tune tu = resolve(tune);
+    tu.grid = grid;
     const int64_t n = (int64_t)W * H * F;
     View a{im, W, H, F, W, (int64_t)W * H, n}, b{ref, W, H, F, W, (int64_t)W * H, n},
         o{out, W, H, F, W, (int64_t)W * H, n};
     return jb_view(c, a, C, b, Cr, o, fw, fh, ff, color_sigma, storage, tu, st);
 }
 
-int gt_joint_bilateral_host(gt_ctx *c, int W, int H, int F, int C, const float *im, int Cr, const float *ref,
-                            float fw, float fh, float ff, float color_sigma, float *out, gt_storage storage,
-                            const gt_tuning *tune, gt_stats *st) {
+int gt_joint_bilateral(gt_ctx *c, int W, int H, int F, int C, const float *im, int Cr, const float *ref, float fw,
+                       float fh, float ff, float color_sigma, float *out, gt_storage storage, const gt_tuning *tune,
+                       gt_stats *st) {
+    return jb_entry(c, W, H, F, C, im, Cr, ref, fw, fh, ff, color_sigma, out, storage, tune, st, false);
+}
+
+int gt_joint_bilateral_grid(gt_ctx *c, int W, int H, int F, int C, const float *im, int Cr, const float *ref,
+                            float fw, float fh, float ff, float color_sigma, float *out, gt_stats *st) {
+    return jb_entry(c, W, H, F, C, im, Cr, ref, fw, fh, ff, color_sigma, out, GT_STORE_F32, nullptr, st, true);
+}
+
+static int jb_host_entry(gt_ctx *c, int W, int H, int F, int C, const float *im, int Cr, const float *ref,
+                         float fw, float fh, float ff, float color_sigma, float *out, gt_storage storage,
+                         const gt_tuning *tune, gt_stats *st, bool grid) {
     if (!c) return fail(GT_EINVAL, "null context");
     if (W < 1 || H < 1 || F < 1 || C < 1 || Cr < 1) return fail(GT_EINVAL, "bad image size %dx%dx%d", W, H, F);
     if (!im || !ref || !out) return fail(GT_EINVAL, "null image pointer");
@@ -844,9 +968,77 @@ int gt_joint_bilateral_host(gt_ctx *c, int W, int H, int F, int C, const float *
     float *di = (float *)c->stage_in.p, *dr = di + n * C;
     GT_HIP(hipMemcpyAsync(di, im, n * C * 4, hipMemcpyHostToDevice, c->stream));
     GT_HIP(hipMemcpyAsync(dr, ref, n * Cr * 4, hipMemcpyHostToDevice, c->stream));
-    GT_TRY(gt_joint_bilateral(c, W, H, F, C, di, Cr, dr, fw, fh, ff, color_sigma, (float *)c->stage_out.p, storage,
-                              tune, st));
+    GT_TRY(jb_entry(c, W, H, F, C, di, Cr, dr, fw, fh, ff, color_sigma, (float *)c->stage_out.p, storage, tune, st,
+                    grid));
     GT_HIP(hipMemcpy(out, c->stage_out.p, n * C * 4, hipMemcpyDeviceToHost));
+    return GT_OK;
+}
+
+int gt_joint_bilateral_host(gt_ctx *c, int W, int H, int F, int C, const float *im, int Cr, const float *ref,
+                            float fw, float fh, float ff, float color_sigma, float *out, gt_storage storage,
+                            const gt_tuning *tune, gt_stats *st) {
+    return jb_host_entry(c, W, H, F, C, im, Cr, ref, fw, fh, ff, color_sigma, out, storage, tune, st, false);
+}
+
+int gt_joint_bilateral_grid_host(gt_ctx *c, int W, int H, int F, int C, const float *im, int Cr, const float *ref,
+                                 float fw, float fh, float ff, float color_sigma, float *out, gt_stats *st) {
+    return jb_host_entry(c, W, H, F, C, im, Cr, ref, fw, fh, ff, color_sigma, out, GT_STORE_F32, nullptr, st, true);
+}
+
+int gt_gauss_transform_grid(gt_ctx *c, int n, int d, int vd, const float *splat, const float *values, int n_slice,
+                            const float *slice, const float *inv_sigma, float *out, gt_stats *st) {
+    if (!c) return fail(GT_EINVAL, "null context");
+    GT_TRY(check_dims(d, vd));
+    if (n < 0 || n_slice < 0) return fail(GT_EINVAL, "negative pixel count");
+    if (n > 0 && (!splat || !values || !inv_sigma)) return fail(GT_EINVAL, "null input pointer");
+    if (slice == splat) slice = nullptr;
+    const int ns = slice ? n_slice : n;
+    if (ns > 0 && !out) return fail(GT_EINVAL, "null output pointer");
+    if (slice && n_slice > 0 && !inv_sigma) return fail(GT_EINVAL, "null inverse sigmas");
+    GT_HIP(hipSetDevice(c->device));
+    if (st) memset(st, 0, sizeof(*st));
+    if (n == 0) {
+        if (ns > 0) GT_HIP(hipMemsetAsync(out, 0, (size_t)ns * vd * 4, c->stream));
+        GT_HIP(hipStreamSynchronize(c->stream));
+        return GT_OK;
+    }
+    gt::PixSrc src{};
+    src.pos = splat;
+    src.pos_cs = n;
+    for (int i = 0; i < d; i++) src.inv[i] = inv_sigma[i];
+    src.val = values;
+    src.val_cs = n;
+    src.nval = vd;
+    gt::PixSrc sl = src;
+    if (slice) {
+        sl.pos = slice;
+        sl.pos_cs = ns;
+    }
+    return run_grid(c, src, n, d, vd, sl, ns, gt::OutDst{out, (int64_t)ns, 0}, false, st);
+}
+
+int gt_gauss_transform_grid_host(gt_ctx *c, int n, int d, int vd, const float *splat, const float *values,
+                                 int n_slice, const float *slice, const float *inv_sigma, float *out,
+                                 gt_stats *st) {
+    if (!c) return fail(GT_EINVAL, "null context");
+    GT_TRY(check_dims(d, vd));
+    if (n < 0 || n_slice < 0) return fail(GT_EINVAL, "negative pixel count");
+    const bool replay = (slice == nullptr || slice == splat);
+    const int ns = replay ? n : n_slice;
+    GT_HIP(hipSetDevice(c->device));
+    const size_t in_floats = (size_t)n * (d + vd) + (replay ? 0 : (size_t)ns * d);
+    GT_TRY(ensure(c->stage_in, in_floats * 4));
+    GT_TRY(ensure(c->stage_out, (size_t)ns * vd * 4));
+    float *dsp = (float *)c->stage_in.p;
+    float *dva = dsp + (size_t)n * d;
+    float *dsl = replay ? nullptr : dva + (size_t)n * vd;
+    if (n) {
+        GT_HIP(hipMemcpyAsync(dsp, splat, (size_t)n * d * 4, hipMemcpyHostToDevice, c->stream));
+        GT_HIP(hipMemcpyAsync(dva, values, (size_t)n * vd * 4, hipMemcpyHostToDevice, c->stream));
+    }
+    if (!replay && ns) GT_HIP(hipMemcpyAsync(dsl, slice, (size_t)ns * d * 4, hipMemcpyHostToDevice, c->stream));
+    GT_TRY(gt_gauss_transform_grid(c, n, d, vd, dsp, dva, ns, dsl, inv_sigma, (float *)c->stage_out.p, st));
+    if (ns) GT_HIP(hipMemcpy(out, c->stage_out.p, (size_t)ns * vd * 4, hipMemcpyDeviceToHost));
     return GT_OK;
 }
 

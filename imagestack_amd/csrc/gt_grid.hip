@@ -1,0 +1,202 @@
+// Bilateral grid (the GRID method of GaussTransform::apply,
+// src/GaussTransform.cpp:227-297, on src/DenseGrid.h with 5 taps), for
+// gfx950.  Positions scaled by sf = sqrtf(1/3 + 1) index a dense grid of
+// ceil(max - min) + 1 cells per dimension with vd interleaved values.
+//
+//   k_grid_splat  : one thread per pixel: cell indices and fractions in FP32
+//                   exactly as the reference, 2^D corner weights multiplied in
+//                   dimension order, and exact 64-bit fixed-point atomics (the
+//                   sums do not depend on the order of the atomics).
+//   k_grid_finish : fixed point -> FP32.
+//   k_grid_blur   : one element per thread, one of the 2 rounds of the
+//                   [1/4 1/2 1/4] blur along one dimension; each output is the
+//                   reference's in-place row recurrence written out for that
+//                   element (it only ever reads the row's previous values),
+//                   with the same FP32 roundings and the same double-precision
+//                   step for the last element of a row.
+//   k_grid_slice  : one thread per pixel: the same corners in the same order,
+//                   FP32 sums, then (JointBilateral) division by the
+//                   homogeneous channel.
+// A corner past the end of a dimension has weight exactly 0 (the fraction is
+// 0 there) and is skipped.
+#include "gt_kernels.h"
+
+#include <algorithm>
+#include <utility>
+
+namespace gt {
+
+// Cell and fractions of pixel p (src/DenseGrid.h:300-305).  The bounds are
+// the minimum and maximum of these same products, so every index lies in
+// its dimension; a pixel that does not (which would take a bound computed
+// from other values) is reported as outside and skipped.
+template <int D, bool BILAT>
+__device__ __forceinline__ bool grid_cell(const PixSrc &src, int p, const GridArgs &g, int64_t &base,
+                                          float (&F)[D], float (&Fi)[D], int (&I)[D]) {
+    float pos[D];
+    load_position<D, BILAT>(src, p, pos);
+    base = 0;
+    bool in = true;
+#pragma unroll
+    for (int i = 0; i < D; i++) {
+        const float f = pos[i] * g.sf - g.mn[i];
+        I[i] = (short)floorf(f);
+        F[i] = f - (float)I[i];
+        Fi[i] = 1 - F[i];
+        in = in && I[i] >= 0 && I[i] < g.size[i];
+        base += (int64_t)I[i] * g.stride[i];
+    }
+    return in;
+}
+
+template <int D, bool BILAT>
+__global__ __launch_bounds__(kBlock) void k_grid_splat(PixSrc src, int n, GridArgs g, ValScale vsc,
+                                                       unsigned long long *__restrict__ acc) {
+    const int p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= n) return;
+    int64_t base;
+    float F[D], Fi[D];
+    int I[D];
+    if (!grid_cell<D, BILAT>(src, p, g, base, F, Fi, I)) return;
+    float v[kMaxVD];
+    load_values<kMaxVD>(src, p, v);
+    for (int c = 0; c < (1 << D); c++) {
+        float w = 1;
+        int64_t idx = base;
+        bool inside = true;
+#pragma unroll
+        for (int j = 0; j < D; j++) {
+            if (c & (1 << j)) {
+                idx += g.stride[j];
+                w *= F[j];
+                inside = inside && I[j] + 1 < g.size[j];
+            } else {
+                w *= Fi[j];
+            }
+        }
+        if (!inside) continue;
+#pragma unroll
+        for (int k = 0; k < kMaxVD; k++)
+            if (k < g.vd) atomicAdd(acc + idx + k, to_fixed(w * v[k], vsc.s[k]));
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_grid_zero(unsigned long long *__restrict__ acc, int64_t total) {
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < total; i += (int64_t)gridDim.x * kBlock)
+        acc[i] = 0ull;
+}
+
+__global__ __launch_bounds__(kBlock) void k_grid_finish(const unsigned long long *__restrict__ acc, int64_t total,
+                                                        int vd, ValScale vsc, float *__restrict__ out) {
+    const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (e >= total) return;
+    const int k = (int)(e % vd);
+    out[e] = (float)((double)(long long)acc[e] * (double)vsc.inv_s[k]);
+}
+
+// One round along a dimension with element stride s and n cells:
+//   first   : t1 = x/2
+//   others  : t1 = 0.5 * (prev + x)          (the previous step's average)
+//   not last: out = 0.5 * (t1 + 0.5 * (x + next))
+//   last    : out = 0.5 * (t1 + 0.5 * x), in double (src/DenseGrid.h:268-271)
+__global__ __launch_bounds__(kBlock) void k_grid_blur(const float *__restrict__ in, float *__restrict__ out,
+                                                      int64_t total, int64_t s, int n) {
+    const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (e >= total) return;
+    const int i = (int)((e / s) % n);
+    const float x = in[e];
+    const float t1 = i == 0 ? x / 2 : 0.5f * (in[e - s] + x);
+    float r;
+    if (i < n - 1) {
+        const float t2 = 0.5f * (x + in[e + s]);
+        r = 0.5f * (t1 + t2);
+    } else {
+        r = (float)(0.5 * ((double)t1 + 0.5 * (double)x));
+    }
+    out[e] = r;
+}
+
+template <int D, bool BILAT>
+__global__ __launch_bounds__(kBlock) void k_grid_slice(PixSrc src, int n, GridArgs g,
+                                                       const float *__restrict__ grid, OutDst dst) {
+    const int p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= n) return;
+    int64_t base;
+    float F[D], Fi[D];
+    int I[D];
+    const bool in = grid_cell<D, BILAT>(src, p, g, base, F, Fi, I);
+    float acc[kMaxVD];
+#pragma unroll
+    for (int k = 0; k < kMaxVD; k++) acc[k] = 0.0f;
+    if (!in) {
+        write_pixel<kMaxVD>(dst, p, g.vd, acc);
+        return;
+    }
+    for (int c = 0; c < (1 << D); c++) {
+        float w = 1;
+        int64_t idx = base;
+        bool inside = true;
+#pragma unroll
+        for (int j = 0; j < D; j++) {
+            if (c & (1 << j)) {
+                idx += g.stride[j];
+                w *= F[j];
+                inside = inside && I[j] + 1 < g.size[j];
+            } else {
+                w *= Fi[j];
+            }
+        }
+        if (!inside) continue;
+#pragma unroll
+        for (int k = 0; k < kMaxVD; k++)
+            if (k < g.vd) acc[k] = acc[k] + w * grid[idx + k];
+    }
+    write_pixel<kMaxVD>(dst, p, g.vd, acc);
+}
+
+#define GT_DISPATCH_GRID(d, MACRO)                                                \
+    switch (d) {                                                                  \
+    case 1: MACRO(1); break; case 2: MACRO(2); break; case 3: MACRO(3); break;    \
+    case 4: MACRO(4); break; case 5: MACRO(5); break; case 6: MACRO(6); break;    \
+    default: return hipErrorInvalidValue;                                         \
+    }
+
+hipError_t launch_grid(int d, bool bilat, const PixSrc &splat, int n, const PixSrc &slice, int n_slice,
+                       const OutDst &dst, const GridArgs &g, int64_t cells, const ValScale &vsc,
+                       unsigned long long *acc, float *a, float *b, hipEvent_t *ev, hipStream_t st) {
+    const int64_t total = cells * g.vd;
+    const unsigned zb = (unsigned)std::min<int64_t>(8192, (total + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(k_grid_zero, dim3(zb > 0 ? zb : 1), dim3(kBlock), 0, st, acc, total);
+    if (ev) (void)hipEventRecord(ev[0], st);
+    const dim3 gp((n + kBlock - 1) / kBlock), gs((n_slice + kBlock - 1) / kBlock), blk(kBlock);
+#define GT_GSPLAT(D)                                                                                        \
+    {                                                                                                       \
+        if (bilat) hipLaunchKernelGGL((k_grid_splat<D, true>), gp, blk, 0, st, splat, n, g, vsc, acc);    \
+        else hipLaunchKernelGGL((k_grid_splat<D, false>), gp, blk, 0, st, splat, n, g, vsc, acc);         \
+    }
+    if (n > 0) GT_DISPATCH_GRID(d, GT_GSPLAT)
+#undef GT_GSPLAT
+    if (ev) (void)hipEventRecord(ev[1], st);
+    const dim3 ge((unsigned)((total + kBlock - 1) / kBlock));
+    hipLaunchKernelGGL(k_grid_finish, ge, blk, 0, st, acc, total, g.vd, vsc, a);
+    if (ev) (void)hipEventRecord(ev[2], st);
+    if (ev) (void)hipEventRecord(ev[3], st);
+    float *cur = a, *nxt = b;
+    for (int j = 0; j < d; j++)
+        for (int it = 0; it < 2; it++) {
+            hipLaunchKernelGGL(k_grid_blur, ge, blk, 0, st, cur, nxt, total, g.stride[j], g.size[j]);
+            std::swap(cur, nxt);
+        }
+    if (ev) (void)hipEventRecord(ev[4], st);
+#define GT_GSLICE(D)                                                                                        \
+    {                                                                                                       \
+        if (bilat) hipLaunchKernelGGL((k_grid_slice<D, true>), gs, blk, 0, st, slice, n_slice, g, cur, dst); \
+        else hipLaunchKernelGGL((k_grid_slice<D, false>), gs, blk, 0, st, slice, n_slice, g, cur, dst);     \
+    }
+    if (n_slice > 0) GT_DISPATCH_GRID(d, GT_GSLICE)
+#undef GT_GSLICE
+    if (ev) (void)hipEventRecord(ev[5], st);
+    return hipGetLastError();
+}
+
+} // namespace gt

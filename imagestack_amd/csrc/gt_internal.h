@@ -153,6 +153,21 @@ struct SortBufs {
 size_t sort_state_words();
 size_t sort_status_entries(int m_max, int pw);
 size_t sort_hist_words(int m_max, int pw);
+// Bilateral grid (gt_grid.hip): at most kMaxGridD position dimensions.
+constexpr int kMaxGridD = 6;
+struct GridArgs {
+    int vd;
+    float sf;                     // sqrtf(1/3 + (taps - 1)/4), taps = 5
+    float mn[kMaxGridD];          // minimum of position * sf per dimension
+    int size[kMaxGridD];          // ceil(max - min) + 1 cells
+    int64_t stride[kMaxGridD];    // in elements: stride[0] = vd
+};
+// Zero, splat, finish, 2 blur rounds per dimension, slice.  ev (optional):
+// 6 events recorded at the stage boundaries (splat, -, -, blur, slice).
+hipError_t launch_grid(int d, bool bilat, const PixSrc &splat, int n, const PixSrc &slice, int n_slice,
+                       const OutDst &dst, const GridArgs &g, int64_t cells, const ValScale &vsc,
+                       unsigned long long *acc, float *a, float *b, hipEvent_t *ev, hipStream_t st);
+
 // Sets the thread's gt_last_error() message; returns code.
 int set_error(int code, const char *msg);
 // Forgets the context's cached key-packing plan, so the next call bounds its

@@ -25,6 +25,8 @@ EXPORTS = [
     "gt_build_lattice", "gt_lattice_export", "gt_joint_bilateral_batch",
     "gt_autotune_joint_bilateral", "gt_autotune_joint_bilateral_batch",
     "gt_server_create", "gt_server_destroy", "gt_server_submit", "gt_server_wait", "gt_server_replay",
+    "gt_gauss_transform_grid", "gt_gauss_transform_grid_host", "gt_joint_bilateral_grid",
+    "gt_joint_bilateral_grid_host",
 ]
 
 
@@ -114,6 +116,13 @@ def lib():
                                                   C.POINTER(Tuning), C.POINTER(TuneReport)]
         L.gt_autotune_joint_bilateral_batch.argtypes = [_vp, C.c_int, C.POINTER(BilateralImage), C.c_int, C.c_int,
                                                         C.c_int, C.c_int, C.POINTER(Tuning), C.POINTER(TuneReport)]
+        gsig = [_vp, C.c_int, C.c_int, C.c_int, _vp, _vp, C.c_int, _vp, _fp, _vp, C.POINTER(Stats)]
+        L.gt_gauss_transform_grid.argtypes = gsig
+        L.gt_gauss_transform_grid_host.argtypes = gsig
+        jgsig = [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp, C.c_int, _vp, C.c_float, C.c_float, C.c_float,
+                 C.c_float, _vp, C.POINTER(Stats)]
+        L.gt_joint_bilateral_grid.argtypes = jgsig
+        L.gt_joint_bilateral_grid_host.argtypes = jgsig
         L.gt_server_create.argtypes = [C.POINTER(_vp), C.c_int, C.POINTER(ServerConfig)]
         L.gt_server_destroy.argtypes = [_vp]
         L.gt_server_destroy.restype = None
@@ -214,6 +223,43 @@ class Context:
         _check(lib().gt_joint_bilateral_batch(self._h, len(items), arr, C_, Cr, STORAGE[storage], _tuning(tune),
                                               C.byref(st)))
         return st.as_dict() if stats else None
+
+    def gauss_transform_grid(self, splat, values, inv_sigma, slice_pos=None, out=None, stats: bool = False):
+        """GaussTransform::apply(GRID): splat (d, N) device f32, values (vd, N);
+        returns (vd, Ns) unnormalised."""
+        import torch
+        d, n = splat.shape[0], splat[0].numel()
+        vd = values.shape[0]
+        ns = n if slice_pos is None else slice_pos[0].numel()
+        if out is None:
+            out = torch.empty((vd, ns), dtype=torch.float32, device=splat.device)
+        inv = np.ascontiguousarray(inv_sigma, dtype=np.float32)
+        st = Stats()
+        _check(lib().gt_gauss_transform_grid(self._h, n, d, vd, _dptr(splat), _dptr(values), ns, _dptr(slice_pos),
+                                             inv.ctypes.data_as(_fp), _dptr(out), C.byref(st)))
+        return (out, st.as_dict()) if stats else out
+
+    def joint_bilateral_grid(self, im, ref, fw, fh, ff, color_sigma, out=None, stats: bool = False):
+        """JointBilateral::apply(GRID): im (C, F, H, W) device f32, ref (Cr, F, H, W)."""
+        import torch
+        Cc, F, H, W = im.shape
+        if out is None:
+            out = torch.empty_like(im)
+        st = Stats()
+        _check(lib().gt_joint_bilateral_grid(self._h, W, H, F, Cc, _dptr(im), ref.shape[0], _dptr(ref), fw, fh,
+                                             ff, color_sigma, _dptr(out), C.byref(st)))
+        return (out, st.as_dict()) if stats else out
+
+    def joint_bilateral_grid_host(self, im, ref, fw, fh, ff, color_sigma):
+        im = np.ascontiguousarray(im, dtype=np.float32)
+        ref = np.ascontiguousarray(ref, dtype=np.float32)
+        Cc, F, H, W = im.shape
+        out = np.zeros_like(im)
+        st = Stats()
+        _check(lib().gt_joint_bilateral_grid_host(self._h, W, H, F, Cc, im.ctypes.data, ref.shape[0],
+                                                  ref.ctypes.data, fw, fh, ff, color_sigma, out.ctypes.data,
+                                                  C.byref(st)))
+        return out
 
     @staticmethod
     def _batch_array(items):

@@ -12,6 +12,9 @@
 //   ImageStack::GPU::jointBilateral(im, ref, fw, fh, ff, colorSigma)
 //       drop-in for JointBilateral::apply(..., PERMUTOHEDRAL)
 //       (src/GaussTransform.cpp:504-678): filters im in place.
+//   ImageStack::GPU::gaussTransformGrid / GPU::jointBilateralGrid
+//       the same for the GRID method (the bilateral grid,
+//       src/DenseGrid.h; at most 6 position channels).
 //
 // Library failures become ImageStack::panic() with the library's message,
 // exactly how the reference reports errors (src/Exception.cpp:18-33).
@@ -86,6 +89,30 @@ inline Image gaussTransform(Image slice, Image splat, Image values, std::vector<
     return out;
 }
 
+// GaussTransform::apply(..., GRID) (src/GaussTransform.cpp:227-297) on the GPU.
+inline Image gaussTransformGrid(Image slice, Image splat, Image values, std::vector<float> sigmas) {
+    assert(splat.width == values.width && splat.height == values.height && splat.frames == values.frames,
+           "Weights and locations of the Gaussians must be the same size\n");
+    assert(slice.channels == splat.channels,
+           "The evaluation locations and the locations of the Gaussians must have"
+           " the same number of channels.\n");
+    assert((int)sigmas.size() == splat.channels, "One standard deviation per position channel is required\n");
+    std::vector<float> invSigma(sigmas.size());
+    for (size_t i = 0; i < sigmas.size(); i++) invSigma[i] = 1.0f / sigmas[i];
+    const int n = splat.width * splat.height * splat.frames;
+    const bool replay = (slice == splat);
+    std::vector<float> sp = packPlanar(splat), va = packPlanar(values), sl;
+    if (!replay) sl = packPlanar(slice);
+    const int ns = slice.width * slice.height * slice.frames;
+    Image out(slice.width, slice.height, slice.frames, values.channels);
+    std::vector<float> ob((size_t)ns * values.channels);
+    if (gt_gauss_transform_grid_host(context(), n, splat.channels, values.channels, sp.data(), va.data(), ns,
+                                     replay ? nullptr : sl.data(), invSigma.data(), ob.data(), nullptr) != GT_OK)
+        panic("%s\n", gt_last_error());
+    unpackPlanar(ob, out);
+    return out;
+}
+
 inline void jointBilateral(Image im, Image ref, float filterWidth, float filterHeight, float filterFrames,
                            float colorSigma) {
     assert(im.width == ref.width && im.height == ref.height && im.frames == ref.frames,
@@ -94,6 +121,19 @@ inline void jointBilateral(Image im, Image ref, float filterWidth, float filterH
     if (gt_joint_bilateral_host(context(), im.width, im.height, im.frames, im.channels, a.data(), ref.channels,
                                 b.data(), filterWidth, filterHeight, filterFrames, colorSigma, o.data(),
                                 GT_STORE_F32, nullptr, nullptr) != GT_OK)
+        panic("%s\n", gt_last_error());
+    unpackPlanar(o, im);
+}
+
+// JointBilateral::apply(..., GRID) on the GPU: filters im in place.
+inline void jointBilateralGrid(Image im, Image ref, float filterWidth, float filterHeight, float filterFrames,
+                           float colorSigma) {
+    assert(im.width == ref.width && im.height == ref.height && im.frames == ref.frames,
+           "Image and reference must be the same size\n");
+    std::vector<float> a = packPlanar(im), b = packPlanar(ref), o(a.size());
+    if (gt_joint_bilateral_grid_host(context(), im.width, im.height, im.frames, im.channels, a.data(),
+                                     ref.channels, b.data(), filterWidth, filterHeight, filterFrames, colorSigma,
+                                     o.data(), nullptr) != GT_OK)
         panic("%s\n", gt_last_error());
     unpackPlanar(o, im);
 }

@@ -7,8 +7,9 @@
 // replaces -gausstransform, -jointbilateral and -bilateral with subclasses
 // that keep the reference's argument grammar (src/GaussTransform.cpp:79-113,
 // :473-502, :699-728) and its AUTO method choice (:540-564).  Whenever the
-// method resolves to PERMUTOHEDRAL the lattice runs in libgtlattice.so; the
-// other methods (exact, grid, gkdtree) run the reference's own code.
+// method resolves to PERMUTOHEDRAL the lattice, and to GRID (at most 6
+// position channels) the bilateral grid, runs in libgtlattice.so; the other
+// methods (exact, gkdtree) run the reference's own code.
 //
 // Built against the reference's headers (main.h is force-included, as the
 // reference's own build does), so Image and Operation match the host binary.
@@ -54,8 +55,11 @@ GaussTransform::Method resolveMethod(Image im, Image ref, float fw, float fh, fl
 }
 
 void jointBilateralDispatch(Image im, Image ref, float fw, float fh, float ff, float cs, GaussTransform::Method m) {
-    if (resolveMethod(im, ref, fw, fh, ff, m) == GaussTransform::PERMUTOHEDRAL)
+    const GaussTransform::Method r = resolveMethod(im, ref, fw, fh, ff, m);
+    if (r == GaussTransform::PERMUTOHEDRAL)
         jointBilateral(im, ref, fw, fh, ff, cs);
+    else if (r == GaussTransform::GRID && ref.channels + 3 <= 6)
+        jointBilateralGrid(im, ref, fw, fh, ff, cs);
     else
         JointBilateral::apply(im, ref, fw, fh, ff, cs, m);
 }
@@ -124,8 +128,9 @@ public:
             panic("-gausstransform takes one argument, two arguments, or one plus the"
                   " number of channels in the second image on the stack arguments\n");
         }
-        Image im = (m == PERMUTOHEDRAL) ? gaussTransform(stack(0), stack(1), stack(2), sigmas)
-                                        : GaussTransform::apply(stack(0), stack(1), stack(2), sigmas, m);
+        Image im = (m == PERMUTOHEDRAL)                           ? gaussTransform(stack(0), stack(1), stack(2), sigmas)
+                   : (m == GRID && stack(1).channels <= 6) ? gaussTransformGrid(stack(0), stack(1), stack(2), sigmas)
+                                                           : GaussTransform::apply(stack(0), stack(1), stack(2), sigmas, m);
         pop();
         push(im);
     }

@@ -142,6 +142,34 @@ int gt_joint_bilateral_host(gt_ctx *ctx, int W, int H, int F, int C, const float
                             const gt_tuning *tune, gt_stats *stats);
 
 /*
+ * The bilateral grid (src/DenseGrid.h, 5 taps), the GRID method that
+ * JointBilateral's AUTO choice picks for at most 4 position channels
+ * (src/GaussTransform.cpp:553-564):
+ *   gt_gauss_transform_grid: GaussTransform::apply(slice, splat, values,
+ *     sigmas, GRID) (src/GaussTransform.cpp:227-297); arguments as
+ *     gt_gauss_transform, at most 6 position channels.
+ *   gt_joint_bilateral_grid: JointBilateral::apply(im, ref, fw, fh, ff,
+ *     colorSigma, GRID) (src/GaussTransform.cpp:504-678); arguments as
+ *     gt_joint_bilateral.
+ * Grid cells, fractions and the blur follow the reference's FP32 (and its
+ * double-precision row end) exactly; the splat sums are exact 64-bit fixed
+ * point.  gt_stats: M = capacity = grid cells; ms_sort = the fixed-point
+ * finish.
+ */
+int gt_gauss_transform_grid(gt_ctx *ctx, int n, int d, int vd, const float *splat,
+                            const float *values, int n_slice, const float *slice,
+                            const float *inv_sigma, float *out, gt_stats *stats);
+int gt_gauss_transform_grid_host(gt_ctx *ctx, int n, int d, int vd, const float *splat,
+                                 const float *values, int n_slice, const float *slice,
+                                 const float *inv_sigma, float *out, gt_stats *stats);
+int gt_joint_bilateral_grid(gt_ctx *ctx, int W, int H, int F, int C, const float *im,
+                            int Cr, const float *ref, float fw, float fh, float ff,
+                            float color_sigma, float *out, gt_stats *stats);
+int gt_joint_bilateral_grid_host(gt_ctx *ctx, int W, int H, int F, int C, const float *im,
+                                 int Cr, const float *ref, float fw, float fh, float ff,
+                                 float color_sigma, float *out, gt_stats *stats);
+
+/*
  * A batch of joint-bilateral filters: many images of uneven sizes through one
  * lattice (the image index is part of every vertex key, so the images never
  * mix), one graph launch for the whole pipeline.  Each image is filtered as

@@ -36,6 +36,8 @@
 #include <numeric>
 #include <vector>
 
+#include "grid_oracle.h"
+
 namespace {
 
 enum { ORC_F32 = 0, ORC_F64_LAT = 1, ORC_F64_PURE = 2 };
@@ -542,16 +544,18 @@ void orc_lattice_free(void *handle) { delete (LatticeHandle *)handle; }
 namespace {
 struct View { const float *base; int W, H, F, C; int64_t ys, ts, cs; };
 
+// JointBilateral::apply (src/GaussTransform.cpp:504-678) with the lattice
+// (grid = false) or the bilateral grid (grid = true) as the Gauss transform.
 template <typename T>
 int jb_rec(int mode, View im, View ref, T *out, int64_t oys, int64_t ots, int64_t ocs,
-                  float fw, float fh, float ff, float colorSigma) {
+                  float fw, float fh, float ff, float colorSigma, bool grid = false) {
     if (im.W == 1) fw = 0;
     if (im.H == 1) fh = 0;
     if (im.F == 1) ff = 0;
     if (im.W > 1 && fw == 0) {
         for (int x = 0; x < im.W; x++) {
             View a = im, b = ref; a.base += x; b.base += x; a.W = b.W = 1;
-            int rc = jb_rec<T>(mode, a, b, out + x, oys, ots, ocs, 0, fh, ff, colorSigma);
+            int rc = jb_rec<T>(mode, a, b, out + x, oys, ots, ocs, 0, fh, ff, colorSigma, grid);
             if (rc < 0) return rc;
         }
         return 0;
@@ -559,7 +563,7 @@ int jb_rec(int mode, View im, View ref, T *out, int64_t oys, int64_t ots, int64_
     if (im.H > 1 && fh == 0) {
         for (int y = 0; y < im.H; y++) {
             View a = im, b = ref; a.base += y * im.ys; b.base += y * ref.ys; a.H = b.H = 1;
-            int rc = jb_rec<T>(mode, a, b, out + y * oys, oys, ots, ocs, fw, 0, ff, colorSigma);
+            int rc = jb_rec<T>(mode, a, b, out + y * oys, oys, ots, ocs, fw, 0, ff, colorSigma, grid);
             if (rc < 0) return rc;
         }
         return 0;
@@ -567,7 +571,7 @@ int jb_rec(int mode, View im, View ref, T *out, int64_t oys, int64_t ots, int64_
     if (im.F > 1 && ff == 0) {
         for (int t = 0; t < im.F; t++) {
             View a = im, b = ref; a.base += t * im.ts; b.base += t * ref.ts; a.F = b.F = 1;
-            int rc = jb_rec<T>(mode, a, b, out + t * ots, oys, ots, ocs, fw, fh, 0, colorSigma);
+            int rc = jb_rec<T>(mode, a, b, out + t * ots, oys, ots, ocs, fw, fh, 0, colorSigma, grid);
             if (rc < 0) return rc;
         }
         return 0;
@@ -596,10 +600,16 @@ int jb_rec(int mode, View im, View ref, T *out, int64_t oys, int64_t ots, int64_
                 val[(size_t)im.C * n + i] = 1.0f;
             }
     std::vector<float> ones(d, 1.0f / 1.0f);
-    Lattice<T> lat(d, vd, mode);
-    run_transform(lat, n, pos.data(), n, val.data(), n, ones.data(), true);
     std::vector<T> res((size_t)vd * n);
-    write_out(lat, n, n, nullptr, 0, ones.data(), res.data(), n);
+    bool overflow = false;
+    if (grid) {
+        grid_oracle::transform<T>(n, d, vd, pos.data(), n, val.data(), n, 0, nullptr, 0, ones.data(), res.data(), n);
+    } else {
+        Lattice<T> lat(d, vd, mode);
+        run_transform(lat, n, pos.data(), n, val.data(), n, ones.data(), true);
+        write_out(lat, n, n, nullptr, 0, ones.data(), res.data(), n);
+        overflow = lat.overflow;
+    }
     // Normalise by the homogeneous channel, true division (:673-676).
     i = 0;
     for (int t = 0; t < im.F; t++)
@@ -607,7 +617,7 @@ int jb_rec(int mode, View im, View ref, T *out, int64_t oys, int64_t ots, int64_
             for (int x = 0; x < im.W; x++, i++)
                 for (int k = 0; k < im.C; k++)
                     out[k * ocs + t * ots + y * oys + x] = res[(size_t)k * n + i] / res[(size_t)im.C * n + i];
-    return lat.overflow ? 1 : 0;
+    return overflow ? 1 : 0;
 }
 } // namespace
 
@@ -620,6 +630,33 @@ int orc_joint_bilateral(int mode, int W, int H, int F, int C, const float *im,
     View a{im, W, H, F, C, ys, ts, cs}, b{ref, W, H, F, Cr, ys, ts, cs};
     if (mode == ORC_F32) return jb_rec<float>(mode, a, b, (float *)out, ys, ts, cs, fw, fh, ff, colorSigma);
     return jb_rec<double>(mode, a, b, (double *)out, ys, ts, cs, fw, fh, ff, colorSigma);
+}
+
+// JointBilateral::apply(..., GRID): the same positions and values, the
+// bilateral grid as the Gauss transform (mode F32 or F64_LAT: FP64 weights,
+// sums, blur and slice on the FP32 cell indices).
+int orc_joint_bilateral_grid(int mode, int W, int H, int F, int C, const float *im,
+                             int Cr, const float *ref, float fw, float fh, float ff,
+                             float colorSigma, void *out) {
+    if (W < 1 || H < 1 || F < 1 || C < 1 || Cr < 1 || mode < 0 || mode > 2) return -1;
+    const int64_t ys = W, ts = (int64_t)W * H, cs = (int64_t)W * H * F;
+    View a{im, W, H, F, C, ys, ts, cs}, b{ref, W, H, F, Cr, ys, ts, cs};
+    if (mode == ORC_F32) return jb_rec<float>(mode, a, b, (float *)out, ys, ts, cs, fw, fh, ff, colorSigma, true);
+    return jb_rec<double>(mode, a, b, (double *)out, ys, ts, cs, fw, fh, ff, colorSigma, true);
+}
+
+// GaussTransform::apply(slice, splat, values, sigma, GRID)
+// (src/GaussTransform.cpp:227-297); slice == NULL slices at the splat
+// positions.  Returns the grid's cell count (or -1 on bad dimensions).
+int64_t orc_grid_transform(int mode, int n, int d, int vd, const float *splat, int64_t splat_cs,
+                           const float *values, int64_t values_cs, int n_slice, const float *slice,
+                           int64_t slice_cs, const float *inv_sigma, void *out, int64_t out_cs) {
+    if (d < 1 || d > 8 || vd < 1 || vd > 16 || n < 0 || mode < 0 || mode > 2) return -1;
+    if (mode == ORC_F32)
+        return grid_oracle::transform<float>(n, d, vd, splat, splat_cs, values, values_cs, n_slice, slice,
+                                             slice_cs, inv_sigma, (float *)out, out_cs);
+    return grid_oracle::transform<double>(n, d, vd, splat, splat_cs, values, values_cs, n_slice, slice,
+                                          slice_cs, inv_sigma, (double *)out, out_cs);
 }
 
 } // extern "C"

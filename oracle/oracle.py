@@ -57,6 +57,10 @@ def lib():
         L.orc_joint_bilateral.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _fp,
                                           C.c_int, _fp, C.c_float, C.c_float, C.c_float,
                                           C.c_float, C.c_void_p]
+        L.orc_joint_bilateral_grid.argtypes = L.orc_joint_bilateral.argtypes
+        L.orc_grid_transform.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _fp, C.c_int64, _fp, C.c_int64,
+                                         C.c_int, _fp, C.c_int64, _fp, C.c_void_p, C.c_int64]
+        L.orc_grid_transform.restype = C.c_int64
         _lib = L
     return _lib
 
@@ -158,6 +162,41 @@ def joint_bilateral(im, ref, fw, fh, ff, color_sigma, mode: int = F32):
     return out
 
 
+def grid_transform(splat, values, inv_sigma, slice_pos=None, mode: int = F32):
+    """GaussTransform::apply(GRID) restated (src/DenseGrid.h, taps 5).
+    Returns (out (vd, Ns), grid cells)."""
+    sp = _planar(splat)
+    va = _planar(values)
+    d, n = sp.shape
+    vd = va.shape[0]
+    inv = np.ascontiguousarray(inv_sigma, dtype=np.float32)
+    if slice_pos is not None:
+        sl = _planar(slice_pos)
+        ns = sl.shape[1]
+        slp = _ptr(sl, _fp)
+    else:
+        sl, ns, slp = None, n, None
+    out = np.zeros((vd, ns), np.float32 if mode == F32 else np.float64)
+    cells = lib().orc_grid_transform(mode, n, d, vd, _ptr(sp, _fp), n, _ptr(va, _fp), n, ns, slp, ns,
+                                     _ptr(inv, _fp), out.ctypes.data_as(C.c_void_p), ns)
+    if cells < 0:
+        raise ValueError("bad dimensions")
+    return out, cells
+
+
+def joint_bilateral_grid(im, ref, fw, fh, ff, color_sigma, mode: int = F32):
+    """JointBilateral::apply(GRID) restated on (C,F,H,W) images."""
+    im = np.ascontiguousarray(im, dtype=np.float32)
+    ref = np.ascontiguousarray(ref, dtype=np.float32)
+    Cc, F, H, W = im.shape
+    out = np.zeros(im.shape, np.float32 if mode == F32 else np.float64)
+    rc = lib().orc_joint_bilateral_grid(mode, W, H, F, Cc, _ptr(im, _fp), ref.shape[0], _ptr(ref, _fp),
+                                        fw, fh, ff, color_sigma, out.ctypes.data_as(C.c_void_p))
+    if rc < 0:
+        raise ValueError("bad arguments")
+    return out
+
+
 # --- the reference, compiled from its own sources (oracle/_ref) -------------
 
 def ref_available(kind: str = "strict") -> bool:
@@ -174,6 +213,8 @@ def ref_lib(kind: str = "strict"):
                                           C.c_float, C.c_float, C.c_float, C.c_float, _fp]
         L.ref_lattice.argtypes = [C.c_int, C.c_int, C.c_int, _fp, _fp, _fp, C.c_int, C.c_int,
                                   _sp, _fp]
+        L.ref_grid_transform.argtypes = L.ref_gauss_transform.argtypes
+        L.ref_joint_bilateral_grid.argtypes = L.ref_joint_bilateral.argtypes
         _ref[kind] = L
     return _ref[kind]
 
@@ -195,13 +236,31 @@ def ref_gauss_transform(splat, values, sigmas, slice_pos=None, kind: str = "stri
     return out
 
 
-def ref_joint_bilateral(im, ref, fw, fh, ff, color_sigma, kind: str = "strict"):
+def ref_joint_bilateral(im, ref, fw, fh, ff, color_sigma, kind: str = "strict", grid: bool = False):
     im = np.ascontiguousarray(im, dtype=np.float32)
     ref = np.ascontiguousarray(ref, dtype=np.float32)
     Cc, F, H, W = im.shape
     out = np.zeros(im.shape, np.float32)
-    rc = ref_lib(kind).ref_joint_bilateral(W, H, F, Cc, _ptr(im, _fp), ref.shape[0], _ptr(ref, _fp),
-                                           fw, fh, ff, color_sigma, _ptr(out, _fp))
+    L = ref_lib(kind)
+    fn = L.ref_joint_bilateral_grid if grid else L.ref_joint_bilateral
+    rc = fn(W, H, F, Cc, _ptr(im, _fp), ref.shape[0], _ptr(ref, _fp), fw, fh, ff, color_sigma, _ptr(out, _fp))
+    if rc != 0:
+        raise RuntimeError("reference raised")
+    return out
+
+
+def ref_grid_transform(splat, values, sigmas, slice_pos=None, kind: str = "strict"):
+    """Reference GaussTransform::apply(GRID) on (C,F,H,W) images."""
+    sp = np.ascontiguousarray(splat, dtype=np.float32)
+    va = np.ascontiguousarray(values, dtype=np.float32)
+    d, F, H, W = sp.shape
+    vd = va.shape[0]
+    sig = np.ascontiguousarray(sigmas, dtype=np.float32)
+    out = np.zeros((vd, F, H, W), np.float32)
+    sl = None if slice_pos is None else np.ascontiguousarray(slice_pos, dtype=np.float32)
+    rc = ref_lib(kind).ref_grid_transform(W, H, F, d, _ptr(sp, _fp), vd, _ptr(va, _fp),
+                                          None if sl is None else _ptr(sl, _fp), _ptr(sig, _fp),
+                                          _ptr(out, _fp))
     if rc != 0:
         raise RuntimeError("reference raised")
     return out

@@ -52,6 +52,39 @@ int ref_gauss_transform(int W, int H, int F, int d, const float *splat, int vd,
     }
 }
 
+// GaussTransform::apply(slice, splat, values, sigmas, GRID)
+// (src/GaussTransform.cpp:227-297).  slice == NULL passes the splat image.
+int ref_grid_transform(int W, int H, int F, int d, const float *splat, int vd,
+                       const float *values, const float *slice, const float *sigmas,
+                       float *out) {
+    try {
+        Image sp = planar_to_image(W, H, F, d, splat);
+        Image va = planar_to_image(W, H, F, vd, values);
+        Image sl = slice ? planar_to_image(W, H, F, d, slice) : sp;
+        vector<float> s(sigmas, sigmas + d);
+        Image o = GaussTransform::apply(sl, sp, va, s, GaussTransform::GRID);
+        image_to_planar(o, out);
+        return 0;
+    } catch (...) {
+        return -1;
+    }
+}
+
+// JointBilateral::apply(im, ref, fw, fh, ff, colorSigma, GRID).
+int ref_joint_bilateral_grid(int W, int H, int F, int C, const float *im, int Cr,
+                             const float *ref, float fw, float fh, float ff, float colorSigma,
+                             float *out) {
+    try {
+        Image a = planar_to_image(W, H, F, C, im);
+        Image b = planar_to_image(W, H, F, Cr, ref);
+        JointBilateral::apply(a, b, fw, fh, ff, colorSigma, GaussTransform::GRID);
+        image_to_planar(a, out);
+        return 0;
+    } catch (...) {
+        return -1;
+    }
+}
+
 // JointBilateral::apply(im, ref, fw, fh, ff, colorSigma, PERMUTOHEDRAL)
 // (src/GaussTransform.cpp:504-678); the filtered image is written to out.
 int ref_joint_bilateral(int W, int H, int F, int C, const float *im, int Cr,

@@ -96,6 +96,26 @@ def highdim_small():
     np.savez_compressed(os.path.join(HERE, "highdim_d16_32x32.npz"), pos=pos, values=val, out=out, keys=keys)
 
 
+def grid_cases():
+    """The GRID method (src/DenseGrid.h via src/GaussTransform.cpp:227-297):
+    a grey bilateral (d=3), an RGB bilateral (d=5) and a Gauss transform with
+    distinct slice positions (d=3, vd=2), reference outputs."""
+    grey = S.rect_image(60, 44, 1, K=6, noise=0.05, seed=31)
+    out_grey = O.ref_joint_bilateral(grey, grey, 4.0, 4.0, 0.0, 0.1, grid=True)
+    rgb = S.rect_image(48, 36, 3, K=6, noise=0.05, seed=32)
+    out_rgb = O.ref_joint_bilateral(rgb, rgb, 3.0, 3.0, 0.0, 0.2, grid=True)
+    s = S.Stream(13)
+    pos = s.normal(3 * 30 * 20).reshape(3, 1, 20, 30).astype(np.float32)
+    slc = s.normal(3 * 30 * 20).reshape(3, 1, 20, 30).astype(np.float32)
+    vals = np.stack([s.uniform(600), np.ones(600)]).reshape(2, 1, 20, 30).astype(np.float32)
+    sig = np.array([0.5, 0.7, 1.2], np.float32)
+    out_gt = O.ref_grid_transform(pos, vals, sig, slice_pos=slc)
+    out_gt_replay = O.ref_grid_transform(pos, vals, sig)
+    np.savez_compressed(os.path.join(HERE, "grid_cases.npz"), grey=grey, out_grey=out_grey, rgb=rgb,
+                        out_rgb=out_rgb, pos=pos, slice=slc, values=vals, sigmas=sig, out_gt=out_gt,
+                        out_gt_replay=out_gt_replay)
+
+
 def digests():
     """cfg1 (512^2 bilateral) and cfg2 (2048^2 joint bilateral): M and digests."""
     rec = {}
@@ -124,5 +144,6 @@ if __name__ == "__main__":
     jointbilateral_testcase()
     zero_sigma_case()
     highdim_small()
+    grid_cases()
     digests()
     print("fixtures written to", HERE)

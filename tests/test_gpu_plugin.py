@@ -96,3 +96,22 @@ def test_plugin_gausstransform_recipe(tmp_path, oracle_lib):
     o64, _ = oracle_lib.gauss_transform(pos, vals, inv, slice_pos=pos, mode=oracle_lib.F64_LAT)
     assert rel_err(outs["gpu"].reshape(4, -1), o64) < FP32_REL_TOL
     assert rel_err(outs["gpu"], outs["cpu"]) < 2 * FP32_REL_TOL
+
+
+def test_plugin_auto_grey_runs_the_grid(tmp_path, oracle_lib):
+    """A grey -bilateral with AUTO has 3 position channels, so the reference
+    picks GRID (src/GaussTransform.cpp:553-564); the plugin runs it on the
+    GPU grid.  Checked against the FP64 grid oracle and the CPU CLI."""
+    need_cli()
+    im = S.rect_image(160, 120, 1, K=12, noise=0.05, seed=41)
+    src = str(tmp_path / "grey.tmp")
+    write_tmp(src, im)
+    cpu_path, gpu_path, grid_path = (str(tmp_path / n) for n in ("cpu.tmp", "gpu.tmp", "grid.tmp"))
+    run(["-load", src, "-bilateral", "0.1", "6", "-save", cpu_path])
+    run(["-plugin", PLUGIN, "-load", src, "-bilateral", "0.1", "6", "-save", gpu_path])
+    run(["-plugin", PLUGIN, "-load", src, "-bilateral", "0.1", "6", "6", "0", "grid", "-save", grid_path])
+    o64 = oracle_lib.joint_bilateral_grid(im, im, 6.0, 6.0, 0.0, 0.1, mode=oracle_lib.F64_LAT)
+    gpu = read_tmp(gpu_path)
+    assert rel_err(gpu, o64) < FP32_REL_TOL
+    assert rel_err(gpu, read_tmp(cpu_path)) < 2 * FP32_REL_TOL
+    assert rel_err(read_tmp(grid_path), o64) < FP32_REL_TOL
