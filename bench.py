@@ -89,6 +89,47 @@ def progress(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def grid_leg(gtlattice, synth, torch, steps, side):
+    """The GRID method (bilateral grid, what AUTO picks for grey images) on a
+    grey side^2 bilateral: the cfg2 guide's first channel filtered by itself,
+    sigma_s 16, sigma_r 0.1.  Returns (report, GPU output)."""
+    _, guide, _, _ = synth.cfg2(side=side)
+    grey = guide[:1].copy()
+    d = torch.from_numpy(grey).cuda()
+    out = torch.empty_like(d)
+    ctx = gtlattice.Context(0, profiling=1)
+    for _ in range(2):
+        ctx.joint_bilateral_grid(d, d, 16.0, 16.0, 0.0, 0.1, out=out)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        _, st = ctx.joint_bilateral_grid(d, d, 16.0, 16.0, 0.0, 0.1, out=out, stats=True)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    ctx.close()
+    rep = {"workload": f"grey {side}x{side} bilateral by the bilateral grid (GRID): guide channel 0 "
+                       "(256 rectangles + N(0,0.03)), sigma_s 16, sigma_r 0.1; FP32",
+           "ms": round(ms, 4), "value": round(side * side / ms / 1e3, 2), "unit": "Mpix/s", "cells": st["M"],
+           "stages_ms": {"splat": round(st["ms_splat"], 4), "finish": round(st["ms_sort"], 4),
+                         "blur": round(st["ms_blur"], 4), "slice": round(st["ms_slice"], 4)}}
+    return rep, out.cpu().numpy(), grey
+
+
+def grid_cpu_check(rep, out_gpu, grey):
+    """CPU leg of the grid: the reference's GRID on one core (child process)
+    and the GPU result's error against the FP64 grid oracle."""
+    from oracle import oracle as O
+    from tests.util import rel_err
+    env = dict(os.environ, OMP_NUM_THREADS="1")
+    r = subprocess.run([sys.executable, "-m", "oracle.cpu_baseline", "--side", str(grey.shape[-1]), "--reps", "3",
+                        "--workload", "grid"], cwd=ROOT, env=env, capture_output=True, text=True, timeout=600)
+    base = json.loads(r.stdout.strip().splitlines()[-1])
+    rep["cpu_baseline"] = {"value": round(base["mpix_s"], 4), "unit": "Mpix/s", "cores": 1, "kind": base["kind"],
+                           "median_s": round(base["median_s"], 4)}
+    o64 = O.joint_bilateral_grid(grey, grey, 16.0, 16.0, 0.0, 0.1, mode=O.F64_LAT)
+    rep["rel_err_vs_fp64"] = rel_err(out_gpu, o64)
+
+
 def batch_leg(gtlattice, synth, torch, steps):
     """cfg4: 64 RGB images of uneven sizes as one batched lattice (one graph
     launch per call), tuned by the autotuner; also the same images one call
@@ -247,6 +288,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-lattice leg (64 images)")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--no-highdim", action="store_true", help="skip the d=16 precision leg")
+    ap.add_argument("--no-grid", action="store_true", help="skip the bilateral-grid leg")
     ap.add_argument("--serve-requests", type=int, default=2000)
     ap.add_argument("--qps", type=float, default=100.0)
     ap.add_argument("--slots", type=int, default=4)
@@ -359,6 +401,9 @@ def main():
     if not a.no_batch:
         progress("batched lattice leg (64 images)")
         line["batch"] = batch_leg(gtlattice, synth, torch, max(4, a.steps // 2))
+    if not a.no_grid:
+        progress("bilateral grid leg (grey 2048^2)")
+        line["grid"], grid_out, grid_in = grid_leg(gtlattice, synth, torch, max(4, a.steps // 2), a.side)
     if not a.no_highdim:
         progress("d=16 precision leg (1024^2, FP32/BF16/FP16 storage)")
         line["highdim"], hd_crop = highdim_leg(gtlattice, synth, torch, 3)
@@ -374,6 +419,8 @@ def main():
         cpu, err = cpu_baseline_leg(out_gpu, im, guide, ss, sr)
         if not a.no_highdim:
             highdim_oracle_check(line["highdim"], hd_crop)
+        if not a.no_grid:
+            grid_cpu_check(line["grid"], grid_out, grid_in)
         line["cpu_baseline"] = cpu
         line["gpu_over_cpu"] = round(value / cpu["value"], 1)
         line["accuracy"] = err

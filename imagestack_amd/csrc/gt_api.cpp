@@ -666,7 +666,7 @@ int run_grid(gt_ctx *c, const gt::PixSrc &src, int n, int d, int vd, const gt::P
     GT_TRY(ensure(c->grid_acc, total * 8));
     GT_TRY(ensure(c->grid_a, total * 4));
     GT_TRY(ensure(c->grid_b, total * 4));
-    GT_HIP(gt::launch_grid(d, bilat, src, n, slice, n_slice, dst, g, cells, vsc,
+    GT_HIP(gt::launch_grid(d, bilat, src, n, bilat ? src.W : 0, slice, n_slice, dst, g, cells, vsc,
                            (unsigned long long *)c->grid_acc.p, (float *)c->grid_a.p, (float *)c->grid_b.p,
                            c->profiling ? c->ev : nullptr, c->stream));
     GT_HIP(hipStreamSynchronize(c->stream));

@@ -19,7 +19,7 @@
 //                   homogeneous channel.
 // A corner past the end of a dimension has weight exactly 0 (the fraction is
 // 0 there) and is skipped.
-#include "gt_kernels.h"
+#include "gt_launch.h"
 
 #include <algorithm>
 #include <utility>
@@ -49,35 +49,112 @@ __device__ __forceinline__ bool grid_cell(const PixSrc &src, int p, const GridAr
     return in;
 }
 
+// Splat, one pixel tile per workgroup (tile_w x 256/tile_w pixels of an
+// image, or 256 consecutive pixels).  The tile's cells lie in the box
+// spanned by its pixels' cell indices (+1 for the upper corners); when that
+// box holds at most kGridLds values the tile pre-reduces its contributions in
+// LDS with 64-bit integer atomics and adds each non-zero box value to the
+// grid once; otherwise every contribution goes to the grid directly.  Both
+// are exact fixed-point sums, so the result is the same either way.
+constexpr int kGridLds = 4096;
+
 template <int D, bool BILAT>
-__global__ __launch_bounds__(kBlock) void k_grid_splat(PixSrc src, int n, GridArgs g, ValScale vsc,
-                                                       unsigned long long *__restrict__ acc) {
-    const int p = blockIdx.x * kBlock + threadIdx.x;
-    if (p >= n) return;
-    int64_t base;
+__global__ __launch_bounds__(kBlock) void k_grid_splat(PixSrc src, int n, int row_w, int tile_w, GridArgs g,
+                                                       ValScale vsc, unsigned long long *__restrict__ acc) {
+    __shared__ unsigned long long lacc[kGridLds];
+    __shared__ int blo[D], bhi[D];
+    const int tid = threadIdx.x;
+    if (tid < D) {
+        blo[tid] = 0x7fffffff;
+        bhi[tid] = -1;
+    }
+    const int p = pixel_index(blockIdx.x, tid, n, row_w, tile_w);
+    int64_t base = 0;
     float F[D], Fi[D];
     int I[D];
-    if (!grid_cell<D, BILAT>(src, p, g, base, F, Fi, I)) return;
+    const bool in = p >= 0 && grid_cell<D, BILAT>(src, p, g, base, F, Fi, I);
     float v[kMaxVD];
-    load_values<kMaxVD>(src, p, v);
-    for (int c = 0; c < (1 << D); c++) {
-        float w = 1;
-        int64_t idx = base;
-        bool inside = true;
 #pragma unroll
-        for (int j = 0; j < D; j++) {
-            if (c & (1 << j)) {
-                idx += g.stride[j];
-                w *= F[j];
-                inside = inside && I[j] + 1 < g.size[j];
-            } else {
-                w *= Fi[j];
+    for (int k = 0; k < kMaxVD; k++) v[k] = 0.0f;
+    if (in) load_values<kMaxVD>(src, p, v);
+    __syncthreads();
+    if (in) {
+#pragma unroll
+        for (int i = 0; i < D; i++) {
+            atomicMin(&blo[i], I[i]);
+            atomicMax(&bhi[i], min(I[i] + 1, g.size[i] - 1));
+        }
+    }
+    __syncthreads();
+    // the box: extents and value strides (lstride[0] = vd)
+    int lo[D], lstride[D];
+    int64_t vol = g.vd;
+    bool any = true;
+#pragma unroll
+    for (int i = 0; i < D; i++) {
+        lo[i] = blo[i];
+        const int e = bhi[i] - blo[i] + 1;
+        any = any && e > 0;
+        lstride[i] = (int)vol;
+        vol *= e > 0 ? e : 1;
+    }
+    if (!any) return;   // no pixel of the tile lies in the grid (uniform)
+    const bool local = vol <= kGridLds;   // uniform over the workgroup
+    if (local) {
+        for (int e = tid; e < (int)vol; e += kBlock) lacc[e] = 0ull;
+        __syncthreads();
+    }
+    if (in) {
+        // Lane i starts at corner i mod 2^D: neighbouring pixels share cells,
+        // and visiting different corners in the same instruction spreads the
+        // LDS atomics over more addresses (the sums are exact, so the order
+        // does not matter).
+        const int rot = tid & ((1 << D) - 1);
+        for (int cc = 0; cc < (1 << D); cc++) {
+            const int c = (cc + rot) & ((1 << D) - 1);
+            float w = 1;
+            int64_t idx = base;
+            int li = 0;
+            bool inside = true;
+#pragma unroll
+            for (int j = 0; j < D; j++) {
+                const int bit = (c >> j) & 1;
+                if (bit) {
+                    idx += g.stride[j];
+                    w *= F[j];
+                    inside = inside && I[j] + 1 < g.size[j];
+                } else {
+                    w *= Fi[j];
+                }
+                li += (I[j] + bit - lo[j]) * lstride[j];
+            }
+            if (!inside) continue;
+#pragma unroll
+            for (int k = 0; k < kMaxVD; k++) {
+                if (k >= g.vd) continue;
+                const unsigned long long q = to_fixed(w * v[k], vsc.s[k]);
+                if (local) atomicAdd(&lacc[li + k], q);
+                else atomicAdd(acc + idx + k, q);
             }
         }
-        if (!inside) continue;
+    }
+    if (!local) return;
+    __syncthreads();
+    // flush: box value e -> grid element
+    for (int e = tid; e < (int)vol; e += kBlock) {
+        const unsigned long long q = lacc[e];
+        if (q == 0ull) continue;
+        int r = e;
+        const int k = r % g.vd;
+        r /= g.vd;
+        int64_t idx = k;
 #pragma unroll
-        for (int k = 0; k < kMaxVD; k++)
-            if (k < g.vd) atomicAdd(acc + idx + k, to_fixed(w * v[k], vsc.s[k]));
+        for (int j = 0; j < D; j++) {
+            const int ext = bhi[j] - blo[j] + 1;
+            idx += (int64_t)(lo[j] + r % ext) * g.stride[j];
+            r /= ext;
+        }
+        atomicAdd(acc + idx, q);
     }
 }
 
@@ -161,18 +238,22 @@ __global__ __launch_bounds__(kBlock) void k_grid_slice(PixSrc src, int n, GridAr
     default: return hipErrorInvalidValue;                                         \
     }
 
-hipError_t launch_grid(int d, bool bilat, const PixSrc &splat, int n, const PixSrc &slice, int n_slice,
+hipError_t launch_grid(int d, bool bilat, const PixSrc &splat, int n, int row_w, const PixSrc &slice, int n_slice,
                        const OutDst &dst, const GridArgs &g, int64_t cells, const ValScale &vsc,
                        unsigned long long *acc, float *a, float *b, hipEvent_t *ev, hipStream_t st) {
     const int64_t total = cells * g.vd;
     const unsigned zb = (unsigned)std::min<int64_t>(8192, (total + kBlock - 1) / kBlock);
     hipLaunchKernelGGL(k_grid_zero, dim3(zb > 0 ? zb : 1), dim3(kBlock), 0, st, acc, total);
     if (ev) (void)hipEventRecord(ev[0], st);
-    const dim3 gp((n + kBlock - 1) / kBlock), gs((n_slice + kBlock - 1) / kBlock), blk(kBlock);
-#define GT_GSPLAT(D)                                                                                        \
-    {                                                                                                       \
-        if (bilat) hipLaunchKernelGGL((k_grid_splat<D, true>), gp, blk, 0, st, splat, n, g, vsc, acc);    \
-        else hipLaunchKernelGGL((k_grid_splat<D, false>), gp, blk, 0, st, splat, n, g, vsc, acc);         \
+    int tile_w = row_w > 0 ? 16 : 0;   // 16 x 16 pixel tiles of an image, else 256 consecutive pixels
+    const int64_t ntiles = pixel_blocks(n, row_w, tile_w);
+    const dim3 gp((unsigned)ntiles), gs((n_slice + kBlock - 1) / kBlock), blk(kBlock);
+#define GT_GSPLAT(D)                                                                                          \
+    {                                                                                                         \
+        if (bilat) hipLaunchKernelGGL((k_grid_splat<D, true>), gp, blk, 0, st, splat, n, row_w, tile_w, g, vsc, \
+                                      acc);                                                                   \
+        else hipLaunchKernelGGL((k_grid_splat<D, false>), gp, blk, 0, st, splat, n, row_w, tile_w, g, vsc,    \
+                                acc);                                                                         \
     }
     if (n > 0) GT_DISPATCH_GRID(d, GT_GSPLAT)
 #undef GT_GSPLAT

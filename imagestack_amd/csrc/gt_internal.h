@@ -164,7 +164,7 @@ struct GridArgs {
 };
 // Zero, splat, finish, 2 blur rounds per dimension, slice.  ev (optional):
 // 6 events recorded at the stage boundaries (splat, -, -, blur, slice).
-hipError_t launch_grid(int d, bool bilat, const PixSrc &splat, int n, const PixSrc &slice, int n_slice,
+hipError_t launch_grid(int d, bool bilat, const PixSrc &splat, int n, int row_w, const PixSrc &slice, int n_slice,
                        const OutDst &dst, const GridArgs &g, int64_t cells, const ValScale &vsc,
                        unsigned long long *acc, float *a, float *b, hipEvent_t *ev, hipStream_t st);
 

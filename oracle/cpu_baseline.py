@@ -1,5 +1,8 @@
 """CPU baseline for bench.py: times ImageStack's own JointBilateral::apply
-(PERMUTOHEDRAL) on the host cores.  TEST/MEASUREMENT INFRASTRUCTURE ONLY.
+on the host cores: PERMUTOHEDRAL on the 2048^2 joint bilateral (--workload
+lattice), or GRID on a grey 2048^2 bilateral (--workload grid: the guide's
+first channel filtered by itself, sigma_s 16, sigma_r 0.1).
+TEST/MEASUREMENT INFRASTRUCTURE ONLY.
 
 Kinds:
   reference : oracle/_ref/libref_fast.so, the reference compiled from its own
@@ -27,14 +30,28 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 
-def run_one(side: int, reps: int, kind: str) -> dict:
+def grid_workload(side: int):
+    from imagestack_amd import synth as S
+    _, guide, _, _ = S.cfg2(side=side)
+    grey = guide[:1].copy()
+    return grey, 16.0, 0.1
+
+
+def run_one(side: int, reps: int, kind: str, workload: str = "lattice") -> dict:
     from imagestack_amd import synth as S
     from oracle import oracle as O
-    im, guide, ss, sr = S.cfg2(side=side)
-    if kind == "reference":
-        fn = lambda: O.ref_joint_bilateral(im, guide, ss, ss, 0.0, sr, kind="fast")  # noqa: E731
+    if workload == "grid":
+        g, ss, sr = grid_workload(side)
+        if kind == "reference":
+            fn = lambda: O.ref_joint_bilateral(g, g, ss, ss, 0.0, sr, kind="fast", grid=True)  # noqa: E731
+        else:
+            fn = lambda: O.joint_bilateral_grid(g, g, ss, ss, 0.0, sr, mode=O.F32)  # noqa: E731
     else:
-        fn = lambda: O.joint_bilateral(im, guide, ss, ss, 0.0, sr, mode=O.F32)  # noqa: E731
+        im, guide, ss, sr = S.cfg2(side=side)
+        if kind == "reference":
+            fn = lambda: O.ref_joint_bilateral(im, guide, ss, ss, 0.0, sr, kind="fast")  # noqa: E731
+        else:
+            fn = lambda: O.joint_bilateral(im, guide, ss, ss, 0.0, sr, mode=O.F32)  # noqa: E731
     fn()  # warm-up
     times = []
     for _ in range(reps):
@@ -51,19 +68,20 @@ def main():
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--kind", default="auto", choices=["auto", "reference", "port"])
     ap.add_argument("--procs", type=int, default=1)
+    ap.add_argument("--workload", default="lattice", choices=["lattice", "grid"])
     a = ap.parse_args()
     from oracle import oracle as O
     kind = a.kind
     if kind == "auto":
         kind = "reference" if O.ref_available("fast") else "port"
     if a.procs <= 1:
-        r = run_one(a.side, a.reps, kind)
-        r.update({"kind": kind, "cores": 1, "side": a.side})
+        r = run_one(a.side, a.reps, kind, a.workload)
+        r.update({"kind": kind, "cores": 1, "side": a.side, "workload": a.workload})
         print(json.dumps(r))
         return
     env = dict(os.environ, OMP_NUM_THREADS="1")
     cmd = [sys.executable, "-m", "oracle.cpu_baseline", "--side", str(a.side), "--reps", str(a.reps),
-           "--kind", kind]
+           "--kind", kind, "--workload", a.workload]
     t0 = time.perf_counter()
     procs = [subprocess.Popen(cmd, cwd=ROOT, env=env, stdout=subprocess.PIPE, text=True) for _ in range(a.procs)]
     outs = [json.loads(p.communicate()[0].strip().splitlines()[-1]) for p in procs]
