@@ -83,6 +83,7 @@ struct gt_ctx {
         vals_a, vals_b, nbr, splat_copy, mm_part, mm_out;
     DevBuf stage_in, stage_out, view_a, view_b, view_o;
     DevBuf grid_acc, grid_a, grid_b;   // bilateral grid: fixed-point sums, FP32 ping-pong
+    DevBuf patch_w, patch_guide;       // non-local means: reordered filters, projected guide
     gt::Counters *h_ctr = nullptr;   // pinned
     float *h_mm = nullptr;           // pinned, 3 * 2 * kMaxDH
     // capacity memory: last lattice size seen for (n, d)
@@ -837,7 +838,7 @@ void gt_ctx_destroy(gt_ctx *c) {
                       &c->pk_a,        &c->pk_b,      &c->pay_b,    &c->hist,      &c->sort_status,
                       &c->sort_hist,   &c->vkeys,     &c->vals_a,   &c->vals_b,    &c->nbr,       &c->splat_copy,
                       &c->mm_part,     &c->mm_out,    &c->stage_in, &c->stage_out, &c->view_a,    &c->view_b,
-                      &c->view_o,      &c->grid_acc,  &c->grid_a,   &c->grid_b};
+                      &c->view_o,      &c->grid_acc,  &c->grid_a,   &c->grid_b,    &c->patch_w,   &c->patch_guide};
     for (DevBuf *b : bufs) free_buf(*b);
     for (auto &e : c->ev)
         if (e) (void)hipEventDestroy(e);
@@ -1174,6 +1175,91 @@ int gt_lattice_export(gt_ctx *c, int16_t *keys, int32_t *nbr, float *splat_vals,
         GT_HIP(hipMemcpyAsync(blur_vals, c->stage_out.p, (size_t)M * vd * 4, hipMemcpyDeviceToHost, c->stream));
         GT_HIP(hipStreamSynchronize(c->stream));
     }
+    return GT_OK;
+}
+
+// --- non-local means (NLMeans::apply / NLMeans3D::apply after the PCA) ------
+
+static int patch_args(int W, int H, int F, int C, const float *im, int fw, int fh, int ft, int K,
+                      const float *filters, const float *out) {
+    if (W < 1 || H < 1 || F < 1 || C < 1) return fail(GT_EINVAL, "bad image size %dx%dx%dx%d", W, H, F, C);
+    if ((int64_t)W * H * F >= (1ll << 31)) return fail(GT_EINVAL, "image of more than 2^31 pixels");
+    if (F > 65535 || (H + 15) / 16 > 65535) return fail(GT_EINVAL, "more than 65535 frames or 2^20 rows");
+    if (K < 1 || K > GT_MAX_PATCH_DIMS)
+        return fail(GT_EINVAL, "patch dimensions K=%d outside [1, %d]", K, GT_MAX_PATCH_DIMS);
+    if (fw < 1 || fh < 1 || ft < 1 || fw > 64 || fh > 64 || ft > 64)
+        return fail(GT_EINVAL, "filter size %dx%dx%d outside [1, 64] per axis", fw, fh, ft);
+    if (((int64_t)fw * fh * ft) % 2 != 1)   // src/Convolve.cpp:144-145
+        return fail(GT_EINVAL, "filter must have odd size (%d %d %d)", fw, fh, ft);
+    if (!im || !filters || !out) return fail(GT_EINVAL, "null pointer");
+    return GT_OK;
+}
+
+// JointBilateral's AUTO choice (src/GaussTransform.cpp:512-564) for the call
+// NLMeans makes, JointBilateral::apply(image, guide, s, s, INF, patch_sigma):
+// 1 = GRID, 2 = PERMUTOHEDRAL, 0 = EXACT, GKDTREE or the zero-sigma split.
+static int nlmeans_method(int W, int H, int F, int K, float s) {
+    float sx = s, sy = s, st = INFINITY;
+    if (W == 1) sx = 0;
+    if (H == 1) sy = 0;
+    if (F == 1) st = 0;
+    if ((W > 1 && sx == 0) || (H > 1 && sy == 0)) return 0;
+    const bool fx = W > 1 && sx < 10 * W, fy = H > 1 && sy < 10 * H, ft = F > 1 && st < 10 * F;
+    const int pos = K + (fx ? 1 : 0) + (fy ? 1 : 0) + (ft ? 1 : 0);
+    const int zx = fx ? ((int)(sx * 6 + 1) | 1) : 1, zy = fy ? ((int)(sy * 6 + 1) | 1) : 1;
+    const int zt = ft ? ((int)(st * 6 + 1) | 1) : 1;
+    if (zx * zy * zt < 16) return 0;
+    if (pos <= 4) return 1;
+    if (pos <= 10) return 2;
+    return 0;
+}
+
+int gt_patch_project(gt_ctx *c, int W, int H, int F, int C, const float *im, int fw, int fh, int ft, int K,
+                     const float *filters, float *out) {
+    if (!c) return fail(GT_EINVAL, "null context");
+    GT_TRY(patch_args(W, H, F, C, im, fw, fh, ft, K, filters, out));
+    GT_HIP(hipSetDevice(c->device));
+    GT_TRY(ensure(c->patch_w, (size_t)K * C * fw * fh * ft * 4));
+    GT_HIP(gt::launch_patch_project(im, W, H, F, C, filters, fw, fh, ft, K, (float *)c->patch_w.p, out,
+                                    c->stream));
+    GT_HIP(hipStreamSynchronize(c->stream));
+    return GT_OK;
+}
+
+int gt_nlmeans(gt_ctx *c, int W, int H, int F, int C, const float *im, int fw, int fh, int ft, int K,
+               const float *filters, float spatial_sigma, float patch_sigma, float *out, gt_storage storage,
+               const gt_tuning *tune, gt_stats *st) {
+    if (!c) return fail(GT_EINVAL, "null context");
+    GT_TRY(patch_args(W, H, F, C, im, fw, fh, ft, K, filters, out));
+    const int m = nlmeans_method(W, H, F, K, spatial_sigma);
+    if (m == 0)
+        return fail(GT_EUNSUPPORTED, "JointBilateral's AUTO method for this call is EXACT, GKDTREE or a "
+                                     "per-row/column split (not on the GPU)");
+    GT_HIP(hipSetDevice(c->device));
+    const size_t n = (size_t)W * H * F;
+    GT_TRY(ensure(c->patch_w, (size_t)K * C * fw * fh * ft * 4));
+    GT_TRY(ensure(c->patch_guide, n * K * 4));
+    GT_HIP(gt::launch_patch_project(im, W, H, F, C, filters, fw, fh, ft, K, (float *)c->patch_w.p,
+                                    (float *)c->patch_guide.p, c->stream));
+    return jb_entry(c, W, H, F, C, im, K, (const float *)c->patch_guide.p, spatial_sigma, spatial_sigma, INFINITY,
+                    patch_sigma, out, storage, tune, st, m == 1);
+}
+
+int gt_nlmeans_host(gt_ctx *c, int W, int H, int F, int C, const float *im, int fw, int fh, int ft, int K,
+                    const float *filters, float spatial_sigma, float patch_sigma, float *out, gt_storage storage,
+                    const gt_tuning *tune, gt_stats *st) {
+    if (!c) return fail(GT_EINVAL, "null context");
+    GT_TRY(patch_args(W, H, F, C, im, fw, fh, ft, K, filters, out));
+    GT_HIP(hipSetDevice(c->device));
+    const size_t n = (size_t)W * H * F, nf = (size_t)K * C * fw * fh * ft;
+    GT_TRY(ensure(c->stage_in, (n * C + nf) * 4));
+    GT_TRY(ensure(c->stage_out, n * C * 4));
+    float *di = (float *)c->stage_in.p, *df = di + n * C;
+    GT_HIP(hipMemcpyAsync(di, im, n * C * 4, hipMemcpyHostToDevice, c->stream));
+    GT_HIP(hipMemcpyAsync(df, filters, nf * 4, hipMemcpyHostToDevice, c->stream));
+    GT_TRY(gt_nlmeans(c, W, H, F, C, di, fw, fh, ft, K, df, spatial_sigma, patch_sigma, (float *)c->stage_out.p,
+                      storage, tune, st));
+    GT_HIP(hipMemcpy(out, c->stage_out.p, n * C * 4, hipMemcpyDeviceToHost));
     return GT_OK;
 }
 

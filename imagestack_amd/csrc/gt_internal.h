@@ -168,6 +168,13 @@ hipError_t launch_grid(int d, bool bilat, const PixSrc &splat, int n, int row_w,
                        const OutDst &dst, const GridArgs &g, int64_t cells, const ValScale &vsc,
                        unsigned long long *acc, float *a, float *b, hipEvent_t *ev, hipStream_t st);
 
+// Patch projection (Convolve zero/inner with a K*C filter bank, K <= 16):
+// filters [k][c][tap] are reordered into wt (K*C*fw*fh*ft floats), then
+// out (K planes of W*H*F) is written.
+hipError_t launch_patch_project(const float *im, int W, int H, int F, int C, const float *filters, int fw, int fh,
+                                int ft, int K, float *wt, float *out, hipStream_t s);
+size_t patch_lds_bytes(int fw, int fh);
+
 // Sets the thread's gt_last_error() message; returns code.
 int set_error(int code, const char *msg);
 // Forgets the context's cached key-packing plan, so the next call bounds its

@@ -17,7 +17,8 @@ LIB_PATH = os.path.join(HERE, "libgtlattice.so")
 
 GT_STORE_F32, GT_STORE_BF16, GT_STORE_F16 = 0, 1, 2
 STORAGE = {"f32": GT_STORE_F32, "bf16": GT_STORE_BF16, "f16": GT_STORE_F16}
-ERRORS = {1: "GT_EINVAL", 2: "GT_ENOMEM", 3: "GT_ERANGE", 4: "GT_EDEVICE", 5: "GT_ESTATE"}
+ERRORS = {1: "GT_EINVAL", 2: "GT_ENOMEM", 3: "GT_ERANGE", 4: "GT_EDEVICE", 5: "GT_ESTATE", 6: "GT_EUNSUPPORTED"}
+GT_EUNSUPPORTED = 6
 
 EXPORTS = [
     "gt_ctx_create", "gt_ctx_destroy", "gt_ctx_set_stream", "gt_ctx_set_profiling", "gt_last_error",
@@ -26,7 +27,7 @@ EXPORTS = [
     "gt_autotune_joint_bilateral", "gt_autotune_joint_bilateral_batch",
     "gt_server_create", "gt_server_destroy", "gt_server_submit", "gt_server_wait", "gt_server_replay",
     "gt_gauss_transform_grid", "gt_gauss_transform_grid_host", "gt_joint_bilateral_grid",
-    "gt_joint_bilateral_grid_host",
+    "gt_joint_bilateral_grid_host", "gt_patch_project", "gt_nlmeans", "gt_nlmeans_host",
 ]
 
 
@@ -123,6 +124,12 @@ def lib():
                  C.c_float, _vp, C.POINTER(Stats)]
         L.gt_joint_bilateral_grid.argtypes = jgsig
         L.gt_joint_bilateral_grid_host.argtypes = jgsig
+        L.gt_patch_project.argtypes = [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp, C.c_int, C.c_int, C.c_int,
+                                       C.c_int, _vp, _vp]
+        nsig = [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp, C.c_float,
+                C.c_float, _vp, C.c_int, C.POINTER(Tuning), C.POINTER(Stats)]
+        L.gt_nlmeans.argtypes = nsig
+        L.gt_nlmeans_host.argtypes = nsig
         L.gt_server_create.argtypes = [C.POINTER(_vp), C.c_int, C.POINTER(ServerConfig)]
         L.gt_server_destroy.argtypes = [_vp]
         L.gt_server_destroy.restype = None
@@ -259,6 +266,51 @@ class Context:
         _check(lib().gt_joint_bilateral_grid_host(self._h, W, H, F, Cc, im.ctypes.data, ref.shape[0],
                                                   ref.ctypes.data, fw, fh, ff, color_sigma, out.ctypes.data,
                                                   C.byref(st)))
+        return out
+
+    def patch_project(self, im, filters, out=None):
+        """Convolve::apply(im, filters, Zero, Inner): im (C, F, H, W) device f32,
+        filters (K*C, ft, fh, fw) device f32; returns (K, F, H, W)."""
+        import torch
+        Cc, F, H, W = im.shape
+        KC, ft, fh, fw = filters.shape
+        if KC % Cc:
+            raise ValueError("filter planes must be a multiple of the image channels")
+        K = KC // Cc
+        if out is None:
+            out = torch.empty((K, F, H, W), dtype=torch.float32, device=im.device)
+        _check(lib().gt_patch_project(self._h, W, H, F, Cc, _dptr(im), fw, fh, ft, K, _dptr(filters), _dptr(out)))
+        return out
+
+    def nlmeans(self, im, filters, spatial_sigma, patch_sigma, out=None, storage: str = "f32", tune=None,
+                stats: bool = False):
+        """NLMeans::apply after the PCA: im (C, F, H, W) device f32, filters
+        (K*C, ft, fh, fw) device f32 (PatchPCA's output); returns (C, F, H, W)."""
+        import torch
+        Cc, F, H, W = im.shape
+        KC, ft, fh, fw = filters.shape
+        if KC % Cc:
+            raise ValueError("filter planes must be a multiple of the image channels")
+        if out is None:
+            out = torch.empty_like(im)
+        st = Stats()
+        _check(lib().gt_nlmeans(self._h, W, H, F, Cc, _dptr(im), fw, fh, ft, KC // Cc, _dptr(filters), spatial_sigma,
+                                patch_sigma, _dptr(out), STORAGE[storage], _tuning(tune), C.byref(st)))
+        return (out, st.as_dict()) if stats else out
+
+    def nlmeans_host(self, im, filters, spatial_sigma, patch_sigma, storage: str = "f32"):
+        """nlmeans on numpy arrays (host memory)."""
+        im = np.ascontiguousarray(im, dtype=np.float32)
+        filt = np.ascontiguousarray(filters, dtype=np.float32)
+        Cc, F, H, W = im.shape
+        KC, ft, fh, fw = filt.shape
+        if KC % Cc:
+            raise ValueError("filter planes must be a multiple of the image channels")
+        out = np.zeros_like(im)
+        st = Stats()
+        _check(lib().gt_nlmeans_host(self._h, W, H, F, Cc, im.ctypes.data, fw, fh, ft, KC // Cc, filt.ctypes.data,
+                                     spatial_sigma, patch_sigma, out.ctypes.data, STORAGE[storage], None,
+                                     C.byref(st)))
         return out
 
     @staticmethod

@@ -15,6 +15,9 @@
 //   ImageStack::GPU::gaussTransformGrid / GPU::jointBilateralGrid
 //       the same for the GRID method (the bilateral grid,
 //       src/DenseGrid.h; at most 6 position channels).
+//   ImageStack::GPU::nlmeans(im, filters, spatialSigma, patchSigma)
+//       NLMeans::apply / NLMeans3D::apply after their PatchPCA call
+//       (src/GaussTransform.cpp:867-874, 934-941): filters in place.
 //
 // Library failures become ImageStack::panic() with the library's message,
 // exactly how the reference reports errors (src/Exception.cpp:18-33).
@@ -136,6 +139,24 @@ inline void jointBilateralGrid(Image im, Image ref, float filterWidth, float fil
                                      o.data(), nullptr) != GT_OK)
         panic("%s\n", gt_last_error());
     unpackPlanar(o, im);
+}
+
+// NLMeans::apply after PatchPCA on the GPU: the patch projection
+// Convolve::apply(im, filters, Zero, Inner) and JointBilateral::apply(im,
+// projection, s, s, INF, patchSigma) with AUTO's choice.  Returns false, with
+// im untouched, when that choice (EXACT, GKDTREE) is not on the GPU.
+inline bool nlmeans(Image im, Image filters, float spatialSigma, float patchSigma) {
+    assert(filters.channels % im.channels == 0,
+           "To perform an inner or matrix product, the channel count of either the image or the filter must be a"
+           " multiple of the channel count of the other.");
+    std::vector<float> a = packPlanar(im), f = packPlanar(filters), o(a.size());
+    const int rc = gt_nlmeans_host(context(), im.width, im.height, im.frames, im.channels, a.data(), filters.width,
+                                   filters.height, filters.frames, filters.channels / im.channels, f.data(),
+                                   spatialSigma, patchSigma, o.data(), GT_STORE_F32, nullptr, nullptr);
+    if (rc == GT_EUNSUPPORTED) return false;
+    if (rc != GT_OK) panic("%s\n", gt_last_error());
+    unpackPlanar(o, im);
+    return true;
 }
 
 } // namespace GPU

@@ -4,9 +4,10 @@
 //     ImageStack -plugin libgt_plugin.so -load im.tmp -bilateral 0.1 8 8 0 permutohedral -save out.tmp
 //
 // init_imagestack_plugin (src/Plugin.cpp:28-35, examples/plugin.cpp:73-77)
-// replaces -gausstransform, -jointbilateral and -bilateral with subclasses
-// that keep the reference's argument grammar (src/GaussTransform.cpp:79-113,
-// :473-502, :699-728) and its AUTO method choice (:540-564).  Whenever the
+// replaces -gausstransform, -jointbilateral, -bilateral, -nlmeans and
+// -nlmeans3d with subclasses that keep the reference's argument grammar
+// (src/GaussTransform.cpp:79-113, :473-502, :699-728, :840-865, :907-932) and
+// its AUTO method choice (:540-564).  Whenever the
 // method resolves to PERMUTOHEDRAL the lattice, and to GRID (at most 6
 // position channels) the bilateral grid, runs in libgtlattice.so; the other
 // methods (exact, gkdtree) run the reference's own code.
@@ -16,6 +17,8 @@
 #include "Operation.h"
 #include "GaussTransform.h"
 #include "Statistics.h"
+#include "Convolve.h"
+#include "Arithmetic.h"
 #include "gt_imagestack.h"
 
 namespace ImageStack {
@@ -194,6 +197,94 @@ public:
     }
 };
 
+// NLMeans::parse (src/GaussTransform.cpp:840-865): the same arguments; the
+// method is parsed and, as in the reference, not passed on (:873).
+static void parseNLMeansArgs(const vector<string> &args, float &patchSize, int &dims, float &spatialSigma,
+                             float &patchSigma) {
+    assert(args.size() == 4 || args.size() == 5, "-nlmeans takes four or five arguments\n");
+    patchSize = readFloat(args[0]);
+    dims = readInt(args[1]);
+    spatialSigma = readFloat(args[2]);
+    patchSigma = readFloat(args[3]);
+    if (args.size() > 4) methodByName(args[4]);
+}
+
+// The steps of NLMeans::apply / NLMeans3D::apply (:867-874, :934-941) with
+// the host's own PatchPCA (so the random patches are the ones the reference
+// draws), then the projection and joint bilateral on the GPU; where AUTO
+// picks a method the GPU does not run, the reference's own Convolve and
+// JointBilateral finish the job with the same filters.
+static void nlmeansSteps(Image image, Image filters, float spatialSigma, float patchSigma) {
+    if (nlmeans(image, filters, spatialSigma, patchSigma)) return;
+    Image pca = Convolve::apply(image, filters, Convolve::Zero, Multiply::Inner);
+    JointBilateral::apply(image, pca, spatialSigma, spatialSigma, INF, patchSigma);
+}
+
+// NLMeans::test's check (src/GaussTransform.cpp:824-838) on a synthetic image
+// (the reference's uses pics/dog1.jpg): noise is removed, and the GPU result
+// is within 1e-4 of the reference's NLMeans::apply on the same random stream.
+static bool nlmeansTest(bool threeD) {
+    Image clean(threeD ? 24 : 80, threeD ? 20 : 60, threeD ? 8 : 1, 3);
+    for (int c = 0; c < 3; c++)
+        for (int t = 0; t < clean.frames; t++)
+            for (int y = 0; y < clean.height; y++)
+                for (int x = 0; x < clean.width; x++)
+                    clean(x, y, t, c) = ((x / 10 + y / 12 + c) % 3) * 0.4f + 0.1f;
+    Image noisy = clean.copy();
+    Noise::apply(noisy, -0.2, 0.2);
+    const float sigma = threeD ? 0.5f : 1.0f;
+    const unsigned seed = (unsigned)rand();
+    Image cpu = noisy.copy();
+    srand(seed);
+    if (threeD) NLMeans3D::apply(cpu, sigma, 4, 4, 0.4);
+    else NLMeans::apply(cpu, sigma, 6, 16, 0.4);
+    Image gpu = noisy.copy();
+    srand(seed);
+    Image filters = threeD ? PatchPCA3D::apply(gpu, sigma, 4) : PatchPCA::apply(gpu, sigma, 6);
+    if (!nlmeans(gpu, filters, threeD ? 4 : 16, 0.4)) return false;
+    double errNoisy = 0, errOut = 0;
+    for (int c = 0; c < 3; c++)
+        for (int t = 0; t < clean.frames; t++)
+            for (int y = 0; y < clean.height; y++)
+                for (int x = 0; x < clean.width; x++) {
+                    errNoisy += fabs(noisy(x, y, t, c) - clean(x, y, t, c));
+                    errOut += fabs(gpu(x, y, t, c) - clean(x, y, t, c));
+                }
+    return closeTo(gpu, cpu, 1e-4f) && errOut < 0.5 * errNoisy;
+}
+
+class NLMeansGPU : public NLMeans {
+public:
+    void help() {
+        NLMeans::help();
+        pprintf("(GPU plugin: the patch projection and the joint bilateral run on the GPU.)\n");
+    }
+    bool test() { return nlmeansTest(false); }
+    void parse(vector<string> args) {
+        float patchSize, spatialSigma, patchSigma;
+        int dims;
+        parseNLMeansArgs(args, patchSize, dims, spatialSigma, patchSigma);
+        Image filters = PatchPCA::apply(stack(0), patchSize, dims);
+        nlmeansSteps(stack(0), filters, spatialSigma, patchSigma);
+    }
+};
+
+class NLMeans3DGPU : public NLMeans3D {
+public:
+    void help() {
+        NLMeans3D::help();
+        pprintf("(GPU plugin: the patch projection and the joint bilateral run on the GPU.)\n");
+    }
+    bool test() { return nlmeansTest(true); }
+    void parse(vector<string> args) {
+        float patchSize, spatialSigma, patchSigma;
+        int dims;
+        parseNLMeansArgs(args, patchSize, dims, spatialSigma, patchSigma);
+        Image filters = PatchPCA3D::apply(stack(0), patchSize, dims);
+        nlmeansSteps(stack(0), filters, spatialSigma, patchSigma);
+    }
+};
+
 } // namespace GPU
 } // namespace ImageStack
 
@@ -201,4 +292,6 @@ extern "C" void init_imagestack_plugin(map<string, ImageStack::Operation *> &ope
     operationMap["-gausstransform"] = new ImageStack::GPU::GaussTransformGPU();
     operationMap["-jointbilateral"] = new ImageStack::GPU::JointBilateralGPU();
     operationMap["-bilateral"] = new ImageStack::GPU::BilateralGPU();
+    operationMap["-nlmeans"] = new ImageStack::GPU::NLMeansGPU();
+    operationMap["-nlmeans3d"] = new ImageStack::GPU::NLMeans3DGPU();
 }

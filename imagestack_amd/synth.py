@@ -74,6 +74,15 @@ def rect_image(W: int, H: int, C: int = 3, K: int = 64, noise: float = 0.05,
     return img.astype(np.float32)
 
 
+def noisy_rect_image(W: int, H: int, C: int = 3, K: int = 16, amp: float = 0.2, seed: int = 41,
+                     F: int = 1) -> np.ndarray:
+    """Denoising input: K rectangles plus iid U[-amp, amp] noise, the kind of
+    noise NLMeans::test adds (src/GaussTransform.cpp:824-838). (C,F,H,W) f32."""
+    clean = rect_image(W, H, C, K=K, noise=0.0, seed=seed, F=F).astype(np.float64)
+    u = Stream(seed, 3).uniform(clean.size).reshape(clean.shape)
+    return (clean + amp * (2.0 * u - 1.0)).astype(np.float32)
+
+
 def uniform_image(W: int, H: int, C: int = 3, seed: int = 2, F: int = 1) -> np.ndarray:
     s = Stream(seed, 2)
     return s.uniform(C * F * H * W).reshape(C, F, H, W).astype(np.float32)

@@ -42,6 +42,7 @@ extern "C" {
 
 #define GT_MAX_D 16
 #define GT_MAX_VD 16
+#define GT_MAX_PATCH_DIMS 16
 
 enum {
     GT_OK = 0,
@@ -49,7 +50,8 @@ enum {
     GT_ENOMEM = 2,      /* device allocation failed / over the memory budget  */
     GT_ERANGE = 3,      /* lattice key outside int16, or NaN/Inf position     */
     GT_EDEVICE = 4,     /* HIP runtime error                                  */
-    GT_ESTATE = 5       /* call out of order (e.g. export before build)       */
+    GT_ESTATE = 5,      /* call out of order (e.g. export before build)       */
+    GT_EUNSUPPORTED = 6 /* the reference would use a method not on the GPU     */
 };
 
 /* Storage type of lattice values between splat, blur and slice.  Blur and
@@ -168,6 +170,38 @@ int gt_joint_bilateral_grid(gt_ctx *ctx, int W, int H, int F, int C, const float
 int gt_joint_bilateral_grid_host(gt_ctx *ctx, int W, int H, int F, int C, const float *im,
                                  int Cr, const float *ref, float fw, float fh, float ff,
                                  float color_sigma, float *out, gt_stats *stats);
+
+/*
+ * Non-local means: the lattice's second consumer in the reference.
+ *
+ * gt_patch_project replaces Convolve::apply(im, filters, Convolve::Zero,
+ * Multiply::Inner) (src/Convolve.cpp:248-270, convolveSingle :139-173) for a
+ * filter bank of K*C planes of fw x fh x ft (PatchPCA::apply's output,
+ * src/Statistics.cpp:1710-1770, or PatchPCA3D's): output channel k is the sum
+ * over image channels c of the zero-boundary convolution of channel c with
+ * filter plane k*C + c.  Same summation order and roundings as the
+ * reference, so the result is bit-identical to it.  1 <= K <= 16; the filter
+ * volume must be odd.  im: C planes of W*H*F; out: K planes (device).
+ *
+ * gt_nlmeans replaces NLMeans::apply (src/GaussTransform.cpp:867-874) and
+ * NLMeans3D::apply (:934-941) after their PatchPCA call: the guide is the
+ * patch projection of im, then JointBilateral::apply(im, guide, spatial_sigma,
+ * spatial_sigma, INF, patch_sigma) with JointBilateral's AUTO choice
+ * (:553-564): the lattice for 5..10 position channels, the bilateral grid for
+ * 4 or fewer; EXACT (small filters), GKDTREE (more than 10) and the zero-sigma
+ * split return GT_EUNSUPPORTED.  The reference parses a method argument but
+ * does not pass it on (:873), so there is none here.  out (C planes) may be im.
+ * The PCA itself (random patch sampling with the host's rand()) stays with the
+ * caller.
+ */
+int gt_patch_project(gt_ctx *ctx, int W, int H, int F, int C, const float *im, int fw, int fh,
+                     int ft, int K, const float *filters, float *out);
+int gt_nlmeans(gt_ctx *ctx, int W, int H, int F, int C, const float *im, int fw, int fh, int ft,
+               int K, const float *filters, float spatial_sigma, float patch_sigma, float *out,
+               gt_storage storage, const gt_tuning *tune, gt_stats *stats);
+int gt_nlmeans_host(gt_ctx *ctx, int W, int H, int F, int C, const float *im, int fw, int fh,
+                    int ft, int K, const float *filters, float spatial_sigma, float patch_sigma,
+                    float *out, gt_storage storage, const gt_tuning *tune, gt_stats *stats);
 
 /*
  * A batch of joint-bilateral filters: many images of uneven sizes through one

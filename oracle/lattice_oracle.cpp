@@ -660,3 +660,99 @@ int64_t orc_grid_transform(int mode, int n, int d, int vd, const float *splat, i
 }
 
 } // extern "C"
+
+// --- non-local means (NLMeans::apply, src/GaussTransform.cpp:867-874 and
+// NLMeans3D::apply, :934-941) after the PCA ---------------------------------
+namespace {
+
+// Convolve::apply(im, filters, Zero, Multiply::Inner) (src/Convolve.cpp:248-270)
+// for a bank of K*C filter channels (PatchPCA's output, src/Statistics.cpp:
+// 1710-1770): output channel k is the sum over c = 0..C-1, in that order, of
+// convolveSingle(image channel c, filter channel k*C + c) with the zero
+// boundary (:151-173): a true convolution (the filter is read at xoff-dx,
+// yoff-dy, toff-dt), taps outside the image skipped, taps summed in t, y, x
+// order into v = 0, then added to the output.  T = float is the reference's
+// own arithmetic; T = double sums in FP64.
+template <typename T>
+void patch_project(int W, int H, int F, int C, const float *im, int fw, int fh, int ft, int K,
+                   const float *filt, T *out) {
+    const int64_t plane = (int64_t)W * H, n = plane * F;
+    const int64_t fsz = (int64_t)fw * fh * ft;
+    const int xoff = (fw - 1) / 2, yoff = (fh - 1) / 2, toff = (ft - 1) / 2;
+    for (int64_t i = 0; i < (int64_t)K * n; i++) out[i] = 0;
+    for (int i = 0; i < K * C; i++) {
+        const float *src = im + (int64_t)(i % C) * n;
+        const float *w = filt + (int64_t)i * fsz;
+        T *dst = out + (int64_t)(i / C) * n;
+        for (int t = 0; t < F; t++)
+            for (int y = 0; y < H; y++)
+                for (int x = 0; x < W; x++) {
+                    T v = 0;
+                    for (int dt = -toff; dt <= toff; dt++) {
+                        if (t + dt < 0) continue;
+                        if (t + dt >= F) break;
+                        for (int dy = -yoff; dy <= yoff; dy++) {
+                            if (y + dy < 0) continue;
+                            if (y + dy >= H) break;
+                            for (int dx = -xoff; dx <= xoff; dx++) {
+                                if (x + dx < 0) continue;
+                                if (x + dx >= W) break;
+                                const T wv = (T)w[(xoff - dx) + fw * ((yoff - dy) + (int64_t)fh * (toff - dt))];
+                                v += (T)src[(t + dt) * plane + (int64_t)(y + dy) * W + (x + dx)] * wv;
+                            }
+                        }
+                    }
+                    dst[t * plane + (int64_t)y * W + x] += v;
+                }
+    }
+}
+
+} // namespace
+
+extern "C" {
+
+// Patch projection (see patch_project): mode ORC_F32 writes float, the other
+// modes double.  Returns -1 on bad sizes (even filter volume, as Convolve
+// asserts at src/Convolve.cpp:145).
+int orc_patch_project(int mode, int W, int H, int F, int C, const float *im, int fw, int fh, int ft, int K,
+                      const float *filt, void *out) {
+    if (W < 1 || H < 1 || F < 1 || C < 1 || K < 1 || fw < 1 || fh < 1 || ft < 1) return -1;
+    if (((int64_t)fw * fh * ft) % 2 != 1 || mode < 0 || mode > 2) return -1;
+    if (mode == ORC_F32) patch_project<float>(W, H, F, C, im, fw, fh, ft, K, filt, (float *)out);
+    else patch_project<double>(W, H, F, C, im, fw, fh, ft, K, filt, (double *)out);
+    return 0;
+}
+
+// NLMeans::apply after PatchPCA: the guide is the FP32 patch projection
+// (the reference's arithmetic); then JointBilateral::apply(image, guide,
+// sigma_s, sigma_s, INF, patch_sigma) with JointBilateral's AUTO choice
+// (src/GaussTransform.cpp:553-564).  Returns -2 when AUTO picks EXACT or
+// GKDTREE (not restated here), -1 on bad sizes.
+int orc_nlmeans(int mode, int W, int H, int F, int C, const float *im, int fw, int fh, int ft, int K,
+                const float *filt, float spatial_sigma, float patch_sigma, void *out) {
+    if (W < 1 || H < 1 || F < 1 || C < 1 || K < 1 || fw < 1 || fh < 1 || ft < 1) return -1;
+    if (((int64_t)fw * fh * ft) % 2 != 1 || mode < 0 || mode > 2) return -1;
+    float sx = spatial_sigma, sy = spatial_sigma, st = INFINITY;
+    if (W == 1) sx = 0;
+    if (H == 1) sy = 0;
+    if (F == 1) st = 0;
+    if ((W > 1 && sx == 0) || (H > 1 && sy == 0)) return -2;
+    const bool fx = W > 1 && sx < 10 * W, fy = H > 1 && sy < 10 * H, ftm = F > 1 && st < 10 * F;
+    const int pos = K + (fx ? 1 : 0) + (fy ? 1 : 0) + (ftm ? 1 : 0);
+    const int szx = fx ? ((int)(sx * 6 + 1) | 1) : 1, szy = fy ? ((int)(sy * 6 + 1) | 1) : 1;
+    const int szt = ftm ? ((int)(st * 6 + 1) | 1) : 1;
+    if (szx * szy * szt < 16 || pos > 10) return -2;
+    const bool grid = pos <= 4;
+    const int64_t n = (int64_t)W * H * F;
+    std::vector<float> guide((size_t)(K * n));
+    patch_project<float>(W, H, F, C, im, fw, fh, ft, K, filt, guide.data());
+    const int64_t ys = W, ts = (int64_t)W * H;
+    View a{im, W, H, F, C, ys, ts, n}, b{guide.data(), W, H, F, K, ys, ts, n};
+    if (mode == ORC_F32)
+        return jb_rec<float>(mode, a, b, (float *)out, ys, ts, n, spatial_sigma, spatial_sigma, INFINITY,
+                             patch_sigma, grid);
+    return jb_rec<double>(mode, a, b, (double *)out, ys, ts, n, spatial_sigma, spatial_sigma, INFINITY,
+                          patch_sigma, grid);
+}
+
+} // extern "C"

@@ -61,6 +61,10 @@ def lib():
         L.orc_grid_transform.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _fp, C.c_int64, _fp, C.c_int64,
                                          C.c_int, _fp, C.c_int64, _fp, C.c_void_p, C.c_int64]
         L.orc_grid_transform.restype = C.c_int64
+        L.orc_patch_project.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _fp, C.c_int, C.c_int,
+                                        C.c_int, C.c_int, _fp, C.c_void_p]
+        L.orc_nlmeans.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _fp, C.c_int, C.c_int, C.c_int,
+                                  C.c_int, _fp, C.c_float, C.c_float, C.c_void_p]
         _lib = L
     return _lib
 
@@ -197,6 +201,41 @@ def joint_bilateral_grid(im, ref, fw, fh, ff, color_sigma, mode: int = F32):
     return out
 
 
+def patch_project(im, filters, mode: int = F32):
+    """Convolve::apply(im, filters, Zero, Inner) restated: im (C,F,H,W),
+    filters (K*C, ft, fh, fw) -> (K,F,H,W)."""
+    im = np.ascontiguousarray(im, dtype=np.float32)
+    filt = np.ascontiguousarray(filters, dtype=np.float32)
+    Cc, F, H, W = im.shape
+    KC, ft, fh, fw = filt.shape
+    if KC % Cc:
+        raise ValueError("filter channels must be a multiple of the image channels")
+    K = KC // Cc
+    out = np.zeros((K, F, H, W), np.float32 if mode == F32 else np.float64)
+    rc = lib().orc_patch_project(mode, W, H, F, Cc, _ptr(im, _fp), fw, fh, ft, K, _ptr(filt, _fp),
+                                 out.ctypes.data_as(C.c_void_p))
+    if rc < 0:
+        raise ValueError("bad arguments")
+    return out
+
+
+def nlmeans(im, filters, spatial_sigma, patch_sigma, mode: int = F32):
+    """NLMeans::apply after the PCA (FP32 guide, then the joint bilateral
+    in `mode`).  Raises NotImplementedError where AUTO picks EXACT/GKDTREE."""
+    im = np.ascontiguousarray(im, dtype=np.float32)
+    filt = np.ascontiguousarray(filters, dtype=np.float32)
+    Cc, F, H, W = im.shape
+    KC, ft, fh, fw = filt.shape
+    out = np.zeros(im.shape, np.float32 if mode == F32 else np.float64)
+    rc = lib().orc_nlmeans(mode, W, H, F, Cc, _ptr(im, _fp), fw, fh, ft, KC // Cc, _ptr(filt, _fp),
+                           spatial_sigma, patch_sigma, out.ctypes.data_as(C.c_void_p))
+    if rc == -2:
+        raise NotImplementedError("JointBilateral's AUTO choice is EXACT or GKDTREE")
+    if rc < 0:
+        raise ValueError("bad arguments")
+    return out
+
+
 # --- the reference, compiled from its own sources (oracle/_ref) -------------
 
 def ref_available(kind: str = "strict") -> bool:
@@ -215,6 +254,12 @@ def ref_lib(kind: str = "strict"):
                                   _sp, _fp]
         L.ref_grid_transform.argtypes = L.ref_gauss_transform.argtypes
         L.ref_joint_bilateral_grid.argtypes = L.ref_joint_bilateral.argtypes
+        L.ref_convolve_inner.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _fp, C.c_int, C.c_int, C.c_int,
+                                         C.c_int, _fp, _fp]
+        L.ref_patch_pca.argtypes = [C.c_uint, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _fp, C.c_float,
+                                    C.c_int, C.c_longlong, _fp]
+        L.ref_nlmeans.argtypes = [C.c_uint, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _fp, C.c_float, C.c_int,
+                                  C.c_float, C.c_float, _fp]
         _ref[kind] = L
     return _ref[kind]
 
@@ -281,6 +326,49 @@ def ref_lattice(splat, values, inv_sigma, blur: bool = False, kind: str = "stric
     L.ref_lattice(n, d, vd, _ptr(sp, _fp), _ptr(va, _fp), _ptr(inv, _fp), int(blur), M,
                   _ptr(keys, _sp), _ptr(vals, _fp))
     return keys, vals
+
+
+def ref_convolve_inner(im, filters, kind: str = "strict"):
+    """Reference Convolve::apply(im, filters, Zero, Inner): (C,F,H,W) x
+    (Kc,ft,fh,fw) -> (Kc/C,F,H,W)."""
+    im = np.ascontiguousarray(im, dtype=np.float32)
+    filt = np.ascontiguousarray(filters, dtype=np.float32)
+    Cc, F, H, W = im.shape
+    KC, ft, fh, fw = filt.shape
+    out = np.zeros((max(KC // Cc, 1), F, H, W), np.float32)
+    rc = ref_lib(kind).ref_convolve_inner(W, H, F, Cc, _ptr(im, _fp), fw, fh, ft, KC, _ptr(filt, _fp),
+                                          _ptr(out, _fp))
+    if rc < 0:
+        raise RuntimeError("reference raised")
+    return out
+
+
+def ref_patch_pca(im, sigma, dims, seed: int, three_d: bool = False, kind: str = "strict"):
+    """Reference PatchPCA::apply (or PatchPCA3D) after srand(seed): filters
+    (C*dims, ft, side, side)."""
+    im = np.ascontiguousarray(im, dtype=np.float32)
+    Cc, F, H, W = im.shape
+    L = ref_lib(kind)
+    side = L.ref_patch_pca(seed, int(three_d), W, H, F, Cc, _ptr(im, _fp), sigma, dims, 0, None)
+    if side < 0:
+        raise RuntimeError("reference raised")
+    ft = side if three_d else 1
+    filt = np.zeros((Cc * dims, ft, side, side), np.float32)
+    L.ref_patch_pca(seed, int(three_d), W, H, F, Cc, _ptr(im, _fp), sigma, dims, filt.size, _ptr(filt, _fp))
+    return filt
+
+
+def ref_nlmeans(im, sigma, dims, spatial_sigma, patch_sigma, seed: int, three_d: bool = False,
+                kind: str = "strict"):
+    """Reference NLMeans::apply (or NLMeans3D) after srand(seed)."""
+    im = np.ascontiguousarray(im, dtype=np.float32)
+    Cc, F, H, W = im.shape
+    out = np.zeros(im.shape, np.float32)
+    rc = ref_lib(kind).ref_nlmeans(seed, int(three_d), W, H, F, Cc, _ptr(im, _fp), sigma, dims, spatial_sigma,
+                                   patch_sigma, _ptr(out, _fp))
+    if rc != 0:
+        raise RuntimeError("reference raised")
+    return out
 
 
 def sort_keys(keys: np.ndarray) -> np.ndarray:

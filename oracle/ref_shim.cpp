@@ -8,6 +8,9 @@
 #include "Image.h"
 #include "GaussTransform.h"
 #include "Permutohedral.h"
+#include "Convolve.h"
+#include "Arithmetic.h"
+#include "Statistics.h"
 
 using namespace ImageStack;
 
@@ -121,6 +124,56 @@ int ref_lattice(int n, int d, int vd, const float *splat, const float *values,
         if (vals) memcpy(vals, lat.hashTable.getValues(), sizeof(float) * (size_t)M * vd);
     }
     return M;
+}
+
+// Convolve::apply(im, filters, Zero, Multiply::Inner) (src/Convolve.cpp:248-270)
+// with a filter bank of Kc planes of fw x fh x ft; out holds Kc / C planes.
+int ref_convolve_inner(int W, int H, int F, int C, const float *im, int fw, int fh, int ft, int Kc,
+                       const float *filt, float *out) {
+    try {
+        Image a = planar_to_image(W, H, F, C, im);
+        Image f = planar_to_image(fw, fh, ft, Kc, filt);
+        Image o = Convolve::apply(a, f, Convolve::Zero, Multiply::Inner);
+        image_to_planar(o, out);
+        return o.channels;
+    } catch (...) {
+        return -1;
+    }
+}
+
+// PatchPCA::apply(im, sigma, dims) (src/Statistics.cpp:1710-1770), or
+// PatchPCA3D::apply when three_d, after srand(seed): the random patch choice
+// is the reference's own rand() stream.  Returns the patch side; filters
+// (side^2 or side^3 floats per plane, C*dims planes) are written when
+// capacity allows.
+int ref_patch_pca(unsigned seed, int three_d, int W, int H, int F, int C, const float *im, float sigma,
+                  int dims, long long capacity, float *filters) {
+    try {
+        Image a = planar_to_image(W, H, F, C, im);
+        srand(seed);
+        Image f = three_d ? PatchPCA3D::apply(a, sigma, dims) : PatchPCA::apply(a, sigma, dims);
+        if ((long long)f.width * f.height * f.frames * f.channels <= capacity) image_to_planar(f, filters);
+        return f.width;
+    } catch (...) {
+        return -1;
+    }
+}
+
+// NLMeans::apply(im, sigma, dims, spatialSigma, patchSigma) (src/GaussTransform.cpp:867-874),
+// or NLMeans3D::apply when three_d, after srand(seed); the filtered image is
+// written to out.  With the same seed it runs on the filters ref_patch_pca returns.
+int ref_nlmeans(unsigned seed, int three_d, int W, int H, int F, int C, const float *im, float sigma, int dims,
+                float spatialSigma, float patchSigma, float *out) {
+    try {
+        Image a = planar_to_image(W, H, F, C, im);
+        srand(seed);
+        if (three_d) NLMeans3D::apply(a, sigma, dims, spatialSigma, patchSigma);
+        else NLMeans::apply(a, sigma, dims, spatialSigma, patchSigma);
+        image_to_planar(a, out);
+        return 0;
+    } catch (...) {
+        return -1;
+    }
 }
 
 }

@@ -116,6 +116,27 @@ def grid_cases():
                         out_gt_replay=out_gt_replay)
 
 
+def nlmeans_cases():
+    """NLMeans::apply (src/GaussTransform.cpp:867-874) and NLMeans3D::apply
+    (:934-941) with the reference's own PatchPCA after srand(seed): the
+    filters, the patch projection (Convolve zero/inner) and the filtered image.
+    Cases: RGB 64x48 (7x7 patches, 6 dims: a d=8 lattice, NLMeans::test's
+    sigmas 16 and 0.4); RGB volume 24x20x8 (5x5x5 patches, 4 dims: d=6); grey
+    48x40 with 2 dims (4 position channels: AUTO picks the grid)."""
+    rec = {}
+    cases = [("rgb", S.noisy_rect_image(64, 48, 3, K=10, amp=0.2, seed=41), 1.0, 6, 16.0, 0.4, False, 7),
+             ("vol", S.noisy_rect_image(24, 20, 3, K=6, amp=0.2, seed=42, F=8), 0.5, 4, 4.0, 0.3, True, 8),
+             ("grey", S.noisy_rect_image(48, 40, 1, K=8, amp=0.2, seed=43), 1.0, 2, 6.0, 0.15, False, 9)]
+    for name, im, sigma, dims, ss, sp, three_d, seed in cases:
+        filt = O.ref_patch_pca(im, sigma, dims, seed=seed, three_d=three_d)
+        rec[name + "_im"] = im
+        rec[name + "_filters"] = filt
+        rec[name + "_proj"] = O.ref_convolve_inner(im, filt)
+        rec[name + "_out"] = O.ref_nlmeans(im, sigma, dims, ss, sp, seed=seed, three_d=three_d)
+        rec[name + "_params"] = np.array([sigma, dims, ss, sp], np.float32)
+    np.savez_compressed(os.path.join(HERE, "nlmeans_cases.npz"), **rec)
+
+
 def digests():
     """cfg1 (512^2 bilateral) and cfg2 (2048^2 joint bilateral): M and digests."""
     rec = {}
@@ -145,5 +166,6 @@ if __name__ == "__main__":
     zero_sigma_case()
     highdim_small()
     grid_cases()
+    nlmeans_cases()
     digests()
     print("fixtures written to", HERE)

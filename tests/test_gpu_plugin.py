@@ -1,7 +1,8 @@
 """Drop-in through the reference's own plugin interface: the unmodified
 ImageStack CLI (built from the reference sources into oracle/_ref) loads
 imagestack_amd/libgt_plugin.so with -plugin and runs the GPU lattice behind
-its own -gausstransform / -jointbilateral / -bilateral operations."""
+its own -gausstransform / -jointbilateral / -bilateral / -nlmeans / -nlmeans3d
+operations."""
 import os
 import subprocess
 
@@ -115,3 +116,20 @@ def test_plugin_auto_grey_runs_the_grid(tmp_path, oracle_lib):
     assert rel_err(gpu, o64) < FP32_REL_TOL
     assert rel_err(gpu, read_tmp(cpu_path)) < 2 * FP32_REL_TOL
     assert rel_err(read_tmp(grid_path), o64) < FP32_REL_TOL
+
+
+def test_plugin_nlmeans(tmp_path):
+    """-nlmeans / -nlmeans3d through the plugin: their reference-style tests
+    (GPU result within 1e-4 of the reference's NLMeans::apply on the same
+    rand() stream, and noise removed), then the CLI grammar on a .tmp image."""
+    need_cli()
+    out = run(["-plugin", PLUGIN, "-test", "nlmeans", "-test", "nlmeans3d"], timeout=300)
+    assert out.count("Passed") == 2, out
+    assert "Failed" not in out
+    clean = S.rect_image(96, 72, 3, K=12, noise=0.0, seed=33)
+    noisy = S.noisy_rect_image(96, 72, 3, K=12, amp=0.2, seed=33)
+    src, dst = str(tmp_path / "noisy.tmp"), str(tmp_path / "out.tmp")
+    write_tmp(src, noisy)
+    run(["-plugin", PLUGIN, "-load", src, "-nlmeans", "1", "6", "16", "0.4", "-save", dst])
+    res = read_tmp(dst)
+    assert np.abs(res - clean).mean() < 0.5 * np.abs(noisy - clean).mean()
