@@ -130,6 +130,67 @@ def grid_cpu_check(rep, out_gpu, grey):
     rep["rel_err_vs_fp64"] = rel_err(out_gpu, o64)
 
 
+def nlmeans_leg(gtlattice, synth, torch, steps, side):
+    """Non-local means (NLMeans::apply after its PCA): a noisy RGB side^2
+    image, 7x7 patches reduced to 6 dimensions (host PCA, not timed), sigma_s
+    16, patch sigma 0.4 (NLMeans::test's parameters): the patch projection
+    and the d = 8 joint bilateral on the GPU.  Returns (report, output, input,
+    filters)."""
+    from imagestack_amd import nlmeans as NL
+    im = synth.noisy_rect_image(side, side, 3, K=64, amp=0.2, seed=46)
+    t0 = time.perf_counter()
+    filt = NL.patch_pca(im, 1.0, 6, seed=0)
+    pca_ms = (time.perf_counter() - t0) * 1e3
+    d = torch.from_numpy(im).cuda()
+    f = torch.from_numpy(filt).cuda()
+    out = torch.empty_like(d)
+    proj = torch.empty((6,) + tuple(d.shape[1:]), dtype=torch.float32, device=d.device)
+    ctx = gtlattice.Context(0)
+    for _ in range(2):
+        ctx.nlmeans(d, f, 16.0, 0.4, out=out)
+        ctx.patch_project(d, f, out=proj)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        _, st = ctx.nlmeans(d, f, 16.0, 0.4, out=out, stats=True)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.patch_project(d, f, out=proj)
+    torch.cuda.synchronize()
+    proj_ms = (time.perf_counter() - t0) * 1e3 / steps
+    ctx.close()
+    rep = {"workload": f"non-local means on a noisy RGB {side}x{side} image (64 rectangles + U[-0.2,0.2]): "
+                       "7x7 patches, 6 PCA dimensions, sigma_s 16, patch sigma 0.4 (d = 8 lattice); FP32",
+           "ms": round(ms, 4), "value": round(side * side / ms / 1e3, 2), "unit": "Mpix/s", "M": st["M"],
+           "projection_ms": round(proj_ms, 4), "host_pca_ms": round(pca_ms, 1)}
+    return rep, out.cpu().numpy(), im, filt
+
+
+def nlmeans_cpu_check(rep, out_gpu, im, filt):
+    """CPU leg of non-local means: the reference's Convolve + JointBilateral
+    on one core over a 512x512 crop (child process), and the GPU output's
+    error against the FP64 oracle on a 256x256 crop of the same filters."""
+    import numpy as np
+    from oracle import oracle as O
+    from tests.util import rel_err
+    env = dict(os.environ, OMP_NUM_THREADS="1")
+    r = subprocess.run([sys.executable, "-m", "oracle.cpu_baseline", "--side", "512", "--reps", "3",
+                        "--workload", "nlmeans"], cwd=ROOT, env=env, capture_output=True, text=True, timeout=600)
+    base = json.loads(r.stdout.strip().splitlines()[-1])
+    rep["cpu_baseline"] = {"value": round(base["mpix_s"], 4), "unit": "Mpix/s", "cores": 1, "kind": base["kind"],
+                           "sample": "one 512x512 image of the same kind, median of 3 after 1 warm-up",
+                           "median_s": round(base["median_s"], 4)}
+    crop = np.ascontiguousarray(im[:, :, :256, :256])
+    import torch
+    from imagestack_amd import gtlattice
+    ctx = gtlattice.Context(0)
+    g = ctx.nlmeans(torch.from_numpy(crop).cuda(), torch.from_numpy(filt).cuda(), 16.0, 0.4).cpu().numpy()
+    ctx.close()
+    rep["rel_err_vs_fp64_256crop"] = rel_err(g, O.nlmeans(crop, filt, 16.0, 0.4, mode=O.F64_LAT))
+
+
 def batch_leg(gtlattice, synth, torch, steps):
     """cfg4: 64 RGB images of uneven sizes as one batched lattice (one graph
     launch per call), tuned by the autotuner; also the same images one call
@@ -289,6 +350,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--no-highdim", action="store_true", help="skip the d=16 precision leg")
     ap.add_argument("--no-grid", action="store_true", help="skip the bilateral-grid leg")
+    ap.add_argument("--no-nlmeans", action="store_true", help="skip the non-local means leg")
     ap.add_argument("--serve-requests", type=int, default=2000)
     ap.add_argument("--qps", type=float, default=100.0)
     ap.add_argument("--slots", type=int, default=4)
@@ -404,6 +466,9 @@ def main():
     if not a.no_grid:
         progress("bilateral grid leg (grey 2048^2)")
         line["grid"], grid_out, grid_in = grid_leg(gtlattice, synth, torch, max(4, a.steps // 2), a.side)
+    if not a.no_nlmeans:
+        progress("non-local means leg (2048^2 RGB, d = 8)")
+        line["nlmeans"], nl_out, nl_in, nl_filt = nlmeans_leg(gtlattice, synth, torch, max(3, a.steps // 4), a.side)
     if not a.no_highdim:
         progress("d=16 precision leg (1024^2, FP32/BF16/FP16 storage)")
         line["highdim"], hd_crop = highdim_leg(gtlattice, synth, torch, 3)
@@ -421,6 +486,8 @@ def main():
             highdim_oracle_check(line["highdim"], hd_crop)
         if not a.no_grid:
             grid_cpu_check(line["grid"], grid_out, grid_in)
+        if not a.no_nlmeans:
+            nlmeans_cpu_check(line["nlmeans"], nl_out, nl_in, nl_filt)
         line["cpu_baseline"] = cpu
         line["gpu_over_cpu"] = round(value / cpu["value"], 1)
         line["accuracy"] = err

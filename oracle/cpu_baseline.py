@@ -1,7 +1,10 @@
 """CPU baseline for bench.py: times ImageStack's own JointBilateral::apply
 on the host cores: PERMUTOHEDRAL on the 2048^2 joint bilateral (--workload
-lattice), or GRID on a grey 2048^2 bilateral (--workload grid: the guide's
-first channel filtered by itself, sigma_s 16, sigma_r 0.1).
+lattice), GRID on a grey 2048^2 bilateral (--workload grid: the guide's
+first channel filtered by itself, sigma_s 16, sigma_r 0.1), or non-local
+means after the PCA (--workload nlmeans: Convolve zero/inner by 7x7x18
+filters, then the d = 8 joint bilateral, sigma_s 16, patch sigma 0.4, on a
+noisy RGB image; the filters are fixed, so the PCA is not timed).
 TEST/MEASUREMENT INFRASTRUCTURE ONLY.
 
 Kinds:
@@ -37,10 +40,25 @@ def grid_workload(side: int):
     return grey, 16.0, 0.1
 
 
+def nlmeans_workload(side: int):
+    from imagestack_amd import nlmeans as NL
+    from imagestack_amd import synth as S
+    im = S.noisy_rect_image(side, side, 3, K=64, amp=0.2, seed=46)
+    return im, NL.patch_pca(im, 1.0, 6, seed=0), 16.0, 0.4
+
+
 def run_one(side: int, reps: int, kind: str, workload: str = "lattice") -> dict:
     from imagestack_amd import synth as S
     from oracle import oracle as O
-    if workload == "grid":
+    if workload == "nlmeans":
+        im, filt, ss, sp = nlmeans_workload(side)
+        if kind == "reference":
+            def fn():
+                proj = O.ref_convolve_inner(im, filt, kind="fast")
+                return O.ref_joint_bilateral(im, proj, ss, ss, float("inf"), sp, kind="fast")
+        else:
+            fn = lambda: O.nlmeans(im, filt, ss, sp, mode=O.F32)  # noqa: E731
+    elif workload == "grid":
         g, ss, sr = grid_workload(side)
         if kind == "reference":
             fn = lambda: O.ref_joint_bilateral(g, g, ss, ss, 0.0, sr, kind="fast", grid=True)  # noqa: E731
@@ -68,7 +86,7 @@ def main():
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--kind", default="auto", choices=["auto", "reference", "port"])
     ap.add_argument("--procs", type=int, default=1)
-    ap.add_argument("--workload", default="lattice", choices=["lattice", "grid"])
+    ap.add_argument("--workload", default="lattice", choices=["lattice", "grid", "nlmeans"])
     a = ap.parse_args()
     from oracle import oracle as O
     kind = a.kind
