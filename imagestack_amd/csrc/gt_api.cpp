@@ -789,6 +789,8 @@ int jb_view(gt_ctx *c, View im, int C, View ref, int Cr, View out, float fw, flo
 
 namespace gt {
 void invalidate_plan(gt_ctx *c) { c->plan_valid = false; }
+int ctx_device(gt_ctx *c) { return c->device; }
+hipStream_t ctx_stream(gt_ctx *c) { return c->stream; }
 } // namespace gt
 
 extern "C" {

@@ -175,6 +175,10 @@ hipError_t launch_patch_project(const float *im, int W, int H, int F, int C, con
                                 int ft, int K, float *wt, float *out, hipStream_t s);
 size_t patch_lds_bytes(int fw, int fh);
 
+// The context's device and current stream (for code outside gt_api.cpp).
+int ctx_device(gt_ctx *c);
+hipStream_t ctx_stream(gt_ctx *c);
+
 // Sets the thread's gt_last_error() message; returns code.
 int set_error(int code, const char *msg);
 // Forgets the context's cached key-packing plan, so the next call bounds its

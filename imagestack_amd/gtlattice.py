@@ -28,7 +28,14 @@ EXPORTS = [
     "gt_server_create", "gt_server_destroy", "gt_server_submit", "gt_server_wait", "gt_server_replay",
     "gt_gauss_transform_grid", "gt_gauss_transform_grid_host", "gt_joint_bilateral_grid",
     "gt_joint_bilateral_grid_host", "gt_patch_project", "gt_nlmeans", "gt_nlmeans_host",
+    "gt_tmp_info", "gt_tmp_read", "gt_tmp_write", "gt_tmp_load", "gt_tmp_save",
+    "gt_net_server_create", "gt_net_server_port", "gt_net_server_stats", "gt_net_server_destroy",
 ]
+
+# .tmp type codes (src/FileTMP.cpp:14) and their numpy types
+TMP_TYPES = {"float32": 0, "float64": 1, "uint8": 2, "int8": 3, "uint16": 4, "int16": 5, "uint32": 6,
+             "int32": 7, "uint64": 8, "int64": 9}
+GT_NET_ECHO, GT_NET_BILATERAL = 0, 1
 
 
 class GTError(RuntimeError):
@@ -64,6 +71,17 @@ class Request(C.Structure):
 class RequestResult(C.Structure):
     _fields_ = [("status", C.c_int), ("batch", C.c_int), ("latency_ms", C.c_double), ("queue_ms", C.c_double),
                 ("footprint", C.c_size_t)]
+
+
+class NetConfig(C.Structure):
+    _fields_ = [("port", C.c_int), ("reply_port", C.c_int), ("op", C.c_int), ("sigma_s", C.c_float),
+                ("sigma_r", C.c_float), ("storage", C.c_int), ("max_pixels", C.c_longlong),
+                ("serving", ServerConfig)]
+
+
+class NetStats(C.Structure):
+    _fields_ = [("received", C.c_longlong), ("replied", C.c_longlong), ("failed", C.c_longlong),
+                ("p50_ms", C.c_double), ("p99_ms", C.c_double)]
 
 
 class BilateralImage(C.Structure):
@@ -130,6 +148,16 @@ def lib():
                 C.c_float, _vp, C.c_int, C.POINTER(Tuning), C.POINTER(Stats)]
         L.gt_nlmeans.argtypes = nsig
         L.gt_nlmeans_host.argtypes = nsig
+        L.gt_tmp_info.argtypes = [C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.gt_tmp_read.argtypes = [C.c_char_p, _vp, C.c_size_t, C.POINTER(C.c_int)]
+        L.gt_tmp_write.argtypes = [C.c_char_p, _vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.gt_tmp_load.argtypes = [_vp, C.c_char_p, _vp, C.c_size_t, C.POINTER(C.c_int)]
+        L.gt_tmp_save.argtypes = [_vp, C.c_char_p, _vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.gt_net_server_create.argtypes = [C.POINTER(_vp), C.c_int, C.POINTER(NetConfig)]
+        L.gt_net_server_port.argtypes = [_vp]
+        L.gt_net_server_stats.argtypes = [_vp, C.POINTER(NetStats)]
+        L.gt_net_server_destroy.argtypes = [_vp]
+        L.gt_net_server_destroy.restype = None
         L.gt_server_create.argtypes = [C.POINTER(_vp), C.c_int, C.POINTER(ServerConfig)]
         L.gt_server_destroy.argtypes = [_vp]
         L.gt_server_destroy.restype = None
@@ -313,6 +341,23 @@ class Context:
                                      C.byref(st)))
         return out
 
+    def tmp_load(self, path: str):
+        """Loads a .tmp file into a new (C, F, H, W) float32 device tensor
+        (samples cross PCIe in the file's type, converted on the GPU)."""
+        import torch
+        W, H, F, Cc = tmp_info(path)[0]
+        out = torch.empty((Cc, F, H, W), dtype=torch.float32, device="cuda")
+        dims = (C.c_int * 4)()
+        _check(lib().gt_tmp_load(self._h, os.fsencode(path), _dptr(out), out.numel(), dims))
+        return out
+
+    def tmp_save(self, path: str, im, type_name: str = "float32") -> None:
+        """Saves a (C, F, H, W) float32 device tensor as a .tmp file of the
+        given sample type (converted on the GPU)."""
+        Cc, F, H, W = im.shape
+        _check(lib().gt_tmp_save(self._h, os.fsencode(path), _dptr(im.contiguous()), W, H, F, Cc,
+                                 TMP_TYPES[type_name]))
+
     @staticmethod
     def _batch_array(items):
         arr = (BilateralImage * len(items))()
@@ -456,3 +501,80 @@ class Server:
         if rc != 0:
             raise GTError(rc, lib().gt_last_error().decode())
         return out
+
+
+def tmp_info(path: str):
+    """((width, height, frames, channels), type code) of a .tmp file."""
+    dims = (C.c_int * 4)()
+    t = C.c_int()
+    _check(lib().gt_tmp_info(os.fsencode(path), dims, C.byref(t)))
+    return tuple(dims), t.value
+
+
+def tmp_read(path: str) -> np.ndarray:
+    """FileTMP::load on the host: a (C, F, H, W) float32 array."""
+    W, H, F, Cc = tmp_info(path)[0]
+    out = np.empty((Cc, F, H, W), np.float32)
+    dims = (C.c_int * 4)()
+    _check(lib().gt_tmp_read(os.fsencode(path), out.ctypes.data, out.size, dims))
+    return out
+
+
+def tmp_write(path: str, im: np.ndarray, type_name: str = "float32") -> None:
+    """FileTMP::save on the host from a (C, F, H, W) float32 array."""
+    im = np.ascontiguousarray(im, dtype=np.float32)
+    Cc, F, H, W = im.shape
+    _check(lib().gt_tmp_write(os.fsencode(path), im.ctypes.data, W, H, F, Cc, TMP_TYPES[type_name]))
+
+
+class NetServer:
+    """gt_net_server: the -send / -receive protocol front end.  op is
+    GT_NET_ECHO or GT_NET_BILATERAL; reply_port 0 answers on the same
+    connection."""
+
+    def __init__(self, op: int = GT_NET_BILATERAL, port: int = 0, reply_port: int = 0, sigma_s: float = 8.0,
+                 sigma_r: float = 0.1, storage: str = "f32", max_pixels: int = 0, device: int = 0, slots: int = 4,
+                 max_batch: int = 16, warm_side: int = 0):
+        self._h = _vp()
+        cfg = NetConfig(port, reply_port, op, sigma_s, sigma_r, STORAGE[storage], max_pixels,
+                        ServerConfig(slots, max_batch, 1 << 22, 0, warm_side))
+        _check(lib().gt_net_server_create(C.byref(self._h), device, C.byref(cfg)))
+        self.port = lib().gt_net_server_port(self._h)
+
+    def stats(self) -> dict:
+        st = NetStats()
+        _check(lib().gt_net_server_stats(self._h, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in NetStats._fields_}
+
+    def close(self) -> None:
+        if self._h:
+            lib().gt_net_server_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def send_image(sock, im: np.ndarray) -> None:
+    """TCPConnection::sendImage: header (W, H, F, C) uint32, then planar float32."""
+    im = np.ascontiguousarray(im, dtype=np.float32)
+    Cc, F, H, W = im.shape
+    sock.sendall(np.array([W, H, F, Cc], np.uint32).tobytes() + im.tobytes())
+
+
+def recv_image(sock) -> np.ndarray:
+    """TCPConnection::recvImage: the inverse of send_image."""
+    def take(k):
+        buf = bytearray()
+        while len(buf) < k:
+            chunk = sock.recv(min(1 << 20, k - len(buf)))
+            if not chunk:
+                raise ConnectionError("connection closed mid-image")
+            buf += chunk
+        return bytes(buf)
+    W, H, F, Cc = np.frombuffer(take(16), np.uint32)
+    return np.frombuffer(take(4 * int(W) * int(H) * int(F) * int(Cc)), np.float32).reshape(int(Cc), int(F), int(H),
+                                                                                         int(W)).copy()

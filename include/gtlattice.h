@@ -304,6 +304,76 @@ int gt_server_replay(gt_server *server, int count, const gt_request *requests,
                      const double *arrival_ms, gt_request_result *results);
 
 /*
+ * ImageStack .tmp files (src/FileTMP.cpp:75-233).  Header: five int32 (width,
+ * height, frames, channels, type code); samples follow in channel, frame,
+ * row, column order, the planar layout used throughout this library.  A type
+ * code outside 0..9 is read as an old-style file (four-int header, float
+ * samples), as the reference's loader does (:210-214).
+ *
+ * gt_tmp_info / gt_tmp_read / gt_tmp_write work on host memory and need no
+ * GPU; they replace FileTMP::load / FileTMP::save.  gt_tmp_load / gt_tmp_save
+ * move the image between the file and device memory, the samples crossing
+ * PCIe in the file's own type and converted on the GPU.  Conversions are the
+ * reference's C casts as its x86-64 build evaluates them: (float)sample on
+ * load; on save, truncation toward zero through int32 (through int64 for
+ * uint32), keeping the low bits.  dims receives width, height, frames,
+ * channels; capacity is in floats.
+ */
+typedef enum {
+    GT_TMP_F32 = 0, GT_TMP_F64 = 1, GT_TMP_U8 = 2, GT_TMP_I8 = 3, GT_TMP_U16 = 4,
+    GT_TMP_I16 = 5, GT_TMP_U32 = 6, GT_TMP_I32 = 7, GT_TMP_U64 = 8, GT_TMP_I64 = 9
+} gt_tmp_type;
+
+int gt_tmp_info(const char *path, int *dims, int *type);
+int gt_tmp_read(const char *path, float *out, size_t capacity, int *dims);
+int gt_tmp_write(const char *path, const float *im, int W, int H, int F, int C, int type);
+int gt_tmp_load(gt_ctx *ctx, const char *path, float *dev_out, size_t capacity, int *dims);
+int gt_tmp_save(gt_ctx *ctx, const char *path, const float *dev_im, int W, int H, int F, int C,
+                int type);
+
+/*
+ * A TCP front end that speaks ImageStack's own -send / -receive protocol
+ * (TCPConnection::sendImage / recvImage, src/Network.cpp:161-196; Send::apply
+ * and Receive::apply, src/NetworkOps.cpp:79-121): four uint32 (width,
+ * height, frames, channels), then float32 samples in channel, frame, row,
+ * column order; one image per connection.  Each received image is filtered
+ * (GT_NET_BILATERAL: -bilateral sigma_r sigma_s sigma_s 0 permutohedral,
+ * i.e. JointBilateral::apply(im, im, ...)) or returned as is (GT_NET_ECHO)
+ * and sent back in the same format: on the same connection when reply_port
+ * is 0, else on a new connection to the sender's address at reply_port,
+ * which is what an unmodified client running
+ *     ImageStack -load a.tmp -send HOST PORT -receive REPLY_PORT -save b.tmp
+ * listens for.  Bilateral images of one frame and 1 or 3 channels go through
+ * the serving loop (gt_server, configured by `serving`); other shapes run on
+ * the front end's own context.  Latency is measured from the last byte
+ * received to the last byte of the reply sent.  No reference counterpart for
+ * the server itself: the reference's -receive takes one image per call.
+ */
+typedef enum { GT_NET_ECHO = 0, GT_NET_BILATERAL = 1 } gt_net_op;
+
+typedef struct {
+    int port;                    /* listen port; 0 = any free port             */
+    int reply_port;              /* 0: reply on the same connection            */
+    int op;                      /* gt_net_op                                  */
+    float sigma_s, sigma_r;      /* bilateral spatial and colour sigmas        */
+    gt_storage storage;
+    long long max_pixels;        /* largest accepted W*H*F (default 2^26)      */
+    gt_server_config serving;    /* the request loop behind GT_NET_BILATERAL   */
+} gt_net_config;
+
+typedef struct {
+    long long received, replied, failed;
+    double p50_ms, p99_ms;       /* over the replied images                    */
+} gt_net_stats;
+
+typedef struct gt_net_server gt_net_server;
+
+int gt_net_server_create(gt_net_server **server, int device, const gt_net_config *config);
+int gt_net_server_port(gt_net_server *server);
+int gt_net_server_stats(gt_net_server *server, gt_net_stats *stats);
+void gt_net_server_destroy(gt_net_server *server);
+
+/*
  * Lattice inspection for parity checks: splat (and blur) the given points and
  * keep the lattice in the context.  Replaces reading
  * PermutohedralLattice::hashTable (src/Permutohedral.h:505-507).

@@ -137,6 +137,38 @@ def nlmeans_cases():
     np.savez_compressed(os.path.join(HERE, "nlmeans_cases.npz"), **rec)
 
 
+TMP_TYPES = ["float32", "float64", "uint8", "int8", "uint16", "int16", "uint32", "int32", "uint64", "int64"]
+
+
+def tmp_cases():
+    """The .tmp format as the reference writes and reads it (FileTMP::save /
+    load, src/FileTMP.cpp:75-233), through the reference CLI: a 7x5x2x3 float
+    image with fractional, negative and mid-range samples saved as every type
+    (tmp/as_<type>.tmp), and each of those loaded back and saved as float32
+    (tmp/back_<type>.tmp)."""
+    import subprocess
+    import tempfile
+    cli = os.path.join(ROOT, "oracle", "_ref", "ImageStack")
+    d = os.path.join(HERE, "tmp")
+    os.makedirs(d, exist_ok=True)
+    s = S.Stream(17)
+    im = (s.uniform(3 * 2 * 5 * 7) * 220.0 - 100.0).reshape(3, 2, 5, 7).astype(np.float32)
+    im.reshape(-1)[:6] = [0.0, -0.0, 0.5, -0.5, 99.99, -99.99]
+    with tempfile.TemporaryDirectory() as tdir:
+        src = os.path.join(tdir, "in.tmp")
+        with open(src, "wb") as f:
+            np.array([7, 5, 2, 3, 0], np.int32).tofile(f)
+            im.tofile(f)
+        for t in TMP_TYPES:
+            out = os.path.join(d, f"as_{t}.tmp")
+            subprocess.run([cli, "-load", src, "-save", out, t], check=True, capture_output=True)
+            subprocess.run([cli, "-load", out, "-save", os.path.join(d, f"back_{t}.tmp")], check=True,
+                           capture_output=True)
+        with open(os.path.join(d, "source.tmp"), "wb") as f:
+            np.array([7, 5, 2, 3, 0], np.int32).tofile(f)
+            im.tofile(f)
+
+
 def digests():
     """cfg1 (512^2 bilateral) and cfg2 (2048^2 joint bilateral): M and digests."""
     rec = {}
@@ -167,5 +199,6 @@ if __name__ == "__main__":
     highdim_small()
     grid_cases()
     nlmeans_cases()
+    tmp_cases()
     digests()
     print("fixtures written to", HERE)
