@@ -338,6 +338,73 @@ def serve_leg(gtlattice, synth, torch, n_req, qps, slots, storages=("f32", "bf16
     return out
 
 
+def io_leg(gtlattice, synth, torch):
+    """The formats either side of the path: .tmp files of a 2048^2 RGB image
+    (float32 and uint8 samples, in the page cache) loaded into and saved from
+    HBM, and the -send/-receive front end filtering 512^2 RGB images
+    (bilateral, sigma_s 8, sigma_r 0.1) for 4 closed-loop socket clients over
+    loopback; latency = first byte sent -> last byte of the reply received."""
+    import socket
+    import tempfile
+    import threading
+    import numpy as np
+    rep = {}
+    im = synth.rect_image(2048, 2048, 3, K=256, noise=0.05, seed=1)
+    ctx = gtlattice.Context(0)
+    with tempfile.TemporaryDirectory() as tdir:
+        for t, data in (("float32", im), ("uint8", np.clip(im * 255.0, 0, 255))):
+            path = os.path.join(tdir, f"im_{t}.tmp")
+            gtlattice.tmp_write(path, data, t)
+            nbytes = os.path.getsize(path)
+            d = ctx.tmp_load(path)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(5):
+                d = ctx.tmp_load(path)
+            load_ms = (time.perf_counter() - t0) * 1e3 / 5
+            out = os.path.join(tdir, f"out_{t}.tmp")
+            t0 = time.perf_counter()
+            for _ in range(3):
+                ctx.tmp_save(out, d, t)
+            save_ms = (time.perf_counter() - t0) * 1e3 / 3
+            rep[f"tmp_{t}"] = {"file_bytes": nbytes, "load_ms": round(load_ms, 3),
+                               "load_GB_s": round(nbytes / load_ms / 1e6, 2), "save_ms": round(save_ms, 3),
+                               "save_GB_s": round(nbytes / save_ms / 1e6, 2)}
+    ctx.close()
+    srv = gtlattice.NetServer(op=gtlattice.GT_NET_BILATERAL, sigma_s=8.0, sigma_r=0.1, slots=4, warm_side=512)
+    crops = [np.ascontiguousarray(im[:, :, y:y + 512, x:x + 512]) for y in (0, 512, 1024) for x in (0, 768, 1536)]
+    lat, lock = [], threading.Lock()
+
+    def client(k, n):
+        for i in range(n):
+            c = crops[(k + i) % len(crops)]
+            t0 = time.perf_counter()
+            with socket.create_connection(("127.0.0.1", srv.port), timeout=60) as s:
+                gtlattice.send_image(s, c)
+                gtlattice.recv_image(s)
+            with lock:
+                lat.append((time.perf_counter() - t0) * 1e3)
+    client(0, 8)   # warm-up
+    lat.clear()
+    n_clients, per = 4, 50
+    t0 = time.perf_counter()
+    th = [threading.Thread(target=client, args=(k, per)) for k in range(n_clients)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    a = np.array(lat)
+    rep["net"] = {"requests": len(lat), "clients": n_clients, "p50_ms": round(float(np.percentile(a, 50)), 3),
+                  "p99_ms": round(float(np.percentile(a, 99)), 3),
+                  "mpix_s": round(len(lat) * 512 * 512 / wall / 1e6, 2),
+                  "server_p50_ms": round(st["p50_ms"], 3), "server_p99_ms": round(st["p99_ms"], 3),
+                  "failed": st["failed"]}
+    return rep
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -351,6 +418,7 @@ def main():
     ap.add_argument("--no-highdim", action="store_true", help="skip the d=16 precision leg")
     ap.add_argument("--no-grid", action="store_true", help="skip the bilateral-grid leg")
     ap.add_argument("--no-nlmeans", action="store_true", help="skip the non-local means leg")
+    ap.add_argument("--no-io", action="store_true", help="skip the .tmp and -send/-receive leg")
     ap.add_argument("--serve-requests", type=int, default=2000)
     ap.add_argument("--qps", type=float, default=100.0)
     ap.add_argument("--slots", type=int, default=4)
@@ -469,6 +537,9 @@ def main():
     if not a.no_nlmeans:
         progress("non-local means leg (2048^2 RGB, d = 8)")
         line["nlmeans"], nl_out, nl_in, nl_filt = nlmeans_leg(gtlattice, synth, torch, max(3, a.steps // 4), a.side)
+    if not a.no_io:
+        progress(".tmp and -send/-receive leg")
+        line["io"] = io_leg(gtlattice, synth, torch)
     if not a.no_highdim:
         progress("d=16 precision leg (1024^2, FP32/BF16/FP16 storage)")
         line["highdim"], hd_crop = highdim_leg(gtlattice, synth, torch, 3)

@@ -145,7 +145,7 @@ void free_buf(DevBuf &b) {
 
 struct Tune {
     float lf = 0.35f;
-    int dedup = 0;      // 0 auto (LDS tiles when d <= 6 and vd <= 4), 1 on, -1 off
+    int dedup = 0;      // 0 auto (LDS tiles when d <= 10 and vd <= 4), 1 on, -1 off
     int tile_w = 32;
     int wpc = 0;        // waves-per-CU cap of the tiled and neighbour kernels (0: none)
     bool graph = true;  // replay the pipeline as a cached hipGraph
@@ -397,7 +397,7 @@ int run(gt_ctx *c, const gt::PixSrc &src, int n, int row_w, int d, int vd, int s
     hipStream_t s = c->stream;
     const bool bilat = batch ? true : src.ref != nullptr;
     const bool count = c->profiling >= 2;
-    const bool dedup = batch ? true : (tu.dedup == 0) ? (d <= 6 && vd <= 4) : (tu.dedup > 0);
+    const bool dedup = batch ? true : (tu.dedup == 0) ? (d <= 10 && vd <= 4) : (tu.dedup > 0);
     const int kind = batch ? 2 : bilat ? 1 : 0;
 
     uint64_t est;

@@ -50,6 +50,11 @@ inline int64_t pixel_blocks(int n, int row_w, int &tile_w) {
 }
 
 
+// Splat launcher for d = 7..10 with the LDS tile table (gt_launch_mid.hip).
+hipError_t launch_splat_tile_mid(int d, bool packed, bool bilat, const PixSrc &src, int n, int row_w, int tile_w,
+                                 int vd, const LatConst &lc, const KeyPlan &kp, const ValScale &vsc,
+                                 const TableArgs &ta, Counters *ctr, bool count, int wpc, hipStream_t st);
+
 // Launchers for the lattice dimensions without an LDS tile table (d > 6).
 hipError_t launch_splat_direct_hi(int d, bool packed, bool bilat, const PixSrc &src, int n, int row_w, int tile_w,
                                   int vd, const LatConst &lc, const KeyPlan &kp, const ValScale &vsc,

@@ -1,5 +1,6 @@
 // Splat launchers: LDS-tiled splat for d <= 6 and vd <= 4 (single image or a
-// batch), direct splat otherwise (d > 6 in gt_launch_splat_hi.hip).
+// batch; d = 7..10 in gt_launch_mid.hip), direct splat otherwise (d > 6 in
+// gt_launch_splat_hi.hip).
 #include "gt_launch.h"
 
 namespace gt {
@@ -7,8 +8,12 @@ namespace gt {
 hipError_t launch_splat(int d, bool packed, bool bilat, bool dedup, const PixSrc &src, int n, int row_w, int tile_w,
                         int vd, const LatConst &lc, const KeyPlan &kp, const ValScale &vsc, const TableArgs &ta,
                         Counters *ctr, bool count, int wpc, hipStream_t st) {
-    if (d > 6)
+    if (d > 6) {
+        if (d <= 10 && dedup && vd <= 4)
+            return launch_splat_tile_mid(d, packed, bilat, src, n, row_w, tile_w, vd, lc, kp, vsc, ta, ctr, count,
+                                         wpc, st);
         return launch_splat_direct_hi(d, packed, bilat, src, n, row_w, tile_w, vd, lc, kp, vsc, ta, ctr, count, st);
+    }
     const int64_t ntiles = pixel_blocks(n, row_w, tile_w);
     if (ntiles == 0) return hipSuccess;
     const int cnt = count ? 1 : 0;

@@ -136,10 +136,43 @@ struct DevArena {
     }
 };
 
+// Pinned host staging of one worker, for requests whose pixels live in
+// pageable memory: a 2-D copy from pageable memory goes row by row through
+// the runtime's own bounce buffer (a 512^2 RGB upload took ~14 ms that way),
+// so such planes are packed here by the CPU and cross PCIe in one copy.
+struct HostStage {
+    float *p = nullptr;
+    size_t cap = 0;
+    int ensure(size_t floats) {
+        if (floats <= cap) return GT_OK;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+        if (hipHostMalloc((void **)&p, floats * sizeof(float), hipHostMallocDefault) != hipSuccess) {
+            (void)hipGetLastError();
+            return GT_ENOMEM;
+        }
+        cap = floats;
+        return GT_OK;
+    }
+    ~HostStage() {
+        if (p) (void)hipHostFree(p);
+    }
+};
+
+bool is_pinned(const void *ptr) {
+    hipPointerAttribute_t a{};
+    if (hipPointerGetAttributes(&a, ptr) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return a.type == hipMemoryTypeHost;
+}
+
 // Runs one group on the worker's context and stream; returns a status per
 // request in `rc` and a message in `err`.
-void run_group(gt_ctx *ctx, hipStream_t st, DevArena &arena, const std::vector<Req *> &g, std::vector<int> &rc,
-               std::vector<std::string> &err) {
+void run_group(gt_ctx *ctx, hipStream_t st, DevArena &arena, HostStage &stage, const std::vector<Req *> &g,
+               std::vector<int> &rc, std::vector<std::string> &err) {
     const int C = g[0]->r.C;
     const gt_storage storage = g[0]->r.storage;
     size_t total = 0;
@@ -152,21 +185,38 @@ void run_group(gt_ctx *ctx, hipStream_t st, DevArena &arena, const std::vector<R
     };
     if (arena.ensure(2 * total) != GT_OK) return fail_all(GT_ENOMEM, "server: no device memory for the planes");
     float *din = arena.p, *dout = arena.p + total;
+    std::vector<char> in_pinned(g.size()), out_pinned(g.size());
+    bool staged = false;
+    for (size_t i = 0; i < g.size(); i++) {
+        in_pinned[i] = is_pinned(g[i]->r.im);
+        out_pinned[i] = is_pinned(g[i]->r.out);
+        staged = staged || !in_pinned[i] || !out_pinned[i];
+    }
+    if (staged && stage.ensure(total) != GT_OK) return fail_all(GT_ENOMEM, "server: no pinned host staging");
     size_t off = 0;
     std::vector<gt_bilateral_image> imgs;
-    for (Req *q : g) {
-        const gt_request &r = q->r;
-        for (int c = 0; c < C; c++) {
-            const hipError_t e = hipMemcpy2DAsync(din + off + (size_t)c * r.W * r.H, (size_t)r.W * 4,
-                                                  r.im + (size_t)c * r.plane_pitch, (size_t)r.row_pitch * 4,
-                                                  (size_t)r.W * 4, (size_t)r.H, hipMemcpyHostToDevice, st);
-            if (e != hipSuccess) {
-                (void)hipGetLastError();
-                return fail_all(GT_EDEVICE, std::string("server: upload failed: ") + hipGetErrorString(e));
-            }
+    for (size_t i = 0; i < g.size(); i++) {
+        const gt_request &r = g[i]->r;
+        const size_t plane = (size_t)r.W * r.H;
+        hipError_t e = hipSuccess;
+        if (in_pinned[i]) {
+            for (int c = 0; c < C && e == hipSuccess; c++)
+                e = hipMemcpy2DAsync(din + off + c * plane, (size_t)r.W * 4, r.im + (size_t)c * r.plane_pitch,
+                                     (size_t)r.row_pitch * 4, (size_t)r.W * 4, (size_t)r.H, hipMemcpyHostToDevice, st);
+        } else {
+            float *h = stage.p + off;
+            for (int c = 0; c < C; c++)
+                for (int y = 0; y < r.H; y++)
+                    memcpy(h + c * plane + (size_t)y * r.W, r.im + (size_t)c * r.plane_pitch + (size_t)y * r.row_pitch,
+                           (size_t)r.W * 4);
+            e = hipMemcpyAsync(din + off, h, plane * C * 4, hipMemcpyHostToDevice, st);
+        }
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            return fail_all(GT_EDEVICE, std::string("server: upload failed: ") + hipGetErrorString(e));
         }
         imgs.push_back(gt_bilateral_image{r.W, r.H, din + off, din + off, dout + off, r.sigma_s, r.sigma_r});
-        off += (size_t)r.W * r.H * C;
+        off += plane * C;
     }
     gt::invalidate_plan(ctx);
     gt_tuning tu{};
@@ -184,7 +234,9 @@ void run_group(gt_ctx *ctx, hipStream_t st, DevArena &arena, const std::vector<R
     for (size_t i = 0; i < g.size(); i++) {
         const gt_request &r = g[i]->r;
         const size_t bytes = (size_t)r.W * r.H * C * 4;
-        const hipError_t e = hipMemcpyAsync(r.out, dout + off, bytes, hipMemcpyDeviceToHost, st);
+        // the input staging of a request is free again once its upload ran: reuse it for the download
+        void *dst = out_pinned[i] ? (void *)r.out : (void *)(stage.p + off);
+        const hipError_t e = hipMemcpyAsync(dst, dout + off, bytes, hipMemcpyDeviceToHost, st);
         if (e != hipSuccess) {
             (void)hipGetLastError();
             return fail_all(GT_EDEVICE, std::string("server: download failed: ") + hipGetErrorString(e));
@@ -195,6 +247,12 @@ void run_group(gt_ctx *ctx, hipStream_t st, DevArena &arena, const std::vector<R
     if (e != hipSuccess) {
         (void)hipGetLastError();
         return fail_all(GT_EDEVICE, std::string("server: stream failed: ") + hipGetErrorString(e));
+    }
+    off = 0;
+    for (size_t i = 0; i < g.size(); i++) {
+        const size_t n = (size_t)g[i]->r.W * g[i]->r.H * C;
+        if (!out_pinned[i]) memcpy(g[i]->r.out, stage.p + off, n * 4);
+        off += n;
     }
     for (size_t i = 0; i < g.size(); i++) rc[i] = GT_OK;
 }
@@ -207,12 +265,13 @@ void worker_main(gt_server *s, std::mutex *init_mu, int *init_left, std::conditi
     if (rc == GT_OK && hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) rc = GT_EDEVICE;
     if (rc == GT_OK) rc = gt_ctx_set_stream(ctx, st);
     DevArena arena;
+    HostStage stage;
     // Warm-up: one dense RGB filter of the configured size grows the context's
     // lattice buffers and the arena to their serving size.
     if (rc == GT_OK && s->cfg.warm_side > 0) {
         const int side = s->cfg.warm_side;
         const size_t n = (size_t)side * side;
-        if (arena.ensure(6 * n) != GT_OK) {
+        if (arena.ensure(6 * n) != GT_OK || stage.ensure(3 * n) != GT_OK) {
             rc = GT_ENOMEM;
         } else if (hipMemcpyAsync(arena.p, s->warm.data(), 3 * n * sizeof(float), hipMemcpyHostToDevice, st) !=
                    hipSuccess) {
@@ -246,7 +305,7 @@ void worker_main(gt_server *s, std::mutex *init_mu, int *init_left, std::conditi
         const double t0 = now_ms();
         std::vector<int> st_rc(g.size(), GT_OK);
         std::vector<std::string> st_err(g.size());
-        run_group(ctx, st, arena, g, st_rc, st_err);
+        run_group(ctx, st, arena, stage, g, st_rc, st_err);
         const double t1 = now_ms();
         {
             std::lock_guard<std::mutex> lk(s->mu);
