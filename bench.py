@@ -91,9 +91,9 @@ def progress(msg):
 
 def grid_leg(gtlattice, synth, torch, steps, side):
     """The GRID method (bilateral grid, what AUTO picks for grey images) on a
-    grey side^2 bilateral: the cfg2 guide's first channel filtered by itself,
+    grey side^2 bilateral: the joint-bilateral guide's first channel filtered by itself,
     sigma_s 16, sigma_r 0.1.  Returns (report, GPU output)."""
-    _, guide, _, _ = synth.cfg2(side=side)
+    _, guide, _, _ = synth.joint_bilateral_2048(side=side)
     grey = guide[:1].copy()
     d = torch.from_numpy(grey).cuda()
     out = torch.empty_like(d)
@@ -192,10 +192,10 @@ def nlmeans_cpu_check(rep, out_gpu, im, filt):
 
 
 def batch_leg(gtlattice, synth, torch, steps):
-    """cfg4: 64 RGB images of uneven sizes as one batched lattice (one graph
+    """64 RGB images of uneven sizes as one batched lattice (one graph
     launch per call), tuned by the autotuner; also the same images one call
     each, for comparison."""
-    imgs = synth.cfg4()
+    imgs = synth.batch_images()
     dev = [torch.from_numpy(im[:, None]).cuda() for im, _, _ in imgs]
     outs = [torch.empty_like(d) for d in dev]
     items = [(d, d, o, ss, sr) for d, o, (_, ss, sr) in zip(dev, outs, imgs)]
@@ -228,7 +228,7 @@ def batch_leg(gtlattice, synth, torch, steps):
 
 
 def highdim_leg(gtlattice, synth, torch, steps):
-    """cfg3: d=16 Gauss transform at 1024x1024 (14 N(0,1) features + x/12,
+    """The d=16 Gauss transform at 1024x1024 (14 N(0,1) features + x/12,
     y/12; 4 values U[0,1]; unnormalised output), lattice values stored in
     FP32, BF16 and FP16 with FP32 arithmetic.  At full size each narrow
     storage is compared with the FP32-stored result; the GPU outputs on a 96x96
@@ -289,7 +289,7 @@ def highdim_oracle_check(report, crop):
 
 
 def serve_leg(gtlattice, synth, torch, n_req, qps, slots, storages=("f32", "bf16")):
-    """cfg5: open-loop Poisson replay through the request server; latency is
+    """Open-loop Poisson replay through the request server; latency is
     submit -> result in (pinned) host memory, PCIe copies included."""
     import numpy as np
     pool_side = 4096
@@ -314,12 +314,12 @@ def serve_leg(gtlattice, synth, torch, n_req, qps, slots, storages=("f32", "bf16
                        "40% RGB (d=5, vd=4); sigma_s in {4,16,32}, sigma_r in {0.05,0.2}; crops of a pinned "
                        "4096^2 host image; latency = submit -> result in pinned host memory (PCIe included)",
            "qps": qps, "slots": slots, "requests": n_req}
-    warm = synth.cfg5_schedule(100, qps, seed=50)
+    warm = synth.serving_schedule(100, qps, seed=50)
     srv.replay(requests(warm, "f32"), warm["arrival_ms"])
     for storage in storages:
         n = n_req if storage == "f32" else max(200, n_req // 2)
         progress(f"serving {n} {storage} requests")
-        sched = synth.cfg5_schedule(n, qps, seed=5 if storage == "f32" else 6)
+        sched = synth.serving_schedule(n, qps, seed=5 if storage == "f32" else 6)
         reqs = requests(sched, storage)
         t0 = time.perf_counter()
         res = srv.replay(reqs, sched["arrival_ms"])
@@ -437,7 +437,7 @@ def main():
     from imagestack_amd import gtlattice, synth
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
-    im, guide, ss, sr = synth.cfg2(side=a.side)
+    im, guide, ss, sr = synth.joint_bilateral_2048(side=a.side)
     C, F, H, W = im.shape
     N = W * H * F
     d_im = torch.from_numpy(im).cuda()

@@ -130,13 +130,13 @@ def highdim_positions(W: int, H: int, nfeat: int = 14, sigma_s: float = 12.0,
     return np.concatenate([feats, xx[None], yy[None]]).astype(f32)
 
 
-def cfg1(seed: int = 1):
+def bilateral_512(seed: int = 1):
     """512x512 RGB bilateral: 64 rectangles + N(0,0.05); sigma_s=8, sigma_r=0.1."""
     im = rect_image(512, 512, 3, K=64, noise=0.05, seed=seed)
     return im, im, 8.0, 0.1
 
 
-def cfg2(seed: int = 1, side: int = 2048):
+def joint_bilateral_2048(seed: int = 1, side: int = 2048):
     """2048^2 joint bilateral: guide 256 rectangles + N(0,0.03), image U[0,1];
     sigma_s=16, sigma_r=0.1."""
     guide = rect_image(side, side, 3, K=256, noise=0.03, seed=seed)
@@ -144,9 +144,9 @@ def cfg2(seed: int = 1, side: int = 2048):
     return im, guide, 16.0, 0.1
 
 
-def cfg4(seed: int = 4, count: int = 64):
+def batch_images(seed: int = 4, count: int = 64):
     """Batch of `count` RGB bilateral images: sides log-uniform in [256, 1536],
-    drawn independently per axis; content as cfg1 (64 rectangles +
+    drawn independently per axis; content as bilateral_512 (64 rectangles +
     N(0, 0.05)); sigma_s from {4, 8, 16} and sigma_r from {0.05, 0.1, 0.2}
     per image.  Returns [(im (3, H, W), sigma_s, sigma_r)]."""
     s = Stream(seed, 11)
@@ -162,7 +162,7 @@ def cfg4(seed: int = 4, count: int = 64):
     return out
 
 
-def cfg5_schedule(n: int, qps: float, seed: int = 5, pool_side: int = 4096):
+def serving_schedule(n: int, qps: float, seed: int = 5, pool_side: int = 4096):
     """Serving workload: Poisson arrivals at `qps`; square sides log-uniform
     in [128, 4096]; 60% grey (C=1, d=3, vd=2) and 40% RGB (C=3, d=5, vd=4);
     sigma_s from {4, 16, 32}, sigma_r from {0.05, 0.2}.  Each request is a crop

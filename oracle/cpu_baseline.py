@@ -35,7 +35,7 @@ if ROOT not in sys.path:
 
 def grid_workload(side: int):
     from imagestack_amd import synth as S
-    _, guide, _, _ = S.cfg2(side=side)
+    _, guide, _, _ = S.joint_bilateral_2048(side=side)
     grey = guide[:1].copy()
     return grey, 16.0, 0.1
 
@@ -65,7 +65,7 @@ def run_one(side: int, reps: int, kind: str, workload: str = "lattice") -> dict:
         else:
             fn = lambda: O.joint_bilateral_grid(g, g, ss, ss, 0.0, sr, mode=O.F32)  # noqa: E731
     else:
-        im, guide, ss, sr = S.cfg2(side=side)
+        im, guide, ss, sr = S.joint_bilateral_2048(side=side)
         if kind == "reference":
             fn = lambda: O.ref_joint_bilateral(im, guide, ss, ss, 0.0, sr, kind="fast")  # noqa: E731
         else:

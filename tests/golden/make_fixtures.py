@@ -170,20 +170,20 @@ def tmp_cases():
 
 
 def digests():
-    """cfg1 (512^2 bilateral) and cfg2 (2048^2 joint bilateral): M and digests."""
+    """The 512^2 bilateral and the 2048^2 joint bilateral: M and digests."""
     rec = {}
-    im, ref, ss, sr = S.cfg1()
+    im, ref, ss, sr = S.bilateral_512()
     pos = S.bilateral_positions(ref, ss, sr)
     val = S.homogeneous_values(im)
     keys, _ = sorted_ref_lattice(pos, val, np.ones(5, np.float32), blur=False)
     out = O.ref_joint_bilateral(im, ref, ss, ss, 0.0, sr)
-    rec["cfg1_bilateral_512"] = {"M": int(keys.shape[0]), "keys_sha": digest(keys), "out_sha": digest(out),
+    rec["bilateral_512"] = {"M": int(keys.shape[0]), "keys_sha": digest(keys), "out_sha": digest(out),
                                  "input_sha": digest(im)}
-    im, ref, ss, sr = S.cfg2()
+    im, ref, ss, sr = S.joint_bilateral_2048()
     pos = S.bilateral_positions(ref, ss, sr)
     val = S.homogeneous_values(im)
     keys, _ = sorted_ref_lattice(pos, val, np.ones(5, np.float32), blur=False)
-    rec["cfg2_jointbilateral_2048"] = {"M": int(keys.shape[0]), "keys_sha": digest(keys),
+    rec["joint_bilateral_2048"] = {"M": int(keys.shape[0]), "keys_sha": digest(keys),
                                        "guide_sha": digest(ref), "image_sha": digest(im)}
     with open(os.path.join(HERE, "digests.json"), "w") as f:
         json.dump(rec, f, indent=1, sort_keys=True)

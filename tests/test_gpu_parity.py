@@ -78,11 +78,11 @@ def test_replay_bilateral_small(gpu_ctx, oracle_lib):
     assert e < FP32_REL_TOL
 
 
-def test_joint_bilateral_cfg1(gpu_ctx, oracle_lib):
+def test_joint_bilateral_512(gpu_ctx, oracle_lib):
     """512x512 RGB bilateral, sigma_s=8, sigma_r=0.1 (normalised output)."""
     O = oracle_lib
-    case("joint bilateral cfg1 512x512")
-    im, ref, ss, sr = S.cfg1()
+    case("joint bilateral 512x512")
+    im, ref, ss, sr = S.bilateral_512()
     out = gpu_ctx.joint_bilateral(dev(im), dev(ref), ss, ss, 0.0, sr).cpu().numpy()
     o64 = O.joint_bilateral(im, ref, ss, ss, 0.0, sr, mode=O.F64_LAT)
     e = rel_err(out, o64)
@@ -90,10 +90,10 @@ def test_joint_bilateral_cfg1(gpu_ctx, oracle_lib):
     assert e < FP32_REL_TOL
 
 
-def test_cfg1_key_set(gpu_ctx, oracle_lib):
+def test_bilateral_512_key_set(gpu_ctx, oracle_lib):
     O = oracle_lib
-    case("cfg1 key set and neighbour table")
-    im, ref, ss, sr = S.cfg1()
+    case("512x512 bilateral key set and neighbour table")
+    im, ref, ss, sr = S.bilateral_512()
     pos = planar(S.bilateral_positions(ref, ss, sr))
     val = planar(S.homogeneous_values(im))
     inv = np.ones(5, np.float32)

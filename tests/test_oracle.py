@@ -266,9 +266,9 @@ def test_golden_highdim(oracle_lib):
     assert np.array_equal(lat["keys"], f["keys"])
 
 
-def test_golden_cfg1_digest(oracle_lib):
-    rec = json.load(open(os.path.join(GOLD, "digests.json")))["cfg1_bilateral_512"]
-    im, ref, ss, sr = S.cfg1()
+def test_golden_bilateral_512_digest(oracle_lib):
+    rec = json.load(open(os.path.join(GOLD, "digests.json")))["bilateral_512"]
+    im, ref, ss, sr = S.bilateral_512()
     assert digest(im) == rec["input_sha"]
     pos = S.bilateral_positions(ref, ss, sr)
     val = S.homogeneous_values(im)
