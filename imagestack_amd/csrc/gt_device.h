@@ -288,9 +288,16 @@ __device__ __forceinline__ int lds_insert(uint64_t *lk, const uint64_t (&k)[W]) 
     return -1;
 }
 
-// x * s rounded to the nearest integer, as a two's-complement 64-bit word.
+// x * s rounded to the nearest integer (ties to even), as a two's-complement
+// 64-bit word; |x * s| < 2^62.  y = rint(x * s) is an integer-valued float
+// with at most 24 significant bits, so hi = floor(y / 2^32) and the low word
+// lo = y - hi * 2^32 in [0, 2^32) are exact, and the word is hi:lo -- the
+// same value as a float -> int64 conversion, without its sign handling.
 __device__ __forceinline__ unsigned long long to_fixed(float x, float s) {
-    return (unsigned long long)__float2ll_rn(x * s);
+    const float y = rintf(x * s);
+    const float hi = floorf(y * 2.3283064365386963e-10f);   // 2^-32
+    const float lo = fmaf(hi, -4294967296.0f, y);            // exact
+    return ((unsigned long long)(uint32_t)(int)hi << 32) | (uint32_t)lo;
 }
 
 __device__ __forceinline__ uint64_t lanemask_lt() {
