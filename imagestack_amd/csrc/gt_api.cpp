@@ -150,6 +150,7 @@ struct Tune {
     int wpc = 0;        // waves-per-CU cap of the tiled and neighbour kernels (0: none)
     bool graph = true;  // replay the pipeline as a cached hipGraph
     bool grid = false;  // the bilateral grid instead of the lattice (GRID entry points only)
+    int nbr = 0;        // 0 auto (hash lookups for d >= 11 or <= 2^20 expected vertices), 1 sort + merge, 2 hash
 };
 
 Tune resolve(const gt_tuning *t) {
@@ -160,6 +161,7 @@ Tune resolve(const gt_tuning *t) {
     if (t->tile_w != 0) r.tile_w = t->tile_w;
     if (t->waves_per_cu > 0) r.wpc = std::min(64, std::max(12, t->waves_per_cu));
     if (t->graph < 0) r.graph = false;
+    if (t->neighbours == 1 || t->neighbours == 2) r.nbr = t->neighbours;
     return r;
 }
 
@@ -266,7 +268,7 @@ void sig_add(std::string &k, const T &v) { k.append((const char *)&v, sizeof(T))
 // blur, and (optionally) slice, enqueued on the context's stream.
 struct Launch {
     int d, vd, storage, W, pw, m_max, vs, n, row_w, tile_w, wpc;
-    bool packed, bilat, dedup, count;
+    bool packed, bilat, dedup, count, hashnbr;
     uint32_t cap;
     gt::LatConst lc;
     gt::KeyPlan kp;
@@ -295,21 +297,31 @@ int enqueue(gt_ctx *c, const Launch &L, const gt::PixSrc &src, const SliceSpec &
     if (c->profiling) GT_HIP(mark(c, 1, s));
     GT_HIP(gt::launch_compact(L.W, ta, L.cap, (uint32_t *)c->counts.p, (uint32_t *)c->slot_list.p,
                               (uint64_t *)c->pk_a.p, L.packed, ctr, s));
-    if (!L.packed)
-        GT_HIP(gt::launch_pack(L.d, ta, (const uint32_t *)c->slot_list.p, ctr, L.m_max, L.kp, L.pw,
-                               (uint64_t *)c->pk_a.p, s));
-    gt::SortBufs sb{(uint64_t *)c->pk_a.p,         (uint64_t *)c->pk_b.p,    (uint32_t *)c->slot_list.p,
-                    (uint32_t *)c->pay_b.p,        (uint32_t *)c->hist.p,    (uint64_t *)c->sort_status.p,
-                    (uint32_t *)c->sort_hist.p, kLookbackTiles};
-    bool in_b = false;
-    GT_HIP(gt::radix_sort(sb, ctr, L.m_max, L.pw, L.kp.bits, &in_b, passes, s));
-    const uint32_t *sorted = in_b ? sb.pay_b : sb.pay_a;
-    const uint64_t *sorted_keys = L.packed ? (in_b ? sb.pk_b : sb.pk_a) : nullptr;
+    const uint32_t *sorted = (const uint32_t *)c->slot_list.p;   // compaction order unless sorted below
+    const uint64_t *sorted_keys = nullptr;
+    *passes = 0;
+    if (!L.hashnbr) {
+        if (!L.packed)
+            GT_HIP(gt::launch_pack(L.d, ta, (const uint32_t *)c->slot_list.p, ctr, L.m_max, L.kp, L.pw,
+                                   (uint64_t *)c->pk_a.p, s));
+        gt::SortBufs sb{(uint64_t *)c->pk_a.p,         (uint64_t *)c->pk_b.p,    (uint32_t *)c->slot_list.p,
+                        (uint32_t *)c->pay_b.p,        (uint32_t *)c->hist.p,    (uint64_t *)c->sort_status.p,
+                        (uint32_t *)c->sort_hist.p, kLookbackTiles};
+        bool in_b = false;
+        GT_HIP(gt::radix_sort(sb, ctr, L.m_max, L.pw, L.kp.bits, &in_b, passes, s));
+        sorted = in_b ? sb.pay_b : sb.pay_a;
+        sorted_keys = L.packed ? (in_b ? sb.pk_b : sb.pk_a) : nullptr;
+    }
     GT_HIP(gt::launch_finalize(L.W, L.storage, ta, L.vsc, sorted, sorted_keys, ctr, L.m_max, L.vd, L.vs,
                                (uint64_t *)c->vkeys.p, c->vals_a.p, L.splat_copy, s));
     if (c->profiling) GT_HIP(mark(c, 2, s));
     int2 *nbr = (int2 *)c->nbr.p;
-    GT_HIP(gt::launch_neighbors(L.d, L.packed, L.kp, (const uint64_t *)c->vkeys.p, ctr, L.m_max, nbr, L.wpc, s));
+    if (L.hashnbr)
+        GT_HIP(gt::launch_neighbors_hash(L.d, L.packed, L.kp, (const uint64_t *)c->vkeys.p, ctr, L.m_max, ta, nbr,
+                                         s));
+    else
+        GT_HIP(gt::launch_neighbors(L.d, L.packed, L.kp, (const uint64_t *)c->vkeys.p, ctr, L.m_max, nbr, L.wpc,
+                                    s));
     if (c->profiling) GT_HIP(mark(c, 3, s));
     void *cur = c->vals_a.p, *nxt = c->vals_b.p;
     for (int j = 0; j <= L.d; j++) {
@@ -429,6 +441,15 @@ int run(gt_ctx *c, const gt::PixSrc &src, int n, int row_w, int d, int vd, int s
         L.bilat = bilat;
         L.dedup = dedup;
         L.count = count;
+        // Vertex ids: sorted (merge neighbours) or compaction order (hash
+        // neighbours, no sort); the output is the same bit for bit.  Skipping
+        // the sort pays for small lattices (its ~30 launches dominate: 2048^2
+        // bilateral, 0.41 M vertices, 0.615 -> 0.557 ms) and at high d (many
+        // passes over wide keys: d = 16, 17.8 M vertices, 37.4 -> 31.3 ms);
+        // in between, sorted ids keep the blur's gathers local (d = 8, 2.1 M
+        // vertices: 3.27 ms sorted, 3.42 ms by hash).  A kept lattice
+        // (gt_build_lattice) is always sorted.
+        L.hashnbr = !keep_splat && !batch && (tu.nbr == 2 || (tu.nbr == 0 && (d >= 11 || est <= (1u << 20))));
         L.cap = cap;
         gt::plan_scale_factors(d, L.lc.sf);
         L.kp = c->plan;
@@ -491,7 +512,7 @@ int run(gt_ctx *c, const gt::PixSrc &src, int n, int row_w, int d, int vd, int s
         c->last_n = n;
         c->last_d = d;
         c->last_M = (unsigned)M;
-        c->have = true;
+        c->have = !L.hashnbr;   // a lattice in compaction order is not exported
         c->M = M;
         c->d = d;
         c->vd = vd;

@@ -126,6 +126,10 @@ hipError_t launch_finalize(int words, int storage, const TableArgs &ta, const Va
                            uint64_t *vkeys, void *vals, float *splat_copy, hipStream_t st);
 hipError_t launch_neighbors(int d, bool packed, const KeyPlan &kp, const uint64_t *vkeys, const Counters *ctr,
                             int m_max, int2 *nbr, int wpc, hipStream_t st);
+// The same table by hash lookups in the complete global table, for vertex ids
+// in compaction order (no sort).
+hipError_t launch_neighbors_hash(int d, bool packed, const KeyPlan &kp, const uint64_t *vkeys, const Counters *ctr,
+                                 int m_max, const TableArgs &ta, int2 *nbr, hipStream_t st);
 hipError_t launch_blur(int storage, int vs, const int2 *nbr, const Counters *ctr, int m_max, const void *in, void *out,
                        hipStream_t st);
 hipError_t launch_slice(int d, bool packed, bool bilat, bool tiled, int storage, const PixSrc &src, int n, int row_w,

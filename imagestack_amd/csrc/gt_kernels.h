@@ -442,7 +442,8 @@ __global__ __launch_bounds__(kBlock) void k_pack(TableArgs ta, const uint32_t *_
         if (w < pw) pk[(size_t)w * stride + i] = out[w];
 }
 
-// finalize: sorted position i -> vertex id i.  Records slot -> id, copies
+// finalize: position i of the sorted (or, without a sort, the compaction)
+// order -> vertex id i.  Records slot -> id, copies
 // the key, and converts the fixed-point splat sums to the storage type (one
 // rounding: int64 -> double is exact below 2^53, then one FP32 rounding).
 template <typename T>
@@ -652,6 +653,59 @@ __global__ __launch_bounds__(kBlock) void k_nbr_merge(const uint64_t *__restrict
     for (int r = 0; r < kNbrItems; r++) {
         const unsigned i = b0 + r * kBlock + threadIdx.x;
         if (i < M) nbr[2 * ((size_t)j * stride + i) + sgn] = res[r];
+    }
+}
+
+// Neighbour table by hash lookups, for lattices whose vertex ids are the
+// compaction order (no sort): the same +-(1,...,1) -+ (D+1) e_j moves as
+// k_nbr_merge, each target looked up in the complete global table and its
+// slot mapped to the vertex id.  One thread per (vertex, axis), both signs.
+// Used where the lattice is sparse (high d: most neighbours are absent, and
+// a miss costs one or two probes) and sorting many wide keys would dominate.
+template <int D, bool PACKED>
+__global__ __launch_bounds__(kBlock) void k_nbr_hash(const uint64_t *__restrict__ vk, KeyPlan kp, const Counters *ctr,
+                                                     int stride, TableArgs ta, int *__restrict__ nbr) {
+    constexpr int W = KeyOps<D, PACKED>::W;
+    const unsigned M = ctr->M;
+    const unsigned i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= M) return;
+    const int j = blockIdx.y;
+    uint64_t k[W];
+#pragma unroll
+    for (int w = 0; w < W; w++) k[w] = vk[(size_t)i * W + w];
+    uint32_t probes = 0;
+#pragma unroll
+    for (int sgn = 0; sgn < 2; sgn++) {   // sgn 0: k + 1 - (D+1)e_j, 1: k - 1 + (D+1)e_j
+        uint64_t t[W];
+#pragma unroll
+        for (int w = 0; w < W; w++) t[w] = k[w];
+        bool ok = true;
+#pragma unroll
+        for (int q = 0; q < D; q++) {
+            uint32_t f, lo, hi;
+            if (PACKED) {
+                f = (uint32_t)((k[0] >> kp.shift[q]) & kp.span[q]);
+                lo = 0;
+                hi = kp.span[q];
+            } else {
+                f = (uint32_t)((k[q / 4] >> (48 - 16 * (q % 4))) & 0xffff);
+                lo = 1 + 2 * (D + 1);
+                hi = 65535 - 2 * (D + 1);
+            }
+            const bool up = (q == j) == (sgn != 0);
+            const uint32_t step = (q == j) ? (uint32_t)D : 1u;
+            ok = ok && (up ? f + step <= hi : f >= lo + step);
+            const int64_t dq = (q == j) ? -(int64_t)D : 1;
+            const int64_t v = sgn ? -dq : dq;
+            if (PACKED) t[0] += (uint64_t)v << kp.shift[q];
+            else t[q / 4] += (uint64_t)v << (48 - 16 * (q % 4));
+        }
+        int res = -1;
+        if (ok) {
+            const int s = table_find<W>(ta.tkeys, ta.mask, t, probes);
+            if (s >= 0) res = ta.slot_id[s];
+        }
+        nbr[2 * ((size_t)j * stride + i) + sgn] = res;
     }
 }
 

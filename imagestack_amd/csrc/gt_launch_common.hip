@@ -323,6 +323,20 @@ hipError_t launch_neighbors(int d, bool packed, const KeyPlan &kp, const uint64_
     return hipGetLastError();
 }
 
+hipError_t launch_neighbors_hash(int d, bool packed, const KeyPlan &kp, const uint64_t *vkeys, const Counters *ctr,
+                                 int m_max, const TableArgs &ta, int2 *nbr, hipStream_t st) {
+    if (m_max <= 0) return hipSuccess;
+    const dim3 g(blocks_for(m_max, kBlock), d + 1), b(kBlock);
+#define GT_NBRH(D)                                                                                               \
+    {                                                                                                            \
+        if (packed) hipLaunchKernelGGL((k_nbr_hash<D, true>), g, b, 0, st, vkeys, kp, ctr, m_max, ta, (int *)nbr);  \
+        else hipLaunchKernelGGL((k_nbr_hash<D, false>), g, b, 0, st, vkeys, kp, ctr, m_max, ta, (int *)nbr);        \
+    }
+    GT_DISPATCH_D(d, GT_NBRH)
+#undef GT_NBRH
+    return hipGetLastError();
+}
+
 template <typename T>
 static hipError_t launch_blur_t(int vs, const int2 *nbr, const Counters *ctr, int m_max, const void *in, void *out,
                                 hipStream_t st) {

@@ -52,3 +52,20 @@ def test_mid_d_lattice_and_transform(gpu_ctx, oracle_lib, d, vd, spread, dedup):
     out = gpu_ctx.gauss_transform(dev(pos), dev(val), inv, slice_pos=dev(slc), tune=tune).cpu().numpy()
     o64, _ = O.gauss_transform(pos, val, inv, slice_pos=slc, mode=O.F64_LAT)
     assert rel_err(out, o64) < FP32_REL_TOL
+
+
+@pytest.mark.parametrize("d", [5, 8, 12, 16])
+def test_hash_and_merge_neighbours_agree(gpu_ctx, oracle_lib, d):
+    """Vertex ids in compaction order with hash-lookup neighbours (no sort)
+    give the same output, bit for bit, as sorted ids with merged neighbours."""
+    print(f"\n[case] neighbours by hash vs merge, d={d}", flush=True)
+    W, H = 80, 60
+    pos = positions(d, W, H, 3.0, seed=d)
+    rng = np.random.default_rng(d + 1)
+    val = np.concatenate([rng.uniform(size=(3, W * H)), np.ones((1, W * H))]).astype(np.float32)
+    inv = np.ones(d, np.float32)
+    outs = [gpu_ctx.gauss_transform(dev(pos), dev(val), inv, row_w=W, tune={"neighbours": m}).cpu().numpy()
+            for m in (1, 2, 0)]
+    assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
+    o64, _ = oracle_lib.gauss_transform(pos, val, inv, mode=oracle_lib.F64_LAT)
+    assert rel_err(outs[1], o64) < FP32_REL_TOL
