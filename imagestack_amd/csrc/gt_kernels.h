@@ -673,40 +673,72 @@ __global__ __launch_bounds__(kBlock) void k_nbr_hash(const uint64_t *__restrict_
     uint64_t k[W];
 #pragma unroll
     for (int w = 0; w < W; w++) k[w] = vk[(size_t)i * W + w];
-    uint32_t probes = 0;
+    // the two targets: sgn 0: k + 1 - (D+1)e_j, sgn 1: k - 1 + (D+1)e_j
+    uint64_t t[2][W];
+    bool ok[2] = {true, true};
 #pragma unroll
-    for (int sgn = 0; sgn < 2; sgn++) {   // sgn 0: k + 1 - (D+1)e_j, 1: k - 1 + (D+1)e_j
-        uint64_t t[W];
+    for (int sgn = 0; sgn < 2; sgn++)
 #pragma unroll
-        for (int w = 0; w < W; w++) t[w] = k[w];
-        bool ok = true;
+        for (int w = 0; w < W; w++) t[sgn][w] = k[w];
 #pragma unroll
-        for (int q = 0; q < D; q++) {
-            uint32_t f, lo, hi;
-            if (PACKED) {
-                f = (uint32_t)((k[0] >> kp.shift[q]) & kp.span[q]);
-                lo = 0;
-                hi = kp.span[q];
-            } else {
-                f = (uint32_t)((k[q / 4] >> (48 - 16 * (q % 4))) & 0xffff);
-                lo = 1 + 2 * (D + 1);
-                hi = 65535 - 2 * (D + 1);
-            }
-            const bool up = (q == j) == (sgn != 0);
-            const uint32_t step = (q == j) ? (uint32_t)D : 1u;
-            ok = ok && (up ? f + step <= hi : f >= lo + step);
-            const int64_t dq = (q == j) ? -(int64_t)D : 1;
+    for (int q = 0; q < D; q++) {
+        uint32_t f, lo, hi;
+        if (PACKED) {
+            f = (uint32_t)((k[0] >> kp.shift[q]) & kp.span[q]);
+            lo = 0;
+            hi = kp.span[q];
+        } else {
+            f = (uint32_t)((k[q / 4] >> (48 - 16 * (q % 4))) & 0xffff);
+            lo = 1 + 2 * (D + 1);
+            hi = 65535 - 2 * (D + 1);
+        }
+        const uint32_t step = (q == j) ? (uint32_t)D : 1u;
+        const int64_t dq = (q == j) ? -(int64_t)D : 1;
+#pragma unroll
+        for (int sgn = 0; sgn < 2; sgn++) {
+            const bool up = (q == j) == (sgn != 0);   // this field grows
+            ok[sgn] = ok[sgn] && (up ? f + step <= hi : f >= lo + step);
             const int64_t v = sgn ? -dq : dq;
-            if (PACKED) t[0] += (uint64_t)v << kp.shift[q];
-            else t[q / 4] += (uint64_t)v << (48 - 16 * (q % 4));
+            if (PACKED) t[sgn][0] += (uint64_t)v << kp.shift[q];
+            else t[sgn][q / 4] += (uint64_t)v << (48 - 16 * (q % 4));
         }
-        int res = -1;
-        if (ok) {
-            const int s = table_find<W>(ta.tkeys, ta.mask, t, probes);
-            if (s >= 0) res = ta.slot_id[s];
-        }
-        nbr[2 * ((size_t)j * stride + i) + sgn] = res;
     }
+    // Both lookups probe in lockstep, so their loads are in flight together.
+    uint32_t s[2];
+    bool done[2];
+    int slot[2] = {-1, -1};
+#pragma unroll
+    for (int sgn = 0; sgn < 2; sgn++) {
+        s[sgn] = (uint32_t)hash_key<W>(t[sgn]) & ta.mask;
+        done[sgn] = !ok[sgn];
+    }
+    for (uint32_t it = 0; it < kMaxProbes && !(done[0] && done[1]); it++) {
+        uint64_t w0[2];
+#pragma unroll
+        for (int sgn = 0; sgn < 2; sgn++) w0[sgn] = done[sgn] ? kEmpty : ta.tkeys[(size_t)s[sgn] * W];
+#pragma unroll
+        for (int sgn = 0; sgn < 2; sgn++) {
+            if (done[sgn]) continue;
+            if (w0[sgn] == kEmpty) {
+                done[sgn] = true;
+                continue;
+            }
+            bool match = w0[sgn] == t[sgn][0];
+#pragma unroll
+            for (int w = 1; w < W; w++) match = match && ta.tkeys[(size_t)s[sgn] * W + w] == t[sgn][w];
+            if (match) {
+                slot[sgn] = (int)s[sgn];
+                done[sgn] = true;
+            } else {
+                s[sgn] = (s[sgn] + 1) & ta.mask;
+            }
+        }
+    }
+    int res[2];
+#pragma unroll
+    for (int sgn = 0; sgn < 2; sgn++) res[sgn] = slot[sgn] >= 0 ? ta.slot_id[slot[sgn]] : -1;
+    int2 *out = (int2 *)nbr + (size_t)j * stride + i;
+    *out = make_int2(res[0], res[1]);
 }
 
 // One blur pass along one axis (src/Permutohedral.h:442-475): Jacobi, new =
