@@ -65,6 +65,19 @@ int search(const std::function<int(const gt_tuning &)> &run, int reps, gt_tuning
                     b = t;
                 }
             }
+    // then the neighbour-table form (sorted ids + merge, or hash lookups), on the best of the rest
+    for (int nb : {1, 2}) {
+        gt_tuning t = b;
+        t.neighbours = nb;
+        float ms = 0.f;
+        rc = time_calls([&] { return run(t); }, reps, &ms);
+        if (rc != GT_OK) return rc;
+        trials++;
+        if (ms < b_ms) {
+            b_ms = ms;
+            b = t;
+        }
+    }
     if (best) *best = b;
     if (rep) {
         rep->trials = trials;

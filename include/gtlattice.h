@@ -233,7 +233,8 @@ int gt_joint_bilateral_batch(gt_ctx *ctx, int count, const gt_bilateral_image *i
 /*
  * Autotuner: times the pipeline on these inputs over a grid of tunings
  * (tile_w 16/32/64 x load_factor 0.25/0.35/0.5 x waves_per_cu none/12/20,
- * plus the defaults), `reps` timed calls each after one untimed call, and
+ * plus the defaults; then the best of those with each neighbour form),
+ * `reps` timed calls each after one untimed call, and
  * returns the tuning with the lowest median time.  Trial outputs go to a
  * scratch buffer; the inputs are only read.  No reference counterpart: the
  * reference has no tunable parameters.
