@@ -120,6 +120,20 @@ struct TileView {
     uint64_t tag;
 };
 
+// Workgroup -> tile order that gives each XCD a contiguous run of tiles (a
+// band of the image): workgroups are dispatched round-robin over the 8 XCDs,
+// so workgroup b runs on XCD b % 8.  Neighbouring tiles share most of their
+// lattice vertices; in one band their table lines stay in one XCD's L2
+// (2048^2 splat 190 -> 179 us).  A permutation of [0, g), so any dispatch
+// order stays correct.  Batches keep the round-robin order: their tiles'
+// cost varies from image to image, and bands of it unbalance the XCDs
+// (64-image batch 20.0 -> 23.3 ms banded).
+__device__ __forceinline__ int xcd_tile(int b, int g) {
+    constexpr int kXcd = 8;
+    const int per = g / kXcd, rem = g % kXcd, x = b % kXcd, k = b / kXcd;
+    return x < rem ? x * (per + 1) + k : rem * (per + 1) + (x - rem) * per + k;
+}
+
 template <bool BATCH>
 __device__ __forceinline__ void tile_view(int tile, const PixSrc &src0, const OutDst &dst0, int n0, int row_w0,
                                           const BatchArgs &ba, const KeyPlan &kp, TileView &tv) {
@@ -207,7 +221,7 @@ __global__ __launch_bounds__(kBlock) void k_splat_tile(PixSrc src0, int n0, int 
     uint32_t probes = 0, inserts = 0, atomics = 0;
     const OutDst no_dst{nullptr, 0, 0};
     TileView tv;
-    tile_view<BATCH>(blockIdx.x, src0, no_dst, n0, row_w0, ba, kp, tv);
+    tile_view<BATCH>(BATCH ? (int)blockIdx.x : xcd_tile(blockIdx.x, gridDim.x), src0, no_dst, n0, row_w0, ba, kp, tv);
     const PixSrc &src = tv.src;
     for (int i = threadIdx.x; i < LT * W; i += kBlock) lk[i] = kEmpty;
     for (int i = threadIdx.x; i < LVD * LT; i += kBlock) lv[i] = 0ull;
@@ -801,7 +815,7 @@ __global__ __launch_bounds__(kBlock) void k_slice_tile(PixSrc src0, int n0, int 
     unsigned flags = 0;
     uint32_t probes = 0;
     TileView tv;
-    tile_view<BATCH>(blockIdx.x, src0, dst0, n0, row_w0, ba, kp, tv);
+    tile_view<BATCH>(BATCH ? (int)blockIdx.x : xcd_tile(blockIdx.x, gridDim.x), src0, dst0, n0, row_w0, ba, kp, tv);
     const PixSrc &src = tv.src;
     for (int i = threadIdx.x; i < LT * W; i += kBlock) lk[i] = kEmpty;
     if (threadIdx.x == 0) lcount = 0;
