@@ -90,6 +90,7 @@ struct gt_net_server {
 namespace {
 
 constexpr int kMaxHandlers = 64;
+constexpr size_t kLatWindow = 100000;
 
 int connect_back(gt_net_server *s, const sockaddr_in &peer, int port, double give_up_ms) {
     sockaddr_in a = peer;
@@ -180,7 +181,9 @@ void handle(gt_net_server *s, int fd, sockaddr_in peer) {
     }
     if (ok) {
         s->replied++;
-        s->lat_ms.push_back(t1 - t0);
+        // latencies of the last kLatWindow replies (a ring), for the percentiles
+        if (s->lat_ms.size() < kLatWindow) s->lat_ms.push_back(t1 - t0);
+        else s->lat_ms[(size_t)(s->replied - 1) % kLatWindow] = t1 - t0;
     } else {
         s->failed++;
     }

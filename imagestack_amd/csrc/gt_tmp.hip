@@ -206,11 +206,16 @@ int write_header(FILE *f, int W, int H, int F, int C, int type) {
     return fwrite(h, sizeof(int32_t), 5, f) == 5 ? GT_OK : set_error(GT_EDEVICE, "write failed");
 }
 
+// Two pinned host buffers (and device staging) for the pipelined copies.
+// The destructor first waits for the stream, so no return path frees a
+// buffer that a copy in flight still uses.
 struct Pinned {
     void *p[2] = {nullptr, nullptr};
     void *d[2] = {nullptr, nullptr};
     hipEvent_t ev[2] = {nullptr, nullptr};
+    hipStream_t stream = nullptr;
     ~Pinned() {
+        if (stream) (void)hipStreamSynchronize(stream);
         for (int i = 0; i < 2; i++) {
             if (ev[i]) (void)hipEventDestroy(ev[i]);
             if (p[i]) (void)hipHostFree(p[i]);
@@ -299,6 +304,7 @@ int gt_tmp_load(gt_ctx *ctx, const char *path, float *dev_out, size_t capacity, 
     const int ts = type_size(t.type);
     const bool raw = t.type == GT_TMP_F32;
     Pinned pin;
+    pin.stream = s;
     if ((rc = pin.init(!raw)) != GT_OK) return rc;
     const size_t per = kChunk / ts;
     int64_t done = 0;
@@ -329,6 +335,7 @@ int gt_tmp_save(gt_ctx *ctx, const char *path, const float *dev_im, int W, int H
     if (hipSetDevice(ctx_device(ctx)) != hipSuccess) return set_error(GT_EDEVICE, "device unavailable");
     hipStream_t s = ctx_stream(ctx);
     Pinned pin;
+    pin.stream = s;
     int rc = pin.init(true);
     if (rc != GT_OK) return rc;
     FILE *f = fopen(path, "wb");

@@ -370,7 +370,7 @@ typedef struct {
 
 typedef struct {
     long long received, replied, failed;
-    double p50_ms, p99_ms;       /* over the replied images                    */
+    double p50_ms, p99_ms;       /* over the last 100 000 replied images        */
 } gt_net_stats;
 
 typedef struct gt_net_server gt_net_server;
