@@ -487,6 +487,15 @@ int run(gt_ctx *c, const gt::PixSrc &src, int n, int row_w, int d, int vd, int s
         L.ta.tacc = (unsigned long long *)c->tacc.p;
         L.ta.mask = cap - 1;
         L.ta.slot_id = (int *)c->slot_id.p;
+        // One-word keys leave room above the key for the vertex id: fold it in
+        // at the finalize, so lookups after it read key and id in one word.
+        {
+            int idbits = 1;
+            while (idbits < 31 && (1u << idbits) < (uint32_t)L.m_max) idbits++;
+            const bool fold = L.packed && L.kp.bits + idbits <= 63;
+            L.ta.idshift = fold ? L.kp.bits : 0;
+            L.ta.keymask = fold ? (1ull << L.kp.bits) - 1 : ~0ull;
+        }
 
         void *cur = nullptr;
         GT_TRY(execute(c, L, src, sl, batch, tu.graph && !keep_splat && c->profiling == 0, &passes, &cur));

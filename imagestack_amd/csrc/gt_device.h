@@ -264,6 +264,26 @@ __device__ __forceinline__ int table_find(const uint64_t *__restrict__ tkeys, ui
     return -1;
 }
 
+// Vertex id of key k in a finalized table (after k_finalize), or -1: read
+// from the key's own word when the ids were folded in (TableArgs::idshift),
+// else through slot_id.
+template <int W>
+__device__ __forceinline__ int table_vertex(const TableArgs &ta, const uint64_t (&k)[W], uint32_t &probes) {
+    if (W == 1 && ta.idshift > 0) {
+        uint32_t s = (uint32_t)hash_key<W>(k) & ta.mask;
+        for (uint32_t it = 0; it < kMaxProbes; it++) {
+            probes++;
+            const uint64_t w0 = ta.tkeys[s];
+            if (w0 == kEmpty) return -1;
+            if ((w0 & ta.keymask) == k[0]) return (int)(w0 >> ta.idshift);
+            s = (s + 1) & ta.mask;
+        }
+        return -1;
+    }
+    const int slot = table_find<W>(ta.tkeys, ta.mask, k, probes);
+    return slot >= 0 ? ta.slot_id[slot] : -1;
+}
+
 // Workgroup-local key table in LDS: the distinct vertices of one pixel tile.
 // Same word-claim protocol as the global table.  Returns the LDS slot, or -1
 // after MAXP probes (a tile with more distinct vertices than the table holds

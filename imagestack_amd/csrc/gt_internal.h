@@ -78,7 +78,11 @@ struct TableArgs {
     uint64_t *tkeys;               // capacity * W words
     unsigned long long *tacc;      // capacity * vd fixed-point accumulators (two's complement)
     uint32_t mask;                 // capacity - 1
-    int *slot_id;                  // capacity: slot -> sorted vertex id
+    int *slot_id;                  // capacity: slot -> vertex id (idshift == 0)
+    int idshift;                   // > 0 (one-word keys with room to spare): after the finalize a
+                                   // table word holds (id << idshift) | key, so a lookup reads the
+                                   // vertex id with the key; slot_id is not used
+    uint64_t keymask;              // (1 << idshift) - 1
 };
 
 // Fixed-point splat accumulation: channel c of a weighted value is added as

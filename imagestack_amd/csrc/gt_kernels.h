@@ -457,8 +457,8 @@ __global__ __launch_bounds__(kBlock) void k_pack(TableArgs ta, const uint32_t *_
 }
 
 // finalize: position i of the sorted (or, without a sort, the compaction)
-// order -> vertex id i.  Records slot -> id, copies
-// the key, and converts the fixed-point splat sums to the storage type (one
+// order -> vertex id i.  Records slot -> id (in slot_id, or folded into the
+// table word above the key when ta.idshift > 0), copies the key, and converts the fixed-point splat sums to the storage type (one
 // rounding: int64 -> double is exact below 2^53, then one FP32 rounding).
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_finalize(TableArgs ta, ValScale vsc, int kw,
@@ -470,11 +470,15 @@ __global__ __launch_bounds__(kBlock) void k_finalize(TableArgs ta, ValScale vsc,
     const unsigned i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= M) return;
     const uint32_t s = sorted[i];
-    ta.slot_id[s] = (int)i;
-    if (sorted_keys)   // packed keys: the sort's own output, read in order
-        vkeys[i] = sorted_keys[i];
-    else
+    if (sorted_keys) {   // packed keys: the sort's own output, read in order
+        const uint64_t key = sorted_keys[i];
+        vkeys[i] = key;
+        if (ta.idshift > 0) ta.tkeys[s] = key | ((uint64_t)i << ta.idshift);
+    } else {
         for (int w = 0; w < kw; w++) vkeys[(size_t)i * kw + w] = ta.tkeys[(size_t)s * kw + w];
+        if (ta.idshift > 0) ta.tkeys[s] |= (uint64_t)i << ta.idshift;   // kw == 1
+    }
+    if (ta.idshift == 0) ta.slot_id[s] = (int)i;
     float o[kMaxVD];
 #pragma unroll
     for (int c = 0; c < kMaxVD; c++)
@@ -718,9 +722,14 @@ __global__ __launch_bounds__(kBlock) void k_nbr_hash(const uint64_t *__restrict_
         }
     }
     // Both lookups probe in lockstep, so their loads are in flight together.
+    // With ids folded into the table words (one-word keys), a hit carries its
+    // vertex id and slot_id is not read.
+    const bool fold = W == 1 && ta.idshift > 0;
+    const uint64_t kmask = fold ? ta.keymask : ~0ull;
     uint32_t s[2];
     bool done[2];
     int slot[2] = {-1, -1};
+    uint64_t hit[2] = {0, 0};
 #pragma unroll
     for (int sgn = 0; sgn < 2; sgn++) {
         s[sgn] = (uint32_t)hash_key<W>(t[sgn]) & ta.mask;
@@ -737,11 +746,12 @@ __global__ __launch_bounds__(kBlock) void k_nbr_hash(const uint64_t *__restrict_
                 done[sgn] = true;
                 continue;
             }
-            bool match = w0[sgn] == t[sgn][0];
+            bool match = (w0[sgn] & kmask) == t[sgn][0];
 #pragma unroll
             for (int w = 1; w < W; w++) match = match && ta.tkeys[(size_t)s[sgn] * W + w] == t[sgn][w];
             if (match) {
                 slot[sgn] = (int)s[sgn];
+                hit[sgn] = w0[sgn];
                 done[sgn] = true;
             } else {
                 s[sgn] = (s[sgn] + 1) & ta.mask;
@@ -750,7 +760,8 @@ __global__ __launch_bounds__(kBlock) void k_nbr_hash(const uint64_t *__restrict_
     }
     int res[2];
 #pragma unroll
-    for (int sgn = 0; sgn < 2; sgn++) res[sgn] = slot[sgn] >= 0 ? ta.slot_id[slot[sgn]] : -1;
+    for (int sgn = 0; sgn < 2; sgn++)
+        res[sgn] = slot[sgn] < 0 ? -1 : fold ? (int)(hit[sgn] >> ta.idshift) : ta.slot_id[slot[sgn]];
     int2 *out = (int2 *)nbr + (size_t)j * stride + i;
     *out = make_int2(res[0], res[1]);
 }
@@ -872,11 +883,11 @@ __global__ __launch_bounds__(kBlock) void k_slice_tile(PixSrc src0, int n0, int 
         uint64_t k[W];
 #pragma unroll
         for (int q = 0; q < W; q++) k[q] = lk[ls * W + q];
-        const int slot = table_find<W>(ta.tkeys, ta.mask, k, probes);
+        const int id = table_vertex<W>(ta, k, probes);
         float row[kMaxVD];
 #pragma unroll
         for (int c = 0; c < LVD; c++) row[c] = 0.0f;
-        if (slot >= 0) load_row<T, LVD>(vals, ta.slot_id[slot], row);
+        if (id >= 0) load_row<T, LVD>(vals, id, row);
 #pragma unroll
         for (int c = 0; c < LVD; c++) lv[c * LT + ls] = row[c];
     }
@@ -894,10 +905,10 @@ __global__ __launch_bounds__(kBlock) void k_slice_tile(PixSrc src0, int n0, int 
             } else if (lsl[r] == -2) {
                 uint64_t k[W];
                 K::make(s, r, kp, tv.tag, k);
-                const int slot = table_find<W>(ta.tkeys, ta.mask, k, probes);
-                if (slot >= 0) {
+                const int id = table_vertex<W>(ta, k, probes);
+                if (id >= 0) {
                     float row[kMaxVD];
-                    load_row<T, LVD>(vals, ta.slot_id[slot], row);
+                    load_row<T, LVD>(vals, id, row);
 #pragma unroll
                     for (int c = 0; c < LVD; c++) acc[c] = acc[c] + s.b[r] * row[c];
                 }
@@ -935,10 +946,10 @@ __global__ __launch_bounds__(kBlock) void k_slice_direct(PixSrc src, int n, int 
             for (int r = 0; r <= D; r++) {
                 uint64_t k[W];
                 if (!K::make(s, r, kp, 0ull, k)) continue;
-                const int slot = table_find<W>(ta.tkeys, ta.mask, k, probes);
-                if (slot < 0) continue;
+                const int id = table_vertex<W>(ta, k, probes);
+                if (id < 0) continue;
                 float v[kMaxVD];
-                load_row_rt<T>(vals, ta.slot_id[slot], vs, v);
+                load_row_rt<T>(vals, id, vs, v);
 #pragma unroll
                 for (int c = 0; c < kMaxVD; c++) acc[c] = acc[c] + s.b[r] * v[c];
             }
