@@ -151,6 +151,7 @@ struct Tune {
     bool graph = true;  // replay the pipeline as a cached hipGraph
     bool grid = false;  // the bilateral grid instead of the lattice (GRID entry points only)
     int nbr = 0;        // 0 auto (hash lookups for d >= 11 or <= 2^20 expected vertices), 1 sort + merge, 2 hash
+    bool parts = true;  // row-band table parts where they apply
 };
 
 Tune resolve(const gt_tuning *t) {
@@ -162,6 +163,7 @@ Tune resolve(const gt_tuning *t) {
     if (t->waves_per_cu > 0) r.wpc = std::min(64, std::max(12, t->waves_per_cu));
     if (t->graph < 0) r.graph = false;
     if (t->neighbours == 1 || t->neighbours == 2) r.nbr = t->neighbours;
+    if (t->parts < 0) r.parts = false;
     return r;
 }
 
@@ -274,6 +276,7 @@ struct Launch {
     gt::KeyPlan kp;
     gt::ValScale vsc;
     gt::TableArgs ta;
+    gt::PartMap pm;
     float *splat_copy;
 };
 
@@ -317,15 +320,15 @@ int enqueue(gt_ctx *c, const Launch &L, const gt::PixSrc &src, const SliceSpec &
     if (c->profiling) GT_HIP(mark(c, 2, s));
     int2 *nbr = (int2 *)c->nbr.p;
     if (L.hashnbr)
-        GT_HIP(gt::launch_neighbors_hash(L.d, L.packed, L.kp, (const uint64_t *)c->vkeys.p, ctr, L.m_max, ta, nbr,
-                                         s));
+        GT_HIP(gt::launch_neighbors_hash(L.d, L.packed, L.kp, (const uint64_t *)c->vkeys.p, ctr, L.m_max, ta, L.pm,
+                                         nbr, s));
     else
         GT_HIP(gt::launch_neighbors(L.d, L.packed, L.kp, (const uint64_t *)c->vkeys.p, ctr, L.m_max, nbr, L.wpc,
                                     s));
     if (c->profiling) GT_HIP(mark(c, 3, s));
     void *cur = c->vals_a.p, *nxt = c->vals_b.p;
     for (int j = 0; j <= L.d; j++) {
-        GT_HIP(gt::launch_blur(L.storage, L.vs, nbr + (size_t)j * L.m_max, ctr, L.m_max, cur, nxt, s));
+        GT_HIP(gt::launch_blur(L.storage, L.vs, nbr + (size_t)j * L.m_max, ctr, L.m_max, L.pm, cur, nxt, s));
         std::swap(cur, nxt);
     }
     if (c->profiling) GT_HIP(mark(c, 4, s));
@@ -495,6 +498,29 @@ int run(gt_ctx *c, const gt::PixSrc &src, int n, int row_w, int d, int vd, int s
             const bool fold = L.packed && L.kp.bits + idbits <= 63;
             L.ta.idshift = fold ? L.kp.bits : 0;
             L.ta.keymask = fold ? (1ull << L.kp.bits) - 1 : ~0ull;
+        }
+        // Row bands: in a one-frame bilateral form the last position channel
+        // is the row, y / sigma_s, and the last lattice coordinate of every
+        // vertex is -d sf y / sigma_s to within 2(d + 1).  Eight equal ranges
+        // of it give eight table parts; the splat and slice hand XCD x the
+        // x-th band of tiles, and (ids in compaction order) the neighbour
+        // and blur kernels the x-th part's ids, so the table lines, sums and
+        // values each XCD touches stay mostly in its own L2.
+        if (tu.parts && !batch && bilat && L.packed && src.fy && !src.ft && cap >= (1u << 14)) {
+            double sum_lo = 0;
+            for (int i = 0; i < d; i++) sum_lo += L.kp.lo[i];
+            const double ymax = (double)(src.H - 1) * src.iy;
+            const int margin = 2 * (d + 1);
+            const double range = std::ceil((double)d * L.lc.sf[d - 1] * ymax) + 2.0 * margin;
+            if (range >= 8 && range < (double)(1 << 24)) {
+                L.ta.part_base = (int)sum_lo + margin;
+                L.ta.part_range = (int)range;
+                L.ta.part_mul = (uint32_t)((double)(1u << 27) / (range + 1.0));
+                int lg = 0;
+                while ((1u << lg) < cap) lg++;
+                L.ta.part_shift = lg - 3;
+                if (L.hashnbr) L.pm = {(const uint32_t *)c->counts.p, (int)(cap / 8 / gt::kCompactSlots)};
+            }
         }
 
         void *cur = nullptr;

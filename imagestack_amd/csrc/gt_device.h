@@ -216,8 +216,7 @@ __device__ __forceinline__ uint64_t hash_key(const uint64_t (&k)[W]) {
 // kMaxProbes slots (table full).
 template <int W>
 __device__ __forceinline__ int table_insert(uint64_t *__restrict__ tkeys, uint32_t mask, const uint64_t (&k)[W],
-                                            uint32_t &probes) {
-    uint32_t s = (uint32_t)hash_key<W>(k) & mask;
+                                            uint32_t s, uint32_t &probes) {
     for (uint32_t it = 0; it < kMaxProbes; it++) {
         probes++;
         bool match = true;
@@ -248,8 +247,7 @@ __device__ __forceinline__ int table_insert(uint64_t *__restrict__ tkeys, uint32
 // Read-only lookup in a complete table (a later kernel): plain loads.
 template <int W>
 __device__ __forceinline__ int table_find(const uint64_t *__restrict__ tkeys, uint32_t mask, const uint64_t (&k)[W],
-                                          uint32_t &probes) {
-    uint32_t s = (uint32_t)hash_key<W>(k) & mask;
+                                          uint32_t s, uint32_t &probes) {
     for (uint32_t it = 0; it < kMaxProbes; it++) {
         probes++;
         const uint64_t *a = tkeys + (size_t)s * W;
@@ -264,13 +262,35 @@ __device__ __forceinline__ int table_find(const uint64_t *__restrict__ tkeys, ui
     return -1;
 }
 
+// Home slot of key k (where its linear probe starts): the hash, or in a
+// partitioned table (TableArgs::part_mul) the hash inside the key's part.
+template <int D, bool PACKED>
+__device__ __forceinline__ uint32_t home_slot(const TableArgs &ta, const KeyPlan &kp,
+                                              const uint64_t (&k)[KeyOps<D, PACKED>::W]) {
+    constexpr int W = KeyOps<D, PACKED>::W;
+    const uint32_t h = (uint32_t)hash_key<W>(k);
+    if constexpr (PACKED) {
+        if (ta.part_mul) {
+            int sum = ta.part_base;
+#pragma unroll
+            for (int i = 0; i < D; i++) sum += (int)((k[0] >> kp.shift[i]) & kp.span[i]);
+            const uint32_t u = (uint32_t)min(max(sum, 0), ta.part_range);
+            const uint32_t part = (uint32_t)(((uint64_t)u * ta.part_mul) >> 24);
+            return (part << ta.part_shift) | (h & (ta.mask >> 3));
+        }
+    }
+    return h & ta.mask;
+}
+
 // Vertex id of key k in a finalized table (after k_finalize), or -1: read
 // from the key's own word when the ids were folded in (TableArgs::idshift),
 // else through slot_id.
-template <int W>
-__device__ __forceinline__ int table_vertex(const TableArgs &ta, const uint64_t (&k)[W], uint32_t &probes) {
+template <int D, bool PACKED>
+__device__ __forceinline__ int table_vertex(const TableArgs &ta, const KeyPlan &kp,
+                                            const uint64_t (&k)[KeyOps<D, PACKED>::W], uint32_t &probes) {
+    constexpr int W = KeyOps<D, PACKED>::W;
+    uint32_t s = home_slot<D, PACKED>(ta, kp, k);
     if (W == 1 && ta.idshift > 0) {
-        uint32_t s = (uint32_t)hash_key<W>(k) & ta.mask;
         for (uint32_t it = 0; it < kMaxProbes; it++) {
             probes++;
             const uint64_t w0 = ta.tkeys[s];
@@ -280,7 +300,7 @@ __device__ __forceinline__ int table_vertex(const TableArgs &ta, const uint64_t 
         }
         return -1;
     }
-    const int slot = table_find<W>(ta.tkeys, ta.mask, k, probes);
+    const int slot = table_find<W>(ta.tkeys, ta.mask, k, s, probes);
     return slot >= 0 ? ta.slot_id[slot] : -1;
 }
 

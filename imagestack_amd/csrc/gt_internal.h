@@ -20,6 +20,7 @@ struct Counters {
     unsigned long long inserts;
     unsigned long long atomics;
 };
+constexpr int kCompactSlots = 2048;   // table slots per compaction workgroup (chunk)
 constexpr unsigned kFlagFull = 1u;    // an insert exceeded the probe limit: regrow and redo
 constexpr unsigned kFlagBad = 2u;     // NaN/Inf position or key outside int16
 constexpr unsigned kFlagRange = 4u;   // a key outside the packing plan: replan and redo
@@ -83,6 +84,22 @@ struct TableArgs {
                                    // table word holds (id << idshift) | key, so a lookup reads the
                                    // vertex id with the key; slot_id is not used
     uint64_t keymask;              // (1 << idshift) - 1
+    // part_mul != 0 (one-word keys of a bilateral form over image rows): the
+    // table is eight parts of capacity / 8 slots; a key's home slot lies in
+    // part ((c_0 + ... + c_{D-1} + part_base) clamped to [0, part_range]) *
+    // part_mul >> 24, its hash picking the slot inside the part.  The sum is
+    // -c_D, which follows the last position channel (the row) alone, so each
+    // horizontal band of the image keeps its vertices in one part of the table.
+    uint32_t part_mul;
+    int part_base, part_range, part_shift;   // part_shift = log2(capacity / 8)
+};
+
+// Vertex ids grouped by table part (ids in compaction order of a partitioned
+// table): part x holds ids [off[x * cpp], off[(x + 1) * cpp]) (M for x = 8),
+// off being the compaction's per-chunk offsets.  cpp = 0: no grouping.
+struct PartMap {
+    const uint32_t *off;
+    int cpp;
 };
 
 // Fixed-point splat accumulation: channel c of a weighted value is added as
@@ -132,10 +149,11 @@ hipError_t launch_neighbors(int d, bool packed, const KeyPlan &kp, const uint64_
                             int m_max, int2 *nbr, int wpc, hipStream_t st);
 // The same table by hash lookups in the complete global table, for vertex ids
 // in compaction order (no sort).
+// With pm.cpp > 0, workgroups on XCD x take the vertices of table part x.
 hipError_t launch_neighbors_hash(int d, bool packed, const KeyPlan &kp, const uint64_t *vkeys, const Counters *ctr,
-                                 int m_max, const TableArgs &ta, int2 *nbr, hipStream_t st);
-hipError_t launch_blur(int storage, int vs, const int2 *nbr, const Counters *ctr, int m_max, const void *in, void *out,
-                       hipStream_t st);
+                                 int m_max, const TableArgs &ta, const PartMap &pm, int2 *nbr, hipStream_t st);
+hipError_t launch_blur(int storage, int vs, const int2 *nbr, const Counters *ctr, int m_max, const PartMap &pm,
+                       const void *in, void *out, hipStream_t st);
 hipError_t launch_slice(int d, bool packed, bool bilat, bool tiled, int storage, const PixSrc &src, int n, int row_w,
                         int tile_w, int vd, int vs, const LatConst &lc, const KeyPlan &kp, const TableArgs &ta,
                         const void *vals, const OutDst &dst, Counters *ctr, bool count, int wpc, hipStream_t st);

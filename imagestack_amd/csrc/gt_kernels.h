@@ -10,7 +10,6 @@ namespace gt {
 
 constexpr int kBlock = 256;          // threads per workgroup of every kernel here
 constexpr int kTileKeys = 512;       // LDS tile table slots of the splat and slice tiles
-constexpr int kCompactSlots = 2048;  // table slots per compaction workgroup
 
 // ---------------------------------------------------------------------------
 // pixel sources
@@ -132,6 +131,24 @@ __device__ __forceinline__ int xcd_tile(int b, int g) {
     constexpr int kXcd = 8;
     const int per = g / kXcd, rem = g % kXcd, x = b % kXcd, k = b / kXcd;
     return x < rem ? x * (per + 1) + k : rem * (per + 1) + (x - rem) * per + k;
+}
+
+// Vertices of workgroup b of g (one thread each, kBlock a round): ids i,
+// i + step, ... below end.  Without a part map, one round of b * kBlock +
+// thread.  With one, the workgroups on XCD x (b % 8, g a multiple of 8) walk
+// part x's id range, so that its gathers stay in that XCD's L2.
+__device__ __forceinline__ void part_walk(const PartMap &pm, unsigned M, unsigned b, unsigned g, unsigned &i,
+                                          unsigned &end, unsigned &step) {
+    if (pm.cpp == 0) {
+        i = b * kBlock + threadIdx.x;
+        end = M;
+        step = g * kBlock;
+        return;
+    }
+    const unsigned x = b & 7u;
+    i = pm.off[x * pm.cpp] + (b >> 3) * kBlock + threadIdx.x;
+    end = x == 7 ? M : pm.off[(x + 1) * pm.cpp];
+    step = (g >> 3) * kBlock;
 }
 
 template <bool BATCH>
@@ -275,7 +292,7 @@ __global__ __launch_bounds__(kBlock) void k_splat_tile(PixSrc src0, int n0, int 
                     for (int c = 0; c < LVD; c++)
                         if (c < vd) atomicAdd(&lv[c * LT + ls], to_fixed(wt[r] * v[c], vsc.s[c]));
                 } else if (!full) {
-                    const int slot = table_insert<W>(ta.tkeys, ta.mask, k, probes);
+                    const int slot = table_insert<W>(ta.tkeys, ta.mask, k, home_slot<D, PACKED>(ta, kp, k), probes);
                     inserts++;
                     if (slot < 0) { flags |= kFlagFull; continue; }
                     unsigned long long *dst = ta.tacc + (size_t)slot * vd;
@@ -306,7 +323,7 @@ __global__ __launch_bounds__(kBlock) void k_splat_tile(PixSrc src0, int n0, int 
             uint64_t k[W];
 #pragma unroll
             for (int w = 0; w < W; w++) k[w] = lk[ls * W + w];
-            slot = table_insert<W>(ta.tkeys, ta.mask, k, probes);
+            slot = table_insert<W>(ta.tkeys, ta.mask, k, home_slot<D, PACKED>(ta, kp, k), probes);
             inserts++;
             if (slot < 0) flags |= kFlagFull;
         }
@@ -348,7 +365,7 @@ __global__ __launch_bounds__(kBlock) void k_splat_direct(PixSrc src, int n, int 
             for (int r = 0; r <= D; r++) {
                 uint64_t k[W];
                 if (!K::make(s, r, kp, 0ull, k)) { flags |= kFlagRange; continue; }
-                const int slot = table_insert<W>(ta.tkeys, ta.mask, k, probes);
+                const int slot = table_insert<W>(ta.tkeys, ta.mask, k, home_slot<D, PACKED>(ta, kp, k), probes);
                 inserts++;
                 if (slot < 0) { flags |= kFlagFull; continue; }
                 unsigned long long *dst = ta.tacc + (size_t)slot * vd;
@@ -682,108 +699,112 @@ __global__ __launch_bounds__(kBlock) void k_nbr_merge(const uint64_t *__restrict
 // a miss costs one or two probes) and sorting many wide keys would dominate.
 template <int D, bool PACKED>
 __global__ __launch_bounds__(kBlock) void k_nbr_hash(const uint64_t *__restrict__ vk, KeyPlan kp, const Counters *ctr,
-                                                     int stride, TableArgs ta, int *__restrict__ nbr) {
+                                                     int stride, TableArgs ta, PartMap pm, int *__restrict__ nbr) {
     constexpr int W = KeyOps<D, PACKED>::W;
     const unsigned M = ctr->M;
-    const unsigned i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= M) return;
+    unsigned i, end, step;
+    part_walk(pm, M, blockIdx.x, gridDim.x, i, end, step);
     const int j = blockIdx.y;
-    uint64_t k[W];
+    for (; i < end; i += step) {
+        uint64_t k[W];
 #pragma unroll
-    for (int w = 0; w < W; w++) k[w] = vk[(size_t)i * W + w];
-    // the two targets: sgn 0: k + 1 - (D+1)e_j, sgn 1: k - 1 + (D+1)e_j
-    uint64_t t[2][W];
-    bool ok[2] = {true, true};
+        for (int w = 0; w < W; w++) k[w] = vk[(size_t)i * W + w];
+        // the two targets: sgn 0: k + 1 - (D+1)e_j, sgn 1: k - 1 + (D+1)e_j
+        uint64_t t[2][W];
+        bool ok[2] = {true, true};
 #pragma unroll
-    for (int sgn = 0; sgn < 2; sgn++)
+        for (int sgn = 0; sgn < 2; sgn++)
 #pragma unroll
-        for (int w = 0; w < W; w++) t[sgn][w] = k[w];
+            for (int w = 0; w < W; w++) t[sgn][w] = k[w];
 #pragma unroll
-    for (int q = 0; q < D; q++) {
-        uint32_t f, lo, hi;
-        if (PACKED) {
-            f = (uint32_t)((k[0] >> kp.shift[q]) & kp.span[q]);
-            lo = 0;
-            hi = kp.span[q];
-        } else {
-            f = (uint32_t)((k[q / 4] >> (48 - 16 * (q % 4))) & 0xffff);
-            lo = 1 + 2 * (D + 1);
-            hi = 65535 - 2 * (D + 1);
-        }
-        const uint32_t step = (q == j) ? (uint32_t)D : 1u;
-        const int64_t dq = (q == j) ? -(int64_t)D : 1;
-#pragma unroll
-        for (int sgn = 0; sgn < 2; sgn++) {
-            const bool up = (q == j) == (sgn != 0);   // this field grows
-            ok[sgn] = ok[sgn] && (up ? f + step <= hi : f >= lo + step);
-            const int64_t v = sgn ? -dq : dq;
-            if (PACKED) t[sgn][0] += (uint64_t)v << kp.shift[q];
-            else t[sgn][q / 4] += (uint64_t)v << (48 - 16 * (q % 4));
-        }
-    }
-    // Both lookups probe in lockstep, so their loads are in flight together.
-    // With ids folded into the table words (one-word keys), a hit carries its
-    // vertex id and slot_id is not read.
-    const bool fold = W == 1 && ta.idshift > 0;
-    const uint64_t kmask = fold ? ta.keymask : ~0ull;
-    uint32_t s[2];
-    bool done[2];
-    int slot[2] = {-1, -1};
-    uint64_t hit[2] = {0, 0};
-#pragma unroll
-    for (int sgn = 0; sgn < 2; sgn++) {
-        s[sgn] = (uint32_t)hash_key<W>(t[sgn]) & ta.mask;
-        done[sgn] = !ok[sgn];
-    }
-    for (uint32_t it = 0; it < kMaxProbes && !(done[0] && done[1]); it++) {
-        uint64_t w0[2];
-#pragma unroll
-        for (int sgn = 0; sgn < 2; sgn++) w0[sgn] = done[sgn] ? kEmpty : ta.tkeys[(size_t)s[sgn] * W];
-#pragma unroll
-        for (int sgn = 0; sgn < 2; sgn++) {
-            if (done[sgn]) continue;
-            if (w0[sgn] == kEmpty) {
-                done[sgn] = true;
-                continue;
-            }
-            bool match = (w0[sgn] & kmask) == t[sgn][0];
-#pragma unroll
-            for (int w = 1; w < W; w++) match = match && ta.tkeys[(size_t)s[sgn] * W + w] == t[sgn][w];
-            if (match) {
-                slot[sgn] = (int)s[sgn];
-                hit[sgn] = w0[sgn];
-                done[sgn] = true;
+        for (int q = 0; q < D; q++) {
+            uint32_t f, lo, hi;
+            if (PACKED) {
+                f = (uint32_t)((k[0] >> kp.shift[q]) & kp.span[q]);
+                lo = 0;
+                hi = kp.span[q];
             } else {
-                s[sgn] = (s[sgn] + 1) & ta.mask;
+                f = (uint32_t)((k[q / 4] >> (48 - 16 * (q % 4))) & 0xffff);
+                lo = 1 + 2 * (D + 1);
+                hi = 65535 - 2 * (D + 1);
+            }
+            const uint32_t mv = (q == j) ? (uint32_t)D : 1u;
+            const int64_t dq = (q == j) ? -(int64_t)D : 1;
+#pragma unroll
+            for (int sgn = 0; sgn < 2; sgn++) {
+                const bool up = (q == j) == (sgn != 0);   // this field grows
+                ok[sgn] = ok[sgn] && (up ? f + mv <= hi : f >= lo + mv);
+                const int64_t v = sgn ? -dq : dq;
+                if (PACKED) t[sgn][0] += (uint64_t)v << kp.shift[q];
+                else t[sgn][q / 4] += (uint64_t)v << (48 - 16 * (q % 4));
             }
         }
-    }
-    int res[2];
+        // Both lookups probe in lockstep, so their loads are in flight together.
+        // With ids folded into the table words (one-word keys), a hit carries its
+        // vertex id and slot_id is not read.
+        const bool fold = W == 1 && ta.idshift > 0;
+        const uint64_t kmask = fold ? ta.keymask : ~0ull;
+        uint32_t s[2];
+        bool done[2];
+        int slot[2] = {-1, -1};
+        uint64_t hit[2] = {0, 0};
 #pragma unroll
-    for (int sgn = 0; sgn < 2; sgn++)
-        res[sgn] = slot[sgn] < 0 ? -1 : fold ? (int)(hit[sgn] >> ta.idshift) : ta.slot_id[slot[sgn]];
-    int2 *out = (int2 *)nbr + (size_t)j * stride + i;
-    *out = make_int2(res[0], res[1]);
+        for (int sgn = 0; sgn < 2; sgn++) {
+            s[sgn] = home_slot<D, PACKED>(ta, kp, t[sgn]);
+            done[sgn] = !ok[sgn];
+        }
+        for (uint32_t it = 0; it < kMaxProbes && !(done[0] && done[1]); it++) {
+            uint64_t w0[2];
+#pragma unroll
+            for (int sgn = 0; sgn < 2; sgn++) w0[sgn] = done[sgn] ? kEmpty : ta.tkeys[(size_t)s[sgn] * W];
+#pragma unroll
+            for (int sgn = 0; sgn < 2; sgn++) {
+                if (done[sgn]) continue;
+                if (w0[sgn] == kEmpty) {
+                    done[sgn] = true;
+                    continue;
+                }
+                bool match = (w0[sgn] & kmask) == t[sgn][0];
+#pragma unroll
+                for (int w = 1; w < W; w++) match = match && ta.tkeys[(size_t)s[sgn] * W + w] == t[sgn][w];
+                if (match) {
+                    slot[sgn] = (int)s[sgn];
+                    hit[sgn] = w0[sgn];
+                    done[sgn] = true;
+                } else {
+                    s[sgn] = (s[sgn] + 1) & ta.mask;
+                }
+            }
+        }
+        int res[2];
+#pragma unroll
+        for (int sgn = 0; sgn < 2; sgn++)
+            res[sgn] = slot[sgn] < 0 ? -1 : fold ? (int)(hit[sgn] >> ta.idshift) : ta.slot_id[slot[sgn]];
+        int2 *out = (int2 *)nbr + (size_t)j * stride + i;
+        *out = make_int2(res[0], res[1]);
+    }
 }
 
 // One blur pass along one axis (src/Permutohedral.h:442-475): Jacobi, new =
 // 0.25*a + 0.5*self + 0.25*b evaluated left to right, missing neighbour = 0.
 template <typename T, int VS>
-__global__ __launch_bounds__(kBlock) void k_blur(const int2 *__restrict__ nbr, const Counters *ctr,
+__global__ __launch_bounds__(kBlock) void k_blur(const int2 *__restrict__ nbr, const Counters *ctr, PartMap pm,
                                                  const T *__restrict__ in, T *__restrict__ out) {
     const unsigned M = ctr->M;
-    const unsigned i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= M) return;
-    const int2 nb = nbr[i];
-    float a[kMaxVD], o[kMaxVD], b[kMaxVD], r[kMaxVD];
+    unsigned i, end, step;
+    part_walk(pm, M, blockIdx.x, gridDim.x, i, end, step);
+    for (; i < end; i += step) {
+        const int2 nb = nbr[i];
+        float a[kMaxVD], o[kMaxVD], b[kMaxVD], r[kMaxVD];
 #pragma unroll
-    for (int c = 0; c < VS; c++) { a[c] = 0.0f; b[c] = 0.0f; }
-    if (nb.x >= 0) load_row<T, VS>(in, nb.x, a);
-    load_row<T, VS>(in, i, o);
-    if (nb.y >= 0) load_row<T, VS>(in, nb.y, b);
+        for (int c = 0; c < VS; c++) { a[c] = 0.0f; b[c] = 0.0f; }
+        if (nb.x >= 0) load_row<T, VS>(in, nb.x, a);
+        load_row<T, VS>(in, i, o);
+        if (nb.y >= 0) load_row<T, VS>(in, nb.y, b);
 #pragma unroll
-    for (int c = 0; c < VS; c++) r[c] = (0.25f * a[c] + 0.5f * o[c]) + 0.25f * b[c];
-    store_row<T, VS>(out, i, r);
+        for (int c = 0; c < VS; c++) r[c] = (0.25f * a[c] + 0.5f * o[c]) + 0.25f * b[c];
+        store_row<T, VS>(out, i, r);
+    }
 }
 
 // Final write of one pixel: raw vd channels, or the first vd-1 divided by the
@@ -883,7 +904,7 @@ __global__ __launch_bounds__(kBlock) void k_slice_tile(PixSrc src0, int n0, int 
         uint64_t k[W];
 #pragma unroll
         for (int q = 0; q < W; q++) k[q] = lk[ls * W + q];
-        const int id = table_vertex<W>(ta, k, probes);
+        const int id = table_vertex<D, PACKED>(ta, kp, k, probes);
         float row[kMaxVD];
 #pragma unroll
         for (int c = 0; c < LVD; c++) row[c] = 0.0f;
@@ -905,7 +926,7 @@ __global__ __launch_bounds__(kBlock) void k_slice_tile(PixSrc src0, int n0, int 
             } else if (lsl[r] == -2) {
                 uint64_t k[W];
                 K::make(s, r, kp, tv.tag, k);
-                const int id = table_vertex<W>(ta, k, probes);
+                const int id = table_vertex<D, PACKED>(ta, kp, k, probes);
                 if (id >= 0) {
                     float row[kMaxVD];
                     load_row<T, LVD>(vals, id, row);
@@ -946,7 +967,7 @@ __global__ __launch_bounds__(kBlock) void k_slice_direct(PixSrc src, int n, int 
             for (int r = 0; r <= D; r++) {
                 uint64_t k[W];
                 if (!K::make(s, r, kp, 0ull, k)) continue;
-                const int id = table_vertex<W>(ta, k, probes);
+                const int id = table_vertex<D, PACKED>(ta, kp, k, probes);
                 if (id < 0) continue;
                 float v[kMaxVD];
                 load_row_rt<T>(vals, id, vs, v);

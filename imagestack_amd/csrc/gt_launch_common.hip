@@ -323,14 +323,22 @@ hipError_t launch_neighbors(int d, bool packed, const KeyPlan &kp, const uint64_
     return hipGetLastError();
 }
 
+// Workgroups along x for m_max vertices; a multiple of 8 with a part map,
+// so that workgroup x of every row of a 2-D grid sits on XCD x % 8.
+static unsigned vertex_blocks(int m_max, const PartMap &pm) {
+    const unsigned g = blocks_for(m_max, kBlock);
+    return pm.cpp ? (g + 7u) & ~7u : g;
+}
+
 hipError_t launch_neighbors_hash(int d, bool packed, const KeyPlan &kp, const uint64_t *vkeys, const Counters *ctr,
-                                 int m_max, const TableArgs &ta, int2 *nbr, hipStream_t st) {
+                                 int m_max, const TableArgs &ta, const PartMap &pm, int2 *nbr, hipStream_t st) {
     if (m_max <= 0) return hipSuccess;
-    const dim3 g(blocks_for(m_max, kBlock), d + 1), b(kBlock);
-#define GT_NBRH(D)                                                                                               \
-    {                                                                                                            \
-        if (packed) hipLaunchKernelGGL((k_nbr_hash<D, true>), g, b, 0, st, vkeys, kp, ctr, m_max, ta, (int *)nbr);  \
-        else hipLaunchKernelGGL((k_nbr_hash<D, false>), g, b, 0, st, vkeys, kp, ctr, m_max, ta, (int *)nbr);        \
+    const dim3 g(vertex_blocks(m_max, pm), d + 1), b(kBlock);
+#define GT_NBRH(D)                                                                                                  \
+    {                                                                                                               \
+        if (packed)                                                                                                 \
+            hipLaunchKernelGGL((k_nbr_hash<D, true>), g, b, 0, st, vkeys, kp, ctr, m_max, ta, pm, (int *)nbr);      \
+        else hipLaunchKernelGGL((k_nbr_hash<D, false>), g, b, 0, st, vkeys, kp, ctr, m_max, ta, pm, (int *)nbr);    \
     }
     GT_DISPATCH_D(d, GT_NBRH)
 #undef GT_NBRH
@@ -338,27 +346,27 @@ hipError_t launch_neighbors_hash(int d, bool packed, const KeyPlan &kp, const ui
 }
 
 template <typename T>
-static hipError_t launch_blur_t(int vs, const int2 *nbr, const Counters *ctr, int m_max, const void *in, void *out,
-                                hipStream_t st) {
-    const dim3 g(blocks_for(m_max, kBlock)), b(kBlock);
+static hipError_t launch_blur_t(int vs, const int2 *nbr, const Counters *ctr, int m_max, const PartMap &pm,
+                                const void *in, void *out, hipStream_t st) {
+    const dim3 g(vertex_blocks(m_max, pm)), b(kBlock);
     const T *i = (const T *)in;
     T *o = (T *)out;
     switch (vs) {
-    case 2: hipLaunchKernelGGL((k_blur<T, 2>), g, b, 0, st, nbr, ctr, i, o); break;
-    case 4: hipLaunchKernelGGL((k_blur<T, 4>), g, b, 0, st, nbr, ctr, i, o); break;
-    case 8: hipLaunchKernelGGL((k_blur<T, 8>), g, b, 0, st, nbr, ctr, i, o); break;
-    case 16: hipLaunchKernelGGL((k_blur<T, 16>), g, b, 0, st, nbr, ctr, i, o); break;
+    case 2: hipLaunchKernelGGL((k_blur<T, 2>), g, b, 0, st, nbr, ctr, pm, i, o); break;
+    case 4: hipLaunchKernelGGL((k_blur<T, 4>), g, b, 0, st, nbr, ctr, pm, i, o); break;
+    case 8: hipLaunchKernelGGL((k_blur<T, 8>), g, b, 0, st, nbr, ctr, pm, i, o); break;
+    case 16: hipLaunchKernelGGL((k_blur<T, 16>), g, b, 0, st, nbr, ctr, pm, i, o); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
-hipError_t launch_blur(int storage, int vs, const int2 *nbr, const Counters *ctr, int m_max, const void *in, void *out,
-                       hipStream_t st) {
+hipError_t launch_blur(int storage, int vs, const int2 *nbr, const Counters *ctr, int m_max, const PartMap &pm,
+                       const void *in, void *out, hipStream_t st) {
     if (m_max <= 0) return hipSuccess;
-    if (storage == 0) return launch_blur_t<float>(vs, nbr, ctr, m_max, in, out, st);
-    if (storage == 1) return launch_blur_t<__hip_bfloat16>(vs, nbr, ctr, m_max, in, out, st);
-    return launch_blur_t<__half>(vs, nbr, ctr, m_max, in, out, st);
+    if (storage == 0) return launch_blur_t<float>(vs, nbr, ctr, m_max, pm, in, out, st);
+    if (storage == 1) return launch_blur_t<__hip_bfloat16>(vs, nbr, ctr, m_max, pm, in, out, st);
+    return launch_blur_t<__half>(vs, nbr, ctr, m_max, pm, in, out, st);
 }
 
 hipError_t launch_unpack_keys(int d, bool packed, const KeyPlan &kp, const uint64_t *vkeys, int M, int16_t *keys,

@@ -46,7 +46,7 @@ class GTError(RuntimeError):
 
 class Tuning(C.Structure):
     _fields_ = [("load_factor", C.c_float), ("waves_per_cu", C.c_int), ("dedup", C.c_int), ("tile_w", C.c_int),
-                ("graph", C.c_int), ("neighbours", C.c_int)]
+                ("graph", C.c_int), ("neighbours", C.c_int), ("parts", C.c_int)]
 
 
 class TuneReport(C.Structure):

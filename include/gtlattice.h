@@ -72,6 +72,9 @@ typedef struct {
                             2 compaction-order ids and hash lookups (no sort).  The
                             output is the same bit for bit; gt_build_lattice always
                             sorts (the exported lattice is in key order).          */
+    int parts;           /* hash table in eight row bands, one per XCD's L2: 0 auto
+                            (one-frame bilateral forms with one-word keys and at
+                            least 2^14 slots), -1 off.  The output is the same.   */
 } gt_tuning;
 
 /* Per-call statistics (all optional). */
