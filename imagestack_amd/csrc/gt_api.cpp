@@ -150,7 +150,7 @@ struct Tune {
     int wpc = 0;        // waves-per-CU cap of the tiled and neighbour kernels (0: none)
     bool graph = true;  // replay the pipeline as a cached hipGraph
     bool grid = false;  // the bilateral grid instead of the lattice (GRID entry points only)
-    int nbr = 0;        // 0 auto (hash lookups for d >= 11 or <= 2^20 expected vertices), 1 sort + merge, 2 hash
+    int nbr = 0;        // 0 auto (hash lookups for d >= 11 or few expected vertices), 1 sort + merge, 2 hash
     bool parts = true;  // row-band table parts where they apply
 };
 
@@ -444,15 +444,20 @@ int run(gt_ctx *c, const gt::PixSrc &src, int n, int row_w, int d, int vd, int s
         L.bilat = bilat;
         L.dedup = dedup;
         L.count = count;
+        // Row-band table parts (below) apply to one-frame bilateral forms.
+        const bool parts = tu.parts && !batch && bilat && src.fy && !src.ft && cap >= (1u << 14);
         // Vertex ids: sorted (merge neighbours) or compaction order (hash
         // neighbours, no sort); the output is the same bit for bit.  Skipping
         // the sort pays for small lattices (its ~30 launches dominate: 2048^2
-        // bilateral, 0.41 M vertices, 0.615 -> 0.557 ms) and at high d (many
-        // passes over wide keys: d = 16, 17.8 M vertices, 37.4 -> 31.3 ms);
-        // in between, sorted ids keep the blur's gathers local (d = 8, 2.1 M
-        // vertices: 3.27 ms sorted, 3.42 ms by hash).  A kept lattice
-        // (gt_build_lattice) is always sorted.
-        L.hashnbr = !keep_splat && !batch && (tu.nbr == 2 || (tu.nbr == 0 && (d >= 11 || est <= (1u << 20))));
+        // joint bilateral, 0.41 M vertices, 0.65 -> 0.45 ms with parts) and at
+        // high d (many passes over wide keys: d = 16, 17.8 M vertices, 37.4 ->
+        // 31.3 ms).  In between, sorted ids keep the blur's gathers local,
+        // unless table parts do so up to a few million vertices (d = 8, 2.1 M
+        // vertices: 3.12 ms sorted, 2.93 ms by hash with parts, 3.45 without;
+        // 4.7 M vertices at d = 5: 3.29 ms sorted, 3.83 by hash with parts).
+        // A kept lattice (gt_build_lattice) is always sorted.
+        const uint64_t hash_upto = parts ? (3u << 20) : (1u << 20);
+        L.hashnbr = !keep_splat && !batch && (tu.nbr == 2 || (tu.nbr == 0 && (d >= 11 || est <= hash_upto)));
         L.cap = cap;
         gt::plan_scale_factors(d, L.lc.sf);
         L.kp = c->plan;
@@ -506,9 +511,10 @@ int run(gt_ctx *c, const gt::PixSrc &src, int n, int row_w, int d, int vd, int s
         // x-th band of tiles, and (ids in compaction order) the neighbour
         // and blur kernels the x-th part's ids, so the table lines, sums and
         // values each XCD touches stay mostly in its own L2.
-        if (tu.parts && !batch && bilat && L.packed && src.fy && !src.ft && cap >= (1u << 14)) {
+        if (parts) {
+            // fields hold c_i - lo_i (packed) or c_i + 32768 (wide)
             double sum_lo = 0;
-            for (int i = 0; i < d; i++) sum_lo += L.kp.lo[i];
+            for (int i = 0; i < d; i++) sum_lo += L.packed ? L.kp.lo[i] : -32768;
             const double ymax = (double)(src.H - 1) * src.iy;
             const int margin = 2 * (d + 1);
             const double range = std::ceil((double)d * L.lc.sf[d - 1] * ymax) + 2.0 * margin;

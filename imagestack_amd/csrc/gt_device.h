@@ -269,15 +269,15 @@ __device__ __forceinline__ uint32_t home_slot(const TableArgs &ta, const KeyPlan
                                               const uint64_t (&k)[KeyOps<D, PACKED>::W]) {
     constexpr int W = KeyOps<D, PACKED>::W;
     const uint32_t h = (uint32_t)hash_key<W>(k);
-    if constexpr (PACKED) {
-        if (ta.part_mul) {
-            int sum = ta.part_base;
+    if (ta.part_mul) {
+        int sum = ta.part_base;   // offsets the fields' sum to c_0 + ... + c_{D-1} + margin
 #pragma unroll
-            for (int i = 0; i < D; i++) sum += (int)((k[0] >> kp.shift[i]) & kp.span[i]);
-            const uint32_t u = (uint32_t)min(max(sum, 0), ta.part_range);
-            const uint32_t part = (uint32_t)(((uint64_t)u * ta.part_mul) >> 24);
-            return (part << ta.part_shift) | (h & (ta.mask >> 3));
-        }
+        for (int i = 0; i < D; i++)
+            sum += PACKED ? (int)((k[0] >> kp.shift[i]) & kp.span[i])
+                          : (int)((k[i / 4] >> (48 - 16 * (i % 4))) & 0xffff);
+        const uint32_t u = (uint32_t)min(max(sum, 0), ta.part_range);
+        const uint32_t part = (uint32_t)(((uint64_t)u * ta.part_mul) >> 24);
+        return (part << ta.part_shift) | (h & (ta.mask >> 3));
     }
     return h & ta.mask;
 }

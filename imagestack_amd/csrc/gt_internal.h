@@ -84,7 +84,7 @@ struct TableArgs {
                                    // table word holds (id << idshift) | key, so a lookup reads the
                                    // vertex id with the key; slot_id is not used
     uint64_t keymask;              // (1 << idshift) - 1
-    // part_mul != 0 (one-word keys of a bilateral form over image rows): the
+    // part_mul != 0 (a one-frame bilateral form over image rows): the
     // table is eight parts of capacity / 8 slots; a key's home slot lies in
     // part ((c_0 + ... + c_{D-1} + part_base) clamped to [0, part_range]) *
     // part_mul >> 24, its hash picking the slot inside the part.  The sum is
