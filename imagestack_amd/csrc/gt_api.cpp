@@ -415,8 +415,11 @@ int run(gt_ctx *c, const gt::PixSrc &src, int n, int row_w, int d, int vd, int s
     const bool dedup = batch ? true : (tu.dedup == 0) ? (d <= 10 && vd <= 4) : (tu.dedup > 0);
     const int kind = batch ? 2 : bilat ? 1 : 0;
 
+    // Expected vertex count: the last call's M for the same size and d, else
+    // the pixel count (a guess: M / n runs from ~0.05 to d + 1).
     uint64_t est;
-    if (c->last_n == n && c->last_d == d && c->last_M > 0) est = (uint64_t)(c->last_M * 1.05) + 64;
+    bool est_measured = c->last_n == n && c->last_d == d && c->last_M > 0;
+    if (est_measured) est = (uint64_t)(c->last_M * 1.05) + 64;
     else est = std::min<uint64_t>((uint64_t)n * (d + 1), std::max<uint64_t>(65536, (uint64_t)n));
     uint32_t cap = next_pow2((uint64_t)(est / tu.lf));
     const uint32_t cap_limit = next_pow2((uint64_t)((double)n * (d + 1) / tu.lf));
@@ -454,9 +457,10 @@ int run(gt_ctx *c, const gt::PixSrc &src, int n, int row_w, int d, int vd, int s
         // 31.3 ms).  In between, sorted ids keep the blur's gathers local,
         // unless table parts do so up to a few million vertices (d = 8, 2.1 M
         // vertices: 3.12 ms sorted, 2.93 ms by hash with parts, 3.45 without;
-        // 4.7 M vertices at d = 5: 3.29 ms sorted, 3.83 by hash with parts).
-        // A kept lattice (gt_build_lattice) is always sorted.
-        const uint64_t hash_upto = parts ? (3u << 20) : (1u << 20);
+        // 4.7 M vertices at d = 5: 3.29 ms sorted, 3.83 by hash with parts);
+        // the higher limit needs a measured M.  A kept lattice
+        // (gt_build_lattice) is always sorted.
+        const uint64_t hash_upto = (parts && est_measured) ? (3u << 20) : (1u << 20);
         L.hashnbr = !keep_splat && !batch && (tu.nbr == 2 || (tu.nbr == 0 && (d >= 11 || est <= hash_upto)));
         L.cap = cap;
         gt::plan_scale_factors(d, L.lc.sf);
@@ -546,7 +550,14 @@ int run(gt_ctx *c, const gt::PixSrc &src, int n, int row_w, int d, int vd, int s
         if (hc.flags & gt::kFlagFull) {
             if (++retries > 6 || cap >= cap_limit)
                 return fail(GT_ENOMEM, "lattice does not fit a hash table of %u slots", cap);
-            cap = std::min<uint32_t>(cap_limit, cap * 8);
+            // a full table held about `cap` vertices: decide the redo's
+            // neighbour form on that
+            est = std::max<uint64_t>(est, cap);
+            est_measured = false;
+            // M <= (d + 1) n, and a first table holds n / load_factor slots: x4
+            // covers every lattice up to d = 7 at the default load in one redo
+            // (a larger table only lengthens the redo's clear and compaction)
+            cap = std::min<uint32_t>(cap_limit, cap * (d <= 7 ? 4u : 8u));
             continue;
         }
         const int M = (int)hc.M;
