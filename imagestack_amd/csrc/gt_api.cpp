@@ -86,9 +86,17 @@ struct gt_ctx {
     DevBuf patch_w, patch_guide;       // non-local means: reordered filters, projected guide
     gt::Counters *h_ctr = nullptr;   // pinned
     float *h_mm = nullptr;           // pinned, 3 * 2 * kMaxDH
-    // capacity memory: last lattice size seen for (n, d)
+    // capacity memory: last lattice size seen for (n, d), and vertices per
+    // pixel seen for a filter (form, d, vd and the scales), used when the
+    // same filter comes with a new image size
     int last_n = -1, last_d = -1;
     unsigned last_M = 0;
+    struct Density {
+        uint64_t key = 0;
+        float per_pixel = 0.f;
+    };
+    Density density[16];
+    int density_next = 0;
     // packing plan cache (position-source kind, d): reused until a key falls outside it
     bool plan_valid = false;
     int plan_d = 0, plan_kind = -1, plan_vd = 0;
@@ -406,6 +414,27 @@ int execute(gt_ctx *c, const Launch &L, const gt::PixSrc &src, const SliceSpec &
 int make_plan_batch(gt_ctx *c, const BatchSpec &b, const std::vector<gt::BatchImage> &imgs, int d, int vd,
                     int tile_w);
 
+// Signature of a filter for the vertex-density memory: the position form,
+// d, vd and the scales of every position channel (FNV-1a over their bytes;
+// never 0).
+uint64_t density_key(const gt::PixSrc &src, int kind, int d, int vd) {
+    std::string k;
+    sig_add(k, kind);
+    sig_add(k, d);
+    sig_add(k, vd);
+    if (src.ref) {
+        sig_add(k, src.cr);
+        sig_add(k, src.ic);
+        const float spatial[3] = {src.fx ? src.ix : 0.f, src.fy ? src.iy : 0.f, src.ft ? src.it : 0.f};
+        sig_add(k, spatial);
+    } else {
+        for (int i = 0; i < d; i++) sig_add(k, src.inv[i]);
+    }
+    uint64_t h = 1469598103934665603ull;
+    for (unsigned char ch : k) h = (h ^ ch) * 1099511628211ull;
+    return h ? h : 1;
+}
+
 int run(gt_ctx *c, const gt::PixSrc &src, int n, int row_w, int d, int vd, int storage, const Tune &tu,
         const SliceSpec &sl, bool keep_splat, gt_stats *st, const BatchSpec *batch = nullptr,
         const std::vector<gt::BatchImage> *bimgs = nullptr) {
@@ -415,12 +444,18 @@ int run(gt_ctx *c, const gt::PixSrc &src, int n, int row_w, int d, int vd, int s
     const bool dedup = batch ? true : (tu.dedup == 0) ? (d <= 10 && vd <= 4) : (tu.dedup > 0);
     const int kind = batch ? 2 : bilat ? 1 : 0;
 
-    // Expected vertex count: the last call's M for the same size and d, else
+    // Expected vertex count: the vertices per pixel this filter gave before
+    // (any image size), else the last call's M for the same size and d, else
     // the pixel count (a guess: M / n runs from ~0.05 to d + 1).
+    const uint64_t dkey = batch ? 0 : density_key(src, kind, d, vd);
+    float per_pixel = 0.f;
+    for (const auto &e : c->density)
+        if (dkey && e.key == dkey) per_pixel = e.per_pixel;
     uint64_t est;
-    bool est_measured = c->last_n == n && c->last_d == d && c->last_M > 0;
-    if (est_measured) est = (uint64_t)(c->last_M * 1.05) + 64;
-    else est = std::min<uint64_t>((uint64_t)n * (d + 1), std::max<uint64_t>(65536, (uint64_t)n));
+    if (per_pixel > 0.f) est = std::max<uint64_t>(65536, (uint64_t)((double)n * per_pixel * 1.05) + 64);
+    else if (c->last_n == n && c->last_d == d && c->last_M > 0) est = (uint64_t)(c->last_M * 1.05) + 64;
+    else est = std::max<uint64_t>(65536, (uint64_t)n);
+    est = std::min<uint64_t>(est, (uint64_t)n * (d + 1));
     uint32_t cap = next_pow2((uint64_t)(est / tu.lf));
     const uint32_t cap_limit = next_pow2((uint64_t)((double)n * (d + 1) / tu.lf));
     if (cap > cap_limit) cap = cap_limit;
@@ -447,21 +482,6 @@ int run(gt_ctx *c, const gt::PixSrc &src, int n, int row_w, int d, int vd, int s
         L.bilat = bilat;
         L.dedup = dedup;
         L.count = count;
-        // Row-band table parts (below) apply to one-frame bilateral forms.
-        const bool parts = tu.parts && !batch && bilat && src.fy && !src.ft && cap >= (1u << 14);
-        // Vertex ids: sorted (merge neighbours) or compaction order (hash
-        // neighbours, no sort); the output is the same bit for bit.  Skipping
-        // the sort pays for small lattices (its ~30 launches dominate: 2048^2
-        // joint bilateral, 0.41 M vertices, 0.65 -> 0.45 ms with parts) and at
-        // high d (many passes over wide keys: d = 16, 17.8 M vertices, 37.4 ->
-        // 31.3 ms).  In between, sorted ids keep the blur's gathers local,
-        // unless table parts do so up to a few million vertices (d = 8, 2.1 M
-        // vertices: 3.12 ms sorted, 2.93 ms by hash with parts, 3.45 without;
-        // 4.7 M vertices at d = 5: 3.29 ms sorted, 3.83 by hash with parts);
-        // the higher limit needs a measured M.  A kept lattice
-        // (gt_build_lattice) is always sorted.
-        const uint64_t hash_upto = (parts && est_measured) ? (3u << 20) : (1u << 20);
-        L.hashnbr = !keep_splat && !batch && (tu.nbr == 2 || (tu.nbr == 0 && (d >= 11 || est <= hash_upto)));
         L.cap = cap;
         gt::plan_scale_factors(d, L.lc.sf);
         L.kp = c->plan;
@@ -469,6 +489,49 @@ int run(gt_ctx *c, const gt::PixSrc &src, int n, int row_w, int d, int vd, int s
         L.packed = L.kp.packed != 0;
         if (batch && !L.packed) return fail(GT_EINVAL, "batched lattice keys do not fit 63 bits");
         L.W = L.packed ? 1 : (d + 3) / 4;
+        // Row bands: in a one-frame bilateral form the last position channel
+        // is the row, y / sigma_s, and the last lattice coordinate of every
+        // vertex is -d sf y / sigma_s to within 2(d + 1).  P equal ranges of
+        // it give P table parts (P a multiple of 8); the splat and slice hand
+        // XCD x the x-th eighth of the tiles, in row order, and (ids in
+        // compaction order) the neighbour and blur kernels the ids of parts
+        // [x P / 8, (x + 1) P / 8), in order, so the table lines, sums and
+        // values each XCD touches at a time stay mostly in its own L2.
+        if (tu.parts && !batch && bilat && src.fy && !src.ft && cap >= (1u << 14)) {
+            // fields hold c_i - lo_i (packed) or c_i + 32768 (wide)
+            double sum_lo = 0;
+            for (int i = 0; i < d; i++) sum_lo += L.packed ? L.kp.lo[i] : -32768;
+            const double ymax = (double)(src.H - 1) * src.iy;
+            const int margin = 2 * (d + 1);
+            const double range = std::ceil((double)d * L.lc.sf[d - 1] * ymax) + 2.0 * margin;
+            // About 2 MB of key words per part (each XCD's L2 is 4 MB), and
+            // parts at least 2(d + 1) units of c_d wide; at least eight.
+            int lg = 0;
+            while ((1u << lg) < cap) lg++;
+            int plg = 3;
+            while (plg < lg - 8 && ((uint64_t)cap * L.W * 8 >> (plg + 1)) >= (1u << 21) &&
+                   range / (double)(1 << (plg + 1)) >= 2.0 * (d + 1))
+                plg++;
+            if (range >= (double)(1 << plg) && range < (double)(1 << 24)) {
+                L.ta.part_base = (int)sum_lo + margin;
+                L.ta.part_range = (int)range;
+                L.ta.part_mul = (uint32_t)((double)(1ull << (24 + plg)) / (range + 1.0));
+                L.ta.part_lg = plg;
+                L.ta.part_shift = lg - plg;
+            }
+        }
+        // Vertex ids: sorted (merge neighbours) or compaction order (hash
+        // neighbours, no sort); the output is the same bit for bit.  With
+        // table parts the hash form wins at every size measured (2048^2
+        // joint bilateral, 0.41 M vertices: 0.45 against 0.65 ms; d = 8,
+        // 2.1 M: 2.55 against 3.12 ms; 4096^2 RGB at sigma_s 4, 28.9 M: 16.4
+        // against 17.2 ms).  Without them it pays for small lattices (the
+        // sort's ~30 launches dominate) and at high d (many passes over wide
+        // keys: d = 16, 17.8 M vertices, 37.4 -> 31.3 ms); in between, sorted
+        // ids keep the blur's gathers local.  A kept lattice
+        // (gt_build_lattice) is always sorted.
+        L.hashnbr = !keep_splat && !batch &&
+                    (tu.nbr == 2 || (tu.nbr == 0 && (L.ta.part_mul != 0 || d >= 11 || est <= (1u << 20))));
         L.m_max = (int)std::min<uint64_t>(cap, (uint64_t)n * (d + 1));
         L.pw = L.packed ? 1 : std::max(1, (L.kp.bits + 63) / 64);
         L.vs = value_stride(vd);
@@ -508,30 +571,8 @@ int run(gt_ctx *c, const gt::PixSrc &src, int n, int row_w, int d, int vd, int s
             L.ta.idshift = fold ? L.kp.bits : 0;
             L.ta.keymask = fold ? (1ull << L.kp.bits) - 1 : ~0ull;
         }
-        // Row bands: in a one-frame bilateral form the last position channel
-        // is the row, y / sigma_s, and the last lattice coordinate of every
-        // vertex is -d sf y / sigma_s to within 2(d + 1).  Eight equal ranges
-        // of it give eight table parts; the splat and slice hand XCD x the
-        // x-th band of tiles, and (ids in compaction order) the neighbour
-        // and blur kernels the x-th part's ids, so the table lines, sums and
-        // values each XCD touches stay mostly in its own L2.
-        if (parts) {
-            // fields hold c_i - lo_i (packed) or c_i + 32768 (wide)
-            double sum_lo = 0;
-            for (int i = 0; i < d; i++) sum_lo += L.packed ? L.kp.lo[i] : -32768;
-            const double ymax = (double)(src.H - 1) * src.iy;
-            const int margin = 2 * (d + 1);
-            const double range = std::ceil((double)d * L.lc.sf[d - 1] * ymax) + 2.0 * margin;
-            if (range >= 8 && range < (double)(1 << 24)) {
-                L.ta.part_base = (int)sum_lo + margin;
-                L.ta.part_range = (int)range;
-                L.ta.part_mul = (uint32_t)((double)(1u << 27) / (range + 1.0));
-                int lg = 0;
-                while ((1u << lg) < cap) lg++;
-                L.ta.part_shift = lg - 3;
-                if (L.hashnbr) L.pm = {(const uint32_t *)c->counts.p, (int)(cap / 8 / gt::kCompactSlots)};
-            }
-        }
+        if (L.hashnbr && L.ta.part_mul)
+            L.pm = {(const uint32_t *)c->counts.p, (int)(cap / 8 / gt::kCompactSlots)};
 
         void *cur = nullptr;
         GT_TRY(execute(c, L, src, sl, batch, tu.graph && !keep_splat && c->profiling == 0, &passes, &cur));
@@ -553,7 +594,6 @@ int run(gt_ctx *c, const gt::PixSrc &src, int n, int row_w, int d, int vd, int s
             // a full table held about `cap` vertices: decide the redo's
             // neighbour form on that
             est = std::max<uint64_t>(est, cap);
-            est_measured = false;
             // M <= (d + 1) n, and a first table holds n / load_factor slots: x4
             // covers every lattice up to d = 7 at the default load in one redo
             // (a larger table only lengthens the redo's clear and compaction)
@@ -564,6 +604,17 @@ int run(gt_ctx *c, const gt::PixSrc &src, int n, int row_w, int d, int vd, int s
         c->last_n = n;
         c->last_d = d;
         c->last_M = (unsigned)M;
+        if (dkey) {
+            gt_ctx::Density *slot = nullptr;
+            for (auto &e : c->density)
+                if (e.key == dkey) slot = &e;
+            if (!slot) {
+                slot = &c->density[c->density_next];
+                c->density_next = (c->density_next + 1) % (int)(sizeof(c->density) / sizeof(c->density[0]));
+            }
+            slot->key = dkey;
+            slot->per_pixel = (float)M / (float)n;
+        }
         c->have = !L.hashnbr;   // a lattice in compaction order is not exported
         c->M = M;
         c->d = d;

@@ -277,7 +277,7 @@ __device__ __forceinline__ uint32_t home_slot(const TableArgs &ta, const KeyPlan
                           : (int)((k[i / 4] >> (48 - 16 * (i % 4))) & 0xffff);
         const uint32_t u = (uint32_t)min(max(sum, 0), ta.part_range);
         const uint32_t part = (uint32_t)(((uint64_t)u * ta.part_mul) >> 24);
-        return (part << ta.part_shift) | (h & (ta.mask >> 3));
+        return (part << ta.part_shift) | (h & (ta.mask >> ta.part_lg));
     }
     return h & ta.mask;
 }

@@ -85,18 +85,21 @@ struct TableArgs {
                                    // vertex id with the key; slot_id is not used
     uint64_t keymask;              // (1 << idshift) - 1
     // part_mul != 0 (a one-frame bilateral form over image rows): the
-    // table is eight parts of capacity / 8 slots; a key's home slot lies in
-    // part ((c_0 + ... + c_{D-1} + part_base) clamped to [0, part_range]) *
-    // part_mul >> 24, its hash picking the slot inside the part.  The sum is
-    // -c_D, which follows the last position channel (the row) alone, so each
-    // horizontal band of the image keeps its vertices in one part of the table.
+    // table is P = 2^part_lg parts of capacity / P slots; a key's home slot
+    // lies in part ((c_0 + ... + c_{D-1} + part_base) clamped to [0,
+    // part_range]) * part_mul >> 24, its hash picking the slot inside the
+    // part.  The sum is -c_D, which follows the last position channel (the
+    // row) alone, so each horizontal band of the image keeps its vertices in
+    // one part of the table.
     uint32_t part_mul;
-    int part_base, part_range, part_shift;   // part_shift = log2(capacity / 8)
+    int part_base, part_range;
+    int part_lg;                   // log2 of the part count P (>= 8)
+    int part_shift;                // log2(capacity / P)
 };
 
-// Vertex ids grouped by table part (ids in compaction order of a partitioned
-// table): part x holds ids [off[x * cpp], off[(x + 1) * cpp]) (M for x = 8),
-// off being the compaction's per-chunk offsets.  cpp = 0: no grouping.
+// Vertex ids grouped by eighths of a partitioned table (ids in compaction
+// order): eighth x holds ids [off[x * cpp], off[(x + 1) * cpp]) (M for
+// x = 8), off being the compaction's per-chunk offsets.  cpp = 0: none.
 struct PartMap {
     const uint32_t *off;
     int cpp;

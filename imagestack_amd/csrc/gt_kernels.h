@@ -136,7 +136,8 @@ __device__ __forceinline__ int xcd_tile(int b, int g) {
 // Vertices of workgroup b of g (one thread each, kBlock a round): ids i,
 // i + step, ... below end.  Without a part map, one round of b * kBlock +
 // thread.  With one, the workgroups on XCD x (b % 8, g a multiple of 8) walk
-// part x's id range, so that its gathers stay in that XCD's L2.
+// the ids of the x-th eighth of the table parts in increasing order (part
+// by part), so that their gathers stay in that XCD's L2.
 __device__ __forceinline__ void part_walk(const PartMap &pm, unsigned M, unsigned b, unsigned g, unsigned &i,
                                           unsigned &end, unsigned &step) {
     if (pm.cpp == 0) {

@@ -66,16 +66,16 @@ typedef struct {
     int dedup;           /* LDS tile dedup / pre-reduction: 0 auto, 1 on, -1 off     */
     int tile_w;          /* pixel-tile width of a workgroup: 16, 32 (default), 64    */
     int graph;           /* 0: replay the pipeline as a cached hipGraph; -1: don't  */
-    int neighbours;      /* vertex ids and neighbour table: 0 auto (hash for d >= 11
-                            or an expected lattice of at most 2^20 vertices, 3 *
-                            2^20 with row-band table parts),
+    int neighbours;      /* vertex ids and neighbour table: 0 auto (hash with row-band
+                            table parts, else for d >= 11 or an expected lattice
+                            of at most 2^20 vertices),
                             1 radix-sorted ids and a merge of the sorted keys,
                             2 compaction-order ids and hash lookups (no sort).  The
                             output is the same bit for bit; gt_build_lattice always
                             sorts (the exported lattice is in key order).          */
-    int parts;           /* hash table in eight row bands, one per XCD's L2: 0 auto
-                            (one-frame bilateral forms with at least 2^14 slots),
-                            -1 off.  The output is the same.                       */
+    int parts;           /* hash table in row bands, 8 or more, an eighth of them per
+                            XCD's L2: 0 auto (one-frame bilateral forms with at
+                            least 2^14 slots), -1 off.  The output is the same.    */
 } gt_tuning;
 
 /* Per-call statistics (all optional). */
