@@ -100,6 +100,9 @@ struct gt_ctx {
     // packing plan cache (position-source kind, d): reused until a key falls outside it
     bool plan_valid = false;
     int plan_d = 0, plan_kind = -1, plan_vd = 0;
+    int plan_n = 0;                  // pixels of the call the plan was made for
+    float last_lo = 0, last_hi = 0;  // its last position channel's scaled bounds
+
     gt::KeyPlan plan{};
     float vmax[gt::kMaxDH] = {};   // value bounds behind the fixed-point scales
     // cached pipeline graph
@@ -231,6 +234,9 @@ int make_plan(gt_ctx *c, const gt::PixSrc &src, int n, int d, int vd, bool bilat
     c->plan_d = d;
     c->plan_vd = vd;
     c->plan_kind = bilat ? 1 : 0;
+    c->plan_n = n;
+    c->last_lo = lo[d - 1];
+    c->last_hi = hi[d - 1];
     return GT_OK;
 }
 
@@ -464,9 +470,13 @@ int run(gt_ctx *c, const gt::PixSrc &src, int n, int row_w, int d, int vd, int s
         return make_plan(c, src, n, d, vd, bilat);
     };
     const int images = batch ? batch->ba.count : 1;
+    // A Gauss transform over an image (row_w > 0) may split its table by its
+    // last position channel: that needs the channel's bounds for this call.
+    const bool gauss_rows = tu.parts && !batch && !bilat && row_w > 0 && n >= (1 << 20);
     if (!(c->plan_valid && c->plan_d == d && c->plan_vd == vd && c->plan_kind == kind &&
-          (images <= 1 || (images - 1) >> c->plan.img_bits == 0)))
+          (images <= 1 || (images - 1) >> c->plan.img_bits == 0) && (!gauss_rows || c->plan_n == n)))
         GT_TRY(replan());
+    bool parts_failed = false;   // a split table filled up: redo with the plain one
 
     int retries = 0, passes = 0;
     for (;;) {
@@ -497,23 +507,42 @@ int run(gt_ctx *c, const gt::PixSrc &src, int n, int row_w, int d, int vd, int s
         // compaction order) the neighbour and blur kernels the ids of parts
         // [x P / 8, (x + 1) P / 8), in order, so the table lines, sums and
         // values each XCD touches at a time stay mostly in its own L2.
-        if (tu.parts && !batch && bilat && src.fy && !src.ft && cap >= (1u << 14)) {
+        // A Gauss transform over an image is split the same way by its last
+        // position channel when the caller laid that channel out by rows (an
+        // uneven split only lengthens probes; one that fills the table is
+        // redone without parts).
+        bool rows = false;
+        double plo = 0, phi = 0;   // bounds of the last position channel, scaled
+        if (tu.parts && !parts_failed && !batch && cap >= (1u << 14)) {
+            if (bilat && src.fy && !src.ft) {
+                rows = true;
+                phi = (double)(src.H - 1) * src.iy;
+            } else if (gauss_rows && std::isfinite(c->last_lo) && std::isfinite(c->last_hi) &&
+                       c->last_hi > c->last_lo) {
+                rows = true;
+                plo = c->last_lo;
+                phi = c->last_hi;
+            }
+        }
+        if (rows) {
             // fields hold c_i - lo_i (packed) or c_i + 32768 (wide)
             double sum_lo = 0;
             for (int i = 0; i < d; i++) sum_lo += L.packed ? L.kp.lo[i] : -32768;
-            const double ymax = (double)(src.H - 1) * src.iy;
             const int margin = 2 * (d + 1);
-            const double range = std::ceil((double)d * L.lc.sf[d - 1] * ymax) + 2.0 * margin;
+            const double base = std::floor((double)d * L.lc.sf[d - 1] * plo);
+            const double range = std::ceil((double)d * L.lc.sf[d - 1] * phi - base) + 2.0 * margin;
             // About 2 MB of key words per part (each XCD's L2 is 4 MB), and
-            // parts at least 2(d + 1) units of c_d wide; at least eight.
+            // parts at least 4 units of c_d wide; at least eight.  (A vertex's
+            // neighbours are within d units of its c_d: parts narrower than
+            // that still hold them a few parts, so a few ids, apart.)
             int lg = 0;
             while ((1u << lg) < cap) lg++;
             int plg = 3;
             while (plg < lg - 8 && ((uint64_t)cap * L.W * 8 >> (plg + 1)) >= (1u << 21) &&
-                   range / (double)(1 << (plg + 1)) >= 2.0 * (d + 1))
+                   range / (double)(1 << (plg + 1)) >= 4.0)
                 plg++;
             if (range >= (double)(1 << plg) && range < (double)(1 << 24)) {
-                L.ta.part_base = (int)sum_lo + margin;
+                L.ta.part_base = (int)(sum_lo - base) + margin;
                 L.ta.part_range = (int)range;
                 L.ta.part_mul = (uint32_t)((double)(1ull << (24 + plg)) / (range + 1.0));
                 L.ta.part_lg = plg;
@@ -594,6 +623,7 @@ int run(gt_ctx *c, const gt::PixSrc &src, int n, int row_w, int d, int vd, int s
             // a full table held about `cap` vertices: decide the redo's
             // neighbour form on that
             est = std::max<uint64_t>(est, cap);
+            if (L.ta.part_mul) parts_failed = true;
             // M <= (d + 1) n, and a first table holds n / load_factor slots: x4
             // covers every lattice up to d = 7 at the default load in one redo
             // (a larger table only lengthens the redo's clear and compaction)

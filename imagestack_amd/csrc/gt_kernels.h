@@ -338,15 +338,16 @@ __global__ __launch_bounds__(kBlock) void k_splat_tile(PixSrc src0, int n0, int 
     if (count) report_counts(ctr, probes, inserts, atomics);
 }
 
-// splat, direct form (D > 6 or vd > 4): every (pixel, vertex) goes to the
-// global table with FP32 atomics.
+// splat, direct form (D > 10, vd > 4, or the LDS tiles turned off): every
+// (pixel, vertex) goes to the global table, its values added as fixed-point
+// integers.  Workgroups take the tiles in XCD bands, as the tiled form does.
 template <int D, bool PACKED, bool BILAT>
 __global__ __launch_bounds__(kBlock) void k_splat_direct(PixSrc src, int n, int row_w, int tile_w, int vd,
                                                          LatConst lc, KeyPlan kp, ValScale vsc, TableArgs ta,
                                                          Counters *ctr, int count) {
     using K = KeyOps<D, PACKED>;
     constexpr int W = K::W;
-    const int p = pixel_index(blockIdx.x, threadIdx.x, n, row_w, tile_w);
+    const int p = pixel_index(xcd_tile(blockIdx.x, gridDim.x), threadIdx.x, n, row_w, tile_w);
     unsigned flags = 0;
     uint32_t probes = 0, inserts = 0, atomics = 0;
     const bool full = *(volatile const unsigned *)&ctr->flags & kFlagFull;
@@ -942,7 +943,8 @@ __global__ __launch_bounds__(kBlock) void k_slice_tile(PixSrc src0, int n0, int 
     if (count) report_counts(ctr, probes, 0, 0);
 }
 
-// Slice, direct form (D > 6 or vd > 4): per-pixel global lookups.
+// Slice, direct form (D > 6 or vd > 4): per-pixel global lookups, the pixels
+// in XCD bands.
 template <int D, bool PACKED, bool BILAT, typename T>
 __global__ __launch_bounds__(kBlock) void k_slice_direct(PixSrc src, int n, int vd, int vs, LatConst lc, KeyPlan kp,
                                                          TableArgs ta, const T *__restrict__ vals, OutDst dst,
@@ -950,7 +952,7 @@ __global__ __launch_bounds__(kBlock) void k_slice_direct(PixSrc src, int n, int 
     using K = KeyOps<D, PACKED>;
     constexpr int W = K::W;
     if (*(volatile const unsigned *)&ctr->flags & kFlagFull) return;   // redone with a larger table
-    const int p = blockIdx.x * kBlock + threadIdx.x;
+    const int p = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     uint32_t probes = 0;
     unsigned flags = 0;
     if (p < n) {

@@ -48,3 +48,53 @@ def test_parts_skip_volumes(gpu_ctx, oracle_lib):
     assert np.array_equal(outs[0], outs[1])
     o64 = oracle_lib.joint_bilateral(im, im, 4.0, 4.0, 1.0, 0.1, mode=oracle_lib.F64_LAT)
     assert rel_err(outs[0], o64) < FP32_REL_TOL
+
+
+def image_positions(W, H, nfeat, spread, seed):
+    """nfeat smooth-ish feature channels scaled by `spread`, then x/8 and y/8
+    (the row last): the layout whose table a Gauss transform splits."""
+    im = S.rect_image(W, H, nfeat, K=40, noise=0.05, seed=seed)
+    x = np.tile(np.arange(W, dtype=np.float32) / 8, (H, 1))
+    y = np.tile((np.arange(H, dtype=np.float32) / 8)[:, None], (1, W))
+    return np.concatenate([im.reshape(nfeat, -1) * spread, x.reshape(1, -1), y.reshape(1, -1)]).astype(np.float32)
+
+
+@pytest.mark.parametrize("nfeat,spread,oracle", [(3, 4.0, True), (10, 2.0, False)])
+def test_parts_gauss_transform(gpu_ctx, oracle_lib, nfeat, spread, oracle):
+    """A Gauss transform over a 1024x1024 image (row_w given), rows in its
+    last position channel: split table (d = 5 one-word keys, d = 12 wide
+    keys), plain table and sorted ids give the same output bit for bit."""
+    d = nfeat + 2
+    print(f"\n[case] table parts, Gauss transform 1024x1024 d={d}", flush=True)
+    W = H = 1024
+    pos = image_positions(W, H, nfeat, spread, seed=d)
+    rng = np.random.default_rng(d)
+    val = np.concatenate([rng.uniform(size=(3, W * H)), np.ones((1, W * H))]).astype(np.float32)
+    inv = np.ones(d, np.float32)
+    outs = [gpu_ctx.gauss_transform(dev(pos), dev(val), inv, row_w=W, tune=t).cpu().numpy()
+            for t in ({}, {"parts": -1}, {"neighbours": 1, "parts": -1})]
+    for o in outs[1:]:
+        assert np.array_equal(o.view(np.uint32), outs[0].view(np.uint32))
+    if oracle:
+        o64, _ = oracle_lib.gauss_transform(pos, val, inv, mode=oracle_lib.F64_LAT)
+        assert rel_err(outs[0], o64) < FP32_REL_TOL
+
+
+def test_parts_skewed_last_channel(gpu_ctx):
+    """A last position channel that is one value but for a single far pixel:
+    every key of a dense lattice lands in the first part, the split table
+    fills up, and the call is redone on the plain table; same output as
+    without parts."""
+    print("\n[case] table parts, skewed last channel (redo without parts)", flush=True)
+    W = H = 1024
+    pos = image_positions(W, H, 3, 40.0, seed=11)
+    pos[-1] = 0.0
+    pos[-1, 0] = 4000.0
+    rng = np.random.default_rng(3)
+    val = np.concatenate([rng.uniform(size=(2, W * H)), np.ones((1, W * H))]).astype(np.float32)
+    inv = np.ones(5, np.float32)
+    a, st = gpu_ctx.gauss_transform(dev(pos), dev(val), inv, row_w=W, stats=True)
+    b = gpu_ctx.gauss_transform(dev(pos), dev(val), inv, row_w=W, tune={"parts": -1})
+    print("retries", st["retries"], "M", st["M"])
+    assert np.array_equal(a.cpu().numpy().view(np.uint32), b.cpu().numpy().view(np.uint32))
+    assert st["retries"] >= 1
