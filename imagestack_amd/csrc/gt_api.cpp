@@ -476,7 +476,7 @@ int run(gt_ctx *c, const gt::PixSrc &src, int n, int row_w, int d, int vd, int s
     if (!(c->plan_valid && c->plan_d == d && c->plan_vd == vd && c->plan_kind == kind &&
           (images <= 1 || (images - 1) >> c->plan.img_bits == 0) && (!gauss_rows || c->plan_n == n)))
         GT_TRY(replan());
-    bool parts_failed = false;   // a split table filled up: redo with the plain one
+    bool parts_failed = false;   // a Gauss transform's split table filled up: redo on the plain one
 
     int retries = 0, passes = 0;
     for (;;) {
@@ -623,7 +623,9 @@ int run(gt_ctx *c, const gt::PixSrc &src, int n, int row_w, int d, int vd, int s
             // a full table held about `cap` vertices: decide the redo's
             // neighbour form on that
             est = std::max<uint64_t>(est, cap);
-            if (L.ta.part_mul) parts_failed = true;
+            // (a bilateral form's rows hold equal pixel counts, so its parts
+            // stay; a Gauss transform's last channel may be skewed)
+            if (L.ta.part_mul && !bilat) parts_failed = true;
             // M <= (d + 1) n, and a first table holds n / load_factor slots: x4
             // covers every lattice up to d = 7 at the default load in one redo
             // (a larger table only lengthens the redo's clear and compaction)
