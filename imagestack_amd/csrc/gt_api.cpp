@@ -91,6 +91,7 @@ struct gt_ctx {
     // same filter comes with a new image size
     int last_n = -1, last_d = -1;
     unsigned last_M = 0;
+    size_t acc_zero = 0;   // leading bytes of tacc known to hold zeros
     struct Density {
         uint64_t key = 0;
         float per_pixel = 0.f;
@@ -285,6 +286,7 @@ void sig_add(std::string &k, const T &v) { k.append((const char *)&v, sizeof(T))
 struct Launch {
     int d, vd, storage, W, pw, m_max, vs, n, row_w, tile_w, wpc;
     bool packed, bilat, dedup, count, hashnbr;
+    bool clear_sums;   // clear the fixed-point sums too (not known to be zero)
     uint32_t cap;
     gt::LatConst lc;
     gt::KeyPlan kp;
@@ -303,7 +305,8 @@ int enqueue(gt_ctx *c, const Launch &L, const gt::PixSrc &src, const SliceSpec &
     hipStream_t s = c->stream;
     gt::Counters *ctr = (gt::Counters *)c->ctr.p;
     const gt::TableArgs &ta = L.ta;
-    GT_HIP(gt::launch_clear(ta.tkeys, (size_t)L.cap * L.W, ta.tacc, (size_t)L.cap * L.vd, ctr, s));
+    GT_HIP(gt::launch_clear(ta.tkeys, (size_t)L.cap * L.W, ta.tacc, L.clear_sums ? (size_t)L.cap * L.vd : 0, ctr,
+                            s));
     if (c->profiling) GT_HIP(mark(c, 0, s));
     if (batch)
         GT_HIP(gt::launch_splat_batch(L.d, batch->tmpl, batch->ba, batch->ntiles, L.tile_w, L.vd, L.lc, L.kp, L.vsc,
@@ -566,7 +569,15 @@ int run(gt_ctx *c, const gt::PixSrc &src, int n, int row_w, int d, int vd, int s
         L.vs = value_stride(vd);
         const unsigned ncount = (cap + 2047) / 2048;
         GT_TRY(ensure(c->tkeys, (size_t)cap * L.W * 8));
+        const void *old_acc = c->tacc.p;
         GT_TRY(ensure(c->tacc, (size_t)cap * vd * 8));
+        if (c->tacc.p != old_acc) c->acc_zero = 0;
+        // The sums need clearing unless a complete earlier call left this
+        // much of the array zero; until this call completes, none is known.
+        const size_t acc_bytes = (size_t)cap * vd * 8;
+        const size_t acc_zero = c->acc_zero;
+        L.clear_sums = acc_zero < acc_bytes;
+        c->acc_zero = 0;
         GT_TRY(ensure(c->ctr, sizeof(gt::Counters)));
         GT_TRY(ensure(c->counts, (size_t)ncount * 4));
         GT_TRY(ensure(c->slot_list, (size_t)L.m_max * 4));
@@ -633,6 +644,7 @@ int run(gt_ctx *c, const gt::PixSrc &src, int n, int row_w, int d, int vd, int s
             continue;
         }
         const int M = (int)hc.M;
+        if (hc.flags == 0) c->acc_zero = std::max(acc_zero, acc_bytes);   // the finalize zeroed what the splat summed
         c->last_n = n;
         c->last_d = d;
         c->last_M = (unsigned)M;

@@ -477,8 +477,11 @@ __global__ __launch_bounds__(kBlock) void k_pack(TableArgs ta, const uint32_t *_
 
 // finalize: position i of the sorted (or, without a sort, the compaction)
 // order -> vertex id i.  Records slot -> id (in slot_id, or folded into the
-// table word above the key when ta.idshift > 0), copies the key, and converts the fixed-point splat sums to the storage type (one
-// rounding: int64 -> double is exact below 2^53, then one FP32 rounding).
+// table word above the key when ta.idshift > 0), copies the key, and
+// converts the fixed-point splat sums to the storage type (one rounding:
+// int64 -> double is exact below 2^53, then one FP32 rounding).  The sums it
+// has read are set back to zero, so after a complete call the whole sum
+// array is zero again and the next call need not clear it.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_finalize(TableArgs ta, ValScale vsc, int kw,
                                                      const uint32_t *__restrict__ sorted,
@@ -502,6 +505,7 @@ __global__ __launch_bounds__(kBlock) void k_finalize(TableArgs ta, ValScale vsc,
 #pragma unroll
     for (int c = 0; c < kMaxVD; c++)
         o[c] = (c < vd) ? (float)((double)(long long)ta.tacc[(size_t)s * vd + c] * (double)vsc.inv_s[c]) : 0.0f;
+    for (int c = 0; c < vd; c++) ta.tacc[(size_t)s * vd + c] = 0ull;
     store_row_rt<T>(vals, i, vs, o);
     if (splat_copy)
         for (int c = 0; c < vd; c++) splat_copy[(size_t)i * vd + c] = o[c];
