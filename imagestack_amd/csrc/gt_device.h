@@ -262,6 +262,14 @@ __device__ __forceinline__ int table_find(const uint64_t *__restrict__ tkeys, ui
     return -1;
 }
 
+// Home slot in a partitioned table of a key with hash h whose fields sum to
+// `fields` (part_base offsets that sum to c_0 + ... + c_{D-1} + margin).
+__device__ __forceinline__ uint32_t slot_in_part(const TableArgs &ta, uint32_t h, int fields) {
+    const uint32_t u = (uint32_t)min(max(fields + ta.part_base, 0), ta.part_range);
+    const uint32_t part = (uint32_t)(((uint64_t)u * ta.part_mul) >> 24);
+    return (part << ta.part_shift) | (h & (ta.mask >> ta.part_lg));
+}
+
 // Home slot of key k (where its linear probe starts): the hash, or in a
 // partitioned table (TableArgs::part_mul) the hash inside the key's part.
 template <int D, bool PACKED>
@@ -270,14 +278,12 @@ __device__ __forceinline__ uint32_t home_slot(const TableArgs &ta, const KeyPlan
     constexpr int W = KeyOps<D, PACKED>::W;
     const uint32_t h = (uint32_t)hash_key<W>(k);
     if (ta.part_mul) {
-        int sum = ta.part_base;   // offsets the fields' sum to c_0 + ... + c_{D-1} + margin
+        int sum = 0;
 #pragma unroll
         for (int i = 0; i < D; i++)
             sum += PACKED ? (int)((k[0] >> kp.shift[i]) & kp.span[i])
                           : (int)((k[i / 4] >> (48 - 16 * (i % 4))) & 0xffff);
-        const uint32_t u = (uint32_t)min(max(sum, 0), ta.part_range);
-        const uint32_t part = (uint32_t)(((uint64_t)u * ta.part_mul) >> 24);
-        return (part << ta.part_shift) | (h & (ta.mask >> ta.part_lg));
+        return slot_in_part(ta, h, sum);
     }
     return h & ta.mask;
 }

@@ -791,6 +791,118 @@ __global__ __launch_bounds__(kBlock) void k_nbr_hash(const uint64_t *__restrict_
     }
 }
 
+// The same table, one thread per vertex for all D+1 axes (used from D = 7,
+// where the per-axis form spends most of its time rebuilding the key): the
+// fields, their range checks and their sum are taken once, and each target
+// is the key plus one constant word per sign -- +1 on every field, then
+// -(D+1) (sign 0) or +(D+1) (sign 1) on field j -- as integer arithmetic,
+// exact whenever no field leaves its range (the checks).  The fields of a
+// target sum to the vertex's sum -1 / +1 (j < D) or +D / -D (j = D).
+template <int D, bool PACKED>
+__global__ __launch_bounds__(kBlock) void k_nbr_hash_vertex(const uint64_t *__restrict__ vk, KeyPlan kp,
+                                                            const Counters *ctr, int stride, TableArgs ta, PartMap pm,
+                                                            int *__restrict__ nbr) {
+    constexpr int W = KeyOps<D, PACKED>::W;
+    const unsigned M = ctr->M;
+    unsigned i, end, step;
+    part_walk(pm, M, blockIdx.x, gridDim.x, i, end, step);
+    const bool fold = W == 1 && ta.idshift > 0;
+    const uint64_t kmask = fold ? ta.keymask : ~0ull;
+    for (; i < end; i += step) {
+        uint64_t k[W], one[W];
+#pragma unroll
+        for (int w = 0; w < W; w++) {
+            k[w] = vk[(size_t)i * W + w];
+            one[w] = 0;
+        }
+        uint32_t no_up1 = 0, no_dn1 = 0, ok_upD = 0, ok_dnD = 0;   // bit q: field q
+        int sum = 0;
+#pragma unroll
+        for (int q = 0; q < D; q++) {
+            const int wq = PACKED ? 0 : q / 4;
+            const int sh = PACKED ? kp.shift[q] : 48 - 16 * (q % 4);
+            const uint32_t f = (uint32_t)((k[wq] >> sh) & (PACKED ? (uint64_t)kp.span[q] : 0xffffull));
+            const uint32_t lo = PACKED ? 0u : (uint32_t)(1 + 2 * (D + 1));
+            const uint32_t hi = PACKED ? kp.span[q] : (uint32_t)(65535 - 2 * (D + 1));
+            one[wq] += 1ull << sh;
+            sum += (int)f;
+            if (f + 1 > hi) no_up1 |= 1u << q;
+            if (f < lo + 1) no_dn1 |= 1u << q;
+            if (f + D <= hi) ok_upD |= 1u << q;
+            if (f >= lo + D) ok_dnD |= 1u << q;
+        }
+        for (int j = 0; j <= D; j++) {
+            uint64_t t[2][W];
+            bool ok[2];
+            int tsum[2];
+#pragma unroll
+            for (int w = 0; w < W; w++) {
+                t[0][w] = k[w] + one[w];
+                t[1][w] = k[w] - one[w];
+            }
+            if (j < D) {
+                const uint32_t bit = 1u << j;
+                ok[0] = (no_up1 & ~bit) == 0 && (ok_dnD & bit) != 0;
+                ok[1] = (no_dn1 & ~bit) == 0 && (ok_upD & bit) != 0;
+                const int wj = PACKED ? 0 : j / 4;
+                const int shj = PACKED ? kp.shift[j] : 48 - 16 * (j % 4);
+                const uint64_t big = (uint64_t)(D + 1) << shj;
+#pragma unroll
+                for (int w = 0; w < W; w++)
+                    if (w == wj) {
+                        t[0][w] -= big;
+                        t[1][w] += big;
+                    }
+                tsum[0] = sum - 1;
+                tsum[1] = sum + 1;
+            } else {
+                ok[0] = no_up1 == 0;
+                ok[1] = no_dn1 == 0;
+                tsum[0] = sum + D;
+                tsum[1] = sum - D;
+            }
+            uint32_t s[2];
+            bool done[2];
+            int slot[2] = {-1, -1};
+            uint64_t hit[2] = {0, 0};
+#pragma unroll
+            for (int sgn = 0; sgn < 2; sgn++) {
+                const uint32_t h = (uint32_t)hash_key<W>(t[sgn]);
+                s[sgn] = ta.part_mul ? slot_in_part(ta, h, tsum[sgn]) : (h & ta.mask);
+                done[sgn] = !ok[sgn];
+            }
+            for (uint32_t it = 0; it < kMaxProbes && !(done[0] && done[1]); it++) {
+                uint64_t w0[2];
+#pragma unroll
+                for (int sgn = 0; sgn < 2; sgn++) w0[sgn] = done[sgn] ? kEmpty : ta.tkeys[(size_t)s[sgn] * W];
+#pragma unroll
+                for (int sgn = 0; sgn < 2; sgn++) {
+                    if (done[sgn]) continue;
+                    if (w0[sgn] == kEmpty) {
+                        done[sgn] = true;
+                        continue;
+                    }
+                    bool match = (w0[sgn] & kmask) == t[sgn][0];
+#pragma unroll
+                    for (int w = 1; w < W; w++) match = match && ta.tkeys[(size_t)s[sgn] * W + w] == t[sgn][w];
+                    if (match) {
+                        slot[sgn] = (int)s[sgn];
+                        hit[sgn] = w0[sgn];
+                        done[sgn] = true;
+                    } else {
+                        s[sgn] = (s[sgn] + 1) & ta.mask;
+                    }
+                }
+            }
+            int res[2];
+#pragma unroll
+            for (int sgn = 0; sgn < 2; sgn++)
+                res[sgn] = slot[sgn] < 0 ? -1 : fold ? (int)(hit[sgn] >> ta.idshift) : ta.slot_id[slot[sgn]];
+            *((int2 *)nbr + (size_t)j * stride + i) = make_int2(res[0], res[1]);
+        }
+    }
+}
+
 // One blur pass along one axis (src/Permutohedral.h:442-475): Jacobi, new =
 // 0.25*a + 0.5*self + 0.25*b evaluated left to right, missing neighbour = 0.
 template <typename T, int VS>

@@ -333,6 +333,23 @@ static unsigned vertex_blocks(int m_max, const PartMap &pm) {
 hipError_t launch_neighbors_hash(int d, bool packed, const KeyPlan &kp, const uint64_t *vkeys, const Counters *ctr,
                                  int m_max, const TableArgs &ta, const PartMap &pm, int2 *nbr, hipStream_t st) {
     if (m_max <= 0) return hipSuccess;
+    if (d >= 7) {   // one thread per vertex
+        const dim3 g1(vertex_blocks(m_max, pm)), b1(kBlock);
+#define GT_NBRV(D)                                                                                                  \
+    {                                                                                                               \
+        if (packed) hipLaunchKernelGGL((k_nbr_hash_vertex<D, true>), g1, b1, 0, st, vkeys, kp, ctr, m_max, ta, pm,  \
+                                       (int *)nbr);                                                                 \
+        else hipLaunchKernelGGL((k_nbr_hash_vertex<D, false>), g1, b1, 0, st, vkeys, kp, ctr, m_max, ta, pm,        \
+                                (int *)nbr);                                                                        \
+    } break;
+        switch (d) {
+        case 7: GT_NBRV(7) case 8: GT_NBRV(8) case 9: GT_NBRV(9) case 10: GT_NBRV(10) case 11: GT_NBRV(11)
+        case 12: GT_NBRV(12) case 13: GT_NBRV(13) case 14: GT_NBRV(14) case 15: GT_NBRV(15) case 16: GT_NBRV(16)
+        default: return hipErrorInvalidValue;
+        }
+#undef GT_NBRV
+        return hipGetLastError();
+    }
     const dim3 g(vertex_blocks(m_max, pm), d + 1), b(kBlock);
 #define GT_NBRH(D)                                                                                                  \
     {                                                                                                               \
