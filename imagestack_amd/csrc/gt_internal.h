@@ -153,8 +153,11 @@ hipError_t launch_neighbors(int d, bool packed, const KeyPlan &kp, const uint64_
 // The same table by hash lookups in the complete global table, for vertex ids
 // in compaction order (no sort).
 // With pm.cpp > 0, workgroups on XCD x take the vertices of table part x.
+// per_vertex: one thread per vertex for all axes (k_nbr_hash_vertex), else
+// one per (vertex, axis) (k_nbr_hash, more threads for small lattices).
 hipError_t launch_neighbors_hash(int d, bool packed, const KeyPlan &kp, const uint64_t *vkeys, const Counters *ctr,
-                                 int m_max, const TableArgs &ta, const PartMap &pm, int2 *nbr, hipStream_t st);
+                                 int m_max, const TableArgs &ta, const PartMap &pm, bool per_vertex, int2 *nbr,
+                                 hipStream_t st);
 hipError_t launch_blur(int storage, int vs, const int2 *nbr, const Counters *ctr, int m_max, const PartMap &pm,
                        const void *in, void *out, hipStream_t st);
 hipError_t launch_slice(int d, bool packed, bool bilat, bool tiled, int storage, const PixSrc &src, int n, int row_w,

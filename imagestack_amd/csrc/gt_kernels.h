@@ -791,8 +791,9 @@ __global__ __launch_bounds__(kBlock) void k_nbr_hash(const uint64_t *__restrict_
     }
 }
 
-// The same table, one thread per vertex for all D+1 axes (used from D = 7,
-// where the per-axis form spends most of its time rebuilding the key): the
+// The same table, one thread per vertex for all D+1 axes (used from D = 7
+// and for large lattices, where the per-axis form spends most of its time
+// rebuilding the key): the
 // fields, their range checks and their sum are taken once, and each target
 // is the key plus one constant word per sign -- +1 on every field, then
 // -(D+1) (sign 0) or +(D+1) (sign 1) on field j -- as integer arithmetic,

@@ -331,9 +331,10 @@ static unsigned vertex_blocks(int m_max, const PartMap &pm) {
 }
 
 hipError_t launch_neighbors_hash(int d, bool packed, const KeyPlan &kp, const uint64_t *vkeys, const Counters *ctr,
-                                 int m_max, const TableArgs &ta, const PartMap &pm, int2 *nbr, hipStream_t st) {
+                                 int m_max, const TableArgs &ta, const PartMap &pm, bool per_vertex, int2 *nbr,
+                                 hipStream_t st) {
     if (m_max <= 0) return hipSuccess;
-    if (d >= 7) {   // one thread per vertex
+    if (per_vertex) {
         const dim3 g1(vertex_blocks(m_max, pm)), b1(kBlock);
 #define GT_NBRV(D)                                                                                                  \
     {                                                                                                               \
@@ -343,8 +344,10 @@ hipError_t launch_neighbors_hash(int d, bool packed, const KeyPlan &kp, const ui
                                 (int *)nbr);                                                                        \
     } break;
         switch (d) {
-        case 7: GT_NBRV(7) case 8: GT_NBRV(8) case 9: GT_NBRV(9) case 10: GT_NBRV(10) case 11: GT_NBRV(11)
-        case 12: GT_NBRV(12) case 13: GT_NBRV(13) case 14: GT_NBRV(14) case 15: GT_NBRV(15) case 16: GT_NBRV(16)
+        case 1: GT_NBRV(1) case 2: GT_NBRV(2) case 3: GT_NBRV(3) case 4: GT_NBRV(4) case 5: GT_NBRV(5)
+        case 6: GT_NBRV(6) case 7: GT_NBRV(7) case 8: GT_NBRV(8) case 9: GT_NBRV(9) case 10: GT_NBRV(10)
+        case 11: GT_NBRV(11) case 12: GT_NBRV(12) case 13: GT_NBRV(13) case 14: GT_NBRV(14) case 15: GT_NBRV(15)
+        case 16: GT_NBRV(16)
         default: return hipErrorInvalidValue;
         }
 #undef GT_NBRV
