@@ -316,12 +316,13 @@ int enqueue(gt_ctx *c, const Launch &L, const gt::PixSrc &src, const SliceSpec &
         GT_HIP(gt::launch_splat(L.d, L.packed, L.bilat, L.dedup, src, L.n, L.row_w, L.tile_w, L.vd, L.lc, L.kp, L.vsc,
                                 ta, ctr, L.count, L.wpc, s));
     if (c->profiling) GT_HIP(mark(c, 1, s));
-    GT_HIP(gt::launch_compact(L.W, ta, L.cap, (uint32_t *)c->counts.p, (uint32_t *)c->slot_list.p,
-                              (uint64_t *)c->pk_a.p, L.packed, ctr, s));
-    const uint32_t *sorted = (const uint32_t *)c->slot_list.p;   // compaction order unless sorted below
-    const uint64_t *sorted_keys = nullptr;
     *passes = 0;
-    if (!L.hashnbr) {
+    if (L.hashnbr) {   // ids in compaction order: the finalize rides along with the compaction
+        GT_HIP(gt::launch_compact_final(L.W, L.storage, ta, L.cap, (uint32_t *)c->counts.p, L.vsc, L.vd, L.vs,
+                                        (uint64_t *)c->vkeys.p, c->vals_a.p, ctr, s));
+    } else {
+        GT_HIP(gt::launch_compact(L.W, ta, L.cap, (uint32_t *)c->counts.p, (uint32_t *)c->slot_list.p,
+                                  (uint64_t *)c->pk_a.p, L.packed, ctr, s));
         if (!L.packed)
             GT_HIP(gt::launch_pack(L.d, ta, (const uint32_t *)c->slot_list.p, ctr, L.m_max, L.kp, L.pw,
                                    (uint64_t *)c->pk_a.p, s));
@@ -330,11 +331,11 @@ int enqueue(gt_ctx *c, const Launch &L, const gt::PixSrc &src, const SliceSpec &
                         (uint32_t *)c->sort_hist.p, kLookbackTiles};
         bool in_b = false;
         GT_HIP(gt::radix_sort(sb, ctr, L.m_max, L.pw, L.kp.bits, &in_b, passes, s));
-        sorted = in_b ? sb.pay_b : sb.pay_a;
-        sorted_keys = L.packed ? (in_b ? sb.pk_b : sb.pk_a) : nullptr;
+        const uint32_t *sorted = in_b ? sb.pay_b : sb.pay_a;
+        const uint64_t *sorted_keys = L.packed ? (in_b ? sb.pk_b : sb.pk_a) : nullptr;
+        GT_HIP(gt::launch_finalize(L.W, L.storage, ta, L.vsc, sorted, sorted_keys, ctr, L.m_max, L.vd, L.vs,
+                                   (uint64_t *)c->vkeys.p, c->vals_a.p, L.splat_copy, s));
     }
-    GT_HIP(gt::launch_finalize(L.W, L.storage, ta, L.vsc, sorted, sorted_keys, ctr, L.m_max, L.vd, L.vs,
-                               (uint64_t *)c->vkeys.p, c->vals_a.p, L.splat_copy, s));
     if (c->profiling) GT_HIP(mark(c, 2, s));
     int2 *nbr = (int2 *)c->nbr.p;
     if (L.hashnbr)

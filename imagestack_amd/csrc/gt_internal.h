@@ -150,6 +150,11 @@ hipError_t launch_finalize(int words, int storage, const TableArgs &ta, const Va
                            uint64_t *vkeys, void *vals, float *splat_copy, hipStream_t st);
 hipError_t launch_neighbors(int d, bool packed, const KeyPlan &kp, const uint64_t *vkeys, const Counters *ctr,
                             int m_max, int2 *nbr, int wpc, hipStream_t st);
+// Compaction with the finalize fused in, for ids in compaction order (no
+// sort, no slot list): count, scan, then k_write_final.
+hipError_t launch_compact_final(int words, int storage, const TableArgs &ta, uint32_t capacity, uint32_t *counts,
+                                const ValScale &vsc, int vd, int vs, uint64_t *vkeys, void *vals, Counters *ctr,
+                                hipStream_t st);
 // The same table by hash lookups in the complete global table, for vertex ids
 // in compaction order (no sort).
 // With pm.cpp > 0, workgroups on XCD x take the vertices of table part x.

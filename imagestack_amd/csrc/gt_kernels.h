@@ -438,6 +438,56 @@ __global__ __launch_bounds__(kBlock) void k_write_list(const uint64_t *__restric
     }
 }
 
+// Compaction and finalize in one pass, for ids in compaction order (no
+// sort): occupied slot s gets id pos, its rank among the occupied slots (as
+// k_write_list ranks them), and k_finalize's work for it is done here: the
+// key copied to vkeys[pos], the sums converted to vals[pos] by the same
+// expression and set back to zero, slot -> id recorded (in slot_id, or
+// folded into the key word).  After a full table (M = 0) nothing is written.
+template <int W, typename T>
+__global__ __launch_bounds__(kBlock) void k_write_final(TableArgs ta, uint32_t cap,
+                                                        const uint32_t *__restrict__ offsets, const Counters *ctr,
+                                                        ValScale vsc, int vd, int vs, uint64_t *__restrict__ vkeys,
+                                                        T *__restrict__ vals) {
+    constexpr int NW = kBlock / kWave;
+    __shared__ unsigned ws[NW];
+    if (*(volatile const unsigned *)&ctr->flags & kFlagFull) return;   // the same for every thread
+    const size_t base = (size_t)blockIdx.x * kCompactSlots;
+    unsigned run = offsets[blockIdx.x];
+    const int wave = threadIdx.x / kWave;
+    for (int k = 0; k < kCompactSlots / kBlock; k++) {
+        const size_t s = base + (size_t)k * kBlock + threadIdx.x;
+        uint64_t w0 = kEmpty;
+        if (s < cap) w0 = ta.tkeys[s * W];
+        const bool occ = w0 != kEmpty;
+        const uint64_t m = __ballot(occ);
+        if (__lane_id() == 0) ws[wave] = (unsigned)__popcll(m);
+        __syncthreads();
+        unsigned off = run, tot = 0;
+#pragma unroll
+        for (int w = 0; w < NW; w++) {
+            off += (w < wave) ? ws[w] : 0u;
+            tot += ws[w];
+        }
+        if (occ) {
+            const unsigned pos = off + (unsigned)__popcll(m & lanemask_lt());
+            vkeys[(size_t)pos * W] = w0;
+#pragma unroll
+            for (int w = 1; w < W; w++) vkeys[(size_t)pos * W + w] = ta.tkeys[s * W + w];
+            float o[kMaxVD];
+#pragma unroll
+            for (int c = 0; c < kMaxVD; c++)
+                o[c] = (c < vd) ? (float)((double)(long long)ta.tacc[s * vd + c] * (double)vsc.inv_s[c]) : 0.0f;
+            for (int c = 0; c < vd; c++) ta.tacc[s * vd + c] = 0ull;
+            store_row_rt<T>(vals, pos, vs, o);
+            if (W == 1 && ta.idshift > 0) ta.tkeys[s] = w0 | ((uint64_t)pos << ta.idshift);
+            else ta.slot_id[s] = (int)pos;
+        }
+        run += tot;
+        __syncthreads();
+    }
+}
+
 // Wide keys -> packed sort key (big integer of pw words, word 0 least
 // significant) by the plan's offsets; coordinates outside the plan flag a
 // replan.

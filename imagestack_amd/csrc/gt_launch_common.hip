@@ -307,6 +307,42 @@ hipError_t launch_finalize(int words, int storage, const TableArgs &ta, const Va
     return hipGetLastError();
 }
 
+template <typename T>
+static void launch_write_final_t(int words, const TableArgs &ta, uint32_t capacity, unsigned nblk,
+                                 const uint32_t *offsets, const Counters *ctr, const ValScale &vsc, int vd, int vs,
+                                 uint64_t *vkeys, void *vals, hipStream_t st) {
+    T *v = (T *)vals;
+#define GT_WF(W)                                                                                                 \
+    hipLaunchKernelGGL((k_write_final<W, T>), dim3(nblk), dim3(kBlock), 0, st, ta, capacity, offsets, ctr, vsc,  \
+                       vd, vs, vkeys, v)
+    switch (words) {
+    case 1: GT_WF(1); break;
+    case 2: GT_WF(2); break;
+    case 3: GT_WF(3); break;
+    default: GT_WF(4); break;
+    }
+#undef GT_WF
+}
+
+hipError_t launch_compact_final(int words, int storage, const TableArgs &ta, uint32_t capacity, uint32_t *counts,
+                                const ValScale &vsc, int vd, int vs, uint64_t *vkeys, void *vals, Counters *ctr,
+                                hipStream_t st) {
+    if (words < 1 || words > 4) return hipErrorInvalidValue;
+    const unsigned nblk = blocks_for(capacity, kCompactSlots);
+    switch (words) {
+    case 1: hipLaunchKernelGGL(k_count<1>, dim3(nblk), dim3(kBlock), 0, st, ta.tkeys, capacity, counts); break;
+    case 2: hipLaunchKernelGGL(k_count<2>, dim3(nblk), dim3(kBlock), 0, st, ta.tkeys, capacity, counts); break;
+    case 3: hipLaunchKernelGGL(k_count<3>, dim3(nblk), dim3(kBlock), 0, st, ta.tkeys, capacity, counts); break;
+    default: hipLaunchKernelGGL(k_count<4>, dim3(nblk), dim3(kBlock), 0, st, ta.tkeys, capacity, counts); break;
+    }
+    hipLaunchKernelGGL(k_scan_1wg, dim3(1), dim3(1024), 0, st, counts, (int)nblk, &ctr->M, &ctr->flags);
+    if (storage == 0) launch_write_final_t<float>(words, ta, capacity, nblk, counts, ctr, vsc, vd, vs, vkeys, vals, st);
+    else if (storage == 1)
+        launch_write_final_t<__hip_bfloat16>(words, ta, capacity, nblk, counts, ctr, vsc, vd, vs, vkeys, vals, st);
+    else launch_write_final_t<__half>(words, ta, capacity, nblk, counts, ctr, vsc, vd, vs, vkeys, vals, st);
+    return hipGetLastError();
+}
+
 // Neighbour table by merging with the sorted keys (no hash lookups).
 hipError_t launch_neighbors(int d, bool packed, const KeyPlan &kp, const uint64_t *vkeys, const Counters *ctr,
                             int m_max, int2 *nbr, int wpc, hipStream_t st) {
