@@ -564,8 +564,8 @@ int run(gt_ctx *c, const gt::PixSrc &src, int n, int row_w, int d, int vd, int s
         // keys: d = 16, 17.8 M vertices, 37.4 -> 31.3 ms); in between, sorted
         // ids keep the blur's gathers local.  A kept lattice
         // (gt_build_lattice) is always sorted.
-        L.hashnbr = !keep_splat && !batch &&
-                    (tu.nbr == 2 || (tu.nbr == 0 && (L.ta.part_mul != 0 || d >= 11 || est <= (1u << 20))));
+        L.hashnbr = !keep_splat && (tu.nbr == 2 || (tu.nbr == 0 && !batch && (L.ta.part_mul != 0 || d >= 11 ||
+                                                                                est <= (1u << 20))));
         // One thread per vertex shares the key work of its d + 1 axes, one per
         // (vertex, axis) has d + 1 times the threads: the latter for small,
         // low-d lattices (512^2 bilateral, 0.17 M vertices: 29 against

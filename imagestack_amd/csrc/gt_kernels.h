@@ -146,9 +146,10 @@ __device__ __forceinline__ void part_walk(const PartMap &pm, unsigned M, unsigne
         step = g * kBlock;
         return;
     }
+    // (after a full table M is 0 while the chunk offsets are not: clamp)
     const unsigned x = b & 7u;
     i = pm.off[x * pm.cpp] + (b >> 3) * kBlock + threadIdx.x;
-    end = x == 7 ? M : pm.off[(x + 1) * pm.cpp];
+    end = min(x == 7 ? M : pm.off[(x + 1) * pm.cpp], M);
     step = (g >> 3) * kBlock;
 }
 

@@ -84,7 +84,8 @@ def test_parts_skewed_last_channel(gpu_ctx):
     """A last position channel that is one value but for a single far pixel:
     every key of a dense lattice lands in the first part, the split table
     fills up, and the call is redone on the plain table; same output as
-    without parts."""
+    without parts.  The failed attempt's neighbour and blur kernels walk the
+    parts' id ranges, which must come out empty (M = 0) after a full table."""
     print("\n[case] table parts, skewed last channel (redo without parts)", flush=True)
     W = H = 1024
     pos = image_positions(W, H, 3, 40.0, seed=11)
