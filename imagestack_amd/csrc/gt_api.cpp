@@ -569,8 +569,9 @@ int run(gt_ctx *c, const gt::PixSrc &src, int n, int row_w, int d, int vd, int s
         // One thread per vertex shares the key work of its d + 1 axes, one per
         // (vertex, axis) has d + 1 times the threads: the latter for small,
         // low-d lattices (512^2 bilateral, 0.17 M vertices: 29 against
-        // 35 us; 4096^2 RGB at sigma_s 4, 28.9 M: 5.6 against 4.3 ms).
-        L.nbr_per_vertex = d >= 7 || est >= (1u << 19);
+        // 35 us; 2048^2 joint bilateral, 0.41 M: 57 against 55 us; 4096^2
+        // RGB at sigma_s 4, 28.9 M: 5.6 against 4.3 ms).
+        L.nbr_per_vertex = d >= 7 || est >= (1u << 18);
         L.m_max = (int)std::min<uint64_t>(cap, (uint64_t)n * (d + 1));
         L.pw = L.packed ? 1 : std::max(1, (L.kp.bits + 63) / 64);
         L.vs = value_stride(vd);
