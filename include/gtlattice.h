@@ -75,7 +75,9 @@ typedef struct {
                             sorts (the exported lattice is in key order).          */
     int parts;           /* hash table in row bands, 8 or more, an eighth of them per
                             XCD's L2: 0 auto (one-frame bilateral forms with at
-                            least 2^14 slots), -1 off.  The output is the same.    */
+                            least 2^14 slots; Gauss transforms given row_w, of at
+                            least 2^20 positions, split by their last position
+                            channel), -1 off.  The output is the same.             */
 } gt_tuning;
 
 /* Per-call statistics (all optional). */
