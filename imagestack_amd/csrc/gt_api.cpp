@@ -112,7 +112,11 @@ struct gt_ctx {
     int gpasses = 0;
     void *gvals = nullptr;
     // batch descriptors
-    DevBuf batch_desc;
+    DevBuf batch_desc, tag_starts;
+    // vertices of each image of the last batch whose lattice was sorted,
+    // for the batch with these sizes and filters (the part slices of the next)
+    uint64_t batch_sig = 0;
+    std::vector<uint32_t> batch_verts;
     gt::BatchImage *h_batch = nullptr;   // pinned
     int h_batch_cap = 0;
     // lattice kept for export
@@ -272,6 +276,8 @@ struct BatchSpec {
     int Cr = 0, C = 0;
     gt::PixSrc tmpl{};
     gt::OutDst tmpl_dst{};
+    uint64_t sig = 0;      // the images' sizes and filters
+    bool parts = false;    // the descriptors hold part slices (vertex counts known)
 };
 
 template <class T>
@@ -554,6 +560,22 @@ int run(gt_ctx *c, const gt::PixSrc &src, int n, int row_w, int d, int vd, int s
                 L.ta.part_shift = lg - plg;
             }
         }
+        // A batch whose images' vertex counts are known: each image takes a
+        // run of parts in proportion to its vertices, split by its rows (the
+        // descriptors hold the slices; the key's tag picks the image).
+        if (tu.parts && !parts_failed && batch && batch->parts && L.packed && cap >= (1u << 14)) {
+            double sum_lo = 0;
+            for (int i = 0; i < d; i++) sum_lo += L.kp.lo[i];
+            int lg = 0;
+            while ((1u << lg) < cap) lg++;
+            int plg = 3;
+            while (plg < lg - 8 && ((uint64_t)cap * L.W * 8 >> (plg + 1)) >= (1u << 21)) plg++;
+            L.ta.part_img = batch->ba.img;
+            L.ta.part_base = (int)sum_lo + 2 * (d + 1);
+            L.ta.part_mul = 1;   // on; the per-image descriptors carry the scales
+            L.ta.part_lg = plg;
+            L.ta.part_shift = lg - plg;
+        }
         // Vertex ids: sorted (merge neighbours) or compaction order (hash
         // neighbours, no sort); the output is the same bit for bit.  With
         // table parts the hash form wins at every size measured (2048^2
@@ -564,8 +586,8 @@ int run(gt_ctx *c, const gt::PixSrc &src, int n, int row_w, int d, int vd, int s
         // keys: d = 16, 17.8 M vertices, 37.4 -> 31.3 ms); in between, sorted
         // ids keep the blur's gathers local.  A kept lattice
         // (gt_build_lattice) is always sorted.
-        L.hashnbr = !keep_splat && (tu.nbr == 2 || (tu.nbr == 0 && !batch && (L.ta.part_mul != 0 || d >= 11 ||
-                                                                                est <= (1u << 20))));
+        L.hashnbr = !keep_splat && (tu.nbr == 2 || (tu.nbr == 0 && (L.ta.part_mul != 0 ||
+                                                                   (!batch && (d >= 11 || est <= (1u << 20))))));
         // One thread per vertex shares the key work of its d + 1 axes, one per
         // (vertex, axis) has d + 1 times the threads: the latter for small,
         // low-d lattices (512^2 bilateral, 0.17 M vertices: 29 against
@@ -643,8 +665,9 @@ int run(gt_ctx *c, const gt::PixSrc &src, int n, int row_w, int d, int vd, int s
             // neighbour form on that
             est = std::max<uint64_t>(est, cap);
             // (a bilateral form's rows hold equal pixel counts, so its parts
-            // stay; a Gauss transform's last channel may be skewed)
-            if (L.ta.part_mul && !bilat) parts_failed = true;
+            // stay; a Gauss transform's last channel may be skewed, and a
+            // batch's images may hold other vertex counts than measured)
+            if (L.ta.part_mul && (!bilat || batch)) parts_failed = true;
             // M <= (d + 1) n, and a first table holds n / load_factor slots: x4
             // covers every lattice up to d = 7 at the default load in one redo
             // (a larger table only lengthens the redo's clear and compaction)
@@ -666,6 +689,19 @@ int run(gt_ctx *c, const gt::PixSrc &src, int n, int row_w, int d, int vd, int s
             }
             slot->key = dkey;
             slot->per_pixel = (float)M / (float)n;
+        }
+        if (batch && batch->sig && !L.hashnbr && M > 0 && L.packed && c->batch_sig != batch->sig) {
+            // sorted ids: the images' vertices are consecutive runs, by tag
+            const int nb = batch->ba.count;
+            GT_TRY(ensure(c->tag_starts, (size_t)(nb + 1) * 4));
+            GT_HIP(gt::launch_tag_starts((const uint64_t *)c->vkeys.p, M, L.kp.coord_bits, nb,
+                                         (uint32_t *)c->tag_starts.p, s));
+            std::vector<uint32_t> starts(nb + 1);
+            GT_HIP(hipMemcpyAsync(starts.data(), c->tag_starts.p, (size_t)(nb + 1) * 4, hipMemcpyDeviceToHost, s));
+            GT_HIP(hipStreamSynchronize(s));
+            c->batch_verts.assign(nb, 0);
+            for (int i = 0; i < nb; i++) c->batch_verts[i] = starts[i + 1] - starts[i];
+            c->batch_sig = batch->sig;
         }
         c->have = !L.hashnbr;   // a lattice in compaction order is not exported
         c->M = M;
@@ -1264,6 +1300,46 @@ int gt_joint_bilateral_batch(gt_ctx *c, int count, const gt_bilateral_image *ima
     }
     if (n >= (1ll << 31) || ntiles >= (1ll << 31)) return fail(GT_EINVAL, "batch of %lld pixels exceeds 2^31", (long long)n);
     if (!imgs.empty()) {
+        // The images' signature; when an earlier sorted call measured their
+        // vertex counts, image i takes the vertex-share slice of [0, 2^24)
+        // starting at its vertex offset, spread over its rows by the sum of
+        // its keys' fields, -c_d = d sf y / sigma_s to within 2(d + 1).
+        uint64_t sig = 1469598103934665603ull;
+        auto mix = [&sig](const void *p, size_t len) {
+            for (size_t q = 0; q < len; q++) sig = (sig ^ ((const unsigned char *)p)[q]) * 1099511628211ull;
+        };
+        mix(&Cr, sizeof(Cr));
+        mix(&C, sizeof(C));
+        for (const gt::BatchImage &bi : imgs) {
+            mix(&bi.W, sizeof(bi.W));
+            mix(&bi.H, sizeof(bi.H));
+            mix(&bi.ic, sizeof(bi.ic));
+            mix(&bi.ix, sizeof(bi.ix));
+        }
+        if (sig == 0) sig = 1;
+        bool parts = tu.parts && c->batch_sig == sig && c->batch_verts.size() == imgs.size();
+        uint64_t total = 0;
+        if (parts)
+            for (uint32_t v : c->batch_verts) total += v;
+        parts = parts && total > 0;
+        if (parts) {
+            float sf[gt::kMaxDH];
+            gt::plan_scale_factors(d, sf);
+            const int margin = 2 * (d + 1);
+            uint64_t before = 0;
+            for (size_t q = 0; q < imgs.size(); q++) {
+                gt::BatchImage &bi = imgs[q];
+                const uint64_t v = c->batch_verts[q];
+                const double range = std::ceil((double)d * sf[d - 1] * (double)(bi.H - 1) * bi.iy) + 2.0 * margin;
+                const uint64_t width24 =
+                    std::max<uint64_t>(1, (uint64_t)((double)(1 << 24) * (double)v / (double)total));
+                bi.part_start24 =
+                    (uint32_t)std::min<double>((double)(1 << 24) - 1, (double)(1 << 24) * (double)before / (double)total);
+                bi.part_range = (int)std::min(range, (double)(1 << 24));
+                bi.part_mulw = (uint64_t)((double)width24 * (double)(1 << 24) / (bi.part_range + 1.0));
+                before += v;
+            }
+        }
         const int nb = (int)imgs.size();
         if (c->h_batch_cap < nb) {
             if (c->h_batch) (void)hipHostFree(c->h_batch);
@@ -1293,6 +1369,8 @@ int gt_joint_bilateral_batch(gt_ctx *c, int count, const gt_bilateral_image *ima
         b.tmpl.ref = imgs[0].ref;   // marks the bilateral form; per-image fields come from the descriptors
         zero(b.tmpl_dst);
         b.tmpl_dst.normalize = 1;
+        b.sig = sig;
+        b.parts = parts;
         Tune t2 = tu;
         t2.tile_w = tw;
         SliceSpec none;

@@ -263,11 +263,25 @@ __device__ __forceinline__ int table_find(const uint64_t *__restrict__ tkeys, ui
 }
 
 // Home slot in a partitioned table of a key with hash h whose fields sum to
-// `fields` (part_base offsets that sum to c_0 + ... + c_{D-1} + margin).
-__device__ __forceinline__ uint32_t slot_in_part(const TableArgs &ta, uint32_t h, int fields) {
-    const uint32_t u = (uint32_t)min(max(fields + ta.part_base, 0), ta.part_range);
-    const uint32_t part = (uint32_t)(((uint64_t)u * ta.part_mul) >> 24);
+// `fields` (part_base offsets that sum to c_0 + ... + c_{D-1} + margin) and
+// whose image tag is `tag` (batches).
+__device__ __forceinline__ uint32_t slot_in_part(const TableArgs &ta, uint32_t h, int fields, uint32_t tag) {
+    uint32_t part;
+    if (ta.part_img) {
+        const BatchImage &bi = ta.part_img[tag];
+        const uint32_t u = (uint32_t)min(max(fields + ta.part_base, 0), bi.part_range);
+        const uint64_t pos24 = bi.part_start24 + (((uint64_t)u * bi.part_mulw) >> 24);
+        part = (uint32_t)((pos24 << ta.part_lg) >> 24);
+    } else {
+        const uint32_t u = (uint32_t)min(max(fields + ta.part_base, 0), ta.part_range);
+        part = (uint32_t)(((uint64_t)u * ta.part_mul) >> 24);
+    }
     return (part << ta.part_shift) | (h & (ta.mask >> ta.part_lg));
+}
+
+// Image tag of a one-word key (a batch's keys carry it above the fields).
+__device__ __forceinline__ uint32_t key_tag(const TableArgs &ta, const KeyPlan &kp, uint64_t w0) {
+    return ta.part_img ? (uint32_t)(w0 >> kp.coord_bits) : 0u;
 }
 
 // Home slot of key k (where its linear probe starts): the hash, or in a
@@ -283,7 +297,7 @@ __device__ __forceinline__ uint32_t home_slot(const TableArgs &ta, const KeyPlan
         for (int i = 0; i < D; i++)
             sum += PACKED ? (int)((k[0] >> kp.shift[i]) & kp.span[i])
                           : (int)((k[i / 4] >> (48 - 16 * (i % 4))) & 0xffff);
-        return slot_in_part(ta, h, sum);
+        return slot_in_part(ta, h, sum, PACKED ? key_tag(ta, kp, k[0]) : 0u);
     }
     return h & ta.mask;
 }

@@ -61,6 +61,13 @@ struct BatchImage {
     int W, H;
     float ic, ix, iy;   // 1/sigma_r, 1/sigma_s, 1/sigma_s
     int tile0;          // first tile of this image in the batch's tile order
+    // Table parts of a batch: this image's keys take the slice [start24,
+    // start24 + its vertex share) of [0, 2^24) -- a row band of it by their
+    // fields' sum u in [0, part_range]: start24 + (u * part_mulw >> 24) --
+    // then scaled to the call's P parts.
+    uint32_t part_start24;
+    int part_range;
+    uint64_t part_mulw;
 };
 
 struct BatchArgs {
@@ -93,6 +100,7 @@ struct TableArgs {
     // one part of the table.
     uint32_t part_mul;
     int part_base, part_range;
+    const BatchImage *part_img;    // a batch: per-image part slices, by the key's image tag
     int part_lg;                   // log2 of the part count P (>= 8)
     int part_shift;                // log2(capacity / P)
 };
@@ -170,6 +178,10 @@ hipError_t launch_slice(int d, bool packed, bool bilat, bool tiled, int storage,
                         const void *vals, const OutDst &dst, Counters *ctr, bool count, int wpc, hipStream_t st);
 hipError_t launch_unpack_keys(int d, bool packed, const KeyPlan &kp, const uint64_t *vkeys, int M, int16_t *keys,
                               hipStream_t st);
+// First vertex of each image in a sorted batched lattice (keys ordered by
+// their image tag): starts[i] for i in [0, count], starts[count] = M.
+hipError_t launch_tag_starts(const uint64_t *vkeys, int M, int coord_bits, int count, uint32_t *starts,
+                             hipStream_t st);
 hipError_t launch_store_to_f32(int storage, const void *vals, int M, int vd, int vs, float *out, hipStream_t st);
 hipError_t launch_view_copy(const float *src, int64_t sys, int64_t sts, int64_t scs, float *dst, int64_t dys,
                             int64_t dts, int64_t dcs, int W, int H, int F, int C, hipStream_t st);

@@ -869,6 +869,7 @@ __global__ __launch_bounds__(kBlock) void k_nbr_hash_vertex(const uint64_t *__re
         }
         uint32_t no_up1 = 0, no_dn1 = 0, ok_upD = 0, ok_dnD = 0;   // bit q: field q
         int sum = 0;
+        const uint32_t tag = PACKED ? key_tag(ta, kp, k[0]) : 0u;   // neighbours share it
 #pragma unroll
         for (int q = 0; q < D; q++) {
             const int wq = PACKED ? 0 : q / 4;
@@ -920,7 +921,7 @@ __global__ __launch_bounds__(kBlock) void k_nbr_hash_vertex(const uint64_t *__re
 #pragma unroll
             for (int sgn = 0; sgn < 2; sgn++) {
                 const uint32_t h = (uint32_t)hash_key<W>(t[sgn]);
-                s[sgn] = ta.part_mul ? slot_in_part(ta, h, tsum[sgn]) : (h & ta.mask);
+                s[sgn] = ta.part_mul ? slot_in_part(ta, h, tsum[sgn], tag) : (h & ta.mask);
                 done[sgn] = !ok[sgn];
             }
             for (uint32_t it = 0; it < kMaxProbes && !(done[0] && done[1]); it++) {

@@ -439,6 +439,28 @@ hipError_t launch_unpack_keys(int d, bool packed, const KeyPlan &kp, const uint6
     return hipGetLastError();
 }
 
+// One thread per image: lower bound of tag i in the sorted one-word keys.
+__global__ __launch_bounds__(kBlock) void k_tag_starts(const uint64_t *__restrict__ vkeys, int M, int coord_bits,
+                                                       int count, uint32_t *__restrict__ starts) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i > count) return;
+    int lo = 0, hi = M;
+    while (lo < hi) {
+        const int mid = lo + (hi - lo) / 2;
+        if ((int64_t)(vkeys[mid] >> coord_bits) < (int64_t)i) lo = mid + 1;
+        else hi = mid;
+    }
+    starts[i] = (uint32_t)lo;
+}
+
+hipError_t launch_tag_starts(const uint64_t *vkeys, int M, int coord_bits, int count, uint32_t *starts,
+                             hipStream_t st) {
+    if (count < 0 || M < 0 || coord_bits < 1 || coord_bits > 63) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_tag_starts, dim3(blocks_for(count + 1, kBlock)), dim3(kBlock), 0, st, vkeys, M, coord_bits,
+                       count, starts);
+    return hipGetLastError();
+}
+
 hipError_t launch_store_to_f32(int storage, const void *vals, int M, int vd, int vs, float *out, hipStream_t st) {
     if (M <= 0) return hipSuccess;
     const dim3 g(blocks_for(M, kBlock)), b(kBlock);

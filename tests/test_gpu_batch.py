@@ -92,3 +92,31 @@ def test_single_image_graph_vs_eager(gpu_ctx):
     g2 = gpu_ctx.joint_bilateral(a, a, 8.0, 8.0, 0.0, 0.1).cpu().numpy()
     e = gpu_ctx.joint_bilateral(a, a, 8.0, 8.0, 0.0, 0.1, tune={"graph": -1}).cpu().numpy()
     assert np.array_equal(g1, g2) and np.array_equal(g1, e)
+
+
+def test_batch_parts_from_measured_vertices(gpu_ctx):
+    """A batch's first call sorts and measures each image's vertex count; a
+    repeat of the same images splits the table into per-image part runs in
+    proportion to those counts and numbers vertices by hash.  Every call's
+    output equals the plain table's bit for bit."""
+    import torch
+    from imagestack_amd import gtlattice as G
+    print("\n[case] batch table parts from measured vertex counts", flush=True)
+    imgs = S.batch_images(seed=7, count=12)
+    items = []
+    for im, ss, sr in imgs:
+        a = torch.from_numpy(im).cuda()[:, None]
+        items.append((a, a, torch.empty_like(a), ss, sr))
+    ctx = G.Context(0)
+    try:
+        outs = []
+        for _ in range(3):
+            ctx.joint_bilateral_batch(items)
+            outs.append([o.cpu().numpy().copy() for (_, _, o, _, _) in items])
+        ctx.joint_bilateral_batch(items, tune={"parts": -1, "neighbours": 1})
+        ref = [o.cpu().numpy() for (_, _, o, _, _) in items]
+    finally:
+        ctx.close()
+    for run in outs:
+        for a, b in zip(run, ref):
+            assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
