@@ -264,10 +264,13 @@ __device__ __forceinline__ int table_find(const uint64_t *__restrict__ tkeys, ui
 
 // Home slot in a partitioned table of a key with hash h whose fields sum to
 // `fields` (part_base offsets that sum to c_0 + ... + c_{D-1} + margin) and
-// whose image tag is `tag` (batches).
+// whose image tag is `tag`.  BATCH: the caller may be working on a batch
+// (per-image slices, TableArgs::part_img); single-image kernels compile
+// without that path.
+template <bool BATCH = true>
 __device__ __forceinline__ uint32_t slot_in_part(const TableArgs &ta, uint32_t h, int fields, uint32_t tag) {
     uint32_t part;
-    if (ta.part_img) {
+    if (BATCH && ta.part_img) {
         const BatchImage &bi = ta.part_img[tag];
         const uint32_t u = (uint32_t)min(max(fields + ta.part_base, 0), bi.part_range);
         const uint64_t pos24 = bi.part_start24 + (((uint64_t)u * bi.part_mulw) >> 24);
@@ -286,7 +289,7 @@ __device__ __forceinline__ uint32_t key_tag(const TableArgs &ta, const KeyPlan &
 
 // Home slot of key k (where its linear probe starts): the hash, or in a
 // partitioned table (TableArgs::part_mul) the hash inside the key's part.
-template <int D, bool PACKED>
+template <int D, bool PACKED, bool BATCH = true>
 __device__ __forceinline__ uint32_t home_slot(const TableArgs &ta, const KeyPlan &kp,
                                               const uint64_t (&k)[KeyOps<D, PACKED>::W]) {
     constexpr int W = KeyOps<D, PACKED>::W;
@@ -297,7 +300,7 @@ __device__ __forceinline__ uint32_t home_slot(const TableArgs &ta, const KeyPlan
         for (int i = 0; i < D; i++)
             sum += PACKED ? (int)((k[0] >> kp.shift[i]) & kp.span[i])
                           : (int)((k[i / 4] >> (48 - 16 * (i % 4))) & 0xffff);
-        return slot_in_part(ta, h, sum, PACKED ? key_tag(ta, kp, k[0]) : 0u);
+        return slot_in_part<BATCH>(ta, h, sum, (PACKED && BATCH) ? key_tag(ta, kp, k[0]) : 0u);
     }
     return h & ta.mask;
 }
@@ -305,11 +308,11 @@ __device__ __forceinline__ uint32_t home_slot(const TableArgs &ta, const KeyPlan
 // Vertex id of key k in a finalized table (after k_finalize), or -1: read
 // from the key's own word when the ids were folded in (TableArgs::idshift),
 // else through slot_id.
-template <int D, bool PACKED>
+template <int D, bool PACKED, bool BATCH>
 __device__ __forceinline__ int table_vertex(const TableArgs &ta, const KeyPlan &kp,
                                             const uint64_t (&k)[KeyOps<D, PACKED>::W], uint32_t &probes) {
     constexpr int W = KeyOps<D, PACKED>::W;
-    uint32_t s = home_slot<D, PACKED>(ta, kp, k);
+    uint32_t s = home_slot<D, PACKED, BATCH>(ta, kp, k);
     if (W == 1 && ta.idshift > 0) {
         for (uint32_t it = 0; it < kMaxProbes; it++) {
             probes++;

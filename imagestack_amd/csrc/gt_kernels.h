@@ -294,7 +294,7 @@ __global__ __launch_bounds__(kBlock) void k_splat_tile(PixSrc src0, int n0, int 
                     for (int c = 0; c < LVD; c++)
                         if (c < vd) atomicAdd(&lv[c * LT + ls], to_fixed(wt[r] * v[c], vsc.s[c]));
                 } else if (!full) {
-                    const int slot = table_insert<W>(ta.tkeys, ta.mask, k, home_slot<D, PACKED>(ta, kp, k), probes);
+                    const int slot = table_insert<W>(ta.tkeys, ta.mask, k, home_slot<D, PACKED, BATCH>(ta, kp, k), probes);
                     inserts++;
                     if (slot < 0) { flags |= kFlagFull; continue; }
                     unsigned long long *dst = ta.tacc + (size_t)slot * vd;
@@ -325,7 +325,7 @@ __global__ __launch_bounds__(kBlock) void k_splat_tile(PixSrc src0, int n0, int 
             uint64_t k[W];
 #pragma unroll
             for (int w = 0; w < W; w++) k[w] = lk[ls * W + w];
-            slot = table_insert<W>(ta.tkeys, ta.mask, k, home_slot<D, PACKED>(ta, kp, k), probes);
+            slot = table_insert<W>(ta.tkeys, ta.mask, k, home_slot<D, PACKED, BATCH>(ta, kp, k), probes);
             inserts++;
             if (slot < 0) flags |= kFlagFull;
         }
@@ -368,7 +368,7 @@ __global__ __launch_bounds__(kBlock) void k_splat_direct(PixSrc src, int n, int 
             for (int r = 0; r <= D; r++) {
                 uint64_t k[W];
                 if (!K::make(s, r, kp, 0ull, k)) { flags |= kFlagRange; continue; }
-                const int slot = table_insert<W>(ta.tkeys, ta.mask, k, home_slot<D, PACKED>(ta, kp, k), probes);
+                const int slot = table_insert<W>(ta.tkeys, ta.mask, k, home_slot<D, PACKED, false>(ta, kp, k), probes);
                 inserts++;
                 if (slot < 0) { flags |= kFlagFull; continue; }
                 unsigned long long *dst = ta.tacc + (size_t)slot * vd;
@@ -1075,7 +1075,7 @@ __global__ __launch_bounds__(kBlock) void k_slice_tile(PixSrc src0, int n0, int 
         uint64_t k[W];
 #pragma unroll
         for (int q = 0; q < W; q++) k[q] = lk[ls * W + q];
-        const int id = table_vertex<D, PACKED>(ta, kp, k, probes);
+        const int id = table_vertex<D, PACKED, BATCH>(ta, kp, k, probes);
         float row[kMaxVD];
 #pragma unroll
         for (int c = 0; c < LVD; c++) row[c] = 0.0f;
@@ -1097,7 +1097,7 @@ __global__ __launch_bounds__(kBlock) void k_slice_tile(PixSrc src0, int n0, int 
             } else if (lsl[r] == -2) {
                 uint64_t k[W];
                 K::make(s, r, kp, tv.tag, k);
-                const int id = table_vertex<D, PACKED>(ta, kp, k, probes);
+                const int id = table_vertex<D, PACKED, BATCH>(ta, kp, k, probes);
                 if (id >= 0) {
                     float row[kMaxVD];
                     load_row<T, LVD>(vals, id, row);
@@ -1139,7 +1139,7 @@ __global__ __launch_bounds__(kBlock) void k_slice_direct(PixSrc src, int n, int 
             for (int r = 0; r <= D; r++) {
                 uint64_t k[W];
                 if (!K::make(s, r, kp, 0ull, k)) continue;
-                const int id = table_vertex<D, PACKED>(ta, kp, k, probes);
+                const int id = table_vertex<D, PACKED, false>(ta, kp, k, probes);
                 if (id < 0) continue;
                 float v[kMaxVD];
                 load_row_rt<T>(vals, id, vs, v);
