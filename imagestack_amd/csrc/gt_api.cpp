@@ -668,6 +668,9 @@ int run(gt_ctx *c, const gt::PixSrc &src, int n, int row_w, int d, int vd, int s
             // stay; a Gauss transform's last channel may be skewed, and a
             // batch's images may hold other vertex counts than measured)
             if (L.ta.part_mul && (!bilat || batch)) parts_failed = true;
+            // a batch's measured vertex counts no longer fit: the sorted redo
+            // measures them again
+            if (L.ta.part_img) c->batch_sig = 0;
             // M <= (d + 1) n, and a first table holds n / load_factor slots: x4
             // covers every lattice up to d = 7 at the default load in one redo
             // (a larger table only lengthens the redo's clear and compaction)

@@ -120,3 +120,32 @@ def test_batch_parts_from_measured_vertices(gpu_ctx):
     for run in outs:
         for a, b in zip(run, ref):
             assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def test_batch_parts_after_content_change(gpu_ctx):
+    """The same image sizes and filters with other content (noise: many more
+    vertices) than the measured counts: the split table may overflow; the
+    call is then redone on the plain table and the counts measured again.
+    Outputs equal the plain table's bit for bit, before and after."""
+    import torch
+    from imagestack_amd import gtlattice as G
+    print("\n[case] batch table parts after the images' content changes", flush=True)
+    imgs = S.batch_images(seed=8, count=10)
+    rng = np.random.default_rng(5)
+    smooth, noisy = [], []
+    for im, ss, sr in imgs:
+        a = torch.from_numpy(im).cuda()[:, None]
+        smooth.append((a, a, torch.empty_like(a), ss, sr))
+        n = torch.from_numpy(rng.uniform(size=im.shape).astype(np.float32)).cuda()[:, None]
+        noisy.append((n, n, torch.empty_like(n), ss, sr))
+    ctx = G.Context(0)
+    try:
+        for items in (smooth, smooth, noisy, noisy, smooth):
+            ctx.joint_bilateral_batch(items)
+            got = [o.cpu().numpy().copy() for (_, _, o, _, _) in items]
+            ctx.joint_bilateral_batch(items, tune={"parts": -1, "neighbours": 1})
+            want = [o.cpu().numpy() for (_, _, o, _, _) in items]
+            for a, b in zip(got, want):
+                assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    finally:
+        ctx.close()
