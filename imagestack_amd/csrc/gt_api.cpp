@@ -15,6 +15,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -73,6 +74,20 @@ namespace gt {
 int set_error(int code, const char *msg) { return fail(code, "%s", msg); }
 } // namespace gt
 
+// Vertices per pixel seen for a filter (form, d, vd and the position
+// scales): the table size of its next call at any image size.  A context
+// keeps its own entries, or uses a table shared with other contexts (the
+// serving loop's workers), locked because those run concurrently.
+struct gt_density {
+    uint64_t key = 0;
+    float per_pixel = 0.f;
+};
+struct gt_density_table {
+    std::mutex mu;
+    gt_density e[32];
+    int next = 0;
+};
+
 struct gt_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -92,12 +107,9 @@ struct gt_ctx {
     int last_n = -1, last_d = -1;
     unsigned last_M = 0;
     size_t acc_zero = 0;   // leading bytes of tacc known to hold zeros
-    struct Density {
-        uint64_t key = 0;
-        float per_pixel = 0.f;
-    };
-    Density density[16];
+    gt_density density[16];
     int density_next = 0;
+    gt_density_table *shared_density = nullptr;   // not owned
     // packing plan cache (position-source kind, d): reused until a key falls outside it
     bool plan_valid = false;
     int plan_d = 0, plan_kind = -1, plan_vd = 0;
@@ -431,6 +443,44 @@ int execute(gt_ctx *c, const Launch &L, const gt::PixSrc &src, const SliceSpec &
 int make_plan_batch(gt_ctx *c, const BatchSpec &b, const std::vector<gt::BatchImage> &imgs, int d, int vd,
                     int tile_w);
 
+template <int N>
+float density_find(const gt_density (&e)[N], uint64_t key) {
+    for (const gt_density &x : e)
+        if (key && x.key == key) return x.per_pixel;
+    return 0.f;
+}
+
+template <int N>
+void density_store(gt_density (&e)[N], int &next, uint64_t key, float per_pixel) {
+    gt_density *slot = nullptr;
+    for (gt_density &x : e)
+        if (x.key == key) slot = &x;
+    if (!slot) {
+        slot = &e[next];
+        next = (next + 1) % N;
+    }
+    slot->key = key;
+    slot->per_pixel = per_pixel;
+}
+
+float density_get(gt_ctx *c, uint64_t key) {
+    if (!key) return 0.f;
+    if (gt_density_table *t = c->shared_density) {
+        std::lock_guard<std::mutex> lk(t->mu);
+        return density_find(t->e, key);
+    }
+    return density_find(c->density, key);
+}
+
+void density_put(gt_ctx *c, uint64_t key, float per_pixel) {
+    if (gt_density_table *t = c->shared_density) {
+        std::lock_guard<std::mutex> lk(t->mu);
+        density_store(t->e, t->next, key, per_pixel);
+    } else {
+        density_store(c->density, c->density_next, key, per_pixel);
+    }
+}
+
 // Signature of a filter for the vertex-density memory: the position form,
 // d, vd and the scales of every position channel (FNV-1a over their bytes;
 // never 0).
@@ -465,9 +515,7 @@ int run(gt_ctx *c, const gt::PixSrc &src, int n, int row_w, int d, int vd, int s
     // (any image size), else the last call's M for the same size and d, else
     // the pixel count (a guess: M / n runs from ~0.05 to d + 1).
     const uint64_t dkey = batch ? 0 : density_key(src, kind, d, vd);
-    float per_pixel = 0.f;
-    for (const auto &e : c->density)
-        if (dkey && e.key == dkey) per_pixel = e.per_pixel;
+    const float per_pixel = density_get(c, dkey);
     uint64_t est;
     if (per_pixel > 0.f) est = std::max<uint64_t>(65536, (uint64_t)((double)n * per_pixel * 1.05) + 64);
     else if (c->last_n == n && c->last_d == d && c->last_M > 0) est = (uint64_t)(c->last_M * 1.05) + 64;
@@ -682,17 +730,7 @@ int run(gt_ctx *c, const gt::PixSrc &src, int n, int row_w, int d, int vd, int s
         c->last_n = n;
         c->last_d = d;
         c->last_M = (unsigned)M;
-        if (dkey) {
-            gt_ctx::Density *slot = nullptr;
-            for (auto &e : c->density)
-                if (e.key == dkey) slot = &e;
-            if (!slot) {
-                slot = &c->density[c->density_next];
-                c->density_next = (c->density_next + 1) % (int)(sizeof(c->density) / sizeof(c->density[0]));
-            }
-            slot->key = dkey;
-            slot->per_pixel = (float)M / (float)n;
-        }
+        if (dkey) density_put(c, dkey, (float)M / (float)n);
         if (batch && batch->sig && !L.hashnbr && M > 0 && L.packed && c->batch_sig != batch->sig) {
             // sorted ids: the images' vertices are consecutive runs, by tag
             const int nb = batch->ba.count;
@@ -1004,6 +1042,10 @@ int jb_view(gt_ctx *c, View im, int C, View ref, int Cr, View out, float fw, flo
 
 namespace gt {
 void invalidate_plan(gt_ctx *c) { c->plan_valid = false; }
+
+gt_density_table *density_table_new() { return new gt_density_table(); }
+void density_table_delete(gt_density_table *t) { delete t; }
+void share_density(gt_ctx *c, gt_density_table *t) { c->shared_density = t; }
 int ctx_device(gt_ctx *c) { return c->device; }
 hipStream_t ctx_stream(gt_ctx *c) { return c->stream; }
 } // namespace gt

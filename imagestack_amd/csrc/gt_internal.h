@@ -7,7 +7,8 @@
 
 #include "gt_keyplan.h"
 
-struct gt_ctx;   // the C ABI's opaque context (gt_api.cpp)
+struct gt_ctx;             // the C ABI's opaque context (gt_api.cpp)
+struct gt_density_table;   // vertex densities shared by contexts (gt_api.cpp)
 
 namespace gt {
 
@@ -233,6 +234,11 @@ int set_error(int code, const char *msg);
 // Forgets the context's cached key-packing plan, so the next call bounds its
 // inputs afresh (a server's next request is an unrelated image).
 void invalidate_plan(::gt_ctx *c);
+// A vertex-density table several contexts share (the serving loop's
+// workers); it must outlive them.
+::gt_density_table *density_table_new();
+void density_table_delete(::gt_density_table *t);
+void share_density(::gt_ctx *c, ::gt_density_table *t);
 // Sorts the first ctr->M (key, payload) pairs by key (bits significant bits,
 // pw words); the result is in the b buffers when *result_in_b.
 hipError_t radix_sort(SortBufs &b, Counters *ctr, int m_max, int pw, int bits, bool *result_in_b, int *passes,

@@ -75,6 +75,7 @@ struct gt_server {
     bool stop = false;
     std::vector<std::thread> workers;
     std::string init_err;
+    gt_density_table *density = nullptr;   // vertex densities the workers share
     int init_rc = GT_OK;
     std::vector<float> warm;   // host warm-up image (3 planes of warm_side^2), shared read-only
 };
@@ -262,6 +263,7 @@ void worker_main(gt_server *s, std::mutex *init_mu, int *init_left, std::conditi
     hipStream_t st = nullptr;
     int rc = hipSetDevice(s->device) == hipSuccess ? GT_OK : GT_EDEVICE;
     if (rc == GT_OK) rc = gt_ctx_create(&ctx, s->device);
+    if (rc == GT_OK) gt::share_density(ctx, s->density);   // a filter learnt by one worker serves all
     if (rc == GT_OK && hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) rc = GT_EDEVICE;
     if (rc == GT_OK) rc = gt_ctx_set_stream(ctx, st);
     DevArena arena;
@@ -343,6 +345,7 @@ int gt_server_create(gt_server **out, int device, const gt_server_config *cfg) {
     }
     gt_server *s = new gt_server();
     s->device = device;
+    s->density = gt::density_table_new();
     if (cfg) s->cfg = *cfg;
     if (s->cfg.slots <= 0) s->cfg.slots = 4;
     if (s->cfg.slots > 16) s->cfg.slots = 16;
@@ -397,6 +400,7 @@ void gt_server_destroy(gt_server *s) {
     s->cv_work.notify_all();
     for (auto &t : s->workers) t.join();
     for (auto &kv : s->reqs) delete kv.second;   // every queued request is also in reqs
+    gt::density_table_delete(s->density);         // after the workers' contexts are gone
     delete s;
 }
 
