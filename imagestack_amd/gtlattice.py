@@ -386,9 +386,11 @@ class Context:
                                                        C.byref(best), C.byref(rep)))
         return tuning_dict(best), {k: getattr(rep, k) for k, _ in TuneReport._fields_}
 
-    def build_lattice(self, splat, values, inv_sigma, row_w: int = 0, storage: str = "f32", tune=None):
+    def build_lattice(self, splat, values, inv_sigma, row_w: int = 0, storage: str = "f32", tune=None,
+                      parts=("keys", "nbr", "splat", "blur")):
         """Splat + blur and export: dict(keys (M,d) int16, nbr (d+1,M,2) int32,
-        splat (M,vd) f32, blur (M,vd) f32)."""
+        splat (M,vd) f32, blur (M,vd) f32); `parts` names the arrays copied
+        out (the others are None)."""
         d, n = splat.shape[0], splat[0].numel()
         vd = values.shape[0]
         inv = np.ascontiguousarray(inv_sigma, dtype=np.float32)
@@ -396,12 +398,15 @@ class Context:
         _check(lib().gt_build_lattice(self._h, n, row_w, d, vd, _dptr(splat), _dptr(values),
                                       inv.ctypes.data_as(_fp), STORAGE[storage], _tuning(tune), C.byref(M)))
         M = M.value
-        keys = np.zeros((M, d), np.int16)
-        nbr = np.zeros((d + 1, M, 2), np.int32)
-        sv = np.zeros((M, vd), np.float32)
-        bv = np.zeros((M, vd), np.float32)
-        _check(lib().gt_lattice_export(self._h, keys.ctypes.data, nbr.ctypes.data, sv.ctypes.data,
-                                       bv.ctypes.data))
+        keys = np.zeros((M, d), np.int16) if "keys" in parts else None
+        nbr = np.zeros((d + 1, M, 2), np.int32) if "nbr" in parts else None
+        sv = np.zeros((M, vd), np.float32) if "splat" in parts else None
+        bv = np.zeros((M, vd), np.float32) if "blur" in parts else None
+
+        def ptr(a):
+            return None if a is None else a.ctypes.data
+
+        _check(lib().gt_lattice_export(self._h, ptr(keys), ptr(nbr), ptr(sv), ptr(bv)))
         return {"M": M, "keys": keys, "nbr": nbr, "splat": sv, "blur": bv}
 
     # --- host-pointer API (numpy) -----------------------------------------

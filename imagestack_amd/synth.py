@@ -130,6 +130,15 @@ def highdim_positions(W: int, H: int, nfeat: int = 14, sigma_s: float = 12.0,
     return np.concatenate([feats, xx[None], yy[None]]).astype(f32)
 
 
+def highdim_1024(side: int = 1024):
+    """d = 16 Gauss transform (non-local-means style) at side x side: 14
+    N(0,1) features + x/12, y/12 (seed 3), values 4 x U[0,1] (seed 33).
+    Returns (positions (16, 1, side, side), values (4, 1, side, side))."""
+    pos = highdim_positions(side, side, nfeat=14, sigma_s=12.0, seed=3)
+    vals = uniform_image(side, side, 4, seed=33)
+    return pos, vals
+
+
 def bilateral_512(seed: int = 1):
     """512x512 RGB bilateral: 64 rectangles + N(0,0.05); sigma_s=8, sigma_r=0.1."""
     im = rect_image(512, 512, 3, K=64, noise=0.05, seed=seed)

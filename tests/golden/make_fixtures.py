@@ -170,7 +170,9 @@ def tmp_cases():
 
 
 def digests():
-    """The 512^2 bilateral and the 2048^2 joint bilateral: M and digests."""
+    """The 512^2 bilateral, the 2048^2 joint bilateral and the 1024^2 d = 16
+    transform: M and digests (key sets; outputs where the reference's FP32
+    bits are the parity target of the oracle)."""
     rec = {}
     im, ref, ss, sr = S.bilateral_512()
     pos = S.bilateral_positions(ref, ss, sr)
@@ -183,8 +185,13 @@ def digests():
     pos = S.bilateral_positions(ref, ss, sr)
     val = S.homogeneous_values(im)
     keys, _ = sorted_ref_lattice(pos, val, np.ones(5, np.float32), blur=False)
-    rec["joint_bilateral_2048"] = {"M": int(keys.shape[0]), "keys_sha": digest(keys),
+    out = O.ref_joint_bilateral(im, ref, ss, ss, 0.0, sr)
+    rec["joint_bilateral_2048"] = {"M": int(keys.shape[0]), "keys_sha": digest(keys), "out_sha": digest(out),
                                        "guide_sha": digest(ref), "image_sha": digest(im)}
+    pos, val = S.highdim_1024()
+    keys, _ = sorted_ref_lattice(pos, val, np.ones(16, np.float32), blur=False)
+    rec["highdim_1024"] = {"M": int(keys.shape[0]), "keys_sha": digest(keys), "pos_sha": digest(pos),
+                           "values_sha": digest(val)}
     with open(os.path.join(HERE, "digests.json"), "w") as f:
         json.dump(rec, f, indent=1, sort_keys=True)
 

@@ -140,8 +140,10 @@ def test_batch_parts_after_content_change(gpu_ctx):
         noisy.append((n, n, torch.empty_like(n), ss, sr))
     ctx = G.Context(0)
     try:
+        retries = []
         for items in (smooth, smooth, noisy, noisy, smooth):
-            ctx.joint_bilateral_batch(items)
+            st = ctx.joint_bilateral_batch(items, stats=True)
+            retries.append(st["retries"])
             got = [o.cpu().numpy().copy() for (_, _, o, _, _) in items]
             ctx.joint_bilateral_batch(items, tune={"parts": -1, "neighbours": 1})
             want = [o.cpu().numpy() for (_, _, o, _, _) in items]
@@ -149,3 +151,9 @@ def test_batch_parts_after_content_change(gpu_ctx):
                 assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
     finally:
         ctx.close()
+    print("retries per call", retries, flush=True)
+    # the first noisy call overflowed the parts sized from the smooth images'
+    # counts and was redone; the redo measured the counts again, so the
+    # second noisy call (and the smooth one after it) fit at once
+    assert retries[2] >= 1
+    assert retries[3] == 0 and retries[4] == 0

@@ -279,6 +279,31 @@ def test_golden_bilateral_512_digest(oracle_lib):
     assert digest(out) == rec["out_sha"]
 
 
+def test_golden_joint_bilateral_2048_digest(oracle_lib):
+    """The headline workload at full size: the oracle's M, sorted key set and
+    FP32 output equal the reference's (digests written from the reference)."""
+    rec = json.load(open(os.path.join(GOLD, "digests.json")))["joint_bilateral_2048"]
+    im, ref, ss, sr = S.joint_bilateral_2048()
+    assert digest(ref) == rec["guide_sha"]
+    assert digest(im) == rec["image_sha"]
+    pos = S.bilateral_positions(ref, ss, sr)
+    val = S.homogeneous_values(im)
+    lat = oracle_lib.lattice(pos, val, np.ones(5, np.float32))
+    assert lat["M"] == rec["M"]
+    assert digest(lat["keys"]) == rec["keys_sha"]
+    out = oracle_lib.joint_bilateral(im, ref, ss, ss, 0.0, sr)
+    assert digest(out) == rec["out_sha"]
+
+
+def test_highdim_1024_inputs_digest():
+    """The d = 16 workload's inputs are the ones the reference's key-set
+    digest was written from (the key set itself is checked on the GPU)."""
+    rec = json.load(open(os.path.join(GOLD, "digests.json")))["highdim_1024"]
+    pos, val = S.highdim_1024()
+    assert digest(pos) == rec["pos_sha"]
+    assert digest(val) == rec["values_sha"]
+
+
 def test_reference_build_agrees(oracle_lib):
     """(R): oracle vs the reference library on fresh seeded inputs."""
     O = oracle_lib
