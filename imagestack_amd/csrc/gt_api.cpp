@@ -94,7 +94,7 @@ struct gt_ctx {
     hipStream_t own = nullptr;
     int profiling = 0;   // 0 off, 1 HIP events, 2 events + probe/atomic counters
     hipEvent_t ev[8] = {};
-    DevBuf tkeys, tacc, ctr, counts, slot_list, slot_id, pk_a, pk_b, pay_b, hist, sort_status, sort_hist, vkeys,
+    DevBuf tkeys, tacc, tnf, vscale, vs_part, ctr, counts, slot_list, slot_id, pk_a, pk_b, pay_b, hist, sort_status, sort_hist, vkeys,
         vals_a, vals_b, nbr, splat_copy, mm_part, mm_out;
     DevBuf stage_in, stage_out, view_a, view_b, view_o;
     DevBuf grid_acc, grid_a, grid_b;   // bilateral grid: fixed-point sums, FP32 ping-pong
@@ -107,6 +107,7 @@ struct gt_ctx {
     int last_n = -1, last_d = -1;
     unsigned last_M = 0;
     size_t acc_zero = 0;   // leading bytes of tacc known to hold zeros
+    bool tnf_clean = false;   // the non-finite marks are all zero (a complete call leaves them so)
     gt_density density[16];
     int density_next = 0;
     gt_density_table *shared_density = nullptr;   // not owned
@@ -117,7 +118,6 @@ struct gt_ctx {
     float last_lo = 0, last_hi = 0;  // its last position channel's scaled bounds
 
     gt::KeyPlan plan{};
-    float vmax[gt::kMaxDH] = {};   // value bounds behind the fixed-point scales
     // cached pipeline graph
     hipGraphExec_t gexec = nullptr;
     std::string gsig;
@@ -201,24 +201,22 @@ float elapsed(hipEvent_t a, hipEvent_t b) {
     return ms;
 }
 
-// Per-channel bounds of the positions and values on the device (one host
-// round trip), then the key packing plan and the value bounds for the
-// fixed-point scales.
+// Per-channel bounds of the positions on the device (one host round trip),
+// then the key packing plan.  (The values' bounds, behind the fixed-point
+// scales, are taken on the device in every call.)
 int make_plan(gt_ctx *c, const gt::PixSrc &src, int n, int d, int vd, bool bilat) {
     const int nplanes = bilat ? src.cr : d;
     const float *planes = bilat ? src.ref : src.pos;
     const int64_t cs = bilat ? src.ref_cs : src.pos_cs;
-    const int nval = src.nval;
     const int nblk = (int)std::min<int64_t>(1024, std::max<int64_t>(1, ((int64_t)n + 4095) / 4096));
-    GT_TRY(ensure(c->mm_part, (size_t)nblk * (nplanes + nval) * 3 * 4));
-    GT_TRY(ensure(c->mm_out, (size_t)(nplanes + nval) * 3 * 4));
+    GT_TRY(ensure(c->mm_part, (size_t)nblk * std::max(nplanes, 1) * 3 * 4));
+    GT_TRY(ensure(c->mm_out, (size_t)std::max(nplanes, 1) * 3 * 4));
     float *part = (float *)c->mm_part.p, *mout = (float *)c->mm_out.p;
-    GT_HIP(gt::launch_minmax(planes, cs, nplanes, src.val, src.val_cs, nval, n, part, nblk, mout, c->stream));
-    GT_HIP(hipMemcpyAsync(c->h_mm, mout, (size_t)(nplanes + nval) * 3 * 4, hipMemcpyDeviceToHost, c->stream));
+    GT_HIP(gt::launch_minmax(planes, cs, nplanes, nullptr, 0, 0, n, part, nblk, mout, c->stream));
+    if (nplanes > 0)
+        GT_HIP(hipMemcpyAsync(c->h_mm, mout, (size_t)nplanes * 3 * 4, hipMemcpyDeviceToHost, c->stream));
     GT_HIP(hipStreamSynchronize(c->stream));
-    const float *mm = c->h_mm, *vm = c->h_mm + (size_t)nplanes * 3;
-    for (int k = 0; k < nval; k++)
-        if (vm[3 * k + 2] != 0.0f) return fail(GT_ERANGE, "value channel %d holds NaN or Inf", k);
+    const float *mm = c->h_mm;
     float lo[gt::kMaxDH], hi[gt::kMaxDH];
     for (int i = 0; i < d; i++) {
         float a, b;
@@ -244,9 +242,6 @@ int make_plan(gt_ctx *c, const gt::PixSrc &src, int n, int d, int vd, bool bilat
     float sf[gt::kMaxDH];
     gt::plan_scale_factors(d, sf);
     gt::plan_from_bounds(d, sf, lo, hi, c->plan);
-    for (int k = 0; k < gt::kMaxDH; k++) c->vmax[k] = 0.0f;
-    for (int k = 0; k < nval && k < vd; k++) c->vmax[k] = std::max(std::fabs(vm[3 * k]), std::fabs(vm[3 * k + 1]));
-    if (src.homog && nval < vd) c->vmax[nval] = 1.0f;
     c->plan_valid = true;
     c->plan_d = d;
     c->plan_vd = vd;
@@ -254,22 +249,6 @@ int make_plan(gt_ctx *c, const gt::PixSrc &src, int n, int d, int vd, bool bilat
     c->plan_n = n;
     c->last_lo = lo[d - 1];
     c->last_hi = hi[d - 1];
-    return GT_OK;
-}
-
-// Fixed-point scales for n contributions of |value| <= vmax and weight <= 1:
-// s = 2^(62 - ceil(log2(n * vmax))) keeps every sum below 2^62 in magnitude.
-int value_scales(const gt_ctx *c, int n, int vd, gt::ValScale &vsc) {
-    for (int k = 0; k < gt::kMaxDH; k++) { vsc.s[k] = 0.0f; vsc.inv_s[k] = 0.0f; vsc.vmax[k] = 0.0f; }
-    for (int k = 0; k < vd; k++) {
-        const double bound = std::max(1e-30, (double)n * (double)c->vmax[k] * 1.001);
-        int p = 62 - (int)std::ceil(std::log2(bound));
-        if (p > 120) p = 120;
-        if (p < -120) return fail(GT_ERANGE, "values of channel %d are too large (%g)", k, (double)c->vmax[k]);
-        vsc.s[k] = std::ldexp(1.0f, p);
-        vsc.inv_s[k] = std::ldexp(1.0f, -p);
-        vsc.vmax[k] = c->vmax[k];
-    }
     return GT_OK;
 }
 
@@ -305,11 +284,14 @@ struct Launch {
     int d, vd, storage, W, pw, m_max, vs, n, row_w, tile_w, wpc;
     bool packed, bilat, dedup, count, hashnbr;
     bool clear_sums;   // clear the fixed-point sums too (not known to be zero)
+    bool clear_marks;  // clear the non-finite marks too (a call that stopped early may have left some)
     bool nbr_per_vertex;   // hash neighbours one thread per vertex (else per vertex and axis)
     uint32_t cap;
     gt::LatConst lc;
     gt::KeyPlan kp;
-    gt::ValScale vsc;
+    gt::ValScale *vsc;   // device: this call's fixed-point scales (computed by the pipeline)
+    float *vs_part;      // device: per-workgroup value bounds behind them
+    int vs_nblk;
     gt::TableArgs ta;
     gt::PartMap pm;
     float *splat_copy;
@@ -324,8 +306,15 @@ int enqueue(gt_ctx *c, const Launch &L, const gt::PixSrc &src, const SliceSpec &
     hipStream_t s = c->stream;
     gt::Counters *ctr = (gt::Counters *)c->ctr.p;
     const gt::TableArgs &ta = L.ta;
-    GT_HIP(gt::launch_clear(ta.tkeys, (size_t)L.cap * L.W, ta.tacc, L.clear_sums ? (size_t)L.cap * L.vd : 0, ctr,
-                            s));
+    GT_HIP(gt::launch_clear(ta.tkeys, (size_t)L.cap * L.W, ta.tacc, L.clear_sums ? (size_t)L.cap * L.vd : 0, ta.tnf,
+                            L.clear_marks ? (size_t)L.cap : 0, ctr, s));
+    // this call's value bounds -> fixed-point scales, on the device
+    if (batch)
+        GT_HIP(gt::launch_value_scale_batch(batch->ba, batch->ntiles, L.tile_w, batch->C, L.n, L.vd, L.vs_part,
+                                            L.vs_nblk, L.vsc, ctr, s));
+    else
+        GT_HIP(gt::launch_value_scale(src.val, src.val_cs, src.nval, src.homog, L.n, L.vd, L.vs_part, L.vs_nblk,
+                                      L.vsc, ctr, s));
     if (c->profiling) GT_HIP(mark(c, 0, s));
     if (batch)
         GT_HIP(gt::launch_splat_batch(L.d, batch->tmpl, batch->ba, batch->ntiles, L.tile_w, L.vd, L.lc, L.kp, L.vsc,
@@ -365,16 +354,18 @@ int enqueue(gt_ctx *c, const Launch &L, const gt::PixSrc &src, const SliceSpec &
     if (c->profiling) GT_HIP(mark(c, 3, s));
     void *cur = c->vals_a.p, *nxt = c->vals_b.p;
     for (int j = 0; j <= L.d; j++) {
-        GT_HIP(gt::launch_blur(L.storage, L.vs, nbr + (size_t)j * L.m_max, ctr, L.m_max, L.pm, cur, nxt, s));
+        GT_HIP(gt::launch_blur(L.storage, L.vs, j, L.vsc, nbr + (size_t)j * L.m_max, ctr, L.m_max, L.pm, cur, nxt,
+                               s));
         std::swap(cur, nxt);
     }
     if (c->profiling) GT_HIP(mark(c, 4, s));
+    const unsigned *sbits = L.storage ? &L.vsc->vmax_bits[L.d][0] : nullptr;   // the last pass's scale
     if (batch)
         GT_HIP(gt::launch_slice_batch(L.d, L.storage, batch->tmpl, batch->ba, batch->ntiles, L.tile_w, L.vd, L.lc,
-                                      L.kp, ta, cur, batch->tmpl_dst, ctr, L.count, L.wpc, s));
+                                      L.kp, ta, cur, sbits, batch->tmpl_dst, ctr, L.count, L.wpc, s));
     else if (sl.enabled)
         GT_HIP(gt::launch_slice(L.d, L.packed, sl.src.ref != nullptr, L.dedup, L.storage, sl.src, sl.n, L.row_w,
-                                L.tile_w, L.vd, L.vs, L.lc, L.kp, ta, cur, sl.dst, ctr, L.count, L.wpc, s));
+                                L.tile_w, L.vd, L.vs, L.lc, L.kp, ta, cur, sbits, sl.dst, ctr, L.count, L.wpc, s));
     if (c->profiling) GT_HIP(mark(c, 5, s));
     GT_HIP(hipMemcpyAsync(c->h_ctr, ctr, sizeof(gt::Counters), hipMemcpyDeviceToHost, s));
     *vals_out = cur;
@@ -402,7 +393,7 @@ int execute(gt_ctx *c, const Launch &L, const gt::PixSrc &src, const SliceSpec &
     sig_add(sig, sl.dst);
     sig_add(sig, c->profiling);
     sig_add(sig, c->stream);
-    const void *bufs[] = {c->tkeys.p,  c->tacc.p,        c->ctr.p,         c->counts.p,    c->slot_list.p,
+    const void *bufs[] = {c->tkeys.p,  c->tacc.p,  c->tnf.p, c->vscale.p, c->vs_part.p, c->ctr.p,         c->counts.p,    c->slot_list.p,
                           c->slot_id.p, c->pk_a.p,       c->pk_b.p,        c->pay_b.p,     c->hist.p,
                           c->sort_status.p, c->sort_hist.p, c->vkeys.p,    c->vals_a.p,    c->vals_b.p,
                           c->nbr.p};
@@ -554,7 +545,6 @@ int run(gt_ctx *c, const gt::PixSrc &src, int n, int row_w, int d, int vd, int s
         L.cap = cap;
         gt::plan_scale_factors(d, L.lc.sf);
         L.kp = c->plan;
-        GT_TRY(value_scales(c, n, vd, L.vsc));
         L.packed = L.kp.packed != 0;
         if (batch && !L.packed) return fail(GT_EINVAL, "batched lattice keys do not fit 63 bits");
         L.W = L.packed ? 1 : (d + 3) / 4;
@@ -656,6 +646,20 @@ int run(gt_ctx *c, const gt::PixSrc &src, int n, int row_w, int d, int vd, int s
         const size_t acc_zero = c->acc_zero;
         L.clear_sums = acc_zero < acc_bytes;
         c->acc_zero = 0;
+        // non-finite marks: zero unless an earlier call stopped before its
+        // finalize read them (a full table); until this call completes, unknown
+        {
+            const void *old_tnf = c->tnf.p;
+            GT_TRY(ensure_zeroed(c->tnf, (size_t)cap * 4, s));
+            if (c->tnf.p != old_tnf) c->tnf_clean = true;
+        }
+        L.clear_marks = !c->tnf_clean;
+        c->tnf_clean = false;
+        L.vs_nblk = (int)std::min<int64_t>(1024, std::max<int64_t>(1, ((int64_t)n + 4095) / 4096));
+        GT_TRY(ensure(c->vs_part, (size_t)L.vs_nblk * std::max(vd, 1) * 3 * 4));
+        GT_TRY(ensure(c->vscale, sizeof(gt::ValScale)));
+        L.vsc = (gt::ValScale *)c->vscale.p;
+        L.vs_part = (float *)c->vs_part.p;
         GT_TRY(ensure(c->ctr, sizeof(gt::Counters)));
         GT_TRY(ensure(c->counts, (size_t)ncount * 4));
         GT_TRY(ensure(c->slot_list, (size_t)L.m_max * 4));
@@ -667,9 +671,10 @@ int run(gt_ctx *c, const gt::PixSrc &src, int n, int row_w, int d, int vd, int s
         GT_TRY(ensure_zeroed(c->sort_status, gt::sort_status_entries(L.m_max, L.pw) * 8, s));
         GT_TRY(ensure(c->sort_hist, gt::sort_hist_words(L.m_max, L.pw) * 4));
         GT_TRY(ensure(c->vkeys, (size_t)L.m_max * L.W * 8));
-        const size_t vbytes = (size_t)L.m_max * L.vs * store_bytes(storage);
-        GT_TRY(ensure(c->vals_a, vbytes));
-        GT_TRY(ensure(c->vals_b, vbytes));
+        // vals_a takes the finalize's FP32 rows (every storage), vals_b the
+        // storage type; the passes alternate between them
+        GT_TRY(ensure(c->vals_a, (size_t)L.m_max * L.vs * 4));
+        GT_TRY(ensure(c->vals_b, (size_t)L.m_max * L.vs * store_bytes(storage)));
         GT_TRY(ensure(c->nbr, (size_t)(d + 1) * L.m_max * 8));
         L.splat_copy = nullptr;
         if (keep_splat) {
@@ -678,6 +683,7 @@ int run(gt_ctx *c, const gt::PixSrc &src, int n, int row_w, int d, int vd, int s
         }
         L.ta.tkeys = (uint64_t *)c->tkeys.p;
         L.ta.tacc = (unsigned long long *)c->tacc.p;
+        L.ta.tnf = (uint32_t *)c->tnf.p;
         L.ta.mask = cap - 1;
         L.ta.slot_id = (int *)c->slot_id.p;
         // One-word keys leave room above the key for the vertex id: fold it in
@@ -696,19 +702,34 @@ int run(gt_ctx *c, const gt::PixSrc &src, int n, int row_w, int d, int vd, int s
         GT_TRY(execute(c, L, src, sl, batch, tu.graph && !keep_splat && c->profiling == 0, &passes, &cur));
         GT_HIP(hipStreamSynchronize(s));
         const gt::Counters hc = *c->h_ctr;
+        // every mark was read by the finalize unless the table filled up
+        if (!((hc.flags & gt::kFlagFull) && (hc.flags & gt::kFlagNonFinite))) c->tnf_clean = true;
+        const unsigned hard = hc.flags & ~gt::kFlagNonFinite;
 
         if (hc.flags & gt::kFlagBad)
             return fail(GT_ERANGE, "position is NaN/Inf or a lattice key leaves the int16 range");
         if (hc.flags & gt::kFlagSort) return fail(GT_EDEVICE, "radix sort look-back timed out");
         if (hc.flags & gt::kFlagRange) {
-            // Keys or values outside the cached bounds: bound these inputs again.
+            // Keys outside the cached plan: bound these positions again.
             if (++retries > 6) return fail(GT_EDEVICE, "inputs keep leaving the computed bounds");
             GT_TRY(replan());
             continue;
         }
         if (hc.flags & gt::kFlagFull) {
-            if (++retries > 6 || cap >= cap_limit)
-                return fail(GT_ENOMEM, "lattice does not fit a hash table of %u slots", cap);
+            if (++retries > 6) return fail(GT_ENOMEM, "lattice does not fit a hash table of %u slots", cap);
+            // a batch's measured vertex counts no longer fit: the sorted redo
+            // measures them again
+            if (L.ta.part_img) c->batch_sig = 0;
+            // One part of a split table filled while the table as a whole held
+            // room (an uneven split: a skewed last channel, rows of uneven
+            // content, stale batch counts): redo at this capacity without
+            // parts.  A table that is really too small fills to ~0.9 before an
+            // insert runs out of probes.
+            if (L.ta.part_mul && !parts_failed && (double)hc.filled < 0.6 * (double)cap) {
+                parts_failed = true;
+                continue;
+            }
+            if (cap >= cap_limit) return fail(GT_ENOMEM, "lattice does not fit a hash table of %u slots", cap);
             // a full table held about `cap` vertices: decide the redo's
             // neighbour form on that
             est = std::max<uint64_t>(est, cap);
@@ -716,9 +737,6 @@ int run(gt_ctx *c, const gt::PixSrc &src, int n, int row_w, int d, int vd, int s
             // stay; a Gauss transform's last channel may be skewed, and a
             // batch's images may hold other vertex counts than measured)
             if (L.ta.part_mul && (!bilat || batch)) parts_failed = true;
-            // a batch's measured vertex counts no longer fit: the sorted redo
-            // measures them again
-            if (L.ta.part_img) c->batch_sig = 0;
             // M <= (d + 1) n, and a first table holds n / load_factor slots: x4
             // covers every lattice up to d = 7 at the default load in one redo
             // (a larger table only lengthens the redo's clear and compaction)
@@ -726,7 +744,7 @@ int run(gt_ctx *c, const gt::PixSrc &src, int n, int row_w, int d, int vd, int s
             continue;
         }
         const int M = (int)hc.M;
-        if (hc.flags == 0) c->acc_zero = std::max(acc_zero, acc_bytes);   // the finalize zeroed what the splat summed
+        if (hard == 0) c->acc_zero = std::max(acc_zero, acc_bytes);   // the finalize zeroed what the splat summed
         c->last_n = n;
         c->last_d = d;
         c->last_M = (unsigned)M;
@@ -776,20 +794,18 @@ int run(gt_ctx *c, const gt::PixSrc &src, int n, int row_w, int d, int vd, int s
     }
 }
 
-// Bounds of a batch: one device pass over every tile of every image.
+// Bounds of a batch's guides: one device pass over every tile of every image.
 int make_plan_batch(gt_ctx *c, const BatchSpec &b, const std::vector<gt::BatchImage> &imgs, int d, int vd,
                     int tile_w) {
-    const int np = b.Cr + b.C;
+    const int np = b.Cr;
     const int nblk = std::max(1, std::min(1024, b.ntiles));
-    GT_TRY(ensure(c->mm_part, (size_t)nblk * np * 3 * 4));
-    GT_TRY(ensure(c->mm_out, (size_t)np * 3 * 4));
-    GT_HIP(gt::launch_bounds_batch(b.ba, b.ntiles, tile_w, b.Cr, b.C, (float *)c->mm_part.p, nblk,
+    GT_TRY(ensure(c->mm_part, (size_t)nblk * std::max(np, 1) * 3 * 4));
+    GT_TRY(ensure(c->mm_out, (size_t)std::max(np, 1) * 3 * 4));
+    GT_HIP(gt::launch_bounds_batch(b.ba, b.ntiles, tile_w, b.Cr, 0, (float *)c->mm_part.p, nblk,
                                    (float *)c->mm_out.p, c->stream));
     GT_HIP(hipMemcpyAsync(c->h_mm, c->mm_out.p, (size_t)np * 3 * 4, hipMemcpyDeviceToHost, c->stream));
     GT_HIP(hipStreamSynchronize(c->stream));
-    const float *mm = c->h_mm, *vm = c->h_mm + (size_t)b.Cr * 3;
-    for (int k = 0; k < b.C; k++)
-        if (vm[3 * k + 2] != 0.0f) return fail(GT_ERANGE, "value channel %d holds NaN or Inf", k);
+    const float *mm = c->h_mm;
     float lo[gt::kMaxDH], hi[gt::kMaxDH];
     for (int i = 0; i < b.Cr; i++) {
         lo[i] = mm[3 * i];
@@ -808,9 +824,6 @@ int make_plan_batch(gt_ctx *c, const BatchSpec &b, const std::vector<gt::BatchIm
     float sf[gt::kMaxDH];
     gt::plan_scale_factors(d, sf);
     gt::plan_from_bounds(d, sf, lo, hi, c->plan, (int)imgs.size());
-    for (int k = 0; k < gt::kMaxDH; k++) c->vmax[k] = 0.0f;
-    for (int k = 0; k < b.C; k++) c->vmax[k] = std::max(std::fabs(vm[3 * k]), std::fabs(vm[3 * k + 1]));
-    c->vmax[b.C] = 1.0f;
     c->plan_valid = true;
     c->plan_d = d;
     c->plan_vd = vd;
@@ -914,7 +927,6 @@ int run_grid(gt_ctx *c, const gt::PixSrc &src, int n, int d, int vd, const gt::P
         if (p < -120) return fail(GT_ERANGE, "values of channel %d are too large", k);
         vsc.s[k] = std::ldexp(1.0f, p);
         vsc.inv_s[k] = std::ldexp(1.0f, -p);
-        vsc.vmax[k] = vmax;
     }
     const size_t total = (size_t)cells * vd;
     GT_TRY(ensure(c->grid_acc, total * 8));
@@ -1472,7 +1484,9 @@ int gt_lattice_export(gt_ctx *c, int16_t *keys, int32_t *nbr, float *splat_vals,
                              hipMemcpyDeviceToHost));
     if (splat_vals) GT_HIP(hipMemcpy(splat_vals, c->splat_copy.p, (size_t)M * vd * 4, hipMemcpyDeviceToHost));
     if (blur_vals) {
-        GT_HIP(gt::launch_store_to_f32(c->storage, c->vals_cur, M, vd, c->vs, (float *)c->stage_out.p, c->stream));
+        const unsigned *sbits = c->storage ? &((const gt::ValScale *)c->vscale.p)->vmax_bits[d][0] : nullptr;
+        GT_HIP(gt::launch_store_to_f32(c->storage, c->vals_cur, sbits, M, vd, c->vs, (float *)c->stage_out.p,
+                                       c->stream));
         GT_HIP(hipMemcpyAsync(blur_vals, c->stage_out.p, (size_t)M * vd * 4, hipMemcpyDeviceToHost, c->stream));
         GT_HIP(hipStreamSynchronize(c->stream));
     }

@@ -363,6 +363,41 @@ __device__ __forceinline__ unsigned long long to_fixed(float x, float s) {
     return ((unsigned long long)(uint32_t)(int)hi << 32) | (uint32_t)lo;
 }
 
+// Exponent e of a narrow storage's scale 2^e for values of largest finite
+// magnitude m (float bits; 0: all zero): e = 14 - ceil(log2(m)), exact
+// through frexpf, clamped to [-100, 100].  Scaled values stay below 2^14.
+__device__ __forceinline__ int store_exp(unsigned mbits) {
+    if (mbits == 0) return 0;
+    int k = 0;
+    const float f = frexpf(__uint_as_float(mbits), &k);   // m = f 2^k, f in [0.5, 1)
+    const int lg = (f == 0.5f) ? k - 1 : k;
+    return min(100, max(-100, 14 - lg));
+}
+
+// Running maximum of the finite magnitudes of o[0..NV) into m.
+template <int NV>
+__device__ __forceinline__ void take_max(float (&m)[kMaxVD], const float (&o)[kMaxVD]) {
+#pragma unroll
+    for (int c = 0; c < NV; c++) {
+        const float a = fabsf(o[c]);
+        m[c] = (a <= 3.402823466e38f) ? fmaxf(m[c], a) : m[c];
+    }
+}
+
+// Wave maximum of m[c] per channel c < nc, one atomicMax of its float bits
+// per wave and channel (non-negative floats order as their bits).
+template <int NV>
+__device__ __forceinline__ void publish_max(unsigned *track, const float (&m)[kMaxVD], int nc) {
+#pragma unroll
+    for (int c = 0; c < NV; c++) {
+        if (c >= nc) continue;
+        float v = m[c];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
+        if (__lane_id() == 0 && v > 0.0f) atomicMax(track + c, __float_as_uint(v));
+    }
+}
+
 __device__ __forceinline__ uint64_t lanemask_lt() {
     const int lane = __lane_id();
     return lane == 0 ? 0ull : (~0ull >> (64 - lane));

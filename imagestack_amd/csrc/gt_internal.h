@@ -16,7 +16,8 @@ namespace gt {
 struct Counters {
     unsigned int M;            // lattice vertices (written by the compaction scan)
     unsigned int flags;        // kFlag* bits
-    unsigned int pad[2];
+    unsigned int filled;       // occupied table slots at the compaction (also after a full table)
+    unsigned int pad;
     unsigned long long probes;   // profiling level 2 only
     unsigned long long inserts;
     unsigned long long atomics;
@@ -26,6 +27,7 @@ constexpr unsigned kFlagFull = 1u;    // an insert exceeded the probe limit: reg
 constexpr unsigned kFlagBad = 2u;     // NaN/Inf position or key outside int16
 constexpr unsigned kFlagRange = 4u;   // a key outside the packing plan: replan and redo
 constexpr unsigned kFlagSort = 8u;    // a radix-sort look-back gave up waiting (a bug, never expected)
+constexpr unsigned kFlagNonFinite = 16u;   // the values hold NaN/Inf (marks in TableArgs::tnf, not an error)
 
 // Where pixel positions and values come from.
 struct PixSrc {
@@ -86,6 +88,9 @@ struct OutDst {
 struct TableArgs {
     uint64_t *tkeys;               // capacity * W words
     unsigned long long *tacc;      // capacity * vd fixed-point accumulators (two's complement)
+    uint32_t *tnf;                 // capacity: NaN/Inf marks of each slot's sums, 2 bits per channel
+                                   // (bit 2c: a +Inf contribution, bit 2c+1: a -Inf one; both: NaN);
+                                   // zero between calls (the finalize clears what it reads)
     uint32_t mask;                 // capacity - 1
     int *slot_id;                  // capacity: slot -> vertex id (idshift == 0)
     int idshift;                   // > 0 (one-word keys with room to spare): after the finalize a
@@ -116,13 +121,29 @@ struct PartMap {
 
 // Fixed-point splat accumulation: channel c of a weighted value is added as
 // round(x * s[c]) to 64-bit integers, s[c] a power of two chosen so that no
-// vertex sum can exceed 2^62 in magnitude (|value| <= vmax[c], weights <= 1,
-// at most n contributions).  Integer sums are exact and independent of the
-// order of the atomics; finalize converts them back with inv_s[c].
+// vertex sum can exceed 2^62 in magnitude (|value| <= the channel's largest
+// finite magnitude in this call, weights <= 1, at most n contributions).
+// Integer sums are exact and independent of the order of the atomics;
+// finalize converts them back with inv_s[c].  The lattice pipeline computes
+// it on the device at the start of every call (k_value_scale), so the scales
+// depend on the call's own values only.  nonfinite: some value is NaN/Inf;
+// those contributions are kept as marks (TableArgs::tnf), not sums.
+//
+// BF16/FP16 value storage keeps each channel scaled by a power of two: the
+// values written by blur pass j are true * 2^e_j with e_j = 14 -
+// ceil(log2(m_j)), m_j the largest finite magnitude of the pass's input
+// (vmax_bits[j], as float bits: [0] the splat sums, [j] pass j-1's output).
+// The blur never raises the largest magnitude (its weights sum to at most 1),
+// so stored values stay below 2^14 and the smallest ones keep FP16's normal
+// range 28 binades further down, where d + 1 unscaled passes over a sparse
+// lattice would halve them into subnormals.  Powers of two scale exactly, so
+// the only rounding is the storage conversion; the slice (and an export)
+// undo e_D with the same exactness.
 struct ValScale {
     float s[kMaxDH];
     float inv_s[kMaxDH];
-    float vmax[kMaxDH];
+    unsigned nonfinite;
+    unsigned vmax_bits[kMaxDH + 2][kMaxDH];
 };
 
 // Launchers (gt_launch_*.hip, gt_sort.hip).  All are asynchronous on `st`.
@@ -134,27 +155,40 @@ hipError_t launch_minmax(const float *pa, int64_t csa, int na, const float *pb, 
                          float *partial, int nblk, float *out, hipStream_t st);
 hipError_t launch_bounds_batch(const BatchArgs &ba, int ntiles, int tile_w, int Cr, int C, float *partial, int nblk,
                                float *out, hipStream_t st);
-hipError_t launch_clear(uint64_t *tkeys, size_t nwords, unsigned long long *tacc, size_t nacc, Counters *ctr,
-                        hipStream_t st);
+// Largest finite magnitude and a non-finite indicator of each of the nval
+// value planes (n pixels each), then this call's fixed-point scales for vd
+// channels (a homogeneous channel of ones after the planes when homog) into
+// the device ValScale; kFlagNonFinite is raised in ctr when a value is NaN or
+// Inf.  partial holds nblk * nval * 3 floats.
+hipError_t launch_value_scale(const float *val, int64_t cs, int nval, int homog, int n, int vd, float *partial,
+                              int nblk, ValScale *out, Counters *ctr, hipStream_t st);
+// The same over the image planes of a batch (n: the batch's pixel count).
+hipError_t launch_value_scale_batch(const BatchArgs &ba, int ntiles, int tile_w, int C, int n, int vd,
+                                    float *partial, int nblk, ValScale *out, Counters *ctr, hipStream_t st);
+// Table words to kEmpty, nacc sums and nmarks non-finite marks to zero, the
+// counters to zero.
+hipError_t launch_clear(uint64_t *tkeys, size_t nwords, unsigned long long *tacc, size_t nacc, uint32_t *tnf,
+                        size_t nmarks, Counters *ctr, hipStream_t st);
 hipError_t launch_splat(int d, bool packed, bool bilat, bool dedup, const PixSrc &src, int n, int row_w, int tile_w,
-                        int vd, const LatConst &lc, const KeyPlan &kp, const ValScale &vsc, const TableArgs &ta,
+                        int vd, const LatConst &lc, const KeyPlan &kp, const ValScale *vsc, const TableArgs &ta,
                         Counters *ctr, bool count, int wpc, hipStream_t st);
 // Batched forms (bilateral, packed keys, d <= 6, vd <= 4): one workgroup per
 // tile, ntiles tiles over all images.  wpc: the waves-per-CU cap of the tiled
 // and neighbour kernels (0: none).
 hipError_t launch_splat_batch(int d, const PixSrc &tmpl, const BatchArgs &ba, int ntiles, int tile_w, int vd,
-                              const LatConst &lc, const KeyPlan &kp, const ValScale &vsc, const TableArgs &ta,
+                              const LatConst &lc, const KeyPlan &kp, const ValScale *vsc, const TableArgs &ta,
                               Counters *ctr, bool count, int wpc, hipStream_t st);
 hipError_t launch_slice_batch(int d, int storage, const PixSrc &tmpl, const BatchArgs &ba, int ntiles, int tile_w,
                               int vd, const LatConst &lc, const KeyPlan &kp, const TableArgs &ta, const void *vals,
-                              const OutDst &tmpl_dst, Counters *ctr, bool count, int wpc, hipStream_t st);
+                              const unsigned *sbits, const OutDst &tmpl_dst, Counters *ctr, bool count, int wpc,
+                              hipStream_t st);
 hipError_t launch_compact(int words, const TableArgs &ta, uint32_t capacity, uint32_t *counts, uint32_t *slot_list,
                           uint64_t *sort_keys, bool packed, Counters *ctr, hipStream_t st);
 hipError_t launch_pack(int d, const TableArgs &ta, const uint32_t *slot_list, const Counters *ctr, int m_max,
                        const KeyPlan &kp, int pw, uint64_t *pk, hipStream_t st);
 // sorted_keys: the sorted packed keys (one word each), or null to gather the
 // table's key words by slot.
-hipError_t launch_finalize(int words, int storage, const TableArgs &ta, const ValScale &vsc, const uint32_t *sorted,
+hipError_t launch_finalize(int words, int storage, const TableArgs &ta, const ValScale *vsc, const uint32_t *sorted,
                            const uint64_t *sorted_keys, const Counters *ctr, int m_max, int vd, int vs,
                            uint64_t *vkeys, void *vals, float *splat_copy, hipStream_t st);
 hipError_t launch_neighbors(int d, bool packed, const KeyPlan &kp, const uint64_t *vkeys, const Counters *ctr,
@@ -162,7 +196,7 @@ hipError_t launch_neighbors(int d, bool packed, const KeyPlan &kp, const uint64_
 // Compaction with the finalize fused in, for ids in compaction order (no
 // sort, no slot list): count, scan, then k_write_final.
 hipError_t launch_compact_final(int words, int storage, const TableArgs &ta, uint32_t capacity, uint32_t *counts,
-                                const ValScale &vsc, int vd, int vs, uint64_t *vkeys, void *vals, Counters *ctr,
+                                const ValScale *vsc, int vd, int vs, uint64_t *vkeys, void *vals, Counters *ctr,
                                 hipStream_t st);
 // The same table by hash lookups in the complete global table, for vertex ids
 // in compaction order (no sort).
@@ -172,18 +206,25 @@ hipError_t launch_compact_final(int words, int storage, const TableArgs &ta, uin
 hipError_t launch_neighbors_hash(int d, bool packed, const KeyPlan &kp, const uint64_t *vkeys, const Counters *ctr,
                                  int m_max, const TableArgs &ta, const PartMap &pm, bool per_vertex, int2 *nbr,
                                  hipStream_t st);
-hipError_t launch_blur(int storage, int vs, const int2 *nbr, const Counters *ctr, int m_max, const PartMap &pm,
-                       const void *in, void *out, hipStream_t st);
+// Blur pass `pass` (0..d): in holds FP32 for pass 0 of a narrow storage (the
+// finalize writes FP32 there) and the storage type otherwise; vsc supplies the
+// narrow storages' scales.
+hipError_t launch_blur(int storage, int vs, int pass, const ValScale *vsc, const int2 *nbr, const Counters *ctr,
+                       int m_max, const PartMap &pm, const void *in, void *out, hipStream_t st);
+// sbits: the scale source of a narrow storage's last blur pass
+// (ValScale::vmax_bits[d]); ignored for FP32.
 hipError_t launch_slice(int d, bool packed, bool bilat, bool tiled, int storage, const PixSrc &src, int n, int row_w,
                         int tile_w, int vd, int vs, const LatConst &lc, const KeyPlan &kp, const TableArgs &ta,
-                        const void *vals, const OutDst &dst, Counters *ctr, bool count, int wpc, hipStream_t st);
+                        const void *vals, const unsigned *sbits, const OutDst &dst, Counters *ctr, bool count,
+                        int wpc, hipStream_t st);
 hipError_t launch_unpack_keys(int d, bool packed, const KeyPlan &kp, const uint64_t *vkeys, int M, int16_t *keys,
                               hipStream_t st);
 // First vertex of each image in a sorted batched lattice (keys ordered by
 // their image tag): starts[i] for i in [0, count], starts[count] = M.
 hipError_t launch_tag_starts(const uint64_t *vkeys, int M, int coord_bits, int count, uint32_t *starts,
                              hipStream_t st);
-hipError_t launch_store_to_f32(int storage, const void *vals, int M, int vd, int vs, float *out, hipStream_t st);
+hipError_t launch_store_to_f32(int storage, const void *vals, const unsigned *sbits, int M, int vd, int vs,
+                               float *out, hipStream_t st);
 hipError_t launch_view_copy(const float *src, int64_t sys, int64_t sts, int64_t scs, float *dst, int64_t dys,
                             int64_t dts, int64_t dcs, int W, int H, int F, int C, hipStream_t st);
 

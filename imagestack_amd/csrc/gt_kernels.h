@@ -3,6 +3,8 @@
 // ids), neighbour table, blur passes and slice.  The launchers (gt_launch_*.hip)
 // map the runtime lattice dimension d and key format to template parameters.
 #pragma once
+#include <type_traits>
+
 #include "gt_device.h"
 #include "gt_internal.h"
 
@@ -226,9 +228,53 @@ constexpr size_t splat_tile_lds() { return (size_t)LT * W * 8 + (size_t)LVD * LT
 template <int W, int LVD, int LT>
 constexpr size_t slice_tile_lds() { return (size_t)LT * W * 8 + (size_t)LT * LVD * 4 + (size_t)LT * 2 + 16; }
 
+// NaN/Inf values (src/Permutohedral.h:377 adds barycentric * value in FP32,
+// so they spread through the sums): the fixed-point sums take such channels
+// as 0, and each of the pixel's vertices gets a mark per channel instead --
+// the class of the FP32 product weight * value: +Inf (bit 2c), -Inf (bit
+// 2c+1) or NaN (both bits; a +Inf and a -Inf term also sum to NaN).  The
+// finalize turns a marked sum into NaN, +Inf or -Inf.  Only calls whose value
+// bounds saw a NaN/Inf take this path.
+template <int D, bool PACKED, bool BATCH>
+__device__ __forceinline__ void mark_nonfinite(const Simplex<D> &s, const KeyPlan &kp, uint64_t tag, const TableArgs &ta,
+                                            const float (&v)[kMaxVD], unsigned nfmask, int vd, uint32_t &probes,
+                                            unsigned &flags) {
+    using K = KeyOps<D, PACKED>;
+    constexpr int W = K::W;
+    for (int r = 0; r <= D; r++) {
+        uint64_t k[W];
+        if (!K::make(s, r, kp, tag, k)) continue;   // reported by the caller
+        const int slot = table_insert<W>(ta.tkeys, ta.mask, k, home_slot<D, PACKED, BATCH>(ta, kp, k), probes);
+        if (slot < 0) { flags |= kFlagFull; continue; }
+        uint32_t bits = 0;
+        for (int c = 0; c < vd; c++) {
+            if (!((nfmask >> c) & 1u)) continue;
+            const float prod = s.b[r] * v[c];
+            const uint32_t cls = (prod != prod) ? 3u : (prod > 0.0f ? 1u : 2u);
+            bits |= cls << (2 * c);
+        }
+        if (bits) atomicOr(ta.tnf + slot, bits);
+    }
+}
+
+// Splits a pixel's values into what the fixed-point sums take (NaN/Inf as 0)
+// and a mask of the non-finite channels (the originals stay in v).
+template <int NV>
+__device__ __forceinline__ unsigned split_nonfinite(const float (&v)[kMaxVD], int vd, float (&fin)[kMaxVD]) {
+    unsigned m = 0;
+#pragma unroll
+    for (int c = 0; c < NV; c++) {
+        const bool bad = c < vd && !(fabsf(v[c]) <= 3.402823466e38f);
+        m |= bad ? (1u << c) : 0u;
+        fin[c] = bad ? 0.0f : v[c];
+    }
+    return m;
+}
+
 template <int D, bool PACKED, bool BILAT, int LVD, int LT, bool BATCH>
 __global__ __launch_bounds__(kBlock) void k_splat_tile(PixSrc src0, int n0, int row_w0, int tile_w, BatchArgs ba,
-                                                       int vd, LatConst lc, KeyPlan kp, ValScale vsc, TableArgs ta,
+                                                       int vd, LatConst lc, KeyPlan kp,
+                                                       const ValScale *__restrict__ vsc, TableArgs ta,
                                                        Counters *ctr, int count) {
     using K = KeyOps<D, PACKED>;
     constexpr int W = K::W;
@@ -249,18 +295,27 @@ __global__ __launch_bounds__(kBlock) void k_splat_tile(PixSrc src0, int n0, int 
     const bool full = *(volatile const unsigned *)&ctr->flags & kFlagFull;
     __syncthreads();
 
+    float sc[LVD];
+#pragma unroll
+    for (int c = 0; c < LVD; c++) sc[c] = vsc->s[c];
+    const bool nfcall = vsc->nonfinite != 0;
+
     if (p >= 0) {
-        float pos[D], v[kMaxVD];
+        float pos[D], v[kMaxVD], vorig[kMaxVD];
         load_position<D, BILAT>(src, p, pos);
         load_values<LVD>(src, p, v);
+        unsigned nfmask = 0;
+        if (nfcall) {
 #pragma unroll
-        for (int c = 0; c < LVD; c++)
-            if (c < vd && !(fabsf(v[c]) <= vsc.vmax[c])) flags |= kFlagRange;
+            for (int c = 0; c < LVD; c++) vorig[c] = v[c];
+            nfmask = split_nonfinite<LVD>(vorig, vd, v);
+        }
         Simplex<D> s;
         simplex<D>(pos, lc.sf, s);
         if (s.bad) {
             flags |= kFlagBad;
         } else {
+            if (nfmask && !full) mark_nonfinite<D, PACKED, BATCH>(s, kp, tv.tag, ta, vorig, nfmask, vd, probes, flags);
             // Lane i visits the vertices starting at rank i mod (D+1): lanes of a
             // wave that share a key (neighbouring pixels, same rank) update it in
             // different instructions, which removes most LDS atomic conflicts.
@@ -292,7 +347,7 @@ __global__ __launch_bounds__(kBlock) void k_splat_tile(PixSrc src0, int n0, int 
                 if (ls >= 0) {
 #pragma unroll
                     for (int c = 0; c < LVD; c++)
-                        if (c < vd) atomicAdd(&lv[c * LT + ls], to_fixed(wt[r] * v[c], vsc.s[c]));
+                        if (c < vd) atomicAdd(&lv[c * LT + ls], to_fixed(wt[r] * v[c], sc[c]));
                 } else if (!full) {
                     const int slot = table_insert<W>(ta.tkeys, ta.mask, k, home_slot<D, PACKED, BATCH>(ta, kp, k), probes);
                     inserts++;
@@ -300,7 +355,7 @@ __global__ __launch_bounds__(kBlock) void k_splat_tile(PixSrc src0, int n0, int 
                     unsigned long long *dst = ta.tacc + (size_t)slot * vd;
 #pragma unroll
                     for (int c = 0; c < LVD; c++)
-                        if (c < vd) atomicAdd(dst + c, to_fixed(wt[r] * v[c], vsc.s[c]));
+                        if (c < vd) atomicAdd(dst + c, to_fixed(wt[r] * v[c], sc[c]));
                     atomics += vd;
                 }
             }
@@ -344,7 +399,8 @@ __global__ __launch_bounds__(kBlock) void k_splat_tile(PixSrc src0, int n0, int 
 // integers.  Workgroups take the tiles in XCD bands, as the tiled form does.
 template <int D, bool PACKED, bool BILAT>
 __global__ __launch_bounds__(kBlock) void k_splat_direct(PixSrc src, int n, int row_w, int tile_w, int vd,
-                                                         LatConst lc, KeyPlan kp, ValScale vsc, TableArgs ta,
+                                                         LatConst lc, KeyPlan kp,
+                                                         const ValScale *__restrict__ vsc, TableArgs ta,
                                                          Counters *ctr, int count) {
     using K = KeyOps<D, PACKED>;
     constexpr int W = K::W;
@@ -353,17 +409,21 @@ __global__ __launch_bounds__(kBlock) void k_splat_direct(PixSrc src, int n, int 
     uint32_t probes = 0, inserts = 0, atomics = 0;
     const bool full = *(volatile const unsigned *)&ctr->flags & kFlagFull;
     if (p >= 0 && !full) {
-        float pos[D], v[kMaxVD];
+        float pos[D], v[kMaxVD], vorig[kMaxVD];
         load_position<D, BILAT>(src, p, pos);
         load_values<kMaxVD>(src, p, v);
+        unsigned nfmask = 0;
+        if (vsc->nonfinite) {
 #pragma unroll
-        for (int c = 0; c < kMaxVD; c++)
-            if (c < vd && !(fabsf(v[c]) <= vsc.vmax[c])) flags |= kFlagRange;
+            for (int c = 0; c < kMaxVD; c++) vorig[c] = v[c];
+            nfmask = split_nonfinite<kMaxVD>(vorig, vd, v);
+        }
         Simplex<D> s;
         simplex<D>(pos, lc.sf, s);
         if (s.bad) {
             flags |= kFlagBad;
         } else {
+            if (nfmask) mark_nonfinite<D, PACKED, false>(s, kp, 0ull, ta, vorig, nfmask, vd, probes, flags);
 #pragma unroll
             for (int r = 0; r <= D; r++) {
                 uint64_t k[W];
@@ -374,7 +434,7 @@ __global__ __launch_bounds__(kBlock) void k_splat_direct(PixSrc src, int n, int 
                 unsigned long long *dst = ta.tacc + (size_t)slot * vd;
 #pragma unroll
                 for (int c = 0; c < kMaxVD; c++)
-                    if (c < vd) atomicAdd(dst + c, to_fixed(s.b[r] * v[c], vsc.s[c]));
+                    if (c < vd) atomicAdd(dst + c, to_fixed(s.b[r] * v[c], vsc->s[c]));
                 atomics += vd;
             }
         }
@@ -445,14 +505,40 @@ __global__ __launch_bounds__(kBlock) void k_write_list(const uint64_t *__restric
 // key copied to vkeys[pos], the sums converted to vals[pos] by the same
 // expression and set back to zero, slot -> id recorded (in slot_id, or
 // folded into the key word).  After a full table (M = 0) nothing is written.
+// The sums of slot s in FP32 (one rounding: int64 -> double is exact below
+// 2^53, then one FP32 rounding), NaN/Inf where the slot holds non-finite
+// marks; the sums and marks read are set back to zero.
+__device__ __forceinline__ void take_sums(const TableArgs &ta, const ValScale *__restrict__ vsc, size_t s, int vd,
+                                          float (&o)[kMaxVD]) {
+#pragma unroll
+    for (int c = 0; c < kMaxVD; c++)
+        o[c] = (c < vd) ? (float)((double)(long long)ta.tacc[s * vd + c] * (double)vsc->inv_s[c]) : 0.0f;
+    for (int c = 0; c < vd; c++) ta.tacc[s * vd + c] = 0ull;
+    if (vsc->nonfinite) {
+        const uint32_t m = ta.tnf[s];
+        if (m) {
+#pragma unroll
+            for (int c = 0; c < kMaxVD; c++) {
+                const uint32_t b = (m >> (2 * c)) & 3u;
+                o[c] = b == 3u ? __builtin_nanf("") : b == 1u ? INFINITY : b == 2u ? -INFINITY : o[c];
+            }
+            ta.tnf[s] = 0u;
+        }
+    }
+}
+
 template <int W, typename T>
 __global__ __launch_bounds__(kBlock) void k_write_final(TableArgs ta, uint32_t cap,
                                                         const uint32_t *__restrict__ offsets, const Counters *ctr,
-                                                        ValScale vsc, int vd, int vs, uint64_t *__restrict__ vkeys,
-                                                        T *__restrict__ vals) {
+                                                        const ValScale *__restrict__ vsc, int vd, int vs,
+                                                        uint64_t *__restrict__ vkeys, T *__restrict__ vals,
+                                                        unsigned *__restrict__ track) {
     constexpr int NW = kBlock / kWave;
     __shared__ unsigned ws[NW];
     if (*(volatile const unsigned *)&ctr->flags & kFlagFull) return;   // the same for every thread
+    float mx[kMaxVD];   // track: largest finite magnitude per channel (a narrow storage's first scale)
+#pragma unroll
+    for (int c = 0; c < kMaxVD; c++) mx[c] = 0.0f;
     const size_t base = (size_t)blockIdx.x * kCompactSlots;
     unsigned run = offsets[blockIdx.x];
     const int wave = threadIdx.x / kWave;
@@ -476,17 +562,16 @@ __global__ __launch_bounds__(kBlock) void k_write_final(TableArgs ta, uint32_t c
 #pragma unroll
             for (int w = 1; w < W; w++) vkeys[(size_t)pos * W + w] = ta.tkeys[s * W + w];
             float o[kMaxVD];
-#pragma unroll
-            for (int c = 0; c < kMaxVD; c++)
-                o[c] = (c < vd) ? (float)((double)(long long)ta.tacc[s * vd + c] * (double)vsc.inv_s[c]) : 0.0f;
-            for (int c = 0; c < vd; c++) ta.tacc[s * vd + c] = 0ull;
+            take_sums(ta, vsc, s, vd, o);
             store_row_rt<T>(vals, pos, vs, o);
+            if (track) take_max<kMaxVD>(mx, o);
             if (W == 1 && ta.idshift > 0) ta.tkeys[s] = w0 | ((uint64_t)pos << ta.idshift);
             else ta.slot_id[s] = (int)pos;
         }
         run += tot;
         __syncthreads();
     }
+    if (track) publish_max<kMaxVD>(track, mx, vd);
 }
 
 // Wide keys -> packed sort key (big integer of pw words, word 0 least
@@ -529,19 +614,19 @@ __global__ __launch_bounds__(kBlock) void k_pack(TableArgs ta, const uint32_t *_
 // finalize: position i of the sorted (or, without a sort, the compaction)
 // order -> vertex id i.  Records slot -> id (in slot_id, or folded into the
 // table word above the key when ta.idshift > 0), copies the key, and
-// converts the fixed-point splat sums to the storage type (one rounding:
-// int64 -> double is exact below 2^53, then one FP32 rounding).  The sums it
-// has read are set back to zero, so after a complete call the whole sum
-// array is zero again and the next call need not clear it.
+// converts the fixed-point splat sums to FP32 rows (one rounding: int64 ->
+// double is exact below 2^53, then one FP32 rounding; a BF16/FP16 storage
+// takes its first rounding in blur pass 0, scaled by the largest magnitude
+// tracked here).  The sums it has read are set back to zero, so after a
+// complete call the whole sum array is zero again and the next call need not
+// clear it.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_finalize(TableArgs ta, ValScale vsc, int kw,
-                                                     const uint32_t *__restrict__ sorted,
-                                                     const uint64_t *__restrict__ sorted_keys, const Counters *ctr,
-                                                     int vd, int vs, uint64_t *__restrict__ vkeys,
-                                                     T *__restrict__ vals, float *__restrict__ splat_copy) {
-    const unsigned M = ctr->M;
-    const unsigned i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= M) return;
+__device__ __forceinline__ void finalize_one(TableArgs &ta, const ValScale *__restrict__ vsc, int kw,
+                                             const uint32_t *__restrict__ sorted,
+                                             const uint64_t *__restrict__ sorted_keys, int vd, int vs,
+                                             uint64_t *__restrict__ vkeys, T *__restrict__ vals,
+                                             float *__restrict__ splat_copy, unsigned i, float (&mx)[kMaxVD],
+                                             bool track) {
     const uint32_t s = sorted[i];
     if (sorted_keys) {   // packed keys: the sort's own output, read in order
         const uint64_t key = sorted_keys[i];
@@ -553,14 +638,29 @@ __global__ __launch_bounds__(kBlock) void k_finalize(TableArgs ta, ValScale vsc,
     }
     if (ta.idshift == 0) ta.slot_id[s] = (int)i;
     float o[kMaxVD];
-#pragma unroll
-    for (int c = 0; c < kMaxVD; c++)
-        o[c] = (c < vd) ? (float)((double)(long long)ta.tacc[(size_t)s * vd + c] * (double)vsc.inv_s[c]) : 0.0f;
-    for (int c = 0; c < vd; c++) ta.tacc[(size_t)s * vd + c] = 0ull;
+    take_sums(ta, vsc, s, vd, o);
     store_row_rt<T>(vals, i, vs, o);
+    if (track) take_max<kMaxVD>(mx, o);
     if (splat_copy)
         for (int c = 0; c < vd; c++) splat_copy[(size_t)i * vd + c] = o[c];
 }
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_finalize(TableArgs ta, const ValScale *__restrict__ vsc, int kw,
+                                                     const uint32_t *__restrict__ sorted,
+                                                     const uint64_t *__restrict__ sorted_keys, const Counters *ctr,
+                                                     int vd, int vs, uint64_t *__restrict__ vkeys,
+                                                     T *__restrict__ vals, float *__restrict__ splat_copy,
+                                                     unsigned *__restrict__ track) {
+    const unsigned M = ctr->M;
+    const unsigned i = blockIdx.x * kBlock + threadIdx.x;
+    float mx[kMaxVD];   // track: largest finite magnitude per channel (a narrow storage's first scale)
+#pragma unroll
+    for (int c = 0; c < kMaxVD; c++) mx[c] = 0.0f;
+    if (i < M) finalize_one(ta, vsc, kw, sorted, sorted_keys, vd, vs, vkeys, vals, splat_copy, i, mx, track != nullptr);
+    if (track) publish_max<kMaxVD>(track, mx, vd);
+}
+
 
 // Neighbour table from the sorted keys, without the hash table.  Moving a
 // key by +-(1,...,1) -+ (D+1) e_j moves every field of its words by a
@@ -958,24 +1058,49 @@ __global__ __launch_bounds__(kBlock) void k_nbr_hash_vertex(const uint64_t *__re
 
 // One blur pass along one axis (src/Permutohedral.h:442-475): Jacobi, new =
 // 0.25*a + 0.5*self + 0.25*b evaluated left to right, missing neighbour = 0.
-template <typename T, int VS>
+// SCALED (BF16/FP16 storage): the input holds true * 2^e_in (in_bits: its
+// scale source; none for the FP32 output of the finalize), the output is
+// written as true * 2^e_out (out_bits: the largest magnitude of this pass's
+// input), and the output's own largest finite magnitude goes to out_max for
+// the next pass.  The FP32 arithmetic is on the true values; the power-of-two
+// factors are exact.
+template <typename TI, typename TO, int VS, bool SCALED>
 __global__ __launch_bounds__(kBlock) void k_blur(const int2 *__restrict__ nbr, const Counters *ctr, PartMap pm,
-                                                 const T *__restrict__ in, T *__restrict__ out) {
+                                                 const TI *__restrict__ in, TO *__restrict__ out,
+                                                 const unsigned *__restrict__ in_bits,
+                                                 const unsigned *__restrict__ out_bits, unsigned *__restrict__ out_max) {
     const unsigned M = ctr->M;
     unsigned i, end, step;
     part_walk(pm, M, blockIdx.x, gridDim.x, i, end, step);
+    float ui[kMaxVD], so[kMaxVD], mx[kMaxVD];
+#pragma unroll
+    for (int c = 0; c < VS; c++) {
+        ui[c] = (SCALED && in_bits) ? ldexpf(1.0f, -store_exp(in_bits[c])) : 1.0f;
+        so[c] = SCALED ? ldexpf(1.0f, store_exp(out_bits[c])) : 1.0f;
+        mx[c] = 0.0f;
+    }
     for (; i < end; i += step) {
         const int2 nb = nbr[i];
         float a[kMaxVD], o[kMaxVD], b[kMaxVD], r[kMaxVD];
 #pragma unroll
         for (int c = 0; c < VS; c++) { a[c] = 0.0f; b[c] = 0.0f; }
-        if (nb.x >= 0) load_row<T, VS>(in, nb.x, a);
-        load_row<T, VS>(in, i, o);
-        if (nb.y >= 0) load_row<T, VS>(in, nb.y, b);
+        if (nb.x >= 0) load_row<TI, VS>(in, nb.x, a);
+        load_row<TI, VS>(in, i, o);
+        if (nb.y >= 0) load_row<TI, VS>(in, nb.y, b);
+        if (SCALED) {
+#pragma unroll
+            for (int c = 0; c < VS; c++) { a[c] *= ui[c]; o[c] *= ui[c]; b[c] *= ui[c]; }
+        }
 #pragma unroll
         for (int c = 0; c < VS; c++) r[c] = (0.25f * a[c] + 0.5f * o[c]) + 0.25f * b[c];
-        store_row<T, VS>(out, i, r);
+        if (SCALED) {
+            take_max<VS>(mx, r);
+#pragma unroll
+            for (int c = 0; c < VS; c++) r[c] *= so[c];
+        }
+        store_row<TO, VS>(out, i, r);
     }
+    if (SCALED) publish_max<VS>(out_max, mx, VS);
 }
 
 // Final write of one pixel: raw vd channels, or the first vd-1 divided by the
@@ -996,6 +1121,15 @@ __device__ __forceinline__ void write_pixel(const OutDst &dst, int p, int vd, co
     }
 }
 
+// 2^-e of each channel of a narrow storage's last blur pass (sbits: its
+// scale source), 1 for FP32 storage.
+template <typename T, int NV>
+__device__ __forceinline__ void slice_unscale(const unsigned *__restrict__ sbits, float (&u)[kMaxVD]) {
+#pragma unroll
+    for (int c = 0; c < NV; c++)
+        u[c] = (std::is_same<T, float>::value || !sbits) ? 1.0f : ldexpf(1.0f, -store_exp(sbits[c]));
+}
+
 // Slice, tiled form (src/Permutohedral.h:405-427): the tile's distinct
 // vertices are looked up once and their blurred values staged in LDS
 // (channel-major); each pixel then sums weight * value over its D+1 vertices
@@ -1005,10 +1139,12 @@ __device__ __forceinline__ void write_pixel(const OutDst &dst, int p, int vd, co
 template <int D, bool PACKED, bool BILAT, typename T, int LVD, int LT, bool BATCH>
 __global__ __launch_bounds__(kBlock) void k_slice_tile(PixSrc src0, int n0, int row_w0, int tile_w, BatchArgs ba,
                                                        int vd, LatConst lc, KeyPlan kp, TableArgs ta,
-                                                       const T *__restrict__ vals, OutDst dst0, Counters *ctr,
-                                                       int count) {
+                                                       const T *__restrict__ vals,
+                                                       const unsigned *__restrict__ sbits, OutDst dst0,
+                                                       Counters *ctr, int count) {
     using K = KeyOps<D, PACKED>;
     constexpr int W = K::W;
+    constexpr bool kScaled = !std::is_same<T, float>::value;
     __shared__ uint64_t lk[LT * W];
     __shared__ float lv[LT * LVD];
     __shared__ uint16_t llist[LT];
@@ -1070,6 +1206,8 @@ __global__ __launch_bounds__(kBlock) void k_slice_tile(PixSrc src0, int n0, int 
     __syncthreads();
 
     const unsigned cnt = lcount;
+    float un[kMaxVD];
+    slice_unscale<T, LVD>(sbits, un);
     for (unsigned j = threadIdx.x; j < cnt; j += kBlock) {
         const int ls = llist[j];
         uint64_t k[W];
@@ -1081,7 +1219,7 @@ __global__ __launch_bounds__(kBlock) void k_slice_tile(PixSrc src0, int n0, int 
         for (int c = 0; c < LVD; c++) row[c] = 0.0f;
         if (id >= 0) load_row<T, LVD>(vals, id, row);
 #pragma unroll
-        for (int c = 0; c < LVD; c++) lv[c * LT + ls] = row[c];
+        for (int c = 0; c < LVD; c++) lv[c * LT + ls] = kScaled ? row[c] * un[c] : row[c];
     }
     __syncthreads();
 
@@ -1102,7 +1240,7 @@ __global__ __launch_bounds__(kBlock) void k_slice_tile(PixSrc src0, int n0, int 
                     float row[kMaxVD];
                     load_row<T, LVD>(vals, id, row);
 #pragma unroll
-                    for (int c = 0; c < LVD; c++) acc[c] = acc[c] + s.b[r] * row[c];
+                    for (int c = 0; c < LVD; c++) acc[c] = acc[c] + s.b[r] * (row[c] * un[c]);
                 }
             }
         }
@@ -1116,7 +1254,8 @@ __global__ __launch_bounds__(kBlock) void k_slice_tile(PixSrc src0, int n0, int 
 // in XCD bands.
 template <int D, bool PACKED, bool BILAT, typename T>
 __global__ __launch_bounds__(kBlock) void k_slice_direct(PixSrc src, int n, int vd, int vs, LatConst lc, KeyPlan kp,
-                                                         TableArgs ta, const T *__restrict__ vals, OutDst dst,
+                                                         TableArgs ta, const T *__restrict__ vals,
+                                                         const unsigned *__restrict__ sbits, OutDst dst,
                                                          Counters *ctr, int count) {
     using K = KeyOps<D, PACKED>;
     constexpr int W = K::W;
@@ -1143,6 +1282,12 @@ __global__ __launch_bounds__(kBlock) void k_slice_direct(PixSrc src, int n, int 
                 if (id < 0) continue;
                 float v[kMaxVD];
                 load_row_rt<T>(vals, id, vs, v);
+                if (!std::is_same<T, float>::value) {
+                    float un[kMaxVD];
+                    slice_unscale<T, kMaxVD>(sbits, un);
+#pragma unroll
+                    for (int c = 0; c < kMaxVD; c++) v[c] *= un[c];
+                }
 #pragma unroll
                 for (int c = 0; c < kMaxVD; c++) acc[c] = acc[c] + s.b[r] * v[c];
             }
@@ -1168,11 +1313,15 @@ __global__ __launch_bounds__(kBlock) void k_unpack_keys(KeyPlan kp, const uint64
 }
 
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_store_to_f32(const T *__restrict__ vals, int M, int vd, int vs,
-                                                         float *__restrict__ out) {
+__global__ __launch_bounds__(kBlock) void k_store_to_f32(const T *__restrict__ vals, const unsigned *__restrict__ sbits,
+                                                         int M, int vd, int vs, float *__restrict__ out) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= M) return;
-    for (int c = 0; c < vd; c++) out[(size_t)i * vd + c] = to_f32<T>(vals[(size_t)i * vs + c]);
+    float un[kMaxVD];
+    slice_unscale<T, kMaxVD>(sbits, un);
+#pragma unroll
+    for (int c = 0; c < kMaxVD; c++)
+        if (c < vd) out[(size_t)i * vd + c] = to_f32<T>(vals[(size_t)i * vs + c]) * un[c];
 }
 
 } // namespace gt

@@ -52,15 +52,19 @@ inline int64_t pixel_blocks(int n, int row_w, int &tile_w) {
 
 // Splat launcher for d = 7..10 with the LDS tile table (gt_launch_mid.hip).
 hipError_t launch_splat_tile_mid(int d, bool packed, bool bilat, const PixSrc &src, int n, int row_w, int tile_w,
-                                 int vd, const LatConst &lc, const KeyPlan &kp, const ValScale &vsc,
+                                 int vd, const LatConst &lc, const KeyPlan &kp, const ValScale *vsc,
                                  const TableArgs &ta, Counters *ctr, bool count, int wpc, hipStream_t st);
 
 // Launchers for the lattice dimensions without an LDS tile table (d > 6).
 hipError_t launch_splat_direct_hi(int d, bool packed, bool bilat, const PixSrc &src, int n, int row_w, int tile_w,
-                                  int vd, const LatConst &lc, const KeyPlan &kp, const ValScale &vsc,
+                                  int vd, const LatConst &lc, const KeyPlan &kp, const ValScale *vsc,
                                   const TableArgs &ta, Counters *ctr, bool count, hipStream_t st);
 hipError_t launch_slice_direct_hi(int d, bool packed, bool bilat, int storage, const PixSrc &src, int n, int vd,
                                   int vs, const LatConst &lc, const KeyPlan &kp, const TableArgs &ta, const void *vals,
-                                  const OutDst &dst, Counters *ctr, bool count, hipStream_t st);
+                                  const unsigned *sbits, const OutDst &dst, Counters *ctr, bool count, hipStream_t st);
+hipError_t launch_slice_direct_top(int d, bool packed, bool bilat, int storage, const PixSrc &src, int n, int vd,
+                                   int vs, const LatConst &lc, const KeyPlan &kp, const TableArgs &ta,
+                                   const void *vals, const unsigned *sbits, const OutDst &dst, Counters *ctr, bool count,
+                                   hipStream_t st);
 
 } // namespace gt

@@ -155,14 +155,17 @@ hipError_t launch_bounds_batch(const BatchArgs &ba, int ntiles, int tile_w, int 
 }
 
 // ---------------------------------------------------------------------------
-// clear: table words to kEmpty, accumulators and counters to zero (a fill
-// kernel rather than hipMemsetAsync, so the sequence can be graph-captured).
+// clear: table words to kEmpty, sums, non-finite marks and counters to zero
+// (a fill kernel rather than hipMemsetAsync, so the sequence can be
+// graph-captured).
 __global__ __launch_bounds__(kBlock) void k_clear(uint64_t *__restrict__ tkeys, size_t nwords,
-                                                  unsigned long long *__restrict__ tacc, size_t nacc, Counters *ctr) {
+                                                  unsigned long long *__restrict__ tacc, size_t nacc,
+                                                  uint32_t *__restrict__ tnf, size_t nmarks, Counters *ctr) {
     const size_t stride = (size_t)gridDim.x * kBlock;
     const size_t t0 = (size_t)blockIdx.x * kBlock + threadIdx.x;
     for (size_t i = t0; i < nwords; i += stride) tkeys[i] = kEmpty;
     for (size_t i = t0; i < nacc; i += stride) tacc[i] = 0ull;
+    for (size_t i = t0; i < nmarks; i += stride) tnf[i] = 0u;
     if (t0 == 0) {
         ctr->M = 0;
         ctr->flags = 0;
@@ -172,11 +175,129 @@ __global__ __launch_bounds__(kBlock) void k_clear(uint64_t *__restrict__ tkeys, 
     }
 }
 
+// ---------------------------------------------------------------------------
+// Value bounds of this call, for the fixed-point scales.  Per workgroup and
+// value plane: the largest finite magnitude and a non-finite indicator, as
+// partial[blk][plane][3] = {-max, max, nonfinite} (the layout k_minmax and
+// k_bounds_batch write: min, max, indicator).  Four loads in flight per lane,
+// 16-byte ones when the planes allow.
+__global__ __launch_bounds__(kBlock) void k_value_bounds(const float *__restrict__ val, int64_t cs, int nval,
+                                                         int n, float *__restrict__ partial) {
+    __shared__ float smx[kBlock / kWave], snf[kBlock / kWave];
+    const bool vec = ((uintptr_t)val % 16) == 0 && (cs % 4) == 0;
+    const int64_t tid = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t nth = (int64_t)gridDim.x * kBlock;
+    for (int c = 0; c < nval; c++) {
+        const float *pl = val + (int64_t)c * cs;
+        float mx = 0.0f, nf = 0.0f;
+        auto take = [&](float v) {
+            const float a = fabsf(v);
+            const bool fin = a <= 3.402823466e38f;   // false for NaN and Inf
+            mx = fin ? fmaxf(mx, a) : mx;
+            nf = fin ? nf : 1.0f;
+        };
+        int64_t done = 0;
+        if (vec) {
+            const float4 *p4 = reinterpret_cast<const float4 *>(pl);
+            const int64_t n4 = n / 4;
+            int64_t i = tid;
+            for (; i + 3 * nth < n4; i += 4 * nth) {
+                float4 q[4];
+#pragma unroll
+                for (int k = 0; k < 4; k++) q[k] = p4[i + k * nth];
+#pragma unroll
+                for (int k = 0; k < 4; k++) { take(q[k].x); take(q[k].y); take(q[k].z); take(q[k].w); }
+            }
+            for (; i < n4; i += nth) {
+                const float4 q = p4[i];
+                take(q.x); take(q.y); take(q.z); take(q.w);
+            }
+            done = n4 * 4;
+        }
+        for (int64_t i = done + tid; i < n; i += nth) take(pl[i]);
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            mx = fmaxf(mx, __shfl_xor(mx, off, kWave));
+            nf = fmaxf(nf, __shfl_xor(nf, off, kWave));
+        }
+        if (__lane_id() == 0) {
+            smx[threadIdx.x / kWave] = mx;
+            snf[threadIdx.x / kWave] = nf;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int w = 1; w < kBlock / kWave; w++) {
+                mx = fmaxf(mx, smx[w]);
+                nf = fmaxf(nf, snf[w]);
+            }
+            float *o = partial + ((size_t)blockIdx.x * nval + c) * 3;
+            o[0] = -mx;
+            o[1] = mx;
+            o[2] = nf;
+        }
+        __syncthreads();
+    }
+}
+
+// This call's fixed-point scales from the per-workgroup bounds of the nval
+// value planes (a workgroup that saw no finite value holds min > max); a
+// homogeneous channel (homog) has magnitude 1.  For n contributions of
+// magnitude <= vmax the scale is 2^p with p = 62 - ceil(log2(n vmax 1.001)),
+// clamped to [-120, 120]: every sum stays below 2^62.  ceil(log2) is exact
+// through frexp.  One workgroup, a wave per channel.
+__global__ __launch_bounds__(kBlock) void k_value_scale(const float *__restrict__ partial, int nblk, int nval,
+                                                        int homog, int n, int vd, ValScale *__restrict__ out,
+                                                        Counters *ctr) {
+    __shared__ unsigned s_nf;
+    if (threadIdx.x == 0) s_nf = 0;
+    // a narrow storage's per-pass magnitudes start from zero in every call
+    for (int k = threadIdx.x; k < (kMaxDH + 2) * kMaxDH; k += kBlock) (&out->vmax_bits[0][0])[k] = 0u;
+    __syncthreads();
+    const int lane = __lane_id();
+    for (int c = threadIdx.x / kWave; c < kMaxDH; c += kBlock / kWave) {
+        float vmax = 0.0f, nf = 0.0f;
+        if (c < nval) {
+            for (int b = lane; b < nblk; b += kWave) {
+                const float *o = partial + ((size_t)b * nval + c) * 3;
+                if (o[0] <= o[1]) vmax = fmaxf(vmax, fmaxf(-o[0], o[1]));
+                nf = fmaxf(nf, o[2]);
+            }
+        } else if (c == nval && homog) {
+            vmax = 1.0f;
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            vmax = fmaxf(vmax, __shfl_xor(vmax, off, kWave));
+            nf = fmaxf(nf, __shfl_xor(nf, off, kWave));
+        }
+        if (lane == 0) {
+            float s = 0.0f, inv = 0.0f;
+            if (c < vd) {
+                const double bound = fmax(1e-30, (double)n * (double)vmax * 1.001);
+                int e = 0;
+                const double f = frexp(bound, &e);   // bound = f 2^e, f in [0.5, 1)
+                const int lg = (f == 0.5) ? e - 1 : e;   // ceil(log2(bound))
+                const int p = min(120, max(-120, 62 - lg));
+                s = ldexpf(1.0f, p);
+                inv = ldexpf(1.0f, -p);
+            }
+            out->s[c] = s;
+            out->inv_s[c] = inv;
+            if (nf != 0.0f) atomicOr(&s_nf, 1u);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        out->nonfinite = s_nf;
+        if (s_nf) atomicOr(&ctr->flags, kFlagNonFinite);
+    }
+}
+
 // In-place exclusive scan of n counts in one workgroup; the total goes to
 // *total, or 0 when `flags` carries kFlagFull (the call is redone with a
 // larger table, so everything after the compaction sees no vertices).
 __global__ __launch_bounds__(1024) void k_scan_1wg(uint32_t *__restrict__ a, int n, unsigned *total,
-                                                   const unsigned *flags) {
+                                                   const unsigned *flags, unsigned *filled) {
     __shared__ uint32_t part[1024];
     const int chunk = (n + 1023) / 1024;
     const int lo = threadIdx.x * chunk;
@@ -208,7 +329,10 @@ __global__ __launch_bounds__(1024) void k_scan_1wg(uint32_t *__restrict__ a, int
             run += v[q];
         }
     }
-    if (threadIdx.x == 1023 && total) *total = (flags && (*flags & kFlagFull)) ? 0u : part[1023];
+    if (threadIdx.x == 1023 && total) {
+        *total = (flags && (*flags & kFlagFull)) ? 0u : part[1023];
+        if (filled) *filled = part[1023];
+    }
 }
 
 // Strided view -> strided view copy, used to gather the non-dense views of
@@ -236,13 +360,34 @@ hipError_t launch_minmax(const float *pa, int64_t csa, int na, const float *pb, 
     return hipGetLastError();
 }
 
-hipError_t launch_clear(uint64_t *tkeys, size_t nwords, unsigned long long *tacc, size_t nacc, Counters *ctr,
-                        hipStream_t st) {
-    const size_t m = nwords > nacc ? nwords : nacc;
+hipError_t launch_clear(uint64_t *tkeys, size_t nwords, unsigned long long *tacc, size_t nacc, uint32_t *tnf,
+                        size_t nmarks, Counters *ctr, hipStream_t st) {
+    size_t m = nwords > nacc ? nwords : nacc;
+    if (nmarks > m) m = nmarks;
     unsigned g = blocks_for((int64_t)m, kBlock);
     if (g > 8192) g = 8192;
     if (g == 0) g = 1;
-    hipLaunchKernelGGL(k_clear, dim3(g), dim3(kBlock), 0, st, tkeys, nwords, tacc, nacc, ctr);
+    hipLaunchKernelGGL(k_clear, dim3(g), dim3(kBlock), 0, st, tkeys, nwords, tacc, nacc, tnf, nmarks, ctr);
+    return hipGetLastError();
+}
+
+hipError_t launch_value_scale(const float *val, int64_t cs, int nval, int homog, int n, int vd, float *partial,
+                              int nblk, ValScale *out, Counters *ctr, hipStream_t st) {
+    if (nval > 0 && n > 0)
+        hipLaunchKernelGGL(k_value_bounds, dim3(nblk), dim3(kBlock), 0, st, val, cs, nval, n, partial);
+    else
+        nval = 0;
+    hipLaunchKernelGGL(k_value_scale, dim3(1), dim3(kBlock), 0, st, partial, nblk, nval, homog, n, vd, out, ctr);
+    return hipGetLastError();
+}
+
+hipError_t launch_value_scale_batch(const BatchArgs &ba, int ntiles, int tile_w, int C, int n, int vd,
+                                    float *partial, int nblk, ValScale *out, Counters *ctr, hipStream_t st) {
+    if (C > 0 && ntiles > 0)
+        hipLaunchKernelGGL(k_bounds_batch, dim3(nblk), dim3(kBlock), 0, st, ba, ntiles, tile_w, 0, C, partial);
+    else
+        C = 0;
+    hipLaunchKernelGGL(k_value_scale, dim3(1), dim3(kBlock), 0, st, partial, nblk, C, 1, n, vd, out, ctr);
     return hipGetLastError();
 }
 
@@ -256,7 +401,8 @@ hipError_t launch_compact(int words, const TableArgs &ta, uint32_t capacity, uin
     case 4: hipLaunchKernelGGL(k_count<4>, dim3(nblk), dim3(kBlock), 0, st, ta.tkeys, capacity, counts); break;
     default: return hipErrorInvalidValue;
     }
-    hipLaunchKernelGGL(k_scan_1wg, dim3(1), dim3(1024), 0, st, counts, (int)nblk, &ctr->M, &ctr->flags);
+    hipLaunchKernelGGL(k_scan_1wg, dim3(1), dim3(1024), 0, st, counts, (int)nblk, &ctr->M, &ctr->flags,
+                       &ctr->filled);
     switch (words) {
     case 1:
         if (packed) hipLaunchKernelGGL((k_write_list<1, true>), dim3(nblk), dim3(kBlock), 0, st, ta.tkeys, capacity,
@@ -290,31 +436,27 @@ hipError_t launch_pack(int d, const TableArgs &ta, const uint32_t *slot_list, co
     return hipGetLastError();
 }
 
-hipError_t launch_finalize(int words, int storage, const TableArgs &ta, const ValScale &vsc, const uint32_t *sorted,
+hipError_t launch_finalize(int words, int storage, const TableArgs &ta, const ValScale *vsc, const uint32_t *sorted,
                            const uint64_t *sorted_keys, const Counters *ctr, int m_max, int vd, int vs,
                            uint64_t *vkeys, void *vals, float *splat_copy, hipStream_t st) {
     if (m_max <= 0) return hipSuccess;
     const dim3 g(blocks_for(m_max, kBlock)), b(kBlock);
-    if (storage == 0)
-        hipLaunchKernelGGL(k_finalize<float>, g, b, 0, st, ta, vsc, words, sorted, sorted_keys, ctr, vd, vs, vkeys,
-                           (float *)vals, splat_copy);
-    else if (storage == 1)
-        hipLaunchKernelGGL(k_finalize<__hip_bfloat16>, g, b, 0, st, ta, vsc, words, sorted, sorted_keys, ctr, vd, vs,
-                           vkeys, (__hip_bfloat16 *)vals, splat_copy);
-    else
-        hipLaunchKernelGGL(k_finalize<__half>, g, b, 0, st, ta, vsc, words, sorted, sorted_keys, ctr, vd, vs, vkeys,
-                           (__half *)vals, splat_copy);
+    // FP32 rows for every storage; a narrow one's first rounding is blur pass
+    // 0, scaled by the largest magnitude tracked here
+    unsigned *track = storage == 0 ? nullptr : const_cast<unsigned *>(&vsc->vmax_bits[0][0]);
+    hipLaunchKernelGGL(k_finalize<float>, g, b, 0, st, ta, vsc, words, sorted, sorted_keys, ctr, vd, vs, vkeys,
+                       (float *)vals, splat_copy, track);
     return hipGetLastError();
 }
 
 template <typename T>
 static void launch_write_final_t(int words, const TableArgs &ta, uint32_t capacity, unsigned nblk,
-                                 const uint32_t *offsets, const Counters *ctr, const ValScale &vsc, int vd, int vs,
-                                 uint64_t *vkeys, void *vals, hipStream_t st) {
+                                 const uint32_t *offsets, const Counters *ctr, const ValScale *vsc, int vd, int vs,
+                                 uint64_t *vkeys, void *vals, unsigned *track, hipStream_t st) {
     T *v = (T *)vals;
 #define GT_WF(W)                                                                                                 \
     hipLaunchKernelGGL((k_write_final<W, T>), dim3(nblk), dim3(kBlock), 0, st, ta, capacity, offsets, ctr, vsc,  \
-                       vd, vs, vkeys, v)
+                       vd, vs, vkeys, v, track)
     switch (words) {
     case 1: GT_WF(1); break;
     case 2: GT_WF(2); break;
@@ -325,7 +467,7 @@ static void launch_write_final_t(int words, const TableArgs &ta, uint32_t capaci
 }
 
 hipError_t launch_compact_final(int words, int storage, const TableArgs &ta, uint32_t capacity, uint32_t *counts,
-                                const ValScale &vsc, int vd, int vs, uint64_t *vkeys, void *vals, Counters *ctr,
+                                const ValScale *vsc, int vd, int vs, uint64_t *vkeys, void *vals, Counters *ctr,
                                 hipStream_t st) {
     if (words < 1 || words > 4) return hipErrorInvalidValue;
     const unsigned nblk = blocks_for(capacity, kCompactSlots);
@@ -335,11 +477,11 @@ hipError_t launch_compact_final(int words, int storage, const TableArgs &ta, uin
     case 3: hipLaunchKernelGGL(k_count<3>, dim3(nblk), dim3(kBlock), 0, st, ta.tkeys, capacity, counts); break;
     default: hipLaunchKernelGGL(k_count<4>, dim3(nblk), dim3(kBlock), 0, st, ta.tkeys, capacity, counts); break;
     }
-    hipLaunchKernelGGL(k_scan_1wg, dim3(1), dim3(1024), 0, st, counts, (int)nblk, &ctr->M, &ctr->flags);
-    if (storage == 0) launch_write_final_t<float>(words, ta, capacity, nblk, counts, ctr, vsc, vd, vs, vkeys, vals, st);
-    else if (storage == 1)
-        launch_write_final_t<__hip_bfloat16>(words, ta, capacity, nblk, counts, ctr, vsc, vd, vs, vkeys, vals, st);
-    else launch_write_final_t<__half>(words, ta, capacity, nblk, counts, ctr, vsc, vd, vs, vkeys, vals, st);
+    hipLaunchKernelGGL(k_scan_1wg, dim3(1), dim3(1024), 0, st, counts, (int)nblk, &ctr->M, &ctr->flags,
+                       &ctr->filled);
+    // FP32 rows for every storage (see launch_finalize)
+    unsigned *track = storage == 0 ? nullptr : const_cast<unsigned *>(&vsc->vmax_bits[0][0]);
+    launch_write_final_t<float>(words, ta, capacity, nblk, counts, ctr, vsc, vd, vs, vkeys, vals, track, st);
     return hipGetLastError();
 }
 
@@ -401,28 +543,45 @@ hipError_t launch_neighbors_hash(int d, bool packed, const KeyPlan &kp, const ui
     return hipGetLastError();
 }
 
-template <typename T>
+template <typename TI, typename TO, bool SCALED>
 static hipError_t launch_blur_t(int vs, const int2 *nbr, const Counters *ctr, int m_max, const PartMap &pm,
-                                const void *in, void *out, hipStream_t st) {
+                                const void *in, void *out, const unsigned *in_bits, const unsigned *out_bits,
+                                unsigned *out_max, hipStream_t st) {
     const dim3 g(vertex_blocks(m_max, pm)), b(kBlock);
-    const T *i = (const T *)in;
-    T *o = (T *)out;
+    const TI *i = (const TI *)in;
+    TO *o = (TO *)out;
+#define GT_BLUR(VS) hipLaunchKernelGGL((k_blur<TI, TO, VS, SCALED>), g, b, 0, st, nbr, ctr, pm, i, o, in_bits, \
+                                       out_bits, out_max)
     switch (vs) {
-    case 2: hipLaunchKernelGGL((k_blur<T, 2>), g, b, 0, st, nbr, ctr, pm, i, o); break;
-    case 4: hipLaunchKernelGGL((k_blur<T, 4>), g, b, 0, st, nbr, ctr, pm, i, o); break;
-    case 8: hipLaunchKernelGGL((k_blur<T, 8>), g, b, 0, st, nbr, ctr, pm, i, o); break;
-    case 16: hipLaunchKernelGGL((k_blur<T, 16>), g, b, 0, st, nbr, ctr, pm, i, o); break;
+    case 2: GT_BLUR(2); break;
+    case 4: GT_BLUR(4); break;
+    case 8: GT_BLUR(8); break;
+    case 16: GT_BLUR(16); break;
     default: return hipErrorInvalidValue;
     }
+#undef GT_BLUR
     return hipGetLastError();
 }
 
-hipError_t launch_blur(int storage, int vs, const int2 *nbr, const Counters *ctr, int m_max, const PartMap &pm,
-                       const void *in, void *out, hipStream_t st) {
+template <typename T>
+static hipError_t launch_blur_narrow(int vs, int pass, const ValScale *vsc, const int2 *nbr, const Counters *ctr,
+                                     int m_max, const PartMap &pm, const void *in, void *out, hipStream_t st) {
+    const unsigned *out_bits = &vsc->vmax_bits[pass][0];
+    unsigned *out_max = const_cast<unsigned *>(&vsc->vmax_bits[pass + 1][0]);
+    if (pass == 0)   // the finalize's FP32 rows, unscaled
+        return launch_blur_t<float, T, true>(vs, nbr, ctr, m_max, pm, in, out, nullptr, out_bits, out_max, st);
+    return launch_blur_t<T, T, true>(vs, nbr, ctr, m_max, pm, in, out, &vsc->vmax_bits[pass - 1][0], out_bits,
+                                     out_max, st);
+}
+
+hipError_t launch_blur(int storage, int vs, int pass, const ValScale *vsc, const int2 *nbr, const Counters *ctr,
+                       int m_max, const PartMap &pm, const void *in, void *out, hipStream_t st) {
     if (m_max <= 0) return hipSuccess;
-    if (storage == 0) return launch_blur_t<float>(vs, nbr, ctr, m_max, pm, in, out, st);
-    if (storage == 1) return launch_blur_t<__hip_bfloat16>(vs, nbr, ctr, m_max, pm, in, out, st);
-    return launch_blur_t<__half>(vs, nbr, ctr, m_max, pm, in, out, st);
+    if (pass < 0 || pass > kMaxD) return hipErrorInvalidValue;
+    if (storage == 0)
+        return launch_blur_t<float, float, false>(vs, nbr, ctr, m_max, pm, in, out, nullptr, nullptr, nullptr, st);
+    if (storage == 1) return launch_blur_narrow<__hip_bfloat16>(vs, pass, vsc, nbr, ctr, m_max, pm, in, out, st);
+    return launch_blur_narrow<__half>(vs, pass, vsc, nbr, ctr, m_max, pm, in, out, st);
 }
 
 hipError_t launch_unpack_keys(int d, bool packed, const KeyPlan &kp, const uint64_t *vkeys, int M, int16_t *keys,
@@ -461,13 +620,16 @@ hipError_t launch_tag_starts(const uint64_t *vkeys, int M, int coord_bits, int c
     return hipGetLastError();
 }
 
-hipError_t launch_store_to_f32(int storage, const void *vals, int M, int vd, int vs, float *out, hipStream_t st) {
+hipError_t launch_store_to_f32(int storage, const void *vals, const unsigned *sbits, int M, int vd, int vs,
+                               float *out, hipStream_t st) {
     if (M <= 0) return hipSuccess;
     const dim3 g(blocks_for(M, kBlock)), b(kBlock);
-    if (storage == 0) hipLaunchKernelGGL(k_store_to_f32<float>, g, b, 0, st, (const float *)vals, M, vd, vs, out);
+    if (storage == 0)
+        hipLaunchKernelGGL(k_store_to_f32<float>, g, b, 0, st, (const float *)vals, sbits, M, vd, vs, out);
     else if (storage == 1)
-        hipLaunchKernelGGL(k_store_to_f32<__hip_bfloat16>, g, b, 0, st, (const __hip_bfloat16 *)vals, M, vd, vs, out);
-    else hipLaunchKernelGGL(k_store_to_f32<__half>, g, b, 0, st, (const __half *)vals, M, vd, vs, out);
+        hipLaunchKernelGGL(k_store_to_f32<__hip_bfloat16>, g, b, 0, st, (const __hip_bfloat16 *)vals, sbits, M, vd,
+                           vs, out);
+    else hipLaunchKernelGGL(k_store_to_f32<__half>, g, b, 0, st, (const __half *)vals, sbits, M, vd, vs, out);
     return hipGetLastError();
 }
 

@@ -6,10 +6,11 @@ namespace gt {
 
 hipError_t launch_slice(int d, bool packed, bool bilat, bool tiled, int storage, const PixSrc &src, int n, int row_w,
                         int tile_w, int vd, int vs, const LatConst &lc, const KeyPlan &kp, const TableArgs &ta,
-                        const void *vals, const OutDst &dst, Counters *ctr, bool count, int wpc, hipStream_t st) {
+                        const void *vals, const unsigned *sbits, const OutDst &dst, Counters *ctr, bool count,
+                        int wpc, hipStream_t st) {
     if (n <= 0) return hipSuccess;
-    if (d > 6) return launch_slice_direct_hi(d, packed, bilat, storage, src, n, vd, vs, lc, kp, ta, vals, dst, ctr,
-                                             count, st);
+    if (d > 6) return launch_slice_direct_hi(d, packed, bilat, storage, src, n, vd, vs, lc, kp, ta, vals, sbits, dst,
+                                             ctr, count, st);
     const int cnt = count ? 1 : 0;
     tiled = tiled && vs <= 4;
     if (tiled) {
@@ -20,11 +21,11 @@ hipError_t launch_slice(int d, bool packed, bool bilat, bool tiled, int storage,
         constexpr int W_ = KeyOps<D, P>::W;                                                                     \
         if (vs <= 2) hipLaunchKernelGGL((k_slice_tile<D, P, B, T, 2, kTileKeys, false>), dim3(ntiles),           \
                                         dim3(kBlock), occupancy_pad(slice_tile_lds<W_, 2, kTileKeys>(), wpc),    \
-                                        st, src, n, row_w, tile_w, none, vd, lc, kp, ta, (const T *)vals, dst,   \
-                                        ctr, cnt);                                                               \
+                                        st, src, n, row_w, tile_w, none, vd, lc, kp, ta, (const T *)vals, sbits, \
+                                        dst, ctr, cnt);                                                          \
         else hipLaunchKernelGGL((k_slice_tile<D, P, B, T, 4, kTileKeys, false>), dim3(ntiles), dim3(kBlock),     \
                                 occupancy_pad(slice_tile_lds<W_, 4, kTileKeys>(), wpc), st, src, n, row_w,       \
-                                tile_w, none, vd, lc, kp, ta, (const T *)vals, dst, ctr, cnt);                   \
+                                tile_w, none, vd, lc, kp, ta, (const T *)vals, sbits, dst, ctr, cnt);            \
     }
 #define GT_SLT_S(D, P, B)                                              \
     {                                                                  \
@@ -44,7 +45,7 @@ hipError_t launch_slice(int d, bool packed, bool bilat, bool tiled, int storage,
     } else {
         const dim3 g(blocks_for(n, kBlock)), b(kBlock);
 #define GT_SLD(D, P, B, T) hipLaunchKernelGGL((k_slice_direct<D, P, B, T>), g, b, 0, st, src, n, vd, vs, lc, kp, ta, \
-                                              (const T *)vals, dst, ctr, cnt)
+                                              (const T *)vals, sbits, dst, ctr, cnt)
 #define GT_SLD_S(D, P, B)                                              \
     {                                                                  \
         if (storage == 0) GT_SLD(D, P, B, float);                      \
@@ -66,7 +67,8 @@ hipError_t launch_slice(int d, bool packed, bool bilat, bool tiled, int storage,
 
 hipError_t launch_slice_batch(int d, int storage, const PixSrc &tmpl, const BatchArgs &ba, int ntiles, int tile_w,
                               int vd, const LatConst &lc, const KeyPlan &kp, const TableArgs &ta, const void *vals,
-                              const OutDst &tmpl_dst, Counters *ctr, bool count, int wpc, hipStream_t st) {
+                              const unsigned *sbits, const OutDst &tmpl_dst, Counters *ctr, bool count, int wpc,
+                              hipStream_t st) {
     if (ntiles <= 0) return hipSuccess;
     if (vd > 4 || !kp.packed) return hipErrorInvalidValue;
     const int cnt = count ? 1 : 0;
@@ -74,11 +76,11 @@ hipError_t launch_slice_batch(int d, int storage, const PixSrc &tmpl, const Batc
     {                                                                                                           \
         if (vd <= 2) hipLaunchKernelGGL((k_slice_tile<D, true, true, T, 2, kTileKeys, true>), dim3(ntiles),      \
                                         dim3(kBlock), occupancy_pad(slice_tile_lds<1, 2, kTileKeys>(), wpc), st, \
-                                        tmpl, 0, 0, tile_w, ba, vd, lc, kp, ta, (const T *)vals, tmpl_dst, ctr,  \
-                                        cnt);                                                                    \
+                                        tmpl, 0, 0, tile_w, ba, vd, lc, kp, ta, (const T *)vals, sbits,          \
+                                        tmpl_dst, ctr, cnt);                                                     \
         else hipLaunchKernelGGL((k_slice_tile<D, true, true, T, 4, kTileKeys, true>), dim3(ntiles),              \
                                 dim3(kBlock), occupancy_pad(slice_tile_lds<1, 4, kTileKeys>(), wpc), st, tmpl,   \
-                                0, 0, tile_w, ba, vd, lc, kp, ta, (const T *)vals, tmpl_dst, ctr, cnt);          \
+                                0, 0, tile_w, ba, vd, lc, kp, ta, (const T *)vals, sbits, tmpl_dst, ctr, cnt);   \
     }
 #define GT_SLB_S(D)                                          \
     {                                                        \

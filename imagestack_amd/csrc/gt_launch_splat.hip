@@ -6,7 +6,7 @@
 namespace gt {
 
 hipError_t launch_splat(int d, bool packed, bool bilat, bool dedup, const PixSrc &src, int n, int row_w, int tile_w,
-                        int vd, const LatConst &lc, const KeyPlan &kp, const ValScale &vsc, const TableArgs &ta,
+                        int vd, const LatConst &lc, const KeyPlan &kp, const ValScale *vsc, const TableArgs &ta,
                         Counters *ctr, bool count, int wpc, hipStream_t st) {
     if (d > 6) {
         if (d <= 10 && dedup && vd <= 4)
@@ -54,7 +54,7 @@ hipError_t launch_splat(int d, bool packed, bool bilat, bool dedup, const PixSrc
 }
 
 hipError_t launch_splat_batch(int d, const PixSrc &tmpl, const BatchArgs &ba, int ntiles, int tile_w, int vd,
-                              const LatConst &lc, const KeyPlan &kp, const ValScale &vsc, const TableArgs &ta,
+                              const LatConst &lc, const KeyPlan &kp, const ValScale *vsc, const TableArgs &ta,
                               Counters *ctr, bool count, int wpc, hipStream_t st) {
     if (ntiles <= 0) return hipSuccess;
     if (vd > 4 || !kp.packed) return hipErrorInvalidValue;

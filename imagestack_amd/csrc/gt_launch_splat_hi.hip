@@ -4,7 +4,7 @@
 namespace gt {
 
 hipError_t launch_splat_direct_hi(int d, bool packed, bool bilat, const PixSrc &src, int n, int row_w, int tile_w,
-                                  int vd, const LatConst &lc, const KeyPlan &kp, const ValScale &vsc,
+                                  int vd, const LatConst &lc, const KeyPlan &kp, const ValScale *vsc,
                                   const TableArgs &ta, Counters *ctr, bool count, hipStream_t st) {
     const int64_t nblk = pixel_blocks(n, row_w, tile_w);
     if (nblk == 0) return hipSuccess;

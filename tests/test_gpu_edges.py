@@ -1,9 +1,9 @@
 """Edge cases of the path on the GPU against the oracle: no Gaussians, a
 single pixel, one-row and one-column images, ragged (prime) sizes, every
 pixel at the same position (one simplex, all collisions), spatial sigmas too
-large to filter (JointBilateral then drops the axis), keys beyond int16 and
-non-finite values (GT_ERANGE, as the reference's short keys cannot hold
-them)."""
+large to filter (JointBilateral then drops the axis), and keys beyond int16
+(GT_ERANGE, as the reference's short keys cannot hold them).  Non-finite
+values are covered in test_gpu_values.py: they spread as in the reference."""
 import numpy as np
 import pytest
 
@@ -61,8 +61,8 @@ def test_all_pixels_collide(gpu_ctx, oracle_lib):
     assert g["M"] == 4   # d + 1 vertices
 
 
-def test_keys_beyond_int16_and_nonfinite_values(gpu_ctx):
-    print("\n[case] keys beyond int16, NaN value", flush=True)
+def test_keys_beyond_int16(gpu_ctx):
+    print("\n[case] keys beyond int16", flush=True)
     pos = np.zeros((3, 64), np.float32)
     pos[1, 7] = 1e6
     val = np.ones((2, 64), np.float32)
@@ -70,9 +70,4 @@ def test_keys_beyond_int16_and_nonfinite_values(gpu_ctx):
         gpu_ctx.gauss_transform(dev(pos), dev(val), np.ones(3, np.float32))
     assert e.value.code == 3
     pos[1, 7] = 0.0
-    val[0, 9] = np.nan
-    with pytest.raises(G.GTError) as e:
-        gpu_ctx.gauss_transform(dev(pos), dev(val), np.ones(3, np.float32))
-    assert e.value.code == 3
-    val[0, 9] = 1.0
     assert np.all(np.isfinite(gpu_ctx.gauss_transform(dev(pos), dev(val), np.ones(3, np.float32)).cpu().numpy()))

@@ -122,6 +122,36 @@ def test_narrow_storage(gpu_ctx, oracle_lib, storage, l2_tol, range_tol):
     assert l2 < l2_tol and rng < range_tol
 
 
+@pytest.mark.parametrize("side,crop", [(64, False), (96, True)])
+def test_narrow_storage_d16(gpu_ctx, oracle_lib, side, crop):
+    """d = 16 (14 N(0,1) features + x/12, y/12): d + 1 blur passes over a
+    lattice of ~17 vertices per pixel leave values ~2^-17 of the splat's,
+    below FP16's normal range unless stored scaled.  FP16 (11-bit
+    significand) must then beat BF16 (8-bit) by several times.  crop: the
+    top-left corner of the 1024^2 workload, else the same recipe at 64^2."""
+    O = oracle_lib
+    case(f"d=16 narrow storage {side}x{side}" + (" crop of 1024^2" if crop else ""))
+    if crop:
+        pos, val = S.highdim_1024()
+        pos = planar(pos[:, :, :side, :side])
+        val = planar(val[:, :, :side, :side])
+    else:
+        pos = planar(S.highdim_positions(side, side, nfeat=14, sigma_s=12.0, seed=3))
+        val = planar(S.uniform_image(side, side, 4, seed=33))
+    inv = np.ones(16, np.float32)
+    o64, _ = O.gauss_transform(pos, val, inv, mode=O.F64_LAT)
+    o64 = o64.reshape(4, -1)
+    err = {}
+    for storage in ("f32", "bf16", "f16"):
+        out = gpu_ctx.gauss_transform(dev(pos), dev(val), inv, row_w=side, storage=storage).cpu().numpy()
+        err[storage] = rel_l2(out, o64)
+    print("rel L2 vs FP64", err, flush=True)
+    assert err["f32"] < 1e-6
+    assert err["bf16"] < 3e-3
+    assert err["f16"] < 5e-4
+    assert err["f16"] * 4 < err["bf16"]
+
+
 def test_highdim_d16_small(gpu_ctx, oracle_lib):
     O = oracle_lib
     case("d=16 gauss transform 64x64")

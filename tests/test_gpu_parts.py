@@ -99,3 +99,38 @@ def test_parts_skewed_last_channel(gpu_ctx):
     print("retries", st["retries"], "M", st["M"])
     assert np.array_equal(a.cpu().numpy().view(np.uint32), b.cpu().numpy().view(np.uint32))
     assert st["retries"] >= 1
+
+
+def test_parts_local_overflow_at_capacity_limit(gpu_ctx):
+    """A split table whose capacity is already the most the call can need
+    (M / n near d + 1, so n (d + 1) / load_factor slots), with a skewed last
+    channel: one part overflows while the table as a whole has room.  The
+    call is redone at the same capacity without parts rather than failing.
+    The second call on the context starts at that capacity (the context
+    remembers the vertex density).  Both equal the plain table bit for bit."""
+    import torch
+    from imagestack_amd import gtlattice as G
+    print("\n[case] table parts overflow locally at the capacity limit", flush=True)
+    W = H = 1024
+    n = W * H
+    rng = np.random.default_rng(17)
+    pos = np.empty((3, n), np.float32)
+    pos[:2] = rng.uniform(-2000.0, 2000.0, size=(2, n))   # nearly every vertex distinct: M ~ 4 n
+    pos[2] = rng.normal(0.0, 0.3, size=n)                 # the last channel: one narrow band ...
+    pos[2, 0], pos[2, 1] = -2000.0, 2000.0                # ... but bounds far apart
+    val = np.concatenate([rng.uniform(size=(1, n)), np.ones((1, n))]).astype(np.float32)
+    inv = np.ones(3, np.float32)
+    ctx = G.Context(0)
+    try:
+        got = []
+        for k in range(2):
+            out, st = ctx.gauss_transform(dev(pos), dev(val), inv, row_w=W, stats=True)
+            print("call", k, "M", st["M"], "capacity", st["capacity"], "retries", st["retries"], flush=True)
+            got.append((out.cpu().numpy(), st))
+        want = ctx.gauss_transform(dev(pos), dev(val), inv, row_w=W, tune={"parts": -1}).cpu().numpy()
+    finally:
+        ctx.close()
+    assert got[0][1]["M"] > 2 * n
+    assert got[1][1]["retries"] >= 1
+    for out, _ in got:
+        assert np.array_equal(out.view(np.uint32), want.view(np.uint32))

@@ -295,6 +295,29 @@ def test_golden_joint_bilateral_2048_digest(oracle_lib):
     assert digest(out) == rec["out_sha"]
 
 
+def test_nonfinite_values_spread_as_in_reference(oracle_lib):
+    """(R) NaN and +-Inf values spread through the lattice's FP32 sums, blur
+    and slice (src/Permutohedral.h:377, :467-469, :423): the F32 oracle's
+    output equals the reference's bit for bit, and the FP64 oracle has the same
+    NaN / +Inf / -Inf pixels (the masks the GPU is held to)."""
+    O = oracle_lib
+    im = S.rect_image(40, 30, 3, K=6, noise=0.05, seed=3)
+    g = S.uniform_image(40, 30, 3, seed=4)
+    g[0, 0, 5, 7] = np.nan
+    g[1, 0, 20, 30] = np.inf
+    g[2, 0, 21, 30] = -np.inf
+    g[1, 0, 10, 10] = np.inf
+    g[1, 0, 10, 11] = -np.inf      # +Inf and -Inf meeting in one vertex: NaN
+    a = O.joint_bilateral(g, im, 3.0, 3.0, 0.0, 0.15)
+    c = O.joint_bilateral(g, im, 3.0, 3.0, 0.0, 0.15, mode=O.F64_LAT)
+    assert np.isnan(a).sum() > 0 and np.isposinf(a).sum() > 0 and np.isneginf(a).sum() > 0
+    for f in (np.isnan, np.isposinf, np.isneginf):
+        assert np.array_equal(f(a), f(c))
+    if O.ref_available("strict"):
+        b = O.ref_joint_bilateral(g, im, 3.0, 3.0, 0.0, 0.15)
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
 def test_highdim_1024_inputs_digest():
     """The d = 16 workload's inputs are the ones the reference's key-set
     digest was written from (the key set itself is checked on the GPU)."""
