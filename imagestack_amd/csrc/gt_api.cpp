@@ -25,15 +25,15 @@
 
 namespace {
 
-thread_local std::string g_err;
+// The thread's last error message (a fixed buffer: reporting an error
+// allocates nothing).
+thread_local char g_err[512];
 
 int fail(int code, const char *fmt, ...) {
-    char buf[512];
     va_list ap;
     va_start(ap, fmt);
-    vsnprintf(buf, sizeof(buf), fmt, ap);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
     va_end(ap);
-    g_err = buf;
     return code;
 }
 
@@ -141,9 +141,16 @@ struct gt_ctx {
 
 namespace {
 
+// Set while the no-allocation entry point runs on this thread: its buffers
+// are fixed shares of the scratch array, and a buffer too small is an error.
+thread_local bool g_no_alloc = false;
+
 int ensure(DevBuf &b, size_t bytes) {
     if (bytes == 0) bytes = 8;
     if (b.cap >= bytes) return GT_OK;
+    if (g_no_alloc)
+        return fail(GT_ENOMEM, "the no-allocation entry point needs %zu bytes where its scratch share holds %zu",
+                    bytes, b.cap);
     if (b.p) (void)hipFree(b.p);
     b.p = nullptr;
     b.cap = 0;
@@ -180,6 +187,10 @@ struct Tune {
     bool grid = false;  // the bilateral grid instead of the lattice (GRID entry points only)
     int nbr = 0;        // 0 auto (hash lookups for d >= 11 or few expected vertices), 1 sort + merge, 2 hash
     bool parts = true;  // row-band table parts where they apply
+    // a caller-bounded arena (the no-allocation entry point): this table
+    // capacity and at most this many vertices, no regrowth
+    uint32_t fixed_cap = 0;
+    int vert_limit = 0;
 };
 
 Tune resolve(const gt_tuning *t) {
@@ -326,10 +337,10 @@ int enqueue(gt_ctx *c, const Launch &L, const gt::PixSrc &src, const SliceSpec &
     *passes = 0;
     if (L.hashnbr) {   // ids in compaction order: the finalize rides along with the compaction
         GT_HIP(gt::launch_compact_final(L.W, L.storage, ta, L.cap, (uint32_t *)c->counts.p, L.vsc, L.vd, L.vs,
-                                        (uint64_t *)c->vkeys.p, c->vals_a.p, ctr, s));
+                                        (uint64_t *)c->vkeys.p, c->vals_a.p, ctr, L.m_max, s));
     } else {
         GT_HIP(gt::launch_compact(L.W, ta, L.cap, (uint32_t *)c->counts.p, (uint32_t *)c->slot_list.p,
-                                  (uint64_t *)c->pk_a.p, L.packed, ctr, s));
+                                  (uint64_t *)c->pk_a.p, L.packed, ctr, L.m_max, s));
         if (!L.packed)
             GT_HIP(gt::launch_pack(L.d, ta, (const uint32_t *)c->slot_list.p, ctr, L.m_max, L.kp, L.pw,
                                    (uint64_t *)c->pk_a.p, s));
@@ -513,8 +524,9 @@ int run(gt_ctx *c, const gt::PixSrc &src, int n, int row_w, int d, int vd, int s
     else est = std::max<uint64_t>(65536, (uint64_t)n);
     est = std::min<uint64_t>(est, (uint64_t)n * (d + 1));
     uint32_t cap = next_pow2((uint64_t)(est / tu.lf));
-    const uint32_t cap_limit = next_pow2((uint64_t)((double)n * (d + 1) / tu.lf));
+    uint32_t cap_limit = next_pow2((uint64_t)((double)n * (d + 1) / tu.lf));
     if (cap > cap_limit) cap = cap_limit;
+    if (tu.fixed_cap) cap = cap_limit = tu.fixed_cap;
     auto replan = [&]() -> int {
         if (batch) return make_plan_batch(c, *batch, *bimgs, d, vd, tu.tile_w);
         return make_plan(c, src, n, d, vd, bilat);
@@ -633,6 +645,7 @@ int run(gt_ctx *c, const gt::PixSrc &src, int n, int row_w, int d, int vd, int s
         // RGB at sigma_s 4, 28.9 M: 5.6 against 4.3 ms).
         L.nbr_per_vertex = d >= 7 || est >= (1u << 18);
         L.m_max = (int)std::min<uint64_t>(cap, (uint64_t)n * (d + 1));
+        if (tu.vert_limit > 0) L.m_max = std::min(L.m_max, tu.vert_limit);
         L.pw = L.packed ? 1 : std::max(1, (L.kp.bits + 63) / 64);
         L.vs = value_stride(vd);
         const unsigned ncount = (cap + 2047) / 2048;
@@ -662,14 +675,16 @@ int run(gt_ctx *c, const gt::PixSrc &src, int n, int row_w, int d, int vd, int s
         L.vs_part = (float *)c->vs_part.p;
         GT_TRY(ensure(c->ctr, sizeof(gt::Counters)));
         GT_TRY(ensure(c->counts, (size_t)ncount * 4));
-        GT_TRY(ensure(c->slot_list, (size_t)L.m_max * 4));
         GT_TRY(ensure(c->slot_id, (size_t)cap * 4));
-        GT_TRY(ensure(c->pk_a, (size_t)L.pw * L.m_max * 8));
-        GT_TRY(ensure(c->pk_b, (size_t)L.pw * L.m_max * 8));
-        GT_TRY(ensure(c->pay_b, (size_t)L.m_max * 4));
-        GT_TRY(ensure_zeroed(c->hist, gt::sort_state_words() * 4, s));
-        GT_TRY(ensure_zeroed(c->sort_status, gt::sort_status_entries(L.m_max, L.pw) * 8, s));
-        GT_TRY(ensure(c->sort_hist, gt::sort_hist_words(L.m_max, L.pw) * 4));
+        if (!L.hashnbr) {   // the slot list and the radix sort's buffers
+            GT_TRY(ensure(c->slot_list, (size_t)L.m_max * 4));
+            GT_TRY(ensure(c->pk_a, (size_t)L.pw * L.m_max * 8));
+            GT_TRY(ensure(c->pk_b, (size_t)L.pw * L.m_max * 8));
+            GT_TRY(ensure(c->pay_b, (size_t)L.m_max * 4));
+            GT_TRY(ensure_zeroed(c->hist, gt::sort_state_words() * 4, s));
+            GT_TRY(ensure_zeroed(c->sort_status, gt::sort_status_entries(L.m_max, L.pw) * 8, s));
+            GT_TRY(ensure(c->sort_hist, gt::sort_hist_words(L.m_max, L.pw) * 4));
+        }
         GT_TRY(ensure(c->vkeys, (size_t)L.m_max * L.W * 8));
         // vals_a takes the finalize's FP32 rows (every storage), vals_b the
         // storage type; the passes alternate between them
@@ -729,6 +744,9 @@ int run(gt_ctx *c, const gt::PixSrc &src, int n, int row_w, int d, int vd, int s
                 parts_failed = true;
                 continue;
             }
+            if (tu.fixed_cap)
+                return fail(GT_ENOMEM, "lattice exceeds the no-allocation maximum of %d vertices (%u table slots)",
+                            L.m_max, cap);
             if (cap >= cap_limit) return fail(GT_ENOMEM, "lattice does not fit a hash table of %u slots", cap);
             // a full table held about `cap` vertices: decide the redo's
             // neighbour form on that
@@ -1064,7 +1082,7 @@ hipStream_t ctx_stream(gt_ctx *c) { return c->stream; }
 
 extern "C" {
 
-const char *gt_last_error(void) { return g_err.c_str(); }
+const char *gt_last_error(void) { return g_err; }
 
 int gt_ctx_create(gt_ctx **out, int device) {
     if (!out) return fail(GT_EINVAL, "null context pointer");
@@ -1105,7 +1123,8 @@ void gt_ctx_destroy(gt_ctx *c) {
     drop_graph(c);
     if (c->h_batch) (void)hipHostFree(c->h_batch);
     free_buf(c->batch_desc);
-    DevBuf *bufs[] = {&c->tkeys,       &c->tacc,      &c->ctr,      &c->counts,    &c->slot_list, &c->slot_id,
+    DevBuf *bufs[] = {&c->tkeys,       &c->tacc,      &c->tnf,      &c->vscale,    &c->vs_part,
+                      &c->ctr,         &c->counts,    &c->slot_list, &c->slot_id,
                       &c->pk_a,        &c->pk_b,      &c->pay_b,    &c->hist,      &c->sort_status,
                       &c->sort_hist,   &c->vkeys,     &c->vals_a,   &c->vals_b,    &c->nbr,       &c->splat_copy,
                       &c->mm_part,     &c->mm_out,    &c->stage_in, &c->stage_out, &c->view_a,    &c->view_b,
@@ -1579,3 +1598,148 @@ int gt_nlmeans_host(gt_ctx *c, int W, int H, int F, int C, const float *im, int 
 }
 
 } // extern "C"
+
+// --- no-allocation entry point ----------------------------------------------
+//
+// gt_joint_bilateral_noalloc runs gt_joint_bilateral's pipeline with every
+// device buffer a fixed share of `scratch`, a file-scope device array, so the
+// call allocates nothing (no hipMalloc/hipFree, no graph objects).  The
+// caller's context lends its stream and pinned counters; its own buffers are
+// set aside for the call and restored afterwards.  Vertex ids are taken in
+// compaction order (no sort buffers) and the hash table has a fixed size.
+
+namespace {
+
+constexpr int kNaPixels = 1 << 20;       // pixels per call
+constexpr int kNaMaxD = 5;               // lattice dimensions: one frame, guide of <= 3 channels, x, y
+constexpr int kNaMaxC = 3;               // image channels (+ the homogeneous one)
+constexpr uint32_t kNaCap = 1u << 21;    // hash-table slots
+constexpr int kNaVerts = 1 << 20;        // lattice vertices (table load <= 0.5)
+constexpr int kNaKeyWords = 2;           // wide keys of d <= 8
+constexpr int kNaMaxDevices = 64;
+
+constexpr size_t na_round(size_t b) { return (b + 255) & ~(size_t)255; }
+constexpr size_t kNaCtr = na_round(sizeof(gt::Counters));
+constexpr size_t kNaKeys = na_round((size_t)kNaCap * kNaKeyWords * 8);
+constexpr size_t kNaAcc = na_round((size_t)kNaCap * (kNaMaxC + 1) * 8);
+constexpr size_t kNaMarks = na_round((size_t)kNaCap * 4);
+constexpr size_t kNaSlotId = na_round((size_t)kNaCap * 4);
+constexpr size_t kNaCounts = na_round((size_t)kNaCap / gt::kCompactSlots * 4);
+constexpr size_t kNaVkeys = na_round((size_t)kNaVerts * kNaKeyWords * 8);
+constexpr size_t kNaVals = na_round((size_t)kNaVerts * 4 * 4);   // value stride 4, FP32 rows
+constexpr size_t kNaNbr = na_round((size_t)(kNaMaxD + 1) * kNaVerts * 8);
+constexpr size_t kNaScale = na_round(sizeof(gt::ValScale));
+constexpr size_t kNaScalePart = na_round((size_t)1024 * (kNaMaxC + 1) * 3 * 4);
+constexpr size_t kNaMmPart = na_round((size_t)1024 * gt::kMaxDH * 3 * 4);
+constexpr size_t kNaMmOut = na_round((size_t)gt::kMaxDH * 3 * 4);
+constexpr size_t kNaBytes = kNaCtr + kNaKeys + kNaAcc + kNaMarks + kNaSlotId + kNaCounts + kNaVkeys + 2 * kNaVals +
+                            kNaNbr + kNaScale + kNaScalePart + kNaMmPart + kNaMmOut;
+
+} // namespace
+
+// The no-allocation entry point's device memory (about 208 MiB), one copy per
+// GPU the library runs on, held until the process ends.  Host code uses the
+// address hipGetSymbolAddress gives for the current device, never this name.
+static __device__ __align__(256) unsigned char scratch[kNaBytes];
+
+namespace {
+
+struct NaDevice {
+    std::mutex mu;          // calls on one device take turns (they share its scratch)
+    void *ptr = nullptr;    // the device's copy of `scratch`
+    size_t acc_zero = 0;    // what a complete call leaves zero (the next call then clears less)
+    bool tnf_clean = false;
+};
+NaDevice g_na[kNaMaxDevices];
+
+// The context's buffers that the pipeline of a one-frame joint bilateral with
+// ids in compaction order touches, in scratch order.
+DevBuf gt_ctx::*const kNaBufs[] = {&gt_ctx::ctr,    &gt_ctx::tkeys,  &gt_ctx::tacc,    &gt_ctx::tnf,
+                                   &gt_ctx::slot_id, &gt_ctx::counts, &gt_ctx::vkeys,   &gt_ctx::vals_a,
+                                   &gt_ctx::vals_b,  &gt_ctx::nbr,    &gt_ctx::vscale,  &gt_ctx::vs_part,
+                                   &gt_ctx::mm_part, &gt_ctx::mm_out};
+const size_t kNaShare[] = {kNaCtr,  kNaKeys,  kNaAcc,   kNaMarks,     kNaSlotId, kNaCounts, kNaVkeys,
+                           kNaVals, kNaVals,  kNaNbr,   kNaScale,     kNaScalePart, kNaMmPart, kNaMmOut};
+constexpr int kNaNumBufs = (int)(sizeof(kNaShare) / sizeof(kNaShare[0]));
+static_assert(sizeof(kNaBufs) / sizeof(kNaBufs[0]) == kNaNumBufs, "one share per buffer");
+
+// Swaps the context's buffers for shares of the device's scratch for the
+// lifetime of the object, and back.
+struct NaArena {
+    gt_ctx *c;
+    NaDevice &dev;
+    DevBuf saved[kNaNumBufs];
+    size_t saved_acc_zero;
+    bool saved_tnf_clean;
+    NaArena(gt_ctx *ctx, NaDevice &d) : c(ctx), dev(d) {
+        saved_acc_zero = c->acc_zero;
+        saved_tnf_clean = c->tnf_clean;
+        unsigned char *p = (unsigned char *)dev.ptr;
+        for (int i = 0; i < kNaNumBufs; i++) {
+            saved[i] = c->*kNaBufs[i];
+            (c->*kNaBufs[i]).p = p;
+            (c->*kNaBufs[i]).cap = kNaShare[i];
+            p += kNaShare[i];
+        }
+        c->acc_zero = dev.acc_zero;
+        c->tnf_clean = dev.tnf_clean;
+        g_no_alloc = true;
+    }
+    ~NaArena() {
+        g_no_alloc = false;
+        dev.acc_zero = c->acc_zero;
+        dev.tnf_clean = c->tnf_clean;
+        for (int i = 0; i < kNaNumBufs; i++) c->*kNaBufs[i] = saved[i];
+        c->acc_zero = saved_acc_zero;
+        c->tnf_clean = saved_tnf_clean;
+        c->have = false;   // the lattice it built lived in the scratch
+    }
+};
+
+} // namespace
+
+int gt_joint_bilateral_noalloc(gt_ctx *c, int W, int H, int F, int C, const float *im, int Cr, const float *ref,
+                               float fw, float fh, float ff, float color_sigma, float *out, gt_storage storage,
+                               const gt_tuning *tune, gt_stats *st) {
+    if (!c) return fail(GT_EINVAL, "null context");
+    if (W < 1 || H < 1 || F < 1 || C < 1 || Cr < 1) return fail(GT_EINVAL, "bad image size %dx%dx%d", W, H, F);
+    if (!im || !ref || !out) return fail(GT_EINVAL, "null image pointer");
+    GT_TRY(check_storage(storage));
+    if ((int64_t)W * H * F > kNaPixels)
+        return fail(GT_EINVAL, "the no-allocation entry point takes at most %d pixels (%lld given)", kNaPixels,
+                    (long long)W * H * F);
+    if (C > kNaMaxC) return fail(GT_EINVAL, "the no-allocation entry point takes at most %d channels", kNaMaxC);
+    // JointBilateral's singleton rule (src/GaussTransform.cpp:512-514); a
+    // zero spatial sigma left after it would split the image into pieces
+    if (W == 1) fw = 0;
+    if (H == 1) fh = 0;
+    if (F == 1) ff = 0;
+    if ((W > 1 && fw == 0) || (H > 1 && fh == 0) || (F > 1 && ff == 0))
+        return fail(GT_EINVAL, "the no-allocation entry point filters whole images: no zero spatial sigma");
+    const int d = Cr + (W > 1 && fw < 10 * W) + (H > 1 && fh < 10 * H) + (F > 1 && ff < 10 * F);
+    if (d > kNaMaxD)
+        return fail(GT_EINVAL, "the no-allocation entry point takes lattices of at most %d dimensions (%d)", kNaMaxD,
+                    d);
+    if (c->device < 0 || c->device >= kNaMaxDevices) return fail(GT_EINVAL, "device %d out of range", c->device);
+    GT_HIP(hipSetDevice(c->device));
+    NaDevice &dev = g_na[c->device];
+    std::lock_guard<std::mutex> lk(dev.mu);
+    if (!dev.ptr) {   // the first call on this device: the address of its copy
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            (void)hipGetLastError();
+            return fail(GT_EDEVICE, "hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+        }
+        dev.ptr = p;
+    }
+    if (st) memset(st, 0, sizeof(*st));
+    Tune tu = resolve(tune);
+    tu.graph = false;         // instantiating a graph allocates
+    tu.nbr = 2;               // ids in compaction order: no sort buffers
+    tu.fixed_cap = kNaCap;
+    tu.vert_limit = kNaVerts;
+    NaArena arena(c, dev);
+    const int64_t n = (int64_t)W * H * F;
+    return jb_dense(c, W, H, F, C, im, n, Cr, ref, n, fw, fh, ff, color_sigma, out, n, storage, tu, st);
+}

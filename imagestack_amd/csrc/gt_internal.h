@@ -182,8 +182,10 @@ hipError_t launch_slice_batch(int d, int storage, const PixSrc &tmpl, const Batc
                               int vd, const LatConst &lc, const KeyPlan &kp, const TableArgs &ta, const void *vals,
                               const unsigned *sbits, const OutDst &tmpl_dst, Counters *ctr, bool count, int wpc,
                               hipStream_t st);
+// limit: the per-vertex buffers' size; a table holding more vertices counts as
+// full (kFlagFull, M = 0).
 hipError_t launch_compact(int words, const TableArgs &ta, uint32_t capacity, uint32_t *counts, uint32_t *slot_list,
-                          uint64_t *sort_keys, bool packed, Counters *ctr, hipStream_t st);
+                          uint64_t *sort_keys, bool packed, Counters *ctr, int limit, hipStream_t st);
 hipError_t launch_pack(int d, const TableArgs &ta, const uint32_t *slot_list, const Counters *ctr, int m_max,
                        const KeyPlan &kp, int pw, uint64_t *pk, hipStream_t st);
 // sorted_keys: the sorted packed keys (one word each), or null to gather the
@@ -197,7 +199,7 @@ hipError_t launch_neighbors(int d, bool packed, const KeyPlan &kp, const uint64_
 // sort, no slot list): count, scan, then k_write_final.
 hipError_t launch_compact_final(int words, int storage, const TableArgs &ta, uint32_t capacity, uint32_t *counts,
                                 const ValScale *vsc, int vd, int vs, uint64_t *vkeys, void *vals, Counters *ctr,
-                                hipStream_t st);
+                                int limit, hipStream_t st);
 // The same table by hash lookups in the complete global table, for vertex ids
 // in compaction order (no sort).
 // With pm.cpp > 0, workgroups on XCD x take the vertices of table part x.

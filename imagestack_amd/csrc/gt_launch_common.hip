@@ -296,8 +296,10 @@ __global__ __launch_bounds__(kBlock) void k_value_scale(const float *__restrict_
 // In-place exclusive scan of n counts in one workgroup; the total goes to
 // *total, or 0 when `flags` carries kFlagFull (the call is redone with a
 // larger table, so everything after the compaction sees no vertices).
+// A table holding more than `limit` vertices (the per-vertex buffers' size,
+// below the table capacity only for a caller-bounded arena) counts as full.
 __global__ __launch_bounds__(1024) void k_scan_1wg(uint32_t *__restrict__ a, int n, unsigned *total,
-                                                   const unsigned *flags, unsigned *filled) {
+                                                   unsigned *flags, unsigned *filled, unsigned limit) {
     __shared__ uint32_t part[1024];
     const int chunk = (n + 1023) / 1024;
     const int lo = threadIdx.x * chunk;
@@ -330,7 +332,12 @@ __global__ __launch_bounds__(1024) void k_scan_1wg(uint32_t *__restrict__ a, int
         }
     }
     if (threadIdx.x == 1023 && total) {
-        *total = (flags && (*flags & kFlagFull)) ? 0u : part[1023];
+        bool full = flags && (*flags & kFlagFull);
+        if (flags && part[1023] > limit && !full) {
+            atomicOr(flags, kFlagFull);
+            full = true;
+        }
+        *total = full ? 0u : part[1023];
         if (filled) *filled = part[1023];
     }
 }
@@ -392,7 +399,7 @@ hipError_t launch_value_scale_batch(const BatchArgs &ba, int ntiles, int tile_w,
 }
 
 hipError_t launch_compact(int words, const TableArgs &ta, uint32_t capacity, uint32_t *counts, uint32_t *slot_list,
-                          uint64_t *sort_keys, bool packed, Counters *ctr, hipStream_t st) {
+                          uint64_t *sort_keys, bool packed, Counters *ctr, int limit, hipStream_t st) {
     const unsigned nblk = blocks_for(capacity, kCompactSlots);
     switch (words) {
     case 1: hipLaunchKernelGGL(k_count<1>, dim3(nblk), dim3(kBlock), 0, st, ta.tkeys, capacity, counts); break;
@@ -402,7 +409,7 @@ hipError_t launch_compact(int words, const TableArgs &ta, uint32_t capacity, uin
     default: return hipErrorInvalidValue;
     }
     hipLaunchKernelGGL(k_scan_1wg, dim3(1), dim3(1024), 0, st, counts, (int)nblk, &ctr->M, &ctr->flags,
-                       &ctr->filled);
+                       &ctr->filled, (unsigned)limit);
     switch (words) {
     case 1:
         if (packed) hipLaunchKernelGGL((k_write_list<1, true>), dim3(nblk), dim3(kBlock), 0, st, ta.tkeys, capacity,
@@ -468,7 +475,7 @@ static void launch_write_final_t(int words, const TableArgs &ta, uint32_t capaci
 
 hipError_t launch_compact_final(int words, int storage, const TableArgs &ta, uint32_t capacity, uint32_t *counts,
                                 const ValScale *vsc, int vd, int vs, uint64_t *vkeys, void *vals, Counters *ctr,
-                                hipStream_t st) {
+                                int limit, hipStream_t st) {
     if (words < 1 || words > 4) return hipErrorInvalidValue;
     const unsigned nblk = blocks_for(capacity, kCompactSlots);
     switch (words) {
@@ -478,7 +485,7 @@ hipError_t launch_compact_final(int words, int storage, const TableArgs &ta, uin
     default: hipLaunchKernelGGL(k_count<4>, dim3(nblk), dim3(kBlock), 0, st, ta.tkeys, capacity, counts); break;
     }
     hipLaunchKernelGGL(k_scan_1wg, dim3(1), dim3(1024), 0, st, counts, (int)nblk, &ctr->M, &ctr->flags,
-                       &ctr->filled);
+                       &ctr->filled, (unsigned)limit);
     // FP32 rows for every storage (see launch_finalize)
     unsigned *track = storage == 0 ? nullptr : const_cast<unsigned *>(&vsc->vmax_bits[0][0]);
     launch_write_final_t<float>(words, ta, capacity, nblk, counts, ctr, vsc, vd, vs, vkeys, vals, track, st);

@@ -23,7 +23,7 @@ GT_EUNSUPPORTED = 6
 EXPORTS = [
     "gt_ctx_create", "gt_ctx_destroy", "gt_ctx_set_stream", "gt_ctx_set_profiling", "gt_last_error",
     "gt_gauss_transform", "gt_gauss_transform_host", "gt_joint_bilateral", "gt_joint_bilateral_host",
-    "gt_build_lattice", "gt_lattice_export", "gt_joint_bilateral_batch",
+    "gt_joint_bilateral_noalloc", "gt_build_lattice", "gt_lattice_export", "gt_joint_bilateral_batch",
     "gt_autotune_joint_bilateral", "gt_autotune_joint_bilateral_batch",
     "gt_server_create", "gt_server_destroy", "gt_server_submit", "gt_server_wait", "gt_server_replay",
     "gt_gauss_transform_grid", "gt_gauss_transform_grid_host", "gt_joint_bilateral_grid",
@@ -125,6 +125,7 @@ def lib():
                 C.c_float, C.c_float, _vp, C.c_int, C.POINTER(Tuning), C.POINTER(Stats)]
         L.gt_joint_bilateral.argtypes = jsig
         L.gt_joint_bilateral_host.argtypes = jsig
+        L.gt_joint_bilateral_noalloc.argtypes = jsig
         L.gt_build_lattice.argtypes = [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp, _fp, C.c_int,
                                        C.POINTER(Tuning), C.POINTER(C.c_int)]
         L.gt_lattice_export.argtypes = [_vp, _vp, _vp, _vp, _vp]
@@ -248,6 +249,17 @@ class Context:
         _check(lib().gt_joint_bilateral(self._h, W, H, F, Cc, _dptr(im), ref.shape[0], _dptr(ref), fw, fh,
                                         ff, color_sigma, _dptr(out), STORAGE[storage], _tuning(tune),
                                         C.byref(st)))
+        return (out, st.as_dict()) if stats else out
+
+    def joint_bilateral_noalloc(self, im, ref, fw, fh, ff, color_sigma, out, storage: str = "f32",
+                                tune=None, stats: bool = False):
+        """gt_joint_bilateral_noalloc: as joint_bilateral, into a given `out`
+        (C, F, H, W) device f32; the call allocates no device memory."""
+        Cc, F, H, W = im.shape
+        st = Stats()
+        _check(lib().gt_joint_bilateral_noalloc(self._h, W, H, F, Cc, _dptr(im), ref.shape[0], _dptr(ref), fw, fh,
+                                                ff, color_sigma, _dptr(out), STORAGE[storage], _tuning(tune),
+                                                C.byref(st)))
         return (out, st.as_dict()) if stats else out
 
     def joint_bilateral_batch(self, items, storage: str = "f32", tune=None, stats: bool = False):

@@ -156,6 +156,27 @@ int gt_joint_bilateral_host(gt_ctx *ctx, int W, int H, int F, int C, const float
                             const gt_tuning *tune, gt_stats *stats);
 
 /*
+ * gt_joint_bilateral without allocating: no hipMalloc, hipFree or graph
+ * objects in the call.  No counterpart in the reference (which allocates per
+ * call): it is for callers that cannot allocate -- serving loops that must
+ * not touch the allocator, harnesses that forbid it.  Every device buffer is
+ * a fixed share of one static device array of about 208 MiB (each GPU the
+ * library runs on holds its own copy until the process ends); ctx lends its
+ * stream and pinned counters, and its own buffers are left as they were.
+ * Limits, each rejected with GT_EINVAL: W*H*F <= 2^20 pixels, C <= 3, a
+ * lattice of at most 5 dimensions (a one-frame guide of <= 3 channels plus
+ * x, y), no zero spatial sigma on an axis longer than 1.  A lattice of more
+ * than 2^20 vertices returns GT_ENOMEM.  The output equals gt_joint_bilateral's
+ * bit for bit.  Calls on one GPU share its array and take turns (a lock); the
+ * array keeps the last call's contents, and each call clears what it needs.
+ * tune: load_factor, neighbours and graph are fixed here; the rest applies.
+ */
+int gt_joint_bilateral_noalloc(gt_ctx *ctx, int W, int H, int F, int C, const float *im,
+                               int Cr, const float *ref, float fw, float fh, float ff,
+                               float color_sigma, float *out, gt_storage storage,
+                               const gt_tuning *tune, gt_stats *stats);
+
+/*
  * The bilateral grid (src/DenseGrid.h, 5 taps), the GRID method that
  * JointBilateral's AUTO choice picks for at most 4 position channels
  * (src/GaussTransform.cpp:553-564):
