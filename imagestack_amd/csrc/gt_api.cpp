@@ -988,6 +988,7 @@ int jb_dense(gt_ctx *c, int W, int H, int F, int C, const float *im, int64_t im_
     src.cr = Cr;
     src.W = W;
     src.H = H;
+    src.F = F;
     src.fx = fx;
     src.fy = fy;
     src.ft = ft;
@@ -1440,6 +1441,7 @@ int gt_joint_bilateral_batch(gt_ctx *c, int count, const gt_bilateral_image *ima
         b.tmpl.fx = 1;
         b.tmpl.fy = 1;
         b.tmpl.ft = 0;
+        b.tmpl.F = 1;
         b.tmpl.nval = C;
         b.tmpl.homog = 1;
         b.tmpl.ref = imgs[0].ref;   // marks the bilateral form; per-image fields come from the descriptors

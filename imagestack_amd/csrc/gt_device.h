@@ -193,6 +193,39 @@ struct KeyOps<D, false> {
     __device__ static __forceinline__ uint64_t tag_of(const uint64_t (&)[W], const KeyPlan &) { return 0ull; }
 };
 
+// Packed keys of all D+1 vertices of a simplex at once (src/Permutohedral.h:
+// 365-369 for every r).  Coordinate i of vertex r is g_i + r - (D+1)[rk_i >=
+// D+1-r]: vertex 0 is g, and each next vertex adds 1 to every coordinate and
+// takes D+1 from the one coordinate of rank D+1-r (none when that is the
+// implicit coordinate D).  In the packed word that is key(r) = key(r-1) +
+// ones - drop_r, exact while every field stays in its range: coordinate i
+// spans [g_i - rk_i, g_i - rk_i + D] over the simplex, so one check per
+// coordinate covers all vertices.  Returns false (keys unusable) when a
+// coordinate leaves the plan.
+template <int D>
+__device__ __forceinline__ bool packed_vertex_keys(const Simplex<D> &s, const KeyPlan &kp, uint64_t tag,
+                                                   uint64_t (&key)[D + 1]) {
+    uint64_t k = tag, ones = 0;
+    bool ok = true;
+#pragma unroll
+    for (int i = 0; i < D; i++) {
+        const int lo_f = s.g[i] - s.rk[i] - kp.lo[i];   // the field's smallest value over the vertices
+        ok &= lo_f >= 0 && lo_f + D <= (int)kp.span[i];
+        k += (uint64_t)(uint32_t)(s.g[i] - kp.lo[i]) << kp.shift[i];
+        ones += 1ull << kp.shift[i];
+    }
+    key[0] = k;
+#pragma unroll
+    for (int r = 1; r <= D; r++) {
+        uint64_t drop = 0;
+#pragma unroll
+        for (int i = 0; i < D; i++) drop = (s.rk[i] == D + 1 - r) ? ((uint64_t)(D + 1) << kp.shift[i]) : drop;
+        k = k + ones - drop;
+        key[r] = k;
+    }
+    return ok;
+}
+
 template <int W>
 __device__ __forceinline__ uint64_t hash_key(const uint64_t (&k)[W]) {
     uint64_t h = 0x9E3779B97F4A7C15ull;
@@ -331,9 +364,22 @@ __device__ __forceinline__ int table_vertex(const TableArgs &ta, const KeyPlan &
 // Same word-claim protocol as the global table.  Returns the LDS slot, or -1
 // after MAXP probes (a tile with more distinct vertices than the table holds
 // then sends the rest straight to the global table).
+//
+// The slot hash of a one-word key is two 32-bit multiplies (a tile's keys
+// differ in a few low fields; the fold and the odd multipliers spread them);
+// wider keys use hash_key.
+template <int W, int LT>
+__device__ __forceinline__ uint32_t lds_home(const uint64_t (&k)[W]) {
+    if (W == 1) {
+        const uint32_t x = (uint32_t)k[0] ^ ((uint32_t)(k[0] >> 32) * 0x85EBCA6Bu);
+        return (x * 0x9E3779B1u) >> (32 - __builtin_ctz((unsigned)LT));
+    }
+    return (uint32_t)(hash_key<W>(k) >> 40) & (LT - 1);
+}
+
 template <int W, int LT, int MAXP>
 __device__ __forceinline__ int lds_insert(uint64_t *lk, const uint64_t (&k)[W]) {
-    uint32_t ls = (uint32_t)(hash_key<W>(k) >> 40) & (LT - 1);
+    uint32_t ls = lds_home<W, LT>(k);
     for (int it = 0; it < MAXP; it++) {
         bool match = true;
 #pragma unroll
@@ -354,12 +400,14 @@ __device__ __forceinline__ int lds_insert(uint64_t *lk, const uint64_t (&k)[W]) 
 // x * s rounded to the nearest integer (ties to even), as a two's-complement
 // 64-bit word; |x * s| < 2^62.  y = rint(x * s) is an integer-valued float
 // with at most 24 significant bits, so hi = floor(y / 2^32) and the low word
-// lo = y - hi * 2^32 in [0, 2^32) are exact, and the word is hi:lo -- the
-// same value as a float -> int64 conversion, without its sign handling.
+// lo = y - hi * 2^32 in [0, 2^32) are exact for y >= 0 and for y <= -2^31,
+// giving the same value as a float -> int64 conversion without its sign
+// handling; for -2^31 < y < 0, lo = 2^32 + y rounds to FP32 and the word lies
+// within 2^8 units of y (deterministic all the same).
 __device__ __forceinline__ unsigned long long to_fixed(float x, float s) {
     const float y = rintf(x * s);
     const float hi = floorf(y * 2.3283064365386963e-10f);   // 2^-32
-    const float lo = fmaf(hi, -4294967296.0f, y);            // exact
+    const float lo = fmaf(hi, -4294967296.0f, y);
     return ((unsigned long long)(uint32_t)(int)hi << 32) | (uint32_t)lo;
 }
 
@@ -396,6 +444,18 @@ __device__ __forceinline__ void publish_max(unsigned *track, const float (&m)[kM
         for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
         if (__lane_id() == 0 && v > 0.0f) atomicMax(track + c, __float_as_uint(v));
     }
+}
+
+// The fixed-point word of a product already in fixed-point units, y = (w v) s
+// with |y| < 2^62, computed as w * (v s) (s a power of two, so the product
+// rounds once, as (w v) s would): hi = floor(y / 2^32) and lo = y - hi 2^32
+// in FP32.  Both are exact for |y| >= 2^31 (y then has no bits below 2^8);
+// below that the word lies within 2^8 units of y.  Deterministic either way,
+// so the integer sums still do not depend on the order of the atomics.
+__device__ __forceinline__ unsigned long long fixed_word(float y) {
+    const float hi = floorf(y * 2.3283064365386963e-10f);   // 2^-32
+    const float lo = fmaf(hi, -4294967296.0f, y);
+    return ((unsigned long long)(uint32_t)(int)hi << 32) | (uint32_t)lo;
 }
 
 __device__ __forceinline__ uint64_t lanemask_lt() {

@@ -40,7 +40,7 @@ struct PixSrc {
     const float *ref;
     int64_t ref_cs;
     int cr;
-    int W, H;
+    int W, H, F;     // image size (F frames: rows t * H + y)
     int fx, fy, ft;
     float ic, ix, iy, it;
     // values: nval planes, plus a constant-1 channel when homog

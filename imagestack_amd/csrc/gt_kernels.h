@@ -17,31 +17,47 @@ constexpr int kTileKeys = 512;       // LDS tile table slots of the splat and sl
 // pixel sources
 
 // Pixel index of thread `tid` of workgroup `blk`: plain linear order, or a
-// tile_w x (kBlock/tile_w) tile of an image with rows of row_w pixels.
-__device__ __forceinline__ int pixel_index(int blk, int tid, int n, int row_w, int tile_w) {
+// tile_w x (kBlock/tile_w) tile of an image with rows of row_w pixels
+// (tile_w a divisor of kBlock, so a power of two).  A tile also gives the
+// pixel's column x and row (t * H + y), which spares load_position its
+// divisions; in linear order x = row = -1.
+__device__ __forceinline__ int pixel_index(int blk, int tid, int n, int row_w, int tile_w, int &x, int &row) {
     if (tile_w <= 0 || row_w <= 0) {
         const int64_t p = (int64_t)blk * kBlock + tid;
+        x = row = -1;
         return p < n ? (int)p : -1;
     }
-    const int tiles_x = (row_w + tile_w - 1) / tile_w;
-    const int th = kBlock / tile_w;
-    const int bx = blk % tiles_x, by = blk / tiles_x;
-    const int x = bx * tile_w + tid % tile_w;
-    const int y = by * th + tid / tile_w;
+    const int lg = __builtin_ctz((unsigned)tile_w);
+    const int tiles_x = (row_w + tile_w - 1) >> lg;
+    const int bx = blk % tiles_x, by = blk / tiles_x;   // the same for the whole workgroup
+    x = (bx << lg) + (tid & (tile_w - 1));
+    row = by * (kBlock >> lg) + (tid >> lg);             // kBlock / tile_w rows per tile
     const int rows = n / row_w;
-    return (x < row_w && y < rows) ? y * row_w + x : -1;
+    return (x < row_w && row < rows) ? row * row_w + x : -1;
 }
 
+__device__ __forceinline__ int pixel_index(int blk, int tid, int n, int row_w, int tile_w) {
+    int x, row;
+    return pixel_index(blk, tid, n, row_w, tile_w, x, row);
+}
+
+// Position of pixel p at column x and row t * H + y (x < 0: not known, found
+// from p).
 template <int D, bool BILAT>
-__device__ __forceinline__ void load_position(const PixSrc &s, int p, float (&pos)[D]) {
+__device__ __forceinline__ void load_position(const PixSrc &s, int p, int x, int row, float (&pos)[D]) {
     if constexpr (BILAT) {
         // (ref_c * (1.0f/sigma_r), x*(1.0f/sigma_x), y*(1.0f/sigma_y),
         //  t*(1.0f/sigma_t)) for the filtered axes, in that order
         // (src/GaussTransform.cpp:654-666).
-        const int x = p % s.W;
-        const int q = p / s.W;
-        const int y = q % s.H;
-        const int t = q / s.H;
+        if (x < 0) {
+            row = p / s.W;
+            x = p - row * s.W;
+        }
+        int y = row, t = 0;
+        if (s.F > 1) {   // (also when t is not filtered: the rows still run over the frames)
+            t = row / s.H;
+            y = row - t * s.H;
+        }
         const float xs = (float)x * s.ix, ys = (float)y * s.iy, ts = (float)t * s.it;
         const float a0 = s.fx ? xs : (s.fy ? ys : ts);
         const float a1 = s.fx ? (s.fy ? ys : ts) : ts;
@@ -61,6 +77,11 @@ __device__ __forceinline__ void load_position(const PixSrc &s, int p, float (&po
 #pragma unroll
         for (int c = 0; c < D; c++) pos[c] = s.pos[(int64_t)c * s.pos_cs + p] * s.inv[c];
     }
+}
+
+template <int D, bool BILAT>
+__device__ __forceinline__ void load_position(const PixSrc &s, int p, float (&pos)[D]) {
+    load_position<D, BILAT>(s, p, -1, -1, pos);
 }
 
 template <int NV>
@@ -291,7 +312,8 @@ __global__ __launch_bounds__(kBlock) void k_splat_tile(PixSrc src0, int n0, int 
     for (int i = threadIdx.x; i < LT * W; i += kBlock) lk[i] = kEmpty;
     for (int i = threadIdx.x; i < LVD * LT; i += kBlock) lv[i] = 0ull;
     if (threadIdx.x == 0) lcount = 0;
-    const int p = pixel_index(tv.local, threadIdx.x, tv.n, tv.row_w, tile_w);
+    int px, prow;
+    const int p = pixel_index(tv.local, threadIdx.x, tv.n, tv.row_w, tile_w, px, prow);
     const bool full = *(volatile const unsigned *)&ctr->flags & kFlagFull;
     __syncthreads();
 
@@ -302,7 +324,7 @@ __global__ __launch_bounds__(kBlock) void k_splat_tile(PixSrc src0, int n0, int 
 
     if (p >= 0) {
         float pos[D], v[kMaxVD], vorig[kMaxVD];
-        load_position<D, BILAT>(src, p, pos);
+        load_position<D, BILAT>(src, p, px, prow, pos);
         load_values<LVD>(src, p, v);
         unsigned nfmask = 0;
         if (nfcall) {
@@ -310,6 +332,9 @@ __global__ __launch_bounds__(kBlock) void k_splat_tile(PixSrc src0, int n0, int 
             for (int c = 0; c < LVD; c++) vorig[c] = v[c];
             nfmask = split_nonfinite<LVD>(vorig, vd, v);
         }
+        // values in fixed-point units once per pixel: w * (v s) rounds as (w v) s
+#pragma unroll
+        for (int c = 0; c < LVD; c++) v[c] *= sc[c];
         Simplex<D> s;
         simplex<D>(pos, lc.sf, s);
         if (s.bad) {
@@ -322,13 +347,22 @@ __global__ __launch_bounds__(kBlock) void k_splat_tile(PixSrc src0, int n0, int 
             uint64_t key[D + 1][W];
             float wt[D + 1];
             int okr[D + 1];
+            if constexpr (PACKED) {
+                uint64_t kk[D + 1];
+                const int ok = packed_vertex_keys<D>(s, kp, tv.tag, kk) ? 1 : 0;
 #pragma unroll
-            for (int r = 0; r <= D; r++) {
-                okr[r] = K::make(s, r, kp, tv.tag, key[r]) ? 1 : 0;
-                wt[r] = s.b[r];
+                for (int r = 0; r <= D; r++) {
+                    key[r][0] = kk[r];
+                    okr[r] = ok;
+                }
+            } else {
+#pragma unroll
+                for (int r = 0; r <= D; r++) okr[r] = K::make(s, r, kp, tv.tag, key[r]) ? 1 : 0;
             }
+#pragma unroll
+            for (int r = 0; r <= D; r++) wt[r] = s.b[r];
             const int rot = (int)(threadIdx.x % (D + 1));
-            rotate_left<D + 1>(okr, rot);
+            if constexpr (!PACKED) rotate_left<D + 1>(okr, rot);   // packed: one flag for all vertices
             rotate_left<D + 1>(wt, rot);
 #pragma unroll
             for (int w = 0; w < W; w++) {
@@ -347,7 +381,7 @@ __global__ __launch_bounds__(kBlock) void k_splat_tile(PixSrc src0, int n0, int 
                 if (ls >= 0) {
 #pragma unroll
                     for (int c = 0; c < LVD; c++)
-                        if (c < vd) atomicAdd(&lv[c * LT + ls], to_fixed(wt[r] * v[c], sc[c]));
+                        if (c < vd) atomicAdd(&lv[c * LT + ls], fixed_word(wt[r] * v[c]));
                 } else if (!full) {
                     const int slot = table_insert<W>(ta.tkeys, ta.mask, k, home_slot<D, PACKED, BATCH>(ta, kp, k), probes);
                     inserts++;
@@ -355,7 +389,7 @@ __global__ __launch_bounds__(kBlock) void k_splat_tile(PixSrc src0, int n0, int 
                     unsigned long long *dst = ta.tacc + (size_t)slot * vd;
 #pragma unroll
                     for (int c = 0; c < LVD; c++)
-                        if (c < vd) atomicAdd(dst + c, to_fixed(wt[r] * v[c], sc[c]));
+                        if (c < vd) atomicAdd(dst + c, fixed_word(wt[r] * v[c]));
                     atomics += vd;
                 }
             }
@@ -404,13 +438,14 @@ __global__ __launch_bounds__(kBlock) void k_splat_direct(PixSrc src, int n, int 
                                                          Counters *ctr, int count) {
     using K = KeyOps<D, PACKED>;
     constexpr int W = K::W;
-    const int p = pixel_index(xcd_tile(blockIdx.x, gridDim.x), threadIdx.x, n, row_w, tile_w);
+    int px, prow;
+    const int p = pixel_index(xcd_tile(blockIdx.x, gridDim.x), threadIdx.x, n, row_w, tile_w, px, prow);
     unsigned flags = 0;
     uint32_t probes = 0, inserts = 0, atomics = 0;
     const bool full = *(volatile const unsigned *)&ctr->flags & kFlagFull;
     if (p >= 0 && !full) {
         float pos[D], v[kMaxVD], vorig[kMaxVD];
-        load_position<D, BILAT>(src, p, pos);
+        load_position<D, BILAT>(src, p, px, prow, pos);
         load_values<kMaxVD>(src, p, v);
         unsigned nfmask = 0;
         if (vsc->nonfinite) {
@@ -418,23 +453,35 @@ __global__ __launch_bounds__(kBlock) void k_splat_direct(PixSrc src, int n, int 
             for (int c = 0; c < kMaxVD; c++) vorig[c] = v[c];
             nfmask = split_nonfinite<kMaxVD>(vorig, vd, v);
         }
+        // values in fixed-point units once per pixel: w * (v s) rounds as (w v) s
+#pragma unroll
+        for (int c = 0; c < kMaxVD; c++)
+            if (c < vd) v[c] *= vsc->s[c];
         Simplex<D> s;
         simplex<D>(pos, lc.sf, s);
         if (s.bad) {
             flags |= kFlagBad;
         } else {
             if (nfmask) mark_nonfinite<D, PACKED, false>(s, kp, 0ull, ta, vorig, nfmask, vd, probes, flags);
+            uint64_t pk[D + 1];
+            bool pk_ok = true;
+            if constexpr (PACKED) pk_ok = packed_vertex_keys<D>(s, kp, 0ull, pk);
 #pragma unroll
             for (int r = 0; r <= D; r++) {
                 uint64_t k[W];
-                if (!K::make(s, r, kp, 0ull, k)) { flags |= kFlagRange; continue; }
+                if constexpr (PACKED) {
+                    k[0] = pk[r];
+                    if (!pk_ok) { flags |= kFlagRange; continue; }
+                } else {
+                    if (!K::make(s, r, kp, 0ull, k)) { flags |= kFlagRange; continue; }
+                }
                 const int slot = table_insert<W>(ta.tkeys, ta.mask, k, home_slot<D, PACKED, false>(ta, kp, k), probes);
                 inserts++;
                 if (slot < 0) { flags |= kFlagFull; continue; }
                 unsigned long long *dst = ta.tacc + (size_t)slot * vd;
 #pragma unroll
                 for (int c = 0; c < kMaxVD; c++)
-                    if (c < vd) atomicAdd(dst + c, to_fixed(s.b[r] * v[c], vsc->s[c]));
+                    if (c < vd) atomicAdd(dst + c, fixed_word(s.b[r] * v[c]));
                 atomics += vd;
             }
         }
@@ -1158,7 +1205,8 @@ __global__ __launch_bounds__(kBlock) void k_slice_tile(PixSrc src0, int n0, int 
     const PixSrc &src = tv.src;
     for (int i = threadIdx.x; i < LT * W; i += kBlock) lk[i] = kEmpty;
     if (threadIdx.x == 0) lcount = 0;
-    const int p = pixel_index(tv.local, threadIdx.x, tv.n, tv.row_w, tile_w);
+    int px, prow;
+    const int p = pixel_index(tv.local, threadIdx.x, tv.n, tv.row_w, tile_w, px, prow);
     __syncthreads();
 
     int lsl[D + 1];   // LDS slot, -1 absent (outside the plan), -2 look up globally
@@ -1168,7 +1216,7 @@ __global__ __launch_bounds__(kBlock) void k_slice_tile(PixSrc src0, int n0, int 
     for (int r = 0; r <= D; r++) lsl[r] = -1;
     if (p >= 0) {
         float pos[D];
-        load_position<D, BILAT>(src, p, pos);
+        load_position<D, BILAT>(src, p, px, prow, pos);
         simplex<D>(pos, lc.sf, s);
         ok = !s.bad;
         if (!ok) {
@@ -1178,8 +1226,20 @@ __global__ __launch_bounds__(kBlock) void k_slice_tile(PixSrc src0, int n0, int 
             // as in the splat), then restore vertex order for the sums.
             uint64_t key[D + 1][W];
             int okr[D + 1];
+            bool all = false;
+            if constexpr (PACKED) {
+                uint64_t kk[D + 1];
+                all = packed_vertex_keys<D>(s, kp, tv.tag, kk);
 #pragma unroll
-            for (int r = 0; r <= D; r++) okr[r] = K::make(s, r, kp, tv.tag, key[r]) ? 1 : 0;
+                for (int r = 0; r <= D; r++) {
+                    key[r][0] = kk[r];
+                    okr[r] = 1;
+                }
+            }
+            if (!all) {   // wide keys, or a simplex across the plan's edge (slice positions of their own)
+#pragma unroll
+                for (int r = 0; r <= D; r++) okr[r] = K::make(s, r, kp, tv.tag, key[r]) ? 1 : 0;
+            }
             const int rot = (int)(threadIdx.x % (D + 1));
             rotate_left<D + 1>(okr, rot);
 #pragma unroll
@@ -1274,17 +1334,21 @@ __global__ __launch_bounds__(kBlock) void k_slice_direct(PixSrc src, int n, int 
         if (s.bad) {
             flags |= kFlagBad;
         } else {
+            float un[kMaxVD];
+            slice_unscale<T, kMaxVD>(sbits, un);
+            uint64_t pk[D + 1];
+            bool all = false;
+            if constexpr (PACKED) all = packed_vertex_keys<D>(s, kp, 0ull, pk);
 #pragma unroll
             for (int r = 0; r <= D; r++) {
                 uint64_t k[W];
-                if (!K::make(s, r, kp, 0ull, k)) continue;
+                if (PACKED && all) k[0] = pk[r];
+                else if (!K::make(s, r, kp, 0ull, k)) continue;   // outside the plan: absent
                 const int id = table_vertex<D, PACKED, false>(ta, kp, k, probes);
                 if (id < 0) continue;
                 float v[kMaxVD];
                 load_row_rt<T>(vals, id, vs, v);
                 if (!std::is_same<T, float>::value) {
-                    float un[kMaxVD];
-                    slice_unscale<T, kMaxVD>(sbits, un);
 #pragma unroll
                     for (int c = 0; c < kMaxVD; c++) v[c] *= un[c];
                 }
