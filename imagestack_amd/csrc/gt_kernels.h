@@ -61,16 +61,20 @@ __device__ __forceinline__ void load_position(const PixSrc &s, int p, int x, int
         const float xs = (float)x * s.ix, ys = (float)y * s.iy, ts = (float)t * s.it;
         const float a0 = s.fx ? xs : (s.fy ? ys : ts);
         const float a1 = s.fx ? (s.fy ? ys : ts) : ts;
+        // Every channel's load is issued before any is used (a branch per
+        // channel would wait for each in turn): channels past the guide's
+        // read its last plane again, a cache hit, and are not used.
+        float g[D];
+#pragma unroll
+        for (int c = 0; c < D; c++) g[c] = 0.0f;
+        if (s.cr > 0) {
+#pragma unroll
+            for (int c = 0; c < D; c++) g[c] = s.ref[(int64_t)min(c, s.cr - 1) * s.ref_cs + p];
+        }
 #pragma unroll
         for (int c = 0; c < D; c++) {
-            float v;
-            if (c < s.cr) {
-                v = s.ref[(int64_t)c * s.ref_cs + p] * s.ic;
-            } else {
-                const int k = c - s.cr;
-                v = (k == 0) ? a0 : (k == 1 ? a1 : ts);
-            }
-            pos[c] = v;
+            const int k = c - s.cr;
+            pos[c] = (c < s.cr) ? g[c] * s.ic : ((k == 0) ? a0 : (k == 1 ? a1 : ts));
         }
     } else {
         // pos[c] = splat(x,y,t,c) * invSigma[c] (src/GaussTransform.cpp:176-178)
@@ -84,15 +88,20 @@ __device__ __forceinline__ void load_position(const PixSrc &s, int p, float (&po
     load_position<D, BILAT>(s, p, -1, -1, pos);
 }
 
+// Values of pixel p: nval planes, then a constant 1 when homog, zeros after.
+// All loads are issued before any is used (channels past the planes read the
+// last plane again, a cache hit).
 template <int NV>
 __device__ __forceinline__ void load_values(const PixSrc &s, int p, float (&v)[kMaxVD]) {
+    float l[kMaxVD];
 #pragma unroll
-    for (int c = 0; c < NV; c++) {
-        float x = 0.0f;
-        if (c < s.nval) x = s.val[(int64_t)c * s.val_cs + p];
-        else if (c == s.nval && s.homog) x = 1.0f;
-        v[c] = x;
+    for (int c = 0; c < NV; c++) l[c] = 0.0f;
+    if (s.nval > 0) {
+#pragma unroll
+        for (int c = 0; c < NV; c++) l[c] = s.val[(int64_t)min(c, s.nval - 1) * s.val_cs + p];
     }
+#pragma unroll
+    for (int c = 0; c < NV; c++) v[c] = (c < s.nval) ? l[c] : ((c == s.nval && s.homog) ? 1.0f : 0.0f);
 }
 
 // OR of per-lane flags, one atomic per wave that has any (rare paths only).
@@ -324,8 +333,8 @@ __global__ __launch_bounds__(kBlock) void k_splat_tile(PixSrc src0, int n0, int 
 
     if (p >= 0) {
         float pos[D], v[kMaxVD], vorig[kMaxVD];
+        load_values<LVD>(src, p, v);   // (issued first: its loads and the positions' are in flight together)
         load_position<D, BILAT>(src, p, px, prow, pos);
-        load_values<LVD>(src, p, v);
         unsigned nfmask = 0;
         if (nfcall) {
 #pragma unroll
@@ -399,29 +408,35 @@ __global__ __launch_bounds__(kBlock) void k_splat_tile(PixSrc src0, int n0, int 
     list_tile<W, LT>(lk, llist, &lcount);
     __syncthreads();
 
-    // Flush: LVD lanes per distinct key, lane c adding channel c, so one
-    // instruction's atomics cover whole contiguous vertex rows.  Once any
-    // insert has found the table full the call is redone with a larger table:
-    // skip the remaining global work (a stale read only costs time).
+    // Flush, once any insert has found the table full skipping the global
+    // work (the call is redone with a larger table; a stale read only costs
+    // time).  First one thread per distinct key inserts it into the global
+    // table, all keys of the tile in one round of table latency, and parks the
+    // slot in the key's tile-table word (-1: table full).  Then LVD lanes per
+    // key, lane c adding channel c, so one instruction's atomics cover whole
+    // contiguous vertex rows; those need no reply and nothing waits on them.
     const unsigned cnt = full ? 0u : lcount;
+    for (unsigned j = threadIdx.x; j < cnt; j += kBlock) {
+        const int ls = (int)llist[j];
+        uint64_t k[W];
+#pragma unroll
+        for (int w = 0; w < W; w++) k[w] = lk[ls * W + w];
+        const int slot = table_insert<W>(ta.tkeys, ta.mask, k, home_slot<D, PACKED, BATCH>(ta, kp, k), probes);
+        inserts++;
+        if (slot < 0) flags |= kFlagFull;
+        lk[ls * W] = (uint64_t)(int64_t)slot;
+    }
+    __syncthreads();
     const int c = (int)(threadIdx.x % LVD);
     for (unsigned base = 0; base < cnt; base += kBlock / LVD) {
         const unsigned j = base + threadIdx.x / LVD;
-        const bool act = j < cnt;
-        const int ls = act ? (int)llist[j] : 0;
-        int slot = -1;
-        if (act && c == 0) {
-            uint64_t k[W];
-#pragma unroll
-            for (int w = 0; w < W; w++) k[w] = lk[ls * W + w];
-            slot = table_insert<W>(ta.tkeys, ta.mask, k, home_slot<D, PACKED, BATCH>(ta, kp, k), probes);
-            inserts++;
-            if (slot < 0) flags |= kFlagFull;
-        }
-        slot = __shfl(slot, (int)(__lane_id() & ~(LVD - 1)), kWave);
-        if (act && slot >= 0 && c < vd) {
-            atomicAdd(ta.tacc + (size_t)slot * vd + c, lv[c * LT + ls]);
-            atomics++;
+        if (j < cnt && c < vd) {
+            const int ls = (int)llist[j];
+            const int slot = (int)(int64_t)lk[ls * W];
+            if (slot >= 0) {
+                atomicAdd(ta.tacc + (size_t)slot * vd + c, lv[c * LT + ls]);
+                atomics++;
+            }
         }
     }
     report_flags(ctr, flags);
@@ -445,8 +460,8 @@ __global__ __launch_bounds__(kBlock) void k_splat_direct(PixSrc src, int n, int 
     const bool full = *(volatile const unsigned *)&ctr->flags & kFlagFull;
     if (p >= 0 && !full) {
         float pos[D], v[kMaxVD], vorig[kMaxVD];
+        load_values<kMaxVD>(src, p, v);   // (issued first: its loads and the positions' are in flight together)
         load_position<D, BILAT>(src, p, px, prow, pos);
-        load_values<kMaxVD>(src, p, v);
         unsigned nfmask = 0;
         if (vsc->nonfinite) {
 #pragma unroll
