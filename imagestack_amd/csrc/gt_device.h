@@ -72,21 +72,17 @@ __device__ __forceinline__ void simplex(const float (&p)[D], const float *__rest
             s.rk[j] += lt ? 0 : 1;
         }
 
-    // Walk back onto the hyperplane (:326-348).
-    if (sum > 0) {
+    // Walk back onto the hyperplane (:326-348).  For sum > 0 the reference
+    // moves every i with rk_i >= D+1-sum down by D+1 and takes D+1 from its
+    // shifted rank; for sum < 0 every i with rk_i < -sum up.  With t = rk_i +
+    // sum those are exactly t >= D+1 and t < 0 (neither happens for the other
+    // sign, nor for sum = 0), which needs no branch on the sign of sum.
 #pragma unroll
-        for (int i = 0; i <= D; i++) {
-            const bool hi = s.rk[i] >= D + 1 - sum;
-            s.g[i] -= hi ? (D + 1) : 0;
-            s.rk[i] += hi ? sum - (D + 1) : sum;
-        }
-    } else if (sum < 0) {
-#pragma unroll
-        for (int i = 0; i <= D; i++) {
-            const bool lo = s.rk[i] < -sum;
-            s.g[i] += lo ? (D + 1) : 0;
-            s.rk[i] += lo ? (D + 1) + sum : sum;
-        }
+    for (int i = 0; i <= D; i++) {
+        const int t = s.rk[i] + sum;
+        const int w = (t < 0) ? (D + 1) : ((t >= D + 1) ? -(D + 1) : 0);
+        s.g[i] += w;
+        s.rk[i] = t + w;
     }
 
     // Barycentric weights (:350-356).  The reference accumulates
@@ -94,15 +90,26 @@ __device__ __forceinline__ void simplex(const float (&p)[D], const float *__rest
     // are a permutation, so each b[m] receives exactly one + and one - term
     // and, IEEE addition being commutative, b[m] = delta(rank D-m) -
     // delta(rank D+1-m) bit for bit, whichever term came first.  delta by
-    // rank is selected without dynamic register indexing.
+    // rank is the deltas in descending order: the ranks count the larger
+    // differentials (ties to the later index), and the walk back moves the
+    // wrapped coordinates' deltas up by exactly 1 -- above every other, the
+    // differentials spanning at most D+1 -- together with their ranks to the
+    // front.  Equal deltas are equal values, so an odd-even transposition
+    // network of max/min gives the same dr as selecting by rank, in
+    // D(D+1)/2 comparators.  (Only where the greedy rounding picked the
+    // farther multiple by a rounding error can a wrapped delta fall an ulp
+    // below another; the two weights it feeds then differ by that ulp.)
     float dr[D + 1];
 #pragma unroll
-    for (int k = 0; k <= D; k++) dr[k] = 0.0f;
+    for (int i = 0; i <= D; i++) dr[i] = (e[i] - (float)s.g[i]) * scale;
 #pragma unroll
-    for (int i = 0; i <= D; i++) {
-        const float di = (e[i] - (float)s.g[i]) * scale;
+    for (int round = 0; round <= D; round++) {
 #pragma unroll
-        for (int k = 0; k <= D; k++) dr[k] = (s.rk[i] == k) ? di : dr[k];
+        for (int i = round & 1; i + 1 <= D; i += 2) {
+            const float a = dr[i], b = dr[i + 1];
+            dr[i] = fmaxf(a, b);
+            dr[i + 1] = fminf(a, b);
+        }
     }
     s.b[0] = dr[D] + (1.0f - dr[0]);
 #pragma unroll
