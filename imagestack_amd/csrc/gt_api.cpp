@@ -849,6 +849,23 @@ int make_plan_batch(gt_ctx *c, const BatchSpec &b, const std::vector<gt::BatchIm
     return GT_OK;
 }
 
+// Sums the statistics of one group of a call run as several lattices.
+void add_stats(gt_stats &t, const gt_stats &g) {
+    t.M = g.M;
+    t.capacity = std::max(t.capacity, g.capacity);
+    t.retries += g.retries;
+    t.sort_passes += g.sort_passes;
+    t.ms_splat += g.ms_splat;
+    t.ms_sort += g.ms_sort;
+    t.ms_nbr += g.ms_nbr;
+    t.ms_blur += g.ms_blur;
+    t.ms_slice += g.ms_slice;
+    t.ms_total += g.ms_total;
+    t.probes += g.probes;
+    t.inserts += g.inserts;
+    t.atomics += g.atomics;
+}
+
 int check_dims(int d, int vd) {
     if (d < 1 || d > GT_MAX_D) return fail(GT_EINVAL, "position dimension d=%d outside [1, %d]", d, GT_MAX_D);
     if (vd < 1 || vd > GT_MAX_VD) return fail(GT_EINVAL, "value dimension vd=%d outside [1, %d]", vd, GT_MAX_VD);
@@ -1155,6 +1172,25 @@ int gt_gauss_transform(gt_ctx *c, int n, int row_w, int d, int vd, const float *
                        int n_slice, const float *slice, const float *inv_sigma, float *out, gt_storage storage,
                        const gt_tuning *tune, gt_stats *st) {
     if (!c) return fail(GT_EINVAL, "null context");
+    if (vd > GT_MAX_VD) {
+        // More value channels than one lattice carries (the reference takes
+        // any number, src/Permutohedral.h:229-240): groups of GT_MAX_VD, each
+        // a lattice over the same positions.  A channel's sums, blur and
+        // slice never involve another channel, so every channel comes out as
+        // it would from one lattice of all of them.
+        if (n < 0 || n_slice < 0) return fail(GT_EINVAL, "negative pixel count");
+        const int ns = (slice && slice != splat) ? n_slice : n;
+        gt_stats total{};
+        for (int c0 = 0; c0 < vd; c0 += GT_MAX_VD) {
+            const int g = std::min(GT_MAX_VD, vd - c0);
+            gt_stats gs{};
+            GT_TRY(gt_gauss_transform(c, n, row_w, d, g, splat, values ? values + (size_t)c0 * n : nullptr, n_slice,
+                                      slice, inv_sigma, out ? out + (size_t)c0 * ns : nullptr, storage, tune, &gs));
+            add_stats(total, gs);
+        }
+        if (st) *st = total;
+        return GT_OK;
+    }
     GT_TRY(check_dims(d, vd));
     GT_TRY(check_storage(storage));
     if (n < 0 || n_slice < 0) return fail(GT_EINVAL, "negative pixel count");
@@ -1196,7 +1232,7 @@ int gt_gauss_transform_host(gt_ctx *c, int n, int row_w, int d, int vd, const fl
                             int n_slice, const float *slice, const float *inv_sigma, float *out,
                             gt_storage storage, const gt_tuning *tune, gt_stats *st) {
     if (!c) return fail(GT_EINVAL, "null context");
-    GT_TRY(check_dims(d, vd));
+    GT_TRY(check_dims(d, std::min(vd, GT_MAX_VD)));   // (more channels run as groups)
     if (n < 0 || n_slice < 0) return fail(GT_EINVAL, "negative pixel count");
     const bool replay = (slice == nullptr || slice == splat);
     const int ns = replay ? n : n_slice;
@@ -1224,16 +1260,28 @@ static int jb_entry(gt_ctx *c, int W, int H, int F, int C, const float *im, int 
     if (!c) return fail(GT_EINVAL, "null context");
     if (W < 1 || H < 1 || F < 1 || C < 1 || Cr < 1) return fail(GT_EINVAL, "bad image size %dx%dx%d", W, H, F);
     if (!im || !ref || !out) return fail(GT_EINVAL, "null image pointer");
-    if (C + 1 > GT_MAX_VD) return fail(GT_EINVAL, "too many channels (%d)", C);
     GT_TRY(check_storage(storage));
     GT_HIP(hipSetDevice(c->device));
     if (st) memset(st, 0, sizeof(*st));
     Tune tu = resolve(tune);
     tu.grid = grid;
     const int64_t n = (int64_t)W * H * F;
-    View a{im, W, H, F, W, (int64_t)W * H, n}, b{ref, W, H, F, W, (int64_t)W * H, n},
-        o{out, W, H, F, W, (int64_t)W * H, n};
-    return jb_view(c, a, C, b, Cr, o, fw, fh, ff, color_sigma, storage, tu, st);
+    // Image channels beyond GT_MAX_VD - 1 (with the homogeneous one, the
+    // values one lattice carries) run as groups against the same guide: a
+    // channel's sums, blur, slice and normalisation never involve another
+    // image channel, so each comes out as from one lattice of all of them.
+    const int per = GT_MAX_VD - 1;
+    gt_stats total{};
+    for (int c0 = 0; c0 < C; c0 += per) {
+        const int g = std::min(per, C - c0);
+        View a{im + (size_t)c0 * n, W, H, F, W, (int64_t)W * H, n}, b{ref, W, H, F, W, (int64_t)W * H, n},
+            o{out + (size_t)c0 * n, W, H, F, W, (int64_t)W * H, n};
+        gt_stats gs{};
+        GT_TRY(jb_view(c, a, g, b, Cr, o, fw, fh, ff, color_sigma, storage, tu, st ? &gs : nullptr));
+        add_stats(total, gs);
+    }
+    if (st) *st = total;
+    return GT_OK;
 }
 
 int gt_joint_bilateral(gt_ctx *c, int W, int H, int F, int C, const float *im, int Cr, const float *ref, float fw,

@@ -33,21 +33,40 @@ GaussTransform::Method methodByName(const string &s) {
     return GaussTransform::AUTO;
 }
 
-// The method JointBilateral::apply would use for this call, or AUTO when the
-// zero-sigma recursion (:512-538) splits the image first (each piece then
-// chooses on its own, so the reference path is used for it).
+// The shape of the pieces JointBilateral::apply filters: a zero spatial sigma
+// on an axis longer than 1 splits the image into columns, rows or frames
+// (:512-538), every piece of the same shape; pw/ph/pf receive it and the
+// sigmas the pieces see.
+void pieceShape(const Image &im, float &fw, float &fh, float &ff, int &pw, int &ph, int &pf) {
+    pw = im.width;
+    ph = im.height;
+    pf = im.frames;
+    if (pw == 1) fw = 0;
+    if (ph == 1) fh = 0;
+    if (pf == 1) ff = 0;
+    if (fw == 0) pw = 1;
+    if (fh == 0) ph = 1;
+    if (ff == 0) pf = 1;
+}
+
+// Position channels of the lattice each piece builds (:540-546).
+int pieceDims(const Image &im, const Image &ref, float fw, float fh, float ff) {
+    int pw, ph, pf;
+    pieceShape(im, fw, fh, ff, pw, ph, pf);
+    return ref.channels + (pw > 1 && fw < 10 * pw ? 1 : 0) + (ph > 1 && fh < 10 * ph ? 1 : 0) +
+           (pf > 1 && ff < 10 * pf ? 1 : 0);
+}
+
+// The method JointBilateral::apply uses for this call: m, or for AUTO the
+// choice each piece makes (:553-564) -- all pieces alike, so one choice.
 GaussTransform::Method resolveMethod(Image im, Image ref, float fw, float fh, float ff, GaussTransform::Method m) {
-    if (im.width == 1) fw = 0;
-    if (im.height == 1) fh = 0;
-    if (im.frames == 1) ff = 0;
-    if ((im.width > 1 && fw == 0) || (im.height > 1 && fh == 0) || (im.frames > 1 && ff == 0))
-        return m == GaussTransform::AUTO ? GaussTransform::AUTO : m;
     if (m != GaussTransform::AUTO) return m;
-    int posChannels = ref.channels;
-    bool fx = im.width > 1 && fw < 10 * im.width;
-    bool fy = im.height > 1 && fh < 10 * im.height;
-    bool ft = im.frames > 1 && ff < 10 * im.frames;
-    posChannels += (fx ? 1 : 0) + (fy ? 1 : 0) + (ft ? 1 : 0);
+    int pw, ph, pf;
+    pieceShape(im, fw, fh, ff, pw, ph, pf);
+    bool fx = pw > 1 && fw < 10 * pw;
+    bool fy = ph > 1 && fh < 10 * ph;
+    bool ft = pf > 1 && ff < 10 * pf;
+    int posChannels = ref.channels + (fx ? 1 : 0) + (fy ? 1 : 0) + (ft ? 1 : 0);
     int sx = fx ? ((int)(fw * 6 + 1) | 1) : 1;
     int sy = fy ? ((int)(fh * 6 + 1) | 1) : 1;
     int st = ft ? ((int)(ff * 6 + 1) | 1) : 1;
@@ -57,14 +76,18 @@ GaussTransform::Method resolveMethod(Image im, Image ref, float fw, float fh, fl
     return GaussTransform::GKDTREE;
 }
 
+// The library filters the zero-sigma pieces itself.  Methods it does not run
+// (EXACT, GKDTREE) and lattices beyond GT_MAX_D position channels (grids
+// beyond 6) go to the reference's own code.
 void jointBilateralDispatch(Image im, Image ref, float fw, float fh, float ff, float cs, GaussTransform::Method m) {
     const GaussTransform::Method r = resolveMethod(im, ref, fw, fh, ff, m);
-    if (r == GaussTransform::PERMUTOHEDRAL)
+    const int d = pieceDims(im, ref, fw, fh, ff);
+    if (r == GaussTransform::PERMUTOHEDRAL && d <= GT_MAX_D)
         jointBilateral(im, ref, fw, fh, ff, cs);
-    else if (r == GaussTransform::GRID && ref.channels + 3 <= 6)
+    else if (r == GaussTransform::GRID && d <= 6)
         jointBilateralGrid(im, ref, fw, fh, ff, cs);
     else
-        JointBilateral::apply(im, ref, fw, fh, ff, cs, m);
+        JointBilateral::apply(im, ref, fw, fh, ff, cs, r);
 }
 
 // Relative difference below tol everywhere (floor of 1e-3 of the largest
@@ -131,9 +154,12 @@ public:
             panic("-gausstransform takes one argument, two arguments, or one plus the"
                   " number of channels in the second image on the stack arguments\n");
         }
-        Image im = (m == PERMUTOHEDRAL)                           ? gaussTransform(stack(0), stack(1), stack(2), sigmas)
-                   : (m == GRID && stack(1).channels <= 6) ? gaussTransformGrid(stack(0), stack(1), stack(2), sigmas)
-                                                           : GaussTransform::apply(stack(0), stack(1), stack(2), sigmas, m);
+        // lattices of more than GT_MAX_D position channels (grids of more
+        // than 6) and the other methods run the reference's own code
+        const int d = stack(1).channels;
+        Image im = (m == PERMUTOHEDRAL && d <= GT_MAX_D) ? gaussTransform(stack(0), stack(1), stack(2), sigmas)
+                   : (m == GRID && d <= 6)               ? gaussTransformGrid(stack(0), stack(1), stack(2), sigmas)
+                                                         : GaussTransform::apply(stack(0), stack(1), stack(2), sigmas, m);
         pop();
         push(im);
     }

@@ -26,9 +26,14 @@
  *     layout of a dense ImageStack Image (src/Image.h:34-37, :57-67).
  *   - Functions without the _host suffix take DEVICE pointers and run on the
  *     context's stream; they return after the results are complete.
- *   - Limits: 1 <= d <= GT_MAX_D, 1 <= vd <= GT_MAX_VD, lattice key
- *     coordinates within int16 (the reference's key type,
- *     src/Permutohedral.h:28), N < 2^31.
+ *   - Limits: 1 <= d <= GT_MAX_D, lattice key coordinates within int16 (the
+ *     reference's key type, src/Permutohedral.h:28), N < 2^31.  Any number
+ *     of value channels: more than GT_MAX_VD (a joint bilateral's image: more
+ *     than GT_MAX_VD - 1) run as groups over the same positions, which gives
+ *     every channel the output of one lattice of all of them (a channel's
+ *     sums, blur and slice never involve another).
+ *   - NaN and Inf values spread through the sums, blur and slice as in the
+ *     reference's FP32 arithmetic; NaN/Inf positions are an error (GT_ERANGE).
  */
 #ifndef GTLATTICE_H
 #define GTLATTICE_H

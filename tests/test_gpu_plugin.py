@@ -133,3 +133,90 @@ def test_plugin_nlmeans(tmp_path):
     run(["-plugin", PLUGIN, "-load", src, "-nlmeans", "1", "6", "16", "0.4", "-save", dst])
     res = read_tmp(dst)
     assert np.abs(res - clean).mean() < 0.5 * np.abs(noisy - clean).mean()
+
+
+def same_bits(a, b):
+    return np.array_equal(np.ascontiguousarray(a, np.float32).view(np.uint32),
+                          np.ascontiguousarray(b, np.float32).view(np.uint32))
+
+
+def test_plugin_auto_zero_sigma_runs_on_the_gpu(tmp_path, oracle_lib, gpu_ctx):
+    """AUTO with a zero spatial sigma: JointBilateral filters each row (or
+    column) by itself and every piece makes the AUTO choice for its own shape
+    (src/GaussTransform.cpp:516-538, :553-564).  A grey image's rows then
+    take the grid, a 4-channel guide's rows the lattice; the plugin runs both
+    on the GPU (its output is the library's bit for bit) and agrees with the
+    reference CLI."""
+    import torch
+    need_cli()
+    im = S.rect_image(200, 48, 1, K=12, noise=0.05, seed=51)
+    src = str(tmp_path / "grey.tmp")
+    write_tmp(src, im)
+    cpu_path, gpu_path = str(tmp_path / "cpu.tmp"), str(tmp_path / "gpu.tmp")
+    run(["-load", src, "-bilateral", "0.1", "6", "0", "-save", cpu_path])
+    run(["-plugin", PLUGIN, "-load", src, "-bilateral", "0.1", "6", "0", "-save", gpu_path])
+    gpu = read_tmp(gpu_path)
+    lib_grid = gpu_ctx.joint_bilateral_grid(torch.from_numpy(im).cuda(), torch.from_numpy(im).cuda(), 6.0, 0.0,
+                                            0.0, 0.1).cpu().numpy()
+    assert same_bits(gpu, lib_grid)
+    assert rel_err(gpu, read_tmp(cpu_path)) < 2 * FP32_REL_TOL
+    guide = S.rect_image(200, 48, 4, K=12, noise=0.05, seed=52)
+    image = S.uniform_image(200, 48, 3, seed=53)
+    gp, ip = str(tmp_path / "guide.tmp"), str(tmp_path / "image.tmp")
+    write_tmp(gp, guide)
+    write_tmp(ip, image)
+    run(["-load", gp, "-load", ip, "-jointbilateral", "0.15", "6", "0", "-save", cpu_path])
+    run(["-plugin", PLUGIN, "-load", gp, "-load", ip, "-jointbilateral", "0.15", "6", "0", "-save", gpu_path])
+    gpu = read_tmp(gpu_path)
+    lib_lat = gpu_ctx.joint_bilateral(torch.from_numpy(image).cuda(), torch.from_numpy(guide).cuda(), 6.0, 0.0,
+                                      0.0, 0.15).cpu().numpy()
+    assert same_bits(gpu, lib_lat)
+    o64 = oracle_lib.joint_bilateral(image, guide, 6.0, 0.0, 0.0, 0.15, mode=oracle_lib.F64_LAT)
+    assert rel_err(gpu, o64) < FP32_REL_TOL
+    assert rel_err(gpu, read_tmp(cpu_path)) < 2 * FP32_REL_TOL
+
+
+def test_plugin_nonfinite_values_spread_as_in_reference(tmp_path):
+    """NaN and Inf pixels: the plugin returns an image, with NaN/+Inf/-Inf
+    exactly where the reference CLI puts them (src/Permutohedral.h:377)."""
+    need_cli()
+    im = S.rect_image(120, 90, 3, K=10, noise=0.05, seed=54)
+    im[0, 0, 10, 20] = np.nan
+    im[1, 0, 50, 70] = np.inf
+    im[2, 0, 51, 70] = -np.inf
+    src = str(tmp_path / "nan.tmp")
+    write_tmp(src, im)
+    cpu_path, gpu_path = str(tmp_path / "cpu.tmp"), str(tmp_path / "gpu.tmp")
+    guide = S.rect_image(120, 90, 3, K=10, noise=0.05, seed=55)
+    gp = str(tmp_path / "guide.tmp")
+    write_tmp(gp, guide)
+    run(["-load", gp, "-load", src, "-jointbilateral", "0.1", "5", "5", "0", "permutohedral", "-save", cpu_path])
+    run(["-plugin", PLUGIN, "-load", gp, "-load", src, "-jointbilateral", "0.1", "5", "5", "0", "permutohedral",
+         "-save", gpu_path])
+    cpu, gpu = read_tmp(cpu_path), read_tmp(gpu_path)
+    for f in (np.isnan, np.isposinf, np.isneginf):
+        assert np.array_equal(f(gpu), f(cpu)), f.__name__
+    assert np.isnan(gpu).any()
+    fin = np.isfinite(cpu)
+    assert rel_err(gpu[fin], cpu[fin]) < 2 * FP32_REL_TOL
+
+
+def test_plugin_lattice_beyond_16_dims_runs_the_reference(tmp_path):
+    """-gausstransform permutohedral over 17 position channels (the reference
+    allocates for any d, src/Permutohedral.h:229-240): beyond the library's
+    16 the plugin hands the call to the reference's own code, so the output
+    is the reference's bit for bit rather than an error."""
+    need_cli()
+    W, H = 24, 20
+    s = S.Stream(56)
+    pos = s.normal(17 * W * H).reshape(17, 1, H, W).astype(np.float32)
+    vals = np.concatenate([s.uniform(2 * W * H).reshape(2, 1, H, W), np.ones((1, 1, H, W))]).astype(np.float32)
+    vp, pp = str(tmp_path / "v.tmp"), str(tmp_path / "p.tmp")
+    write_tmp(vp, vals)
+    write_tmp(pp, pos)
+    outs = {}
+    for tag, plug in (("cpu", []), ("gpu", ["-plugin", PLUGIN])):
+        o = str(tmp_path / f"{tag}.tmp")
+        run(plug + ["-load", vp, "-load", pp, "-dup", "-gausstransform", "permutohedral", "1", "-save", o])
+        outs[tag] = read_tmp(o)
+    assert same_bits(outs["gpu"], outs["cpu"])

@@ -220,3 +220,27 @@ def test_nonfinite_batch(oracle_lib):
             check_nonfinite(out, o32, o64)
         else:
             assert np.all(np.isfinite(out)) and rel_err(out, o64) < FP32_REL_TOL
+
+
+def test_more_than_16_value_channels(gpu_ctx, oracle_lib):
+    """vd = 20 (the reference takes any number of value channels): run as
+    groups of 16 over the same positions, the output equals the groups run
+    one by one bit for bit, and the FP64 oracle of all 20 channels."""
+    O = oracle_lib
+    case("gauss transform d=5 vd=20")
+    W, H = 80, 64
+    guide = S.rect_image(W, H, 3, K=10, noise=0.05, seed=91)
+    pos = planar(S.bilateral_positions(guide, 6.0, 0.1))
+    val = S.Stream(92).uniform(20 * W * H).reshape(20, -1).astype(np.float32)
+    inv = np.ones(5, np.float32)
+    out = gt(gpu_ctx, pos, val, W)
+    a = gt(gpu_ctx, pos, val[:16], W)
+    b = gt(gpu_ctx, pos, val[16:], W)
+    assert same_bits(out, np.concatenate([a, b]))
+    o64, _ = O.gauss_transform(pos, val, inv, mode=O.F64_LAT)
+    assert rel_err(out, o64.reshape(out.shape)) < FP32_REL_TOL
+    case("joint bilateral with 17 image channels")
+    im = S.uniform_image(W, H, 17, seed=93)
+    got = jb(gpu_ctx, im, guide, 6.0, 0.1)
+    o64 = O.joint_bilateral(im, guide, 6.0, 6.0, 0.0, 0.1, mode=O.F64_LAT)
+    assert rel_err(got, o64) < FP32_REL_TOL
