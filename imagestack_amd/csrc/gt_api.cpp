@@ -98,6 +98,7 @@ struct gt_ctx {
         vals_a, vals_b, nbr, splat_copy, mm_part, mm_out;
     DevBuf stage_in, stage_out, view_a, view_b, view_o;
     DevBuf grid_acc, grid_a, grid_b;   // bilateral grid: fixed-point sums, FP32 ping-pong
+    DevBuf grid_marks;                 // bilateral grid: NaN/Inf marks (calls with such values only)
     DevBuf patch_w, patch_guide;       // non-local means: reordered filters, projected guide
     gt::Counters *h_ctr = nullptr;   // pinned
     float *h_mm = nullptr;           // pinned, 3 * 2 * kMaxDH
@@ -908,8 +909,10 @@ int run_grid(gt_ctx *c, const gt::PixSrc &src, int n, int d, int vd, const gt::P
                           c->stream));
     GT_HIP(hipStreamSynchronize(c->stream));
     const float *mm = c->h_mm, *vm = c->h_mm + (size_t)np * 3, *sm = c->h_mm + (size_t)(np + nval) * 3;
-    for (int k = 0; k < (n > 0 ? nval : 0); k++)
-        if (vm[3 * k + 2] != 0.0f) return fail(GT_ERANGE, "value channel %d holds NaN or Inf", k);
+    // NaN/Inf values spread as in the reference (per-element marks, see
+    // k_grid_splat); the scales come from the finite values
+    bool nonfinite = false;
+    for (int k = 0; k < (n > 0 ? nval : 0); k++) nonfinite |= vm[3 * k + 2] != 0.0f;
     gt::GridArgs g{};
     g.vd = vd;
     g.sf = sqrtf((float)(1.0 / 3 + (5 - 1) * 0.25));   // taps = 5 (src/GaussTransform.cpp:229)
@@ -956,7 +959,8 @@ int run_grid(gt_ctx *c, const gt::PixSrc &src, int n, int d, int vd, const gt::P
     }
     gt::ValScale vsc{};
     for (int k = 0; k < vd; k++) {
-        const float vmax = k < nval ? std::max(std::fabs(vm[3 * k]), std::fabs(vm[3 * k + 1])) : 1.0f;
+        float vmax = 1.0f;
+        if (k < nval) vmax = (vm[3 * k] <= vm[3 * k + 1]) ? std::max(std::fabs(vm[3 * k]), std::fabs(vm[3 * k + 1])) : 0.0f;
         const double bound = std::max(1e-30, (double)std::max(n, 1) * (double)vmax * 1.001);
         const int p = std::min(120, 62 - (int)std::ceil(std::log2(bound)));
         if (p < -120) return fail(GT_ERANGE, "values of channel %d are too large", k);
@@ -967,6 +971,10 @@ int run_grid(gt_ctx *c, const gt::PixSrc &src, int n, int d, int vd, const gt::P
     GT_TRY(ensure(c->grid_acc, total * 8));
     GT_TRY(ensure(c->grid_a, total * 4));
     GT_TRY(ensure(c->grid_b, total * 4));
+    if (nonfinite) {
+        GT_TRY(ensure(c->grid_marks, total * 4));
+        g.marks = (uint32_t *)c->grid_marks.p;
+    }
     GT_HIP(gt::launch_grid(d, bilat, src, n, bilat ? src.W : 0, slice, n_slice, dst, g, cells, vsc,
                            (unsigned long long *)c->grid_acc.p, (float *)c->grid_a.p, (float *)c->grid_b.p,
                            c->profiling ? c->ev : nullptr, c->stream));
@@ -1146,7 +1154,8 @@ void gt_ctx_destroy(gt_ctx *c) {
                       &c->pk_a,        &c->pk_b,      &c->pay_b,    &c->hist,      &c->sort_status,
                       &c->sort_hist,   &c->vkeys,     &c->vals_a,   &c->vals_b,    &c->nbr,       &c->splat_copy,
                       &c->mm_part,     &c->mm_out,    &c->stage_in, &c->stage_out, &c->view_a,    &c->view_b,
-                      &c->view_o,      &c->grid_acc,  &c->grid_a,   &c->grid_b,    &c->patch_w,   &c->patch_guide};
+                      &c->view_o,      &c->grid_acc,  &c->grid_a,   &c->grid_b,    &c->grid_marks,
+                      &c->patch_w,     &c->patch_guide};
     for (DevBuf *b : bufs) free_buf(*b);
     for (auto &e : c->ev)
         if (e) (void)hipEventDestroy(e);

@@ -73,10 +73,19 @@ __global__ __launch_bounds__(kBlock) void k_grid_splat(PixSrc src, int n, int ro
     float F[D], Fi[D];
     int I[D];
     const bool in = p >= 0 && grid_cell<D, BILAT>(src, p, g, base, F, Fi, I);
-    float v[kMaxVD];
+    float v[kMaxVD], vorig[kMaxVD];
 #pragma unroll
     for (int k = 0; k < kMaxVD; k++) v[k] = 0.0f;
     if (in) load_values<kMaxVD>(src, p, v);
+    // NaN/Inf values (the reference adds w * v in FP32, src/DenseGrid.h):
+    // the fixed-point sums take them as 0 and the corner cells get a mark of
+    // the product's class instead, which k_grid_finish turns into NaN / +-Inf
+    unsigned nfmask = 0;
+    if (g.marks) {
+#pragma unroll
+        for (int k = 0; k < kMaxVD; k++) vorig[k] = v[k];
+        nfmask = split_nonfinite<kMaxVD>(vorig, g.vd, v);
+    }
     __syncthreads();
     if (in) {
 #pragma unroll
@@ -129,6 +138,13 @@ __global__ __launch_bounds__(kBlock) void k_grid_splat(PixSrc src, int n, int ro
                 li += (I[j] + bit - lo[j]) * lstride[j];
             }
             if (!inside) continue;
+            if (nfmask) {
+                for (int k = 0; k < g.vd; k++) {
+                    if (!((nfmask >> k) & 1u)) continue;
+                    const float prod = w * vorig[k];
+                    atomicOr(g.marks + idx + k, (prod != prod) ? 3u : (prod > 0.0f ? 1u : 2u));
+                }
+            }
 #pragma unroll
             for (int k = 0; k < kMaxVD; k++) {
                 if (k >= g.vd) continue;
@@ -158,17 +174,26 @@ __global__ __launch_bounds__(kBlock) void k_grid_splat(PixSrc src, int n, int ro
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_grid_zero(unsigned long long *__restrict__ acc, int64_t total) {
-    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < total; i += (int64_t)gridDim.x * kBlock)
+__global__ __launch_bounds__(kBlock) void k_grid_zero(unsigned long long *__restrict__ acc, int64_t total,
+                                                      uint32_t *__restrict__ marks) {
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < total; i += (int64_t)gridDim.x * kBlock) {
         acc[i] = 0ull;
+        if (marks) marks[i] = 0u;
+    }
 }
 
 __global__ __launch_bounds__(kBlock) void k_grid_finish(const unsigned long long *__restrict__ acc, int64_t total,
-                                                        int vd, ValScale vsc, float *__restrict__ out) {
+                                                        int vd, ValScale vsc, const uint32_t *__restrict__ marks,
+                                                        float *__restrict__ out) {
     const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (e >= total) return;
     const int k = (int)(e % vd);
-    out[e] = (float)((double)(long long)acc[e] * (double)vsc.inv_s[k]);
+    float o = (float)((double)(long long)acc[e] * (double)vsc.inv_s[k]);
+    if (marks) {
+        const uint32_t m = marks[e];
+        o = m == 3u ? __builtin_nanf("") : m == 1u ? INFINITY : m == 2u ? -INFINITY : o;
+    }
+    out[e] = o;
 }
 
 // One round along a dimension with element stride s and n cells:
@@ -243,7 +268,7 @@ hipError_t launch_grid(int d, bool bilat, const PixSrc &splat, int n, int row_w,
                        unsigned long long *acc, float *a, float *b, hipEvent_t *ev, hipStream_t st) {
     const int64_t total = cells * g.vd;
     const unsigned zb = (unsigned)std::min<int64_t>(8192, (total + kBlock - 1) / kBlock);
-    hipLaunchKernelGGL(k_grid_zero, dim3(zb > 0 ? zb : 1), dim3(kBlock), 0, st, acc, total);
+    hipLaunchKernelGGL(k_grid_zero, dim3(zb > 0 ? zb : 1), dim3(kBlock), 0, st, acc, total, g.marks);
     if (ev) (void)hipEventRecord(ev[0], st);
     int tile_w = row_w > 0 ? 16 : 0;   // 16 x 16 pixel tiles of an image, else 256 consecutive pixels
     const int64_t ntiles = pixel_blocks(n, row_w, tile_w);
@@ -259,7 +284,7 @@ hipError_t launch_grid(int d, bool bilat, const PixSrc &splat, int n, int row_w,
 #undef GT_GSPLAT
     if (ev) (void)hipEventRecord(ev[1], st);
     const dim3 ge((unsigned)((total + kBlock - 1) / kBlock));
-    hipLaunchKernelGGL(k_grid_finish, ge, blk, 0, st, acc, total, g.vd, vsc, a);
+    hipLaunchKernelGGL(k_grid_finish, ge, blk, 0, st, acc, total, g.vd, vsc, g.marks, a);
     if (ev) (void)hipEventRecord(ev[2], st);
     if (ev) (void)hipEventRecord(ev[3], st);
     float *cur = a, *nxt = b;

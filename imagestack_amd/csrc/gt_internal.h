@@ -254,6 +254,8 @@ struct GridArgs {
     float mn[kMaxGridD];          // minimum of position * sf per dimension
     int size[kMaxGridD];          // ceil(max - min) + 1 cells
     int64_t stride[kMaxGridD];    // in elements: stride[0] = vd
+    uint32_t *marks;              // per element: the class of the NaN/Inf contributions (bit 0 +Inf, bit 1
+                                  // -Inf, both NaN), or null when every value is finite
 };
 // Zero, splat, finish, 2 blur rounds per dimension, slice.  ev (optional):
 // 6 events recorded at the stage boundaries (splat, -, -, blur, slice).

@@ -79,3 +79,33 @@ def test_grid_errors(gpu_ctx):
         gpu_ctx.gauss_transform_grid(dev(pos), dev(np.ones((1, 64), np.float32)), np.ones(7, np.float32))
     out = gpu_ctx.joint_bilateral_grid_host(im, im, 4.0, 4.0, 0.0, 0.1)   # context still usable
     assert np.isfinite(out).all()
+
+
+def test_grid_nonfinite_values(gpu_ctx, oracle_lib):
+    """NaN and +-Inf image values through the grid (JointBilateral GRID):
+    NaN / +Inf / -Inf pixels exactly where the reference puts them (the F32
+    grid oracle, bit-identical to the reference on such input), finite pixels
+    within 1e-5 of the FP64 grid oracle; the next finite call is unaffected."""
+    import torch
+    O = oracle_lib
+    print("\n[case] grid with NaN/Inf values", flush=True)
+    guide = S.rect_image(90, 70, 1, K=8, noise=0.05, seed=61)
+    im = S.uniform_image(90, 70, 2, seed=62)
+    im[0, 0, 5, 7] = np.nan
+    im[1, 0, 30, 40] = np.inf
+    im[0, 0, 50, 60] = -np.inf
+    im[1, 0, 10, 10] = np.inf
+    im[1, 0, 10, 11] = -np.inf
+    out = gpu_ctx.joint_bilateral_grid(dev(im), dev(guide), 3.0, 3.0, 0.0, 0.15).cpu().numpy()
+    o32 = O.joint_bilateral_grid(im, guide, 3.0, 3.0, 0.0, 0.15)
+    o64 = O.joint_bilateral_grid(im, guide, 3.0, 3.0, 0.0, 0.15, mode=O.F64_LAT)
+    for f in (np.isnan, np.isposinf, np.isneginf):
+        assert np.array_equal(f(out), f(o32)), f.__name__
+    assert np.isnan(out).any() and np.isinf(out).any()
+    fin = np.isfinite(o64)
+    assert np.array_equal(fin, np.isfinite(out))
+    assert rel_err(out[fin], o64[fin]) < FP32_REL_TOL
+    clean = S.uniform_image(90, 70, 2, seed=62)
+    a = gpu_ctx.joint_bilateral_grid(dev(clean), dev(guide), 3.0, 3.0, 0.0, 0.15).cpu().numpy()
+    assert np.isfinite(a).all()
+    assert rel_err(a, O.joint_bilateral_grid(clean, guide, 3.0, 3.0, 0.0, 0.15, mode=O.F64_LAT)) < FP32_REL_TOL

@@ -199,6 +199,19 @@ def test_plugin_nonfinite_values_spread_as_in_reference(tmp_path):
     assert np.isnan(gpu).any()
     fin = np.isfinite(cpu)
     assert rel_err(gpu[fin], cpu[fin]) < 2 * FP32_REL_TOL
+    # a grey guide with AUTO takes the grid (3 position channels); the NaN/Inf
+    # stay in the filtered image (a NaN position is an error here, and sends
+    # the reference's grid index astray)
+    grey = S.rect_image(120, 90, 1, K=10, noise=0.05, seed=57)
+    gsrc = str(tmp_path / "grey.tmp")
+    write_tmp(gsrc, grey)
+    run(["-load", gsrc, "-load", src, "-jointbilateral", "0.1", "5", "-save", cpu_path])
+    run(["-plugin", PLUGIN, "-load", gsrc, "-load", src, "-jointbilateral", "0.1", "5", "-save", gpu_path])
+    cpu, gpu = read_tmp(cpu_path), read_tmp(gpu_path)
+    for f in (np.isnan, np.isposinf, np.isneginf):
+        assert np.array_equal(f(gpu), f(cpu)), f.__name__
+    fin = np.isfinite(cpu)
+    assert rel_err(gpu[fin], cpu[fin]) < 2 * FP32_REL_TOL
 
 
 def test_plugin_lattice_beyond_16_dims_runs_the_reference(tmp_path):
