@@ -311,7 +311,7 @@ __global__ __launch_bounds__(kBlock) void k_splat_tile(PixSrc src0, int n0, int 
     __shared__ uint64_t lk[LT * W];
     __shared__ unsigned long long lv[LVD * LT];
     __shared__ uint16_t llist[LT];
-    __shared__ unsigned lcount;
+    __shared__ unsigned lcount, lfull;
     unsigned flags = 0;
     uint32_t probes = 0, inserts = 0, atomics = 0;
     const OutDst no_dst{nullptr, 0, 0};
@@ -320,11 +320,17 @@ __global__ __launch_bounds__(kBlock) void k_splat_tile(PixSrc src0, int n0, int 
     const PixSrc &src = tv.src;
     for (int i = threadIdx.x; i < LT * W; i += kBlock) lk[i] = kEmpty;
     for (int i = threadIdx.x; i < LVD * LT; i += kBlock) lv[i] = 0ull;
-    if (threadIdx.x == 0) lcount = 0;
+    if (threadIdx.x == 0) {
+        lcount = 0;
+        lfull = *(volatile const unsigned *)&ctr->flags & kFlagFull;
+    }
     int px, prow;
     const int p = pixel_index(tv.local, threadIdx.x, tv.n, tv.row_w, tile_w, px, prow);
-    const bool full = *(volatile const unsigned *)&ctr->flags & kFlagFull;
     __syncthreads();
+    // One reading of the full flag for the whole workgroup: other workgroups
+    // may set it at any time, and the flush's second phase reads the slots
+    // its first phase parked, so every thread must skip it or none.
+    const bool full = lfull != 0;
 
     float sc[LVD];
 #pragma unroll

@@ -134,3 +134,27 @@ def test_parts_local_overflow_at_capacity_limit(gpu_ctx):
     assert got[1][1]["retries"] >= 1
     for out, _ in got:
         assert np.array_equal(out.view(np.uint32), want.view(np.uint32))
+
+
+def test_table_fills_during_tiled_splat():
+    """A first call whose table fills up while the tiled splat runs (2048^2
+    RGB noise at sigma_s 4, sigma_r 0.1, about 5 vertices per pixel against a
+    first table sized for 1): workgroups that start after the table is found
+    full skip their flush as a whole, the call is redone in a larger table,
+    and its output equals a call that starts at the right size."""
+    from imagestack_amd import gtlattice as G
+    print("\n[case] 2048^2 RGB noise, table full during the first splat", flush=True)
+    im = S.uniform_image(2048, 2048, 3, seed=5)
+    di = dev(im)
+    ctx = G.Context(0)
+    try:
+        first, st0 = ctx.joint_bilateral(di, di, 4.0, 4.0, 0.0, 0.1, stats=True)
+        first = first.cpu().numpy()
+        again, st1 = ctx.joint_bilateral(di, di, 4.0, 4.0, 0.0, 0.1, stats=True)
+        again = again.cpu().numpy()
+    finally:
+        ctx.close()
+    print("first call M", st0["M"], "retries", st0["retries"], "then retries", st1["retries"], flush=True)
+    assert st0["retries"] >= 1 and st1["retries"] == 0
+    assert st0["M"] == st1["M"] > 2048 * 2048
+    assert np.array_equal(first.view(np.uint32), again.view(np.uint32))
