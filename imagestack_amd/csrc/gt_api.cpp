@@ -70,9 +70,32 @@ size_t store_bytes(int storage) { return storage == GT_STORE_F32 ? 4 : 2; }
 
 } // namespace
 
+namespace {
+// gt_debug_fail_allocations: allocations to let through, then to fail
+std::mutex g_fail_mu;
+int g_fail_after = 0, g_fail_count = 0;
+} // namespace
+
 namespace gt {
 int set_error(int code, const char *msg) { return fail(code, "%s", msg); }
+bool alloc_should_fail() {
+    std::lock_guard<std::mutex> lk(g_fail_mu);
+    if (g_fail_count <= 0) return false;
+    if (g_fail_after > 0) {
+        g_fail_after--;
+        return false;
+    }
+    g_fail_count--;
+    return true;
+}
 } // namespace gt
+
+extern "C" int gt_debug_fail_allocations(int after, int count) {
+    std::lock_guard<std::mutex> lk(g_fail_mu);
+    g_fail_after = after > 0 ? after : 0;
+    g_fail_count = count > 0 ? count : 0;
+    return GT_OK;
+}
 
 // Vertices per pixel seen for a filter (form, d, vd and the position
 // scales): the table size of its next call at any image size.  A context
@@ -155,6 +178,7 @@ int ensure(DevBuf &b, size_t bytes) {
     if (b.p) (void)hipFree(b.p);
     b.p = nullptr;
     b.cap = 0;
+    if (gt::alloc_should_fail()) return fail(GT_ENOMEM, "device allocation of %zu bytes failed (injected)", bytes);
     if (hipMalloc(&b.p, bytes) != hipSuccess) {
         (void)hipGetLastError();
         return fail(GT_ENOMEM, "device allocation of %zu bytes failed", bytes);
@@ -509,6 +533,7 @@ int run(gt_ctx *c, const gt::PixSrc &src, int n, int row_w, int d, int vd, int s
         const SliceSpec &sl, bool keep_splat, gt_stats *st, const BatchSpec *batch = nullptr,
         const std::vector<gt::BatchImage> *bimgs = nullptr) {
     hipStream_t s = c->stream;
+    c->have = false;   // until this call completes (a failed call may have freed the arrays)
     const bool bilat = batch ? true : src.ref != nullptr;
     const bool count = c->profiling >= 2;
     const bool dedup = batch ? true : (tu.dedup == 0) ? (d <= 10 && vd <= 4) : (tu.dedup > 0);
@@ -1133,7 +1158,8 @@ int gt_ctx_create(gt_ctx **out, int device) {
             return fail(GT_EDEVICE, "event creation failed");
         }
     }
-    if (hipHostMalloc((void **)&c->h_ctr, sizeof(gt::Counters), hipHostMallocDefault) != hipSuccess ||
+    if (gt::alloc_should_fail() ||
+        hipHostMalloc((void **)&c->h_ctr, sizeof(gt::Counters), hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc((void **)&c->h_mm, sizeof(float) * 6 * gt::kMaxDH, hipHostMallocDefault) != hipSuccess) {
         gt_ctx_destroy(c);
         return fail(GT_ENOMEM, "pinned allocation failed");
@@ -1479,7 +1505,8 @@ int gt_joint_bilateral_batch(gt_ctx *c, int count, const gt_bilateral_image *ima
             if (c->h_batch) (void)hipHostFree(c->h_batch);
             c->h_batch = nullptr;
             c->h_batch_cap = 0;
-            if (hipHostMalloc((void **)&c->h_batch, sizeof(gt::BatchImage) * nb, hipHostMallocDefault) != hipSuccess)
+            if (gt::alloc_should_fail() ||
+                hipHostMalloc((void **)&c->h_batch, sizeof(gt::BatchImage) * nb, hipHostMallocDefault) != hipSuccess)
                 return fail(GT_ENOMEM, "pinned allocation failed");
             c->h_batch_cap = nb;
         }

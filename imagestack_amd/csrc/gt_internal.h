@@ -276,6 +276,9 @@ hipStream_t ctx_stream(gt_ctx *c);
 
 // Sets the thread's gt_last_error() message; returns code.
 int set_error(int code, const char *msg);
+// True when an allocation is to fail (gt_debug_fail_allocations); every
+// device or pinned allocation of the library asks first.
+bool alloc_should_fail();
 // Forgets the context's cached key-packing plan, so the next call bounds its
 // inputs afresh (a server's next request is an unrelated image).
 void invalidate_plan(::gt_ctx *c);

@@ -4,7 +4,9 @@
 // joins its batch), and run by `slots` worker threads, each with its own
 // context (resident lattice buffers) and its own stream, so the groups
 // overlap on the device.  Latency runs from submit to the result in host
-// memory.
+// memory.  A request still queued can be cancelled, and one given a
+// deadline that no worker takes in time completes with GT_ETIMEDOUT; both
+// leave its output untouched.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -56,8 +58,10 @@ struct Req {
     size_t fp = 0;
     double t_submit = 0, t_start = 0, t_done = 0;
     int status = GT_OK, batch = 0;
+    bool taken = false;   // a worker runs it (no longer cancellable)
     bool done = false;
     std::string err;
+    bool expired(double t) const { return r.deadline_ms > 0 && t - t_submit >= r.deadline_ms; }
 };
 
 } // namespace
@@ -90,6 +94,7 @@ namespace {
 std::vector<Req *> take_group(gt_server *s) {
     std::vector<Req *> g;
     Req *head = s->queue.front();
+    head->taken = true;
     s->queue.pop_front();
     g.push_back(head);
     size_t fp = head->fp;
@@ -99,6 +104,7 @@ std::vector<Req *> take_group(gt_server *s) {
         const uint64_t qpx = (uint64_t)q->r.W * q->r.H;
         if (q->r.C == head->r.C && q->r.storage == head->r.storage && px + qpx <= (uint64_t)s->cfg.batch_pixels &&
             s->reserved + fp + q->fp <= s->cfg.hbm_budget) {
+            q->taken = true;
             g.push_back(q);
             fp += q->fp;
             px += qpx;
@@ -110,6 +116,30 @@ std::vector<Req *> take_group(gt_server *s) {
     s->reserved += fp;
     s->running++;
     return g;
+}
+
+// Completes a queued request without running it (under the lock).
+void finish_unrun(Req *q, int status, const char *msg, double t) {
+    q->status = status;
+    q->err = msg;
+    q->t_start = q->t_done = t;
+    q->done = true;
+}
+
+// Completes every queued request past its deadline with GT_ETIMEDOUT (under
+// the lock); true if any was.
+bool expire_queued(gt_server *s, double t) {
+    bool any = false;
+    for (auto it = s->queue.begin(); it != s->queue.end();) {
+        if ((*it)->expired(t)) {
+            finish_unrun(*it, GT_ETIMEDOUT, "server: request not started before its deadline", t);
+            it = s->queue.erase(it);
+            any = true;
+        } else {
+            ++it;
+        }
+    }
+    return any;
 }
 
 bool can_take(gt_server *s) {
@@ -125,6 +155,7 @@ struct DevArena {
         if (p) (void)hipFree(p);
         p = nullptr;
         cap = 0;
+        if (gt::alloc_should_fail()) return GT_ENOMEM;
         if (hipMalloc((void **)&p, floats * sizeof(float)) != hipSuccess) {
             (void)hipGetLastError();
             return GT_ENOMEM;
@@ -149,6 +180,7 @@ struct HostStage {
         if (p) (void)hipHostFree(p);
         p = nullptr;
         cap = 0;
+        if (gt::alloc_should_fail()) return GT_ENOMEM;
         if (hipHostMalloc((void **)&p, floats * sizeof(float), hipHostMallocDefault) != hipSuccess) {
             (void)hipGetLastError();
             return GT_ENOMEM;
@@ -298,12 +330,21 @@ void worker_main(gt_server *s, std::mutex *init_mu, int *init_left, std::conditi
     init_cv->notify_all();
     for (;;) {
         std::vector<Req *> g;
+        bool expired = false;
         {
             std::unique_lock<std::mutex> lk(s->mu);
-            s->cv_work.wait(lk, [&] { return s->stop || (rc == GT_OK && can_take(s)); });
-            if (s->stop) break;
-            g = take_group(s);
+            while (!s->stop) {
+                s->cv_work.wait(lk, [&] { return s->stop || (rc == GT_OK && can_take(s)); });
+                if (s->stop) break;
+                expired = expire_queued(s, now_ms()) || expired;
+                if (can_take(s)) {
+                    g = take_group(s);
+                    break;
+                }
+            }
         }
+        if (expired) s->cv_done.notify_all();
+        if (g.empty()) break;   // stopping
         const double t0 = now_ms();
         std::vector<int> st_rc(g.size(), GT_OK);
         std::vector<std::string> st_err(g.size());
@@ -408,8 +449,8 @@ int gt_server_submit(gt_server *s, const gt_request *r, long long *ticket) {
     if (!s || !r || !ticket) return gt::set_error(GT_EINVAL, "server: null argument");
     if (r->W <= 0 || r->H <= 0 || (r->C != 1 && r->C != 3) || !r->im || !r->out || !(r->sigma_s > 0) ||
         !(r->sigma_r > 0) || r->row_pitch < r->W || r->plane_pitch < (long long)r->row_pitch * (r->H - 1) + r->W ||
-        (int64_t)r->W * r->H >= (int64_t)1 << 31)
-        return gt::set_error(GT_EINVAL, "server: bad request (sizes, C in {1,3}, sigmas > 0, pitches)");
+        (int64_t)r->W * r->H >= (int64_t)1 << 31 || !(r->deadline_ms >= 0))
+        return gt::set_error(GT_EINVAL, "server: bad request (sizes, C in {1,3}, sigmas > 0, pitches, deadline >= 0)");
     Req *q = new Req();
     q->r = *r;
     q->fp = footprint(*r);
@@ -435,6 +476,17 @@ int gt_server_wait(gt_server *s, long long ticket, gt_request_result *res) {
     auto it = s->reqs.find(ticket);
     if (it == s->reqs.end()) return gt::set_error(GT_EINVAL, "server: unknown ticket");
     Req *q = it->second;
+    if (q->r.deadline_ms > 0 && !q->taken && !q->done) {
+        // expire it here at its deadline if every worker is still busy then
+        using namespace std::chrono;
+        const steady_clock::time_point at(
+            duration_cast<steady_clock::duration>(duration<double, std::milli>(q->t_submit + q->r.deadline_ms)));
+        if (!s->cv_done.wait_until(lk, at, [&] { return q->done || q->taken; }) && !q->taken && !q->done) {
+            s->queue.erase(std::find(s->queue.begin(), s->queue.end(), q));
+            finish_unrun(q, GT_ETIMEDOUT, "server: request not started before its deadline", now_ms());
+            s->cv_work.notify_all();   // the queue head may have changed
+        }
+    }
     s->cv_done.wait(lk, [&] { return q->done; });
     if (res) {
         res->status = q->status;
@@ -449,6 +501,22 @@ int gt_server_wait(gt_server *s, long long ticket, gt_request_result *res) {
     delete q;
     lk.unlock();
     if (rc != GT_OK) return gt::set_error(rc, m.c_str());
+    return GT_OK;
+}
+
+int gt_server_cancel(gt_server *s, long long ticket) {
+    if (!s) return gt::set_error(GT_EINVAL, "server: null server");
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        auto it = s->reqs.find(ticket);
+        if (it == s->reqs.end()) return gt::set_error(GT_EINVAL, "server: unknown ticket");
+        Req *q = it->second;
+        if (q->taken || q->done) return gt::set_error(GT_ESTATE, "server: request already running or done");
+        s->queue.erase(std::find(s->queue.begin(), s->queue.end(), q));
+        finish_unrun(q, GT_ECANCELED, "server: request cancelled", now_ms());
+    }
+    s->cv_done.notify_all();
+    s->cv_work.notify_all();   // the queue head may have changed
     return GT_OK;
 }
 

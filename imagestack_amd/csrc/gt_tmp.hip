@@ -224,7 +224,7 @@ struct Pinned {
     }
     int init(bool device_staging) {
         for (int i = 0; i < 2; i++) {
-            if (hipHostMalloc(&p[i], kChunk, hipHostMallocDefault) != hipSuccess ||
+            if (alloc_should_fail() || hipHostMalloc(&p[i], kChunk, hipHostMallocDefault) != hipSuccess ||
                 (device_staging && hipMalloc(&d[i], kChunk) != hipSuccess) ||
                 hipEventCreateWithFlags(&ev[i], hipEventDisableTiming) != hipSuccess) {
                 (void)hipGetLastError();

@@ -101,7 +101,7 @@ extern "C" int gt_autotune_joint_bilateral(gt_ctx *ctx, int W, int H, int F, int
                                            gt_storage storage, int reps, gt_tuning *best, gt_tune_report *rep) {
     if (!ctx || W <= 0 || H <= 0 || F <= 0 || C <= 0) return gt::set_error(GT_EINVAL, "autotune: bad sizes");
     Scratch out;
-    if (hipMalloc((void **)&out.p, (size_t)W * H * F * C * sizeof(float)) != hipSuccess) {
+    if (gt::alloc_should_fail() || hipMalloc((void **)&out.p, (size_t)W * H * F * C * sizeof(float)) != hipSuccess) {
         (void)hipGetLastError();
         return gt::set_error(GT_ENOMEM, "autotune: no device memory for the trial output");
     }
@@ -118,7 +118,7 @@ extern "C" int gt_autotune_joint_bilateral_batch(gt_ctx *ctx, int count, const g
     size_t total = 0;
     for (int i = 0; i < count; i++) total += (size_t)images[i].W * images[i].H * C;
     Scratch out;
-    if (hipMalloc((void **)&out.p, total * sizeof(float)) != hipSuccess) {
+    if (gt::alloc_should_fail() || hipMalloc((void **)&out.p, total * sizeof(float)) != hipSuccess) {
         (void)hipGetLastError();
         return gt::set_error(GT_ENOMEM, "autotune: no device memory for the trial outputs");
     }

@@ -17,16 +17,17 @@ LIB_PATH = os.path.join(HERE, "libgtlattice.so")
 
 GT_STORE_F32, GT_STORE_BF16, GT_STORE_F16 = 0, 1, 2
 STORAGE = {"f32": GT_STORE_F32, "bf16": GT_STORE_BF16, "f16": GT_STORE_F16}
-ERRORS = {1: "GT_EINVAL", 2: "GT_ENOMEM", 3: "GT_ERANGE", 4: "GT_EDEVICE", 5: "GT_ESTATE", 6: "GT_EUNSUPPORTED"}
-GT_EUNSUPPORTED = 6
+ERRORS = {1: "GT_EINVAL", 2: "GT_ENOMEM", 3: "GT_ERANGE", 4: "GT_EDEVICE", 5: "GT_ESTATE", 6: "GT_EUNSUPPORTED",
+          7: "GT_ECANCELED", 8: "GT_ETIMEDOUT"}
+GT_EINVAL, GT_ENOMEM, GT_ESTATE, GT_EUNSUPPORTED, GT_ECANCELED, GT_ETIMEDOUT = 1, 2, 5, 6, 7, 8
 
 EXPORTS = [
     "gt_ctx_create", "gt_ctx_destroy", "gt_ctx_set_stream", "gt_ctx_set_profiling", "gt_last_error",
     "gt_gauss_transform", "gt_gauss_transform_host", "gt_joint_bilateral", "gt_joint_bilateral_host",
     "gt_joint_bilateral_noalloc", "gt_build_lattice", "gt_lattice_export", "gt_joint_bilateral_batch",
     "gt_autotune_joint_bilateral", "gt_autotune_joint_bilateral_batch",
-    "gt_server_create", "gt_server_destroy", "gt_server_submit", "gt_server_wait", "gt_server_replay",
-    "gt_gauss_transform_grid", "gt_gauss_transform_grid_host", "gt_joint_bilateral_grid",
+    "gt_server_create", "gt_server_destroy", "gt_server_submit", "gt_server_wait", "gt_server_cancel",
+    "gt_server_replay", "gt_debug_fail_allocations", "gt_gauss_transform_grid", "gt_gauss_transform_grid_host", "gt_joint_bilateral_grid",
     "gt_joint_bilateral_grid_host", "gt_patch_project", "gt_nlmeans", "gt_nlmeans_host",
     "gt_tmp_info", "gt_tmp_read", "gt_tmp_write", "gt_tmp_load", "gt_tmp_save",
     "gt_net_server_create", "gt_net_server_port", "gt_net_server_stats", "gt_net_server_destroy",
@@ -65,7 +66,7 @@ class ServerConfig(C.Structure):
 class Request(C.Structure):
     _fields_ = [("W", C.c_int), ("H", C.c_int), ("C", C.c_int), ("im", C.c_void_p), ("row_pitch", C.c_longlong),
                 ("plane_pitch", C.c_longlong), ("out", C.c_void_p), ("sigma_s", C.c_float), ("sigma_r", C.c_float),
-                ("storage", C.c_int)]
+                ("storage", C.c_int), ("deadline_ms", C.c_double)]
 
 
 class RequestResult(C.Structure):
@@ -164,6 +165,8 @@ def lib():
         L.gt_server_destroy.restype = None
         L.gt_server_submit.argtypes = [_vp, C.POINTER(Request), C.POINTER(C.c_longlong)]
         L.gt_server_wait.argtypes = [_vp, C.c_longlong, C.POINTER(RequestResult)]
+        L.gt_server_cancel.argtypes = [_vp, C.c_longlong]
+        L.gt_debug_fail_allocations.argtypes = [C.c_int, C.c_int]
         L.gt_server_replay.argtypes = [_vp, C.c_int, C.POINTER(Request), C.POINTER(C.c_double),
                                        C.POINTER(RequestResult)]
         _lib = L
@@ -173,6 +176,13 @@ def lib():
 def _check(rc: int) -> None:
     if rc != 0:
         raise GTError(rc, lib().gt_last_error().decode())
+
+
+def fail_allocations(count: int, after: int = 0) -> None:
+    """gt_debug_fail_allocations: of the library's next device or pinned
+    allocations, `after` succeed and then `count` fail (0 turns it off).
+    For failure-path tests."""
+    _check(lib().gt_debug_fail_allocations(after, count))
 
 
 def _dptr(t) -> int:
@@ -464,7 +474,8 @@ def _host_ptr(a) -> int:
 
 
 def make_request(im, out, sigma_s: float, sigma_r: float, storage: str = "f32", row_pitch: int | None = None,
-                 plane_pitch: int | None = None, W: int | None = None, H: int | None = None) -> Request:
+                 plane_pitch: int | None = None, W: int | None = None, H: int | None = None,
+                 deadline_ms: float = 0.0) -> Request:
     """A bilateral request on host memory.  `im` is (C, H, W) dense float32, or
     (with W, H and the pitches given) a view into a larger planar buffer whose
     first pixel `im` points at; `out` is dense (C, H, W)."""
@@ -474,7 +485,7 @@ def make_request(im, out, sigma_s: float, sigma_r: float, storage: str = "f32", 
     rp = row_pitch if row_pitch is not None else Ww
     pp = plane_pitch if plane_pitch is not None else Ww * Hh
     addr = im if isinstance(im, int) else _host_ptr(im)
-    return Request(Ww, Hh, Cc, addr, rp, pp, _host_ptr(out), sigma_s, sigma_r, STORAGE[storage])
+    return Request(Ww, Hh, Cc, addr, rp, pp, _host_ptr(out), sigma_s, sigma_r, STORAGE[storage], deadline_ms)
 
 
 class Server:
@@ -503,10 +514,18 @@ class Server:
         _check(lib().gt_server_submit(self._h, C.byref(req), C.byref(t)))
         return t.value
 
-    def wait(self, ticket: int) -> dict:
+    def wait(self, ticket: int, check: bool = True) -> dict:
+        """The request's result; with check=False a failed request's status
+        is returned in the dict instead of raised."""
         r = RequestResult()
-        _check(lib().gt_server_wait(self._h, ticket, C.byref(r)))
+        rc = lib().gt_server_wait(self._h, ticket, C.byref(r))
+        if check:
+            _check(rc)
         return {k: getattr(r, k) for k, _ in RequestResult._fields_}
+
+    def cancel(self, ticket: int) -> None:
+        """gt_server_cancel: GTError(GT_ESTATE) once a worker has taken it."""
+        _check(lib().gt_server_cancel(self._h, ticket))
 
     def replay(self, reqs, arrival_ms) -> list:
         n = len(reqs)

@@ -56,7 +56,9 @@ enum {
     GT_ERANGE = 3,      /* lattice key outside int16, or NaN/Inf position     */
     GT_EDEVICE = 4,     /* HIP runtime error                                  */
     GT_ESTATE = 5,      /* call out of order (e.g. export before build)       */
-    GT_EUNSUPPORTED = 6 /* the reference would use a method not on the GPU     */
+    GT_EUNSUPPORTED = 6, /* the reference would use a method not on the GPU    */
+    GT_ECANCELED = 7,   /* a served request cancelled before it ran           */
+    GT_ETIMEDOUT = 8    /* a served request not started by its deadline        */
 };
 
 /* Storage type of lattice values between splat, blur and slice.  Blur and
@@ -318,6 +320,9 @@ typedef struct {
     float *out;                  /* host: dense C x H x W                        */
     float sigma_s, sigma_r;      /* spatial and colour standard deviations       */
     gt_storage storage;
+    double deadline_ms;          /* > 0: not started within this many ms of the
+                                    submit -> completes with GT_ETIMEDOUT, never run
+                                    (a request that has started runs to the end)  */
 } gt_request;
 
 typedef struct {
@@ -337,6 +342,11 @@ void gt_server_destroy(gt_server *server);
 int gt_server_submit(gt_server *server, const gt_request *request, long long *ticket);
 /* Blocks until the request is done; returns its status. */
 int gt_server_wait(gt_server *server, long long ticket, gt_request_result *result);
+/* Cancels a request that no worker has taken yet: it completes at once with
+ * GT_ECANCELED (gt_server_wait still collects it) and its `out` is not
+ * written.  GT_ESTATE if it is running or done (it then runs to the end),
+ * GT_EINVAL for an unknown ticket. */
+int gt_server_cancel(gt_server *server, long long ticket);
 /* Open-loop replay: submits request i at arrival_ms[i] after the call starts
  * (one submitting thread), then waits for all of them. */
 int gt_server_replay(gt_server *server, int count, const gt_request *requests,
@@ -430,6 +440,15 @@ int gt_build_lattice(gt_ctx *ctx, int n, int row_w, int d, int vd,
  */
 int gt_lattice_export(gt_ctx *ctx, int16_t *keys, int32_t *nbr,
                       float *splat_vals, float *blur_vals);
+
+/*
+ * Failure-injection hook for tests: of the library's next device and
+ * pinned-host allocations, the first `after` succeed and the `count` after
+ * them fail as if the memory were exhausted (count <= 0 turns it off).  A
+ * call that meets one returns GT_ENOMEM and leaves its context (or server)
+ * usable: the next call allocates again.  Process-wide.
+ */
+int gt_debug_fail_allocations(int after, int count);
 
 #ifdef __cplusplus
 }
