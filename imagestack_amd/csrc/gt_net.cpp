@@ -14,6 +14,12 @@
 // through the serving loop (gt_server: HBM admission, continuous batching);
 // the shapes it does not take (several frames, channel counts other than 1
 // and 3) run on the front end's own context.
+//
+// Headers are unauthenticated: an image is accepted only within max_pixels
+// and max_bytes, and only while the bytes every handler holds (input and
+// output) stay within max_inflight; a request past that is refused, its
+// connection closed, before anything is allocated.  The listener binds to
+// loopback unless told otherwise.
 #include <arpa/inet.h>
 #include <netinet/in.h>
 #include <netinet/tcp.h>
@@ -80,7 +86,8 @@ struct gt_net_server {
     std::condition_variable cv;
     std::set<int> open_fds;
     int active = 0;                   // detached handler threads still running
-    long long received = 0, replied = 0, failed = 0;
+    long long received = 0, replied = 0, failed = 0, refused = 0;
+    long long inflight = 0;           // bytes reserved by handlers (input + output images)
     std::vector<double> lat_ms;
     gt_server *serving = nullptr;     // bilateral requests of 1 or 3 channels, 1 frame
     gt_ctx *ctx = nullptr;            // the other shapes
@@ -139,13 +146,25 @@ int run_op(gt_net_server *s, const uint32_t h[4], std::vector<float> &im, std::v
 }
 
 void handle(gt_net_server *s, int fd, sockaddr_in peer) {
-    bool ok = false;
+    bool ok = false, refused = false;
+    long long held = 0;   // bytes reserved against max_inflight
     double t0 = 0, t1 = 0;
     uint32_t h[4];
     if (recv_all(fd, h, sizeof(h))) {
         const uint64_t n = (uint64_t)h[0] * h[1] * h[2];
-        const bool sane = h[0] >= 1 && h[1] >= 1 && h[2] >= 1 && h[3] >= 1 && h[3] < GT_MAX_VD &&
-                          n <= (uint64_t)s->cfg.max_pixels;
+        const uint64_t bytes = n * h[3] * sizeof(float);
+        bool sane = h[0] >= 1 && h[1] >= 1 && h[2] >= 1 && h[3] >= 1 && h[3] < GT_MAX_VD &&
+                    n <= (uint64_t)s->cfg.max_pixels && bytes <= (uint64_t)s->cfg.max_bytes;
+        if (sane) {
+            std::lock_guard<std::mutex> lk(s->mu);
+            if (s->inflight + 2 * (long long)bytes > s->cfg.max_inflight) {
+                refused = true;
+                sane = false;
+            } else {
+                held = 2 * (long long)bytes;
+                s->inflight += held;
+            }
+        }
         std::vector<float> im, out;
         if (sane) {
             im.resize(n * h[3]);
@@ -179,7 +198,10 @@ void handle(gt_net_server *s, int fd, sockaddr_in peer) {
         s->open_fds.erase(fd);
         ::close(fd);
     }
-    if (ok) {
+    s->inflight -= held;
+    if (refused) {
+        s->refused++;
+    } else if (ok) {
         s->replied++;
         // latencies of the last kLatWindow replies (a ring), for the percentiles
         if (s->lat_ms.size() < kLatWindow) s->lat_ms.push_back(t1 - t0);
@@ -230,6 +252,15 @@ int gt_net_server_create(gt_net_server **out, int device, const gt_net_config *c
     s->cfg = *cfg;
     s->device = device;
     if (s->cfg.max_pixels <= 0) s->cfg.max_pixels = 1ll << 26;
+    if (s->cfg.max_bytes <= 0) s->cfg.max_bytes = 1ll << 30;
+    if (s->cfg.max_inflight <= 0) s->cfg.max_inflight = 4ll << 30;
+    in_addr bind_to{};
+    const char *addr = (cfg->bind_addr && cfg->bind_addr[0]) ? cfg->bind_addr : "127.0.0.1";
+    s->cfg.bind_addr = nullptr;   // the caller's string is not kept
+    if (::inet_pton(AF_INET, addr, &bind_to) != 1) {
+        delete s;
+        return gt::set_error(GT_EINVAL, "bind address is not an IPv4 address");
+    }
     if (cfg->op == GT_NET_BILATERAL) {
         int rc = gt_server_create(&s->serving, device, &s->cfg.serving);
         if (rc == GT_OK) rc = gt_ctx_create(&s->ctx, device);
@@ -242,7 +273,7 @@ int gt_net_server_create(gt_net_server **out, int device, const gt_net_config *c
     const int one = 1;
     sockaddr_in a{};
     a.sin_family = AF_INET;
-    a.sin_addr.s_addr = htonl(INADDR_ANY);
+    a.sin_addr = bind_to;
     a.sin_port = htons((uint16_t)cfg->port);
     socklen_t len = sizeof(a);
     if (s->listen_fd < 0 || ::setsockopt(s->listen_fd, SOL_SOCKET, SO_REUSEADDR, &one, sizeof(one)) != 0 ||
@@ -266,6 +297,8 @@ int gt_net_server_stats(gt_net_server *s, gt_net_stats *st) {
     st->received = s->received;
     st->replied = s->replied;
     st->failed = s->failed;
+    st->refused = s->refused;
+    st->inflight_bytes = s->inflight;
     std::vector<double> v = s->lat_ms;
     std::sort(v.begin(), v.end());
     if (!v.empty()) {

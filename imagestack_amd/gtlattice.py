@@ -77,12 +77,14 @@ class RequestResult(C.Structure):
 class NetConfig(C.Structure):
     _fields_ = [("port", C.c_int), ("reply_port", C.c_int), ("op", C.c_int), ("sigma_s", C.c_float),
                 ("sigma_r", C.c_float), ("storage", C.c_int), ("max_pixels", C.c_longlong),
-                ("serving", ServerConfig)]
+                ("serving", ServerConfig), ("max_bytes", C.c_longlong), ("max_inflight", C.c_longlong),
+                ("bind_addr", C.c_char_p)]
 
 
 class NetStats(C.Structure):
     _fields_ = [("received", C.c_longlong), ("replied", C.c_longlong), ("failed", C.c_longlong),
-                ("p50_ms", C.c_double), ("p99_ms", C.c_double)]
+                ("p50_ms", C.c_double), ("p99_ms", C.c_double), ("refused", C.c_longlong),
+                ("inflight_bytes", C.c_longlong)]
 
 
 class BilateralImage(C.Structure):
@@ -570,10 +572,12 @@ class NetServer:
 
     def __init__(self, op: int = GT_NET_BILATERAL, port: int = 0, reply_port: int = 0, sigma_s: float = 8.0,
                  sigma_r: float = 0.1, storage: str = "f32", max_pixels: int = 0, device: int = 0, slots: int = 4,
-                 max_batch: int = 16, warm_side: int = 0):
+                 max_batch: int = 16, warm_side: int = 0, max_bytes: int = 0, max_inflight: int = 0,
+                 bind_addr: str | None = None):
         self._h = _vp()
         cfg = NetConfig(port, reply_port, op, sigma_s, sigma_r, STORAGE[storage], max_pixels,
-                        ServerConfig(slots, max_batch, 1 << 22, 0, warm_side))
+                        ServerConfig(slots, max_batch, 1 << 22, 0, warm_side), max_bytes, max_inflight,
+                        bind_addr.encode() if bind_addr else None)
         _check(lib().gt_net_server_create(C.byref(self._h), device, C.byref(cfg)))
         self.port = lib().gt_net_server_port(self._h)
 

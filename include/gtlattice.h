@@ -408,11 +408,23 @@ typedef struct {
     gt_storage storage;
     long long max_pixels;        /* largest accepted W*H*F (default 2^26)      */
     gt_server_config serving;    /* the request loop behind GT_NET_BILATERAL   */
+    long long max_bytes;         /* largest accepted image, W*H*F*C*4 bytes
+                                    (default 1 GiB); larger: dropped, failed   */
+    long long max_inflight;      /* bytes the handlers hold at once, input and
+                                    output of every image being served (default
+                                    4 GiB): a request that would pass it is
+                                    refused (connection closed) before any
+                                    allocation                                 */
+    const char *bind_addr;       /* IPv4 address to listen on; NULL or "" =
+                                    127.0.0.1 (this host only), "0.0.0.0" =
+                                    every interface                            */
 } gt_net_config;
 
 typedef struct {
     long long received, replied, failed;
     double p50_ms, p99_ms;       /* over the last 100 000 replied images        */
+    long long refused;           /* requests turned away by max_inflight        */
+    long long inflight_bytes;    /* bytes the handlers hold now                 */
 } gt_net_stats;
 
 typedef struct gt_net_server gt_net_server;

@@ -7,6 +7,7 @@ unmodified reference CLI (-send / -receive) and by a socket client."""
 import os
 import socket
 import subprocess
+import time
 
 import numpy as np
 import pytest
@@ -93,6 +94,89 @@ def test_echo_server_socket_client():
         assert st["replied"] == 3 and st["failed"] == 1
     finally:
         srv.close()
+
+
+def test_echo_server_byte_limits():
+    """Unauthenticated headers: an image past max_bytes is dropped unread; one
+    that would take the handlers past max_inflight bytes (input + output) is
+    refused before anything is allocated, and accepted again once the bytes
+    held drop."""
+    srv = G.NetServer(op=G.GT_NET_ECHO, max_bytes=6000, max_inflight=10000)
+    try:
+        big = np.zeros((1, 1, 40, 40), np.float32)               # 6400 bytes > max_bytes
+        with socket.create_connection(("127.0.0.1", srv.port), timeout=20) as s:
+            s.sendall(np.array([40, 40, 1, 1], np.uint32).tobytes())
+            assert s.recv(16) == b""
+        assert srv.stats()["failed"] == 1
+        a = np.arange(1000, dtype=np.float32).reshape(1, 1, 25, 40)   # holds 2 x 4000 bytes
+        b = np.ones((1, 1, 10, 50), np.float32)                       # needs 2 x 2000 more
+        with socket.create_connection(("127.0.0.1", srv.port), timeout=20) as sa:
+            sa.sendall(np.array([40, 25, 1, 1], np.uint32).tobytes())   # header only: a handler waits, holding
+            for _ in range(2000):
+                if srv.stats()["inflight_bytes"] == 8000:
+                    break
+                time.sleep(0.005)
+            assert srv.stats()["inflight_bytes"] == 8000
+            with socket.create_connection(("127.0.0.1", srv.port), timeout=20) as sb:
+                assert dropped(sb, b)                           # refused
+            assert srv.stats()["refused"] == 1
+            sa.sendall(a.tobytes())
+            assert np.array_equal(G.recv_image(sa), a)
+        for _ in range(2000):
+            if srv.stats()["inflight_bytes"] == 0:
+                break
+            time.sleep(0.005)
+        with socket.create_connection(("127.0.0.1", srv.port), timeout=20) as sb:
+            G.send_image(sb, b)
+            assert np.array_equal(G.recv_image(sb), b)
+        st = srv.stats()
+        assert st["replied"] == 2 and st["refused"] == 1 and st["failed"] == 1 and st["inflight_bytes"] == 0
+    finally:
+        srv.close()
+
+
+def dropped(sock, im):
+    """Sends im and reports whether the server closed the connection without
+    a reply (the close may reach the client as EOF or as a reset)."""
+    try:
+        G.send_image(sock, im)
+        return sock.recv(16) == b""
+    except (ConnectionResetError, BrokenPipeError):
+        return True
+
+
+def test_echo_server_bind_address():
+    """Loopback by default; an explicit address is honoured; a malformed one
+    is refused."""
+    srv = G.NetServer(op=G.GT_NET_ECHO, bind_addr="0.0.0.0")
+    try:
+        im = np.ones((1, 1, 2, 3), np.float32)
+        with socket.create_connection(("127.0.0.1", srv.port), timeout=20) as s:
+            G.send_image(s, im)
+            assert np.array_equal(G.recv_image(s), im)
+    finally:
+        srv.close()
+    with pytest.raises(G.GTError) as e:
+        G.NetServer(op=G.GT_NET_ECHO, bind_addr="not-an-address")
+    assert e.value.code == G.GT_EINVAL
+    for kw, want in (({}, "0100007F"), ({"bind_addr": "0.0.0.0"}, "00000000")):
+        srv = G.NetServer(op=G.GT_NET_ECHO, **kw)
+        try:
+            assert listen_address(srv.port) == want, kw
+        finally:
+            srv.close()
+
+
+def listen_address(port):
+    """Local address (hex, as /proc/net/tcp writes it) of the socket listening
+    on `port`."""
+    with open("/proc/net/tcp") as f:
+        for line in f.readlines()[1:]:
+            local, state = line.split()[1], line.split()[3]
+            addr, p = local.split(":")
+            if int(p, 16) == port and state == "0A":
+                return addr
+    return None
 
 
 def free_port():
