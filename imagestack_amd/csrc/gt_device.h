@@ -439,17 +439,32 @@ __device__ __forceinline__ void take_max(float (&m)[kMaxVD], const float (&o)[kM
     }
 }
 
-// Wave maximum of m[c] per channel c < nc, one atomicMax of its float bits
-// per wave and channel (non-negative floats order as their bits).
+// Workgroup maximum of m[c] per channel c < nc into track[c] (float bits:
+// non-negative floats order as their bits).  Waves meet in LDS, then one
+// thread per channel raises the global word, and only when the workgroup's
+// maximum exceeds what the word already holds: a pass over millions of
+// vertices launches ~10^5 workgroups, and same-address global atomics from
+// all of them serialise (the maximum only grows, so a stale read can cost an
+// atomic but never skip a needed one).  Every thread of the workgroup must
+// call it.
 template <int NV>
 __device__ __forceinline__ void publish_max(unsigned *track, const float (&m)[kMaxVD], int nc) {
+    __shared__ unsigned wg_max[NV];
+    if (threadIdx.x < NV) wg_max[threadIdx.x] = 0u;
+    __syncthreads();
 #pragma unroll
     for (int c = 0; c < NV; c++) {
         if (c >= nc) continue;
         float v = m[c];
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
-        if (__lane_id() == 0 && v > 0.0f) atomicMax(track + c, __float_as_uint(v));
+        if (__lane_id() == 0 && v > 0.0f) atomicMax(&wg_max[c], __float_as_uint(v));
+    }
+    __syncthreads();
+    if (threadIdx.x < (unsigned)nc && threadIdx.x < NV) {
+        const unsigned b = wg_max[threadIdx.x];
+        if (b > __hip_atomic_load(track + threadIdx.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            atomicMax(track + threadIdx.x, b);
     }
 }
 
