@@ -19,6 +19,7 @@ constexpr int kMaxVD = 16;
 constexpr uint64_t kEmpty = ~0ull;     // an unused table word (no valid key word equals it)
 constexpr int kWave = 64;
 constexpr uint32_t kMaxProbes = 256;   // an insert that probes this far reports a full table
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));   // a 16-byte vector access
 
 // Enclosing simplex of one point: fixed-up remainder-0 point g, ranks rk and
 // barycentric weights b[0..D].  `bad` flags NaN/Inf or keys beyond int16.
@@ -384,9 +385,14 @@ __device__ __forceinline__ uint32_t lds_home(const uint64_t (&k)[W]) {
     return (uint32_t)(hash_key<W>(k) >> 40) & (LT - 1);
 }
 
+//
+// The probe loop stays a loop: unrolled (MAXP times, for each of a pixel's
+// D+1 vertices) every level of it holds its own saved exec mask, and the
+// SGPRs those take spill the kernel's arguments into VGPR lanes.
 template <int W, int LT, int MAXP>
 __device__ __forceinline__ int lds_insert(uint64_t *lk, const uint64_t (&k)[W]) {
     uint32_t ls = lds_home<W, LT>(k);
+#pragma unroll 1
     for (int it = 0; it < MAXP; it++) {
         bool match = true;
 #pragma unroll

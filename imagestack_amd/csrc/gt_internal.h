@@ -111,6 +111,48 @@ struct TableArgs {
     int part_shift;                // log2(capacity / P)
 };
 
+// Replay record of the LDS-tiled splat, so that a slice at the splat's own
+// positions needs neither the positions nor the simplex again (the
+// reference's replay slice, src/Permutohedral.h:405-427, keeps an (offset,
+// weight) pair per pixel and vertex the same way).  Entry q = tile * 256 +
+// thread of the splat's workgroup for that tile; the slice runs the same tile
+// decomposition.  Entry q is replay_dwords(D) words, as 16-byte vectors: word
+// 4i + k of it is word k of uint4 i * stride + q of rec.  For vertex j in the
+// splat's per-lane rotated order (vertex (j + thread % (D+1)) % (D+1)), word j
+// holds its barycentric weight (FP32 bits) and the 10-bit field j % 3 of word
+// D+1 + j / 3 its slot in the tile's LDS table,
+// or kReplayGlobal when the tile table had no room for it and the splat went
+// to the global table directly, at slot gs[j * stride + q].  Per tile t:
+// tcnt[t] distinct keys, entry k of tent[t * kTileKeys + k] = {LDS slot,
+// global table slot}.  rec == nullptr: no record.
+struct Replay {
+    uint32_t *rec;
+    uint32_t *gs;
+    uint2 *tent;
+    uint32_t *tcnt;
+    int64_t stride;   // entries: tiles * 256
+};
+constexpr unsigned kReplayGlobal = 1023u;
+// Words of one replay entry: D+1 weights and ceil((D+1)/3) slot words,
+// rounded up to whole 16-byte vectors (8 words for d = 5).
+constexpr int replay_dwords(int D) { return (D + 1 + (D + 3) / 3 + 3) / 4 * 4; }
+constexpr int kTileKeys = 512;   // LDS tile table slots of the tiled splat and slice
+constexpr int kTilePixels = 256;   // pixels of a tile: one workgroup of the tiled kernels
+
+// Tiles of an n-pixel image with rows of row_w pixels cut into tile_w x
+// (256 / tile_w) tiles (tile_w a divisor of 256); when that does not apply,
+// tile_w becomes 0 and the tiles are runs of 256 pixels.
+inline int64_t pixel_tiles(int n, int row_w, int &tile_w) {
+    if (tile_w > 0 && row_w > 0 && (kTilePixels % tile_w) == 0 && tile_w <= kTilePixels) {
+        const int tiles_x = (row_w + tile_w - 1) / tile_w;
+        const int th = kTilePixels / tile_w;
+        const int rows = n / row_w;
+        return (int64_t)tiles_x * ((rows + th - 1) / th);
+    }
+    tile_w = 0;
+    return ((int64_t)n + kTilePixels - 1) / kTilePixels;
+}
+
 // Vertex ids grouped by eighths of a partitioned table (ids in compaction
 // order): eighth x holds ids [off[x * cpp], off[(x + 1) * cpp]) (M for
 // x = 8), off being the compaction's per-chunk offsets.  cpp = 0: none.
@@ -169,15 +211,17 @@ hipError_t launch_value_scale_batch(const BatchArgs &ba, int ntiles, int tile_w,
 // counters to zero.
 hipError_t launch_clear(uint64_t *tkeys, size_t nwords, unsigned long long *tacc, size_t nacc, uint32_t *tnf,
                         size_t nmarks, Counters *ctr, hipStream_t st);
+// rp: the tiled splat's replay record (rp.rec == nullptr: none; the direct
+// splat writes none).
 hipError_t launch_splat(int d, bool packed, bool bilat, bool dedup, const PixSrc &src, int n, int row_w, int tile_w,
                         int vd, const LatConst &lc, const KeyPlan &kp, const ValScale *vsc, const TableArgs &ta,
-                        Counters *ctr, bool count, int wpc, hipStream_t st);
+                        const Replay &rp, Counters *ctr, bool count, int wpc, hipStream_t st);
 // Batched forms (bilateral, packed keys, d <= 6, vd <= 4): one workgroup per
 // tile, ntiles tiles over all images.  wpc: the waves-per-CU cap of the tiled
 // and neighbour kernels (0: none).
 hipError_t launch_splat_batch(int d, const PixSrc &tmpl, const BatchArgs &ba, int ntiles, int tile_w, int vd,
                               const LatConst &lc, const KeyPlan &kp, const ValScale *vsc, const TableArgs &ta,
-                              Counters *ctr, bool count, int wpc, hipStream_t st);
+                              const Replay &rp, Counters *ctr, bool count, int wpc, hipStream_t st);
 hipError_t launch_slice_batch(int d, int storage, const PixSrc &tmpl, const BatchArgs &ba, int ntiles, int tile_w,
                               int vd, const LatConst &lc, const KeyPlan &kp, const TableArgs &ta, const void *vals,
                               const unsigned *sbits, const OutDst &tmpl_dst, Counters *ctr, bool count, int wpc,
@@ -215,6 +259,13 @@ hipError_t launch_blur(int storage, int vs, int pass, const ValScale *vsc, const
                        int m_max, const PartMap &pm, const void *in, void *out, hipStream_t st);
 // sbits: the scale source of a narrow storage's last blur pass
 // (ValScale::vmax_bits[d]); ignored for FP32.
+// Slice from the tiled splat's replay record (the splat's own positions,
+// d <= 10, vd <= 4; batch: ba.count > 0, over ntiles tiles); the same tile
+// decomposition as that splat (n, row_w, tile_w or the batch's tiles).
+hipError_t launch_slice_replay(int d, int storage, const PixSrc &src, int n, int row_w, int tile_w,
+                               const BatchArgs &ba, int ntiles, int vd, const KeyPlan &kp, const TableArgs &ta,
+                               const Replay &rp, const void *vals, const unsigned *sbits, const OutDst &dst,
+                               Counters *ctr, int wpc, hipStream_t st);
 hipError_t launch_slice(int d, bool packed, bool bilat, bool tiled, int storage, const PixSrc &src, int n, int row_w,
                         int tile_w, int vd, int vs, const LatConst &lc, const KeyPlan &kp, const TableArgs &ta,
                         const void *vals, const unsigned *sbits, const OutDst &dst, Counters *ctr, bool count,

@@ -11,7 +11,6 @@
 namespace gt {
 
 constexpr int kBlock = 256;          // threads per workgroup of every kernel here
-constexpr int kTileKeys = 512;       // LDS tile table slots of the splat and slice tiles
 
 // ---------------------------------------------------------------------------
 // pixel sources
@@ -305,7 +304,7 @@ template <int D, bool PACKED, bool BILAT, int LVD, int LT, bool BATCH>
 __global__ __launch_bounds__(kBlock) void k_splat_tile(PixSrc src0, int n0, int row_w0, int tile_w, BatchArgs ba,
                                                        int vd, LatConst lc, KeyPlan kp,
                                                        const ValScale *__restrict__ vsc, TableArgs ta,
-                                                       Counters *ctr, int count) {
+                                                       Replay rp, Counters *ctr, int count) {
     using K = KeyOps<D, PACKED>;
     constexpr int W = K::W;
     __shared__ uint64_t lk[LT * W];
@@ -316,7 +315,9 @@ __global__ __launch_bounds__(kBlock) void k_splat_tile(PixSrc src0, int n0, int 
     uint32_t probes = 0, inserts = 0, atomics = 0;
     const OutDst no_dst{nullptr, 0, 0};
     TileView tv;
-    tile_view<BATCH>(BATCH ? (int)blockIdx.x : xcd_tile(blockIdx.x, gridDim.x), src0, no_dst, n0, row_w0, ba, kp, tv);
+    const int tile = BATCH ? (int)blockIdx.x : xcd_tile(blockIdx.x, gridDim.x);
+    tile_view<BATCH>(tile, src0, no_dst, n0, row_w0, ba, kp, tv);
+    const unsigned rq = (unsigned)tile * kBlock + threadIdx.x;   // this pixel's replay entry
     const PixSrc &src = tv.src;
     for (int i = threadIdx.x; i < LT * W; i += kBlock) lk[i] = kEmpty;
     for (int i = threadIdx.x; i < LVD * LT; i += kBlock) lv[i] = 0ull;
@@ -388,11 +389,21 @@ __global__ __launch_bounds__(kBlock) void k_splat_tile(PixSrc src0, int n0, int 
 #pragma unroll
                 for (int r = 0; r <= D; r++) key[r][w] = col[r];
             }
+            constexpr int RW = replay_dwords(D);
+            uint32_t rslot[RW - (D + 1)];   // the replay entry's slot words
+#pragma unroll
+            for (int i = 0; i < RW - (D + 1); i++) rslot[i] = 0u;
 #pragma unroll
             for (int r = 0; r <= D; r++) {
                 const uint64_t(&k)[W] = key[r];
-                if (!okr[r]) { flags |= kFlagRange; continue; }
+                if (!okr[r]) {   // (the call is replanned and redone)
+                    flags |= kFlagRange;
+                    rslot[r / 3] |= kReplayGlobal << (10 * (r % 3));
+                    continue;
+                }
                 const int ls = lds_insert<W, LT, kLdsProbes>(lk, k);
+                // (in the rotated order; the slice undoes the rotation)
+                rslot[r / 3] |= (ls >= 0 ? (uint32_t)ls : kReplayGlobal) << (10 * (r % 3));
                 if (ls >= 0) {
 #pragma unroll
                     for (int c = 0; c < LVD; c++)
@@ -400,12 +411,28 @@ __global__ __launch_bounds__(kBlock) void k_splat_tile(PixSrc src0, int n0, int 
                 } else if (!full) {
                     const int slot = table_insert<W>(ta.tkeys, ta.mask, k, home_slot<D, PACKED, BATCH>(ta, kp, k), probes);
                     inserts++;
+                    if (rp.rec) rp.gs[r * rp.stride + rq] = (uint32_t)slot;
                     if (slot < 0) { flags |= kFlagFull; continue; }
                     unsigned long long *dst = ta.tacc + (size_t)slot * vd;
 #pragma unroll
                     for (int c = 0; c < LVD; c++)
                         if (c < vd) atomicAdd(dst + c, fixed_word(wt[r] * v[c]));
                     atomics += vd;
+                }
+            }
+            if (rp.rec) {   // one entry, whole 16-byte vectors
+                uint32_t e[RW];
+#pragma unroll
+                for (int r = 0; r <= D; r++) e[r] = __float_as_uint(wt[r]);
+#pragma unroll
+                for (int i = D + 1; i < RW; i++) e[i] = rslot[i - (D + 1)];
+                // vector i of every entry in plane i (each store a contiguous
+                // run of lines), streamed past the L2 that holds the table
+                u32x4 *dst = reinterpret_cast<u32x4 *>(rp.rec) + rq;
+#pragma unroll
+                for (int i = 0; i < RW / 4; i++) {
+                    const u32x4 v = {e[4 * i], e[4 * i + 1], e[4 * i + 2], e[4 * i + 3]};
+                    __builtin_nontemporal_store(v, dst + i * rp.stride);
                 }
             }
         }
@@ -431,7 +458,9 @@ __global__ __launch_bounds__(kBlock) void k_splat_tile(PixSrc src0, int n0, int 
         inserts++;
         if (slot < 0) flags |= kFlagFull;
         lk[ls * W] = (uint64_t)(int64_t)slot;
+        if (rp.rec) rp.tent[(size_t)tile * LT + j] = make_uint2((unsigned)ls, (unsigned)slot);
     }
+    if (rp.rec && threadIdx.x == 0) rp.tcnt[tile] = cnt;
     __syncthreads();
     const int c = (int)(threadIdx.x % LVD);
     for (unsigned base = 0; base < cnt; base += kBlock / LVD) {
@@ -1329,6 +1358,101 @@ __global__ __launch_bounds__(kBlock) void k_slice_tile(PixSrc src0, int n0, int 
     }
     report_flags(ctr, flags);
     if (count) report_counts(ctr, probes, 0, 0);
+}
+
+// Vertex id of a slot of the finalized table (ids folded into the key word,
+// or slot_id).
+__device__ __forceinline__ int slot_vertex(const TableArgs &ta, uint32_t slot) {
+    return ta.idshift > 0 ? (int)(ta.tkeys[slot] >> ta.idshift) : ta.slot_id[slot];
+}
+
+// Slice at the splat's own positions from the tiled splat's replay record
+// (src/Permutohedral.h:405-427): the workgroup of tile t stages the blurred
+// values of the tile's distinct vertices in LDS at the splat's tile-table
+// slots (their global slots give the ids, no probing), then each pixel sums
+// weight * value over its D+1 vertices in vertex order -- the same FP32
+// operations as the slice that recomputes the simplex, so the output is the
+// same bit for bit.  Records are checked against the table and LDS bounds (a
+// call whose splat flagged an error writes records only partly and is
+// discarded, but its slice still runs).
+template <int D, typename T, int LVD, int LT, bool BATCH>
+__global__ __launch_bounds__(kBlock) void k_slice_replay(PixSrc src0, int n0, int row_w0, int tile_w, BatchArgs ba,
+                                                         int vd, KeyPlan kp, TableArgs ta, Replay rp,
+                                                         const T *__restrict__ vals,
+                                                         const unsigned *__restrict__ sbits, OutDst dst0,
+                                                         Counters *ctr) {
+    constexpr bool kScaled = !std::is_same<T, float>::value;
+    __shared__ float lv[LT * LVD];
+    if (*(volatile const unsigned *)&ctr->flags & kFlagFull) return;   // redone with a larger table
+    const unsigned M = ctr->M;
+    const int tile = BATCH ? (int)blockIdx.x : xcd_tile(blockIdx.x, gridDim.x);
+    TileView tv;
+    tile_view<BATCH>(tile, src0, dst0, n0, row_w0, ba, kp, tv);
+    float un[kMaxVD];
+    slice_unscale<T, LVD>(sbits, un);
+    int px, prow;
+    const int p = pixel_index(tv.local, threadIdx.x, tv.n, tv.row_w, tile_w, px, prow);
+    const unsigned rq = (unsigned)tile * kBlock + threadIdx.x;
+    constexpr int RW = replay_dwords(D);
+    uint32_t e[RW];
+    if (p >= 0) {   // (issued before the staging: the loads overlap it)
+        const u32x4 *src = reinterpret_cast<const u32x4 *>(rp.rec) + rq;
+#pragma unroll
+        for (int i = 0; i < RW / 4; i++) {
+            const u32x4 u = __builtin_nontemporal_load(src + i * rp.stride);
+            e[4 * i] = u.x;
+            e[4 * i + 1] = u.y;
+            e[4 * i + 2] = u.z;
+            e[4 * i + 3] = u.w;
+        }
+    }
+    const unsigned cnt = min(rp.tcnt[tile], (unsigned)LT);
+    for (unsigned j = threadIdx.x; j < cnt; j += kBlock) {
+        const uint2 e = rp.tent[(size_t)tile * LT + j];
+        if (e.x >= (unsigned)LT) continue;
+        const int id = e.y <= ta.mask ? slot_vertex(ta, e.y) : -1;
+        float row[kMaxVD];
+#pragma unroll
+        for (int c = 0; c < LVD; c++) row[c] = 0.0f;
+        if (id >= 0 && (unsigned)id < M) load_row<T, LVD>(vals, id, row);
+#pragma unroll
+        for (int c = 0; c < LVD; c++) lv[c * LT + e.x] = kScaled ? row[c] * un[c] : row[c];
+    }
+    __syncthreads();
+    if (p < 0) return;
+    float w[D + 1];
+    int ls[D + 1];
+#pragma unroll
+    for (int j = 0; j <= D; j++) {
+        w[j] = __uint_as_float(e[j]);
+        ls[j] = (int)((e[D + 1 + j / 3] >> (10 * (j % 3))) & 1023u);
+    }
+    // vertex order: the splat stored vertex (j + rot) % (D+1) at j
+    const int rot = (int)(threadIdx.x % (D + 1));
+    const int back = (D + 1 - rot) % (D + 1);
+    rotate_left<D + 1>(w, back);
+    rotate_left<D + 1>(ls, back);
+    float acc[kMaxVD];
+#pragma unroll
+    for (int c = 0; c < LVD; c++) acc[c] = 0.0f;
+#pragma unroll
+    for (int r = 0; r <= D; r++) {
+        if (ls[r] < LT) {
+#pragma unroll
+            for (int c = 0; c < LVD; c++) acc[c] = acc[c] + w[r] * lv[c * LT + ls[r]];
+        } else if (ls[r] == (int)kReplayGlobal) {
+            const int j = (r + D + 1 - rot) % (D + 1);
+            const uint32_t gs = rp.gs[j * rp.stride + rq];
+            const int id = gs <= ta.mask ? slot_vertex(ta, gs) : -1;
+            if (id >= 0 && (unsigned)id < M) {
+                float row[kMaxVD];
+                load_row<T, LVD>(vals, id, row);
+#pragma unroll
+                for (int c = 0; c < LVD; c++) acc[c] = acc[c] + w[r] * (row[c] * un[c]);
+            }
+        }
+    }
+    write_pixel<LVD>(tv.dst, p, vd, acc);
 }
 
 // Slice, direct form (D > 6 or vd > 4): per-pixel global lookups, the pixels

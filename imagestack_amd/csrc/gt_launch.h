@@ -39,21 +39,16 @@ inline unsigned occupancy_pad(size_t static_lds, int waves_per_cu) {
     }
 
 inline int64_t pixel_blocks(int n, int row_w, int &tile_w) {
-    if (tile_w > 0 && row_w > 0 && (kBlock % tile_w) == 0 && tile_w <= kBlock) {
-        const int tiles_x = (row_w + tile_w - 1) / tile_w;
-        const int th = kBlock / tile_w;
-        const int rows = n / row_w;
-        return (int64_t)tiles_x * ((rows + th - 1) / th);
-    }
-    tile_w = 0;
-    return blocks_for(n, kBlock);
+    static_assert(kBlock == kTilePixels, "a tile is one workgroup's pixels");
+    return pixel_tiles(n, row_w, tile_w);
 }
 
 
 // Splat launcher for d = 7..10 with the LDS tile table (gt_launch_mid.hip).
 hipError_t launch_splat_tile_mid(int d, bool packed, bool bilat, const PixSrc &src, int n, int row_w, int tile_w,
                                  int vd, const LatConst &lc, const KeyPlan &kp, const ValScale *vsc,
-                                 const TableArgs &ta, Counters *ctr, bool count, int wpc, hipStream_t st);
+                                 const TableArgs &ta, const Replay &rp, Counters *ctr, bool count, int wpc,
+                                 hipStream_t st);
 
 // Launchers for the lattice dimensions without an LDS tile table (d > 6).
 hipError_t launch_splat_direct_hi(int d, bool packed, bool bilat, const PixSrc &src, int n, int row_w, int tile_w,

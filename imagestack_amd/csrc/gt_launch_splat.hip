@@ -7,11 +7,11 @@ namespace gt {
 
 hipError_t launch_splat(int d, bool packed, bool bilat, bool dedup, const PixSrc &src, int n, int row_w, int tile_w,
                         int vd, const LatConst &lc, const KeyPlan &kp, const ValScale *vsc, const TableArgs &ta,
-                        Counters *ctr, bool count, int wpc, hipStream_t st) {
+                        const Replay &rp, Counters *ctr, bool count, int wpc, hipStream_t st) {
     if (d > 6) {
         if (d <= 10 && dedup && vd <= 4)
-            return launch_splat_tile_mid(d, packed, bilat, src, n, row_w, tile_w, vd, lc, kp, vsc, ta, ctr, count,
-                                         wpc, st);
+            return launch_splat_tile_mid(d, packed, bilat, src, n, row_w, tile_w, vd, lc, kp, vsc, ta, rp, ctr,
+                                         count, wpc, st);
         return launch_splat_direct_hi(d, packed, bilat, src, n, row_w, tile_w, vd, lc, kp, vsc, ta, ctr, count, st);
     }
     const int64_t ntiles = pixel_blocks(n, row_w, tile_w);
@@ -24,10 +24,10 @@ hipError_t launch_splat(int d, bool packed, bool bilat, bool dedup, const PixSrc
         constexpr int W_ = KeyOps<D, P>::W;                                                                     \
         if (vd <= 2) hipLaunchKernelGGL((k_splat_tile<D, P, B, 2, kTileKeys, false>), dim3(ntiles),             \
                                         dim3(kBlock), occupancy_pad(splat_tile_lds<W_, 2, kTileKeys>(), wpc),    \
-                                        st, src, n, row_w, tile_w, none, vd, lc, kp, vsc, ta, ctr, cnt);         \
+                                        st, src, n, row_w, tile_w, none, vd, lc, kp, vsc, ta, rp, ctr, cnt);         \
         else hipLaunchKernelGGL((k_splat_tile<D, P, B, 4, kTileKeys, false>), dim3(ntiles), dim3(kBlock),        \
                                 occupancy_pad(splat_tile_lds<W_, 4, kTileKeys>(), wpc), st, src, n, row_w,       \
-                                tile_w, none, vd, lc, kp, vsc, ta, ctr, cnt);                                    \
+                                tile_w, none, vd, lc, kp, vsc, ta, rp, ctr, cnt);                                    \
     }
 #define GT_SPLAT_TILE(D)                                                                    \
     {                                                                                       \
@@ -55,7 +55,7 @@ hipError_t launch_splat(int d, bool packed, bool bilat, bool dedup, const PixSrc
 
 hipError_t launch_splat_batch(int d, const PixSrc &tmpl, const BatchArgs &ba, int ntiles, int tile_w, int vd,
                               const LatConst &lc, const KeyPlan &kp, const ValScale *vsc, const TableArgs &ta,
-                              Counters *ctr, bool count, int wpc, hipStream_t st) {
+                              const Replay &rp, Counters *ctr, bool count, int wpc, hipStream_t st) {
     if (ntiles <= 0) return hipSuccess;
     if (vd > 4 || !kp.packed) return hipErrorInvalidValue;
     const int cnt = count ? 1 : 0;
@@ -63,10 +63,10 @@ hipError_t launch_splat_batch(int d, const PixSrc &tmpl, const BatchArgs &ba, in
     {                                                                                                           \
         if (vd <= 2) hipLaunchKernelGGL((k_splat_tile<D, true, true, 2, kTileKeys, true>), dim3(ntiles),         \
                                         dim3(kBlock), occupancy_pad(splat_tile_lds<1, 2, kTileKeys>(), wpc), st, \
-                                        tmpl, 0, 0, tile_w, ba, vd, lc, kp, vsc, ta, ctr, cnt);                  \
+                                        tmpl, 0, 0, tile_w, ba, vd, lc, kp, vsc, ta, rp, ctr, cnt);                  \
         else hipLaunchKernelGGL((k_splat_tile<D, true, true, 4, kTileKeys, true>), dim3(ntiles), dim3(kBlock),   \
                                 occupancy_pad(splat_tile_lds<1, 4, kTileKeys>(), wpc), st, tmpl, 0, 0, tile_w,   \
-                                ba, vd, lc, kp, vsc, ta, ctr, cnt);                                              \
+                                ba, vd, lc, kp, vsc, ta, rp, ctr, cnt);                                              \
     }
     GT_DISPATCH_DT(d, GT_SPLAT_B)
 #undef GT_SPLAT_B

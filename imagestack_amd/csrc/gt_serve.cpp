@@ -39,7 +39,8 @@ uint64_t pow2_at_least(uint64_t v) {
 // Device bytes one request occupies while it runs: its input and output
 // planes plus the lattice buffers sized as the library sizes them for an
 // unseen image (hash table for min(n(d+1), max(65536, n)) vertices at load
-// 0.35; vertex arrays for that many vertices).
+// 0.35; vertex arrays for that many vertices; the splat's replay record over
+// the image padded to whole tiles of up to 64 x 16 pixels).
 size_t footprint(const gt_request &r) {
     const uint64_t n = (uint64_t)r.W * r.H;
     const int d = r.C + 2, vd = r.C + 1, vs = vd <= 2 ? 2 : 4;
@@ -47,9 +48,11 @@ size_t footprint(const gt_request &r) {
     const uint64_t cap = pow2_at_least((uint64_t)(est / 0.35));
     const uint64_t m = std::min<uint64_t>(cap, n * (d + 1));
     const size_t sb = r.storage == GT_STORE_F32 ? 4 : 2;
+    const uint64_t np = (uint64_t)((r.W + 63) / 64 * 64) * (uint64_t)((r.H + 15) / 16 * 16);
     return (size_t)(2 * r.C * n * 4                       // planes in / out
                     + cap * (8 + 8 * vd + 4)              // keys, fixed-point sums, slot ids
-                    + m * (4 + 16 + 4 + 8 + 2 * vs * sb + 8 * (d + 1)));   // compaction, sort, values, nbr
+                    + m * (4 + 16 + 4 + 8 + 2 * vs * sb + 8 * (d + 1))    // compaction, sort, values, nbr
+                    + np * (4 * 16 + 4 * (d + 1)) + np / 256 * (8 * 512 + 4));   // replay entries, tile lists
 }
 
 struct Req {
