@@ -399,11 +399,12 @@ int enqueue(gt_ctx *c, const Launch &L, const gt::PixSrc &src, const SliceSpec &
     if (c->profiling) GT_HIP(mark(c, 4, s));
     const unsigned *sbits = L.storage ? &L.vsc->vmax_bits[L.d][0] : nullptr;   // the last pass's scale
     if (L.rp.rec && batch)
-        GT_HIP(gt::launch_slice_replay(L.d, L.storage, batch->tmpl, 0, 0, L.tile_w, batch->ba, batch->ntiles, L.vd,
-                                       L.kp, ta, L.rp, cur, sbits, batch->tmpl_dst, ctr, L.wpc, s));
+        GT_HIP(gt::launch_slice_replay(L.d, true, L.storage, batch->tmpl, 0, 0, L.tile_w, batch->ba, batch->ntiles,
+                                       L.vd, L.lc, L.kp, ta, L.rp, cur, sbits, batch->tmpl_dst, ctr, L.wpc, s));
     else if (L.rp.rec)
-        GT_HIP(gt::launch_slice_replay(L.d, L.storage, sl.src, sl.n, L.row_w, L.tile_w, gt::BatchArgs{nullptr, 0}, 0,
-                                       L.vd, L.kp, ta, L.rp, cur, sbits, sl.dst, ctr, L.wpc, s));
+        GT_HIP(gt::launch_slice_replay(L.d, sl.src.ref != nullptr, L.storage, sl.src, sl.n, L.row_w, L.tile_w,
+                                       gt::BatchArgs{nullptr, 0}, 0, L.vd, L.lc, L.kp, ta, L.rp, cur, sbits, sl.dst,
+                                       ctr, L.wpc, s));
     else if (batch)
         GT_HIP(gt::launch_slice_batch(L.d, L.storage, batch->tmpl, batch->ba, batch->ntiles, L.tile_w, L.vd, L.lc,
                                       L.kp, ta, cur, sbits, batch->tmpl_dst, ctr, L.count, L.wpc, s));
@@ -725,12 +726,13 @@ int run(gt_ctx *c, const gt::PixSrc &src, int n, int row_w, int d, int vd, int s
         GT_TRY(ensure(c->vals_a, (size_t)L.m_max * L.vs * 4));
         GT_TRY(ensure(c->vals_b, (size_t)L.m_max * L.vs * store_bytes(storage)));
         GT_TRY(ensure(c->nbr, (size_t)(d + 1) * L.m_max * 8));
-        // Replay slice: a tiled splat records each pixel's weights and tile
-        // slots, and the slice at the same positions reads them back instead
-        // of recomputing the simplex and the keys (2048^2 joint bilateral:
-        // slice 116 -> 52 us, splat 146 -> 175 us, 0.429 -> 0.397 ms).  Not in
-        // the no-allocation entry point, whose buffers are fixed shares of its
-        // scratch array.
+        // Replay slice: a tiled splat records each pixel's tile-table slots,
+        // and the slice at the same positions reads them back instead of
+        // building the keys and inserting them into its own tile table
+        // (2048^2 joint bilateral: slice 116 -> 59 us, splat 146 -> 149 us,
+        // 0.429 -> 0.380 ms; recording the weights as well made the splat
+        // 175 us for a 52 us slice).  Not in the no-allocation entry point,
+        // whose buffers are fixed shares of its scratch array.
         zero(L.rp);
         const bool same_pos = sl.enabled && sl.n == n && sl.src.pos == src.pos && sl.src.ref == src.ref;
         if (!g_no_alloc && L.dedup && vd <= 4 && d <= 10 && (batch || same_pos)) {

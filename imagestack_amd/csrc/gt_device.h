@@ -20,6 +20,7 @@ constexpr uint64_t kEmpty = ~0ull;     // an unused table word (no valid key wor
 constexpr int kWave = 64;
 constexpr uint32_t kMaxProbes = 256;   // an insert that probes this far reports a full table
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));   // a 16-byte vector access
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));   // an 8-byte vector access
 
 // Enclosing simplex of one point: fixed-up remainder-0 point g, ranks rk and
 // barycentric weights b[0..D].  `bad` flags NaN/Inf or keys beyond int16.

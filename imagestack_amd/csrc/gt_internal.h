@@ -112,19 +112,19 @@ struct TableArgs {
 };
 
 // Replay record of the LDS-tiled splat, so that a slice at the splat's own
-// positions needs neither the positions nor the simplex again (the
-// reference's replay slice, src/Permutohedral.h:405-427, keeps an (offset,
-// weight) pair per pixel and vertex the same way).  Entry q = tile * 256 +
-// thread of the splat's workgroup for that tile; the slice runs the same tile
-// decomposition.  Entry q is replay_dwords(D) words, as 16-byte vectors: word
-// 4i + k of it is word k of uint4 i * stride + q of rec.  For vertex j in the
-// splat's per-lane rotated order (vertex (j + thread % (D+1)) % (D+1)), word j
-// holds its barycentric weight (FP32 bits) and the 10-bit field j % 3 of word
-// D+1 + j / 3 its slot in the tile's LDS table,
-// or kReplayGlobal when the tile table had no room for it and the splat went
-// to the global table directly, at slot gs[j * stride + q].  Per tile t:
-// tcnt[t] distinct keys, entry k of tent[t * kTileKeys + k] = {LDS slot,
-// global table slot}.  rec == nullptr: no record.
+// positions needs neither the vertex keys nor the tile table's inserts again
+// (the reference's replay slice, src/Permutohedral.h:405-427, keeps an
+// (offset, weight) pair per pixel and vertex the same way; here the slice
+// recomputes the weights, which costs less than moving them through HBM).
+// Entry q = tile * 256 + thread of the splat's workgroup for that tile; the
+// slice runs the same tile decomposition.  Entry q is the replay_dwords(D)
+// words rec[q * that ...]: for vertex j in the splat's per-lane rotated order
+// (vertex (j + thread % (D+1)) % (D+1)), the 10-bit field j % 3 of word j / 3
+// holds its slot in the tile's LDS table, or kReplayGlobal when the tile
+// table had no room for it and the splat went to the global table directly,
+// at slot gs[j * stride + q].  Per tile t: tcnt[t] distinct keys, entry k of
+// tent[t * kTileKeys + k] = {LDS slot, global table slot}.  rec == nullptr:
+// no record.
 struct Replay {
     uint32_t *rec;
     uint32_t *gs;
@@ -133,9 +133,9 @@ struct Replay {
     int64_t stride;   // entries: tiles * 256
 };
 constexpr unsigned kReplayGlobal = 1023u;
-// Words of one replay entry: D+1 weights and ceil((D+1)/3) slot words,
-// rounded up to whole 16-byte vectors (8 words for d = 5).
-constexpr int replay_dwords(int D) { return (D + 1 + (D + 3) / 3 + 3) / 4 * 4; }
+// Words of one replay entry: ceil((D+1)/3) slot words as one 8- or 16-byte
+// vector (2 words up to d = 5).
+constexpr int replay_dwords(int D) { return (D + 3) / 3 <= 2 ? 2 : 4; }
 constexpr int kTileKeys = 512;   // LDS tile table slots of the tiled splat and slice
 constexpr int kTilePixels = 256;   // pixels of a tile: one workgroup of the tiled kernels
 
@@ -262,10 +262,10 @@ hipError_t launch_blur(int storage, int vs, int pass, const ValScale *vsc, const
 // Slice from the tiled splat's replay record (the splat's own positions,
 // d <= 10, vd <= 4; batch: ba.count > 0, over ntiles tiles); the same tile
 // decomposition as that splat (n, row_w, tile_w or the batch's tiles).
-hipError_t launch_slice_replay(int d, int storage, const PixSrc &src, int n, int row_w, int tile_w,
-                               const BatchArgs &ba, int ntiles, int vd, const KeyPlan &kp, const TableArgs &ta,
-                               const Replay &rp, const void *vals, const unsigned *sbits, const OutDst &dst,
-                               Counters *ctr, int wpc, hipStream_t st);
+hipError_t launch_slice_replay(int d, bool bilat, int storage, const PixSrc &src, int n, int row_w, int tile_w,
+                               const BatchArgs &ba, int ntiles, int vd, const LatConst &lc, const KeyPlan &kp,
+                               const TableArgs &ta, const Replay &rp, const void *vals, const unsigned *sbits,
+                               const OutDst &dst, Counters *ctr, int wpc, hipStream_t st);
 hipError_t launch_slice(int d, bool packed, bool bilat, bool tiled, int storage, const PixSrc &src, int n, int row_w,
                         int tile_w, int vd, int vs, const LatConst &lc, const KeyPlan &kp, const TableArgs &ta,
                         const void *vals, const unsigned *sbits, const OutDst &dst, Counters *ctr, bool count,

@@ -390,9 +390,9 @@ __global__ __launch_bounds__(kBlock) void k_splat_tile(PixSrc src0, int n0, int 
                 for (int r = 0; r <= D; r++) key[r][w] = col[r];
             }
             constexpr int RW = replay_dwords(D);
-            uint32_t rslot[RW - (D + 1)];   // the replay entry's slot words
+            uint32_t rslot[RW];   // the replay entry
 #pragma unroll
-            for (int i = 0; i < RW - (D + 1); i++) rslot[i] = 0u;
+            for (int i = 0; i < RW; i++) rslot[i] = 0u;
 #pragma unroll
             for (int r = 0; r <= D; r++) {
                 const uint64_t(&k)[W] = key[r];
@@ -420,19 +420,13 @@ __global__ __launch_bounds__(kBlock) void k_splat_tile(PixSrc src0, int n0, int 
                     atomics += vd;
                 }
             }
-            if (rp.rec) {   // one entry, whole 16-byte vectors
-                uint32_t e[RW];
-#pragma unroll
-                for (int r = 0; r <= D; r++) e[r] = __float_as_uint(wt[r]);
-#pragma unroll
-                for (int i = D + 1; i < RW; i++) e[i] = rslot[i - (D + 1)];
-                // vector i of every entry in plane i (each store a contiguous
-                // run of lines), streamed past the L2 that holds the table
-                u32x4 *dst = reinterpret_cast<u32x4 *>(rp.rec) + rq;
-#pragma unroll
-                for (int i = 0; i < RW / 4; i++) {
-                    const u32x4 v = {e[4 * i], e[4 * i + 1], e[4 * i + 2], e[4 * i + 3]};
-                    __builtin_nontemporal_store(v, dst + i * rp.stride);
+            if (rp.rec) {   // one vector store, streamed past the L2 that holds the table
+                if constexpr (RW == 2) {
+                    const u32x2 v = {rslot[0], rslot[1]};
+                    __builtin_nontemporal_store(v, reinterpret_cast<u32x2 *>(rp.rec) + rq);
+                } else {
+                    const u32x4 v = {rslot[0], rslot[1], rslot[2], rslot[3]};
+                    __builtin_nontemporal_store(v, reinterpret_cast<u32x4 *>(rp.rec) + rq);
                 }
             }
         }
@@ -1369,19 +1363,21 @@ __device__ __forceinline__ int slot_vertex(const TableArgs &ta, uint32_t slot) {
 // Slice at the splat's own positions from the tiled splat's replay record
 // (src/Permutohedral.h:405-427): the workgroup of tile t stages the blurred
 // values of the tile's distinct vertices in LDS at the splat's tile-table
-// slots (their global slots give the ids, no probing), then each pixel sums
+// slots (their global slots give the ids, no probing); each pixel recomputes
+// its simplex for the weights only (no keys, no tile-table inserts) and sums
 // weight * value over its D+1 vertices in vertex order -- the same FP32
-// operations as the slice that recomputes the simplex, so the output is the
-// same bit for bit.  Records are checked against the table and LDS bounds (a
-// call whose splat flagged an error writes records only partly and is
-// discarded, but its slice still runs).
-template <int D, typename T, int LVD, int LT, bool BATCH>
+// operations as the slice that finds its vertices by key, so the output is
+// the same bit for bit.  Records are checked against the table and LDS
+// bounds (a call whose splat flagged an error writes records only partly and
+// is discarded, but its slice still runs).
+template <int D, bool BILAT, typename T, int LVD, int LT, bool BATCH>
 __global__ __launch_bounds__(kBlock) void k_slice_replay(PixSrc src0, int n0, int row_w0, int tile_w, BatchArgs ba,
-                                                         int vd, KeyPlan kp, TableArgs ta, Replay rp,
+                                                         int vd, LatConst lc, KeyPlan kp, TableArgs ta, Replay rp,
                                                          const T *__restrict__ vals,
                                                          const unsigned *__restrict__ sbits, OutDst dst0,
                                                          Counters *ctr) {
     constexpr bool kScaled = !std::is_same<T, float>::value;
+    constexpr int RW = replay_dwords(D);
     __shared__ float lv[LT * LVD];
     if (*(volatile const unsigned *)&ctr->flags & kFlagFull) return;   // redone with a larger table
     const unsigned M = ctr->M;
@@ -1393,45 +1389,44 @@ __global__ __launch_bounds__(kBlock) void k_slice_replay(PixSrc src0, int n0, in
     int px, prow;
     const int p = pixel_index(tv.local, threadIdx.x, tv.n, tv.row_w, tile_w, px, prow);
     const unsigned rq = (unsigned)tile * kBlock + threadIdx.x;
-    constexpr int RW = replay_dwords(D);
     uint32_t e[RW];
+    float pos[D];
     if (p >= 0) {   // (issued before the staging: the loads overlap it)
-        const u32x4 *src = reinterpret_cast<const u32x4 *>(rp.rec) + rq;
-#pragma unroll
-        for (int i = 0; i < RW / 4; i++) {
-            const u32x4 u = __builtin_nontemporal_load(src + i * rp.stride);
-            e[4 * i] = u.x;
-            e[4 * i + 1] = u.y;
-            e[4 * i + 2] = u.z;
-            e[4 * i + 3] = u.w;
+        if constexpr (RW == 2) {
+            const u32x2 u = __builtin_nontemporal_load(reinterpret_cast<const u32x2 *>(rp.rec) + rq);
+            e[0] = u.x;
+            e[1] = u.y;
+        } else {
+            const u32x4 u = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(rp.rec) + rq);
+            e[0] = u.x;
+            e[1] = u.y;
+            e[2] = u.z;
+            e[3] = u.w;
         }
+        load_position<D, BILAT>(tv.src, p, px, prow, pos);
     }
     const unsigned cnt = min(rp.tcnt[tile], (unsigned)LT);
     for (unsigned j = threadIdx.x; j < cnt; j += kBlock) {
-        const uint2 e = rp.tent[(size_t)tile * LT + j];
-        if (e.x >= (unsigned)LT) continue;
-        const int id = e.y <= ta.mask ? slot_vertex(ta, e.y) : -1;
+        const uint2 te = rp.tent[(size_t)tile * LT + j];
+        if (te.x >= (unsigned)LT) continue;
+        const int id = te.y <= ta.mask ? slot_vertex(ta, te.y) : -1;
         float row[kMaxVD];
 #pragma unroll
         for (int c = 0; c < LVD; c++) row[c] = 0.0f;
         if (id >= 0 && (unsigned)id < M) load_row<T, LVD>(vals, id, row);
 #pragma unroll
-        for (int c = 0; c < LVD; c++) lv[c * LT + e.x] = kScaled ? row[c] * un[c] : row[c];
+        for (int c = 0; c < LVD; c++) lv[c * LT + te.x] = kScaled ? row[c] * un[c] : row[c];
     }
+    Simplex<D> s;
+    if (p >= 0) simplex<D>(pos, lc.sf, s);   // (before the barrier: overlaps the staging)
     __syncthreads();
-    if (p < 0) return;
-    float w[D + 1];
+    if (p < 0 || s.bad) return;   // (a bad position fails the call)
     int ls[D + 1];
 #pragma unroll
-    for (int j = 0; j <= D; j++) {
-        w[j] = __uint_as_float(e[j]);
-        ls[j] = (int)((e[D + 1 + j / 3] >> (10 * (j % 3))) & 1023u);
-    }
+    for (int j = 0; j <= D; j++) ls[j] = (int)((e[j / 3] >> (10 * (j % 3))) & 1023u);
     // vertex order: the splat stored vertex (j + rot) % (D+1) at j
     const int rot = (int)(threadIdx.x % (D + 1));
-    const int back = (D + 1 - rot) % (D + 1);
-    rotate_left<D + 1>(w, back);
-    rotate_left<D + 1>(ls, back);
+    rotate_left<D + 1>(ls, (D + 1 - rot) % (D + 1));
     float acc[kMaxVD];
 #pragma unroll
     for (int c = 0; c < LVD; c++) acc[c] = 0.0f;
@@ -1439,7 +1434,7 @@ __global__ __launch_bounds__(kBlock) void k_slice_replay(PixSrc src0, int n0, in
     for (int r = 0; r <= D; r++) {
         if (ls[r] < LT) {
 #pragma unroll
-            for (int c = 0; c < LVD; c++) acc[c] = acc[c] + w[r] * lv[c * LT + ls[r]];
+            for (int c = 0; c < LVD; c++) acc[c] = acc[c] + s.b[r] * lv[c * LT + ls[r]];
         } else if (ls[r] == (int)kReplayGlobal) {
             const int j = (r + D + 1 - rot) % (D + 1);
             const uint32_t gs = rp.gs[j * rp.stride + rq];
@@ -1448,7 +1443,7 @@ __global__ __launch_bounds__(kBlock) void k_slice_replay(PixSrc src0, int n0, in
                 float row[kMaxVD];
                 load_row<T, LVD>(vals, id, row);
 #pragma unroll
-                for (int c = 0; c < LVD; c++) acc[c] = acc[c] + w[r] * (row[c] * un[c]);
+                for (int c = 0; c < LVD; c++) acc[c] = acc[c] + s.b[r] * (row[c] * un[c]);
             }
         }
     }

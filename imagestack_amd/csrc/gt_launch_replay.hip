@@ -1,29 +1,30 @@
 // Slice launcher from the tiled splat's replay record (d <= 10, vd <= 4,
-// single image or batch).  The kernel reads no positions, so it is templated
-// on the vertex count, the storage type, the channel group and the batch form
-// only.
+// single image or batch).  The kernel builds no keys, so it is templated on
+// the lattice dimension, the position form, the storage type, the channel
+// group and the batch form only.
 #include "gt_launch.h"
 
 namespace gt {
 
-hipError_t launch_slice_replay(int d, int storage, const PixSrc &src, int n, int row_w, int tile_w,
-                               const BatchArgs &ba, int ntiles, int vd, const KeyPlan &kp, const TableArgs &ta,
-                               const Replay &rp, const void *vals, const unsigned *sbits, const OutDst &dst,
-                               Counters *ctr, int wpc, hipStream_t st) {
+hipError_t launch_slice_replay(int d, bool bilat, int storage, const PixSrc &src, int n, int row_w, int tile_w,
+                               const BatchArgs &ba, int ntiles, int vd, const LatConst &lc, const KeyPlan &kp,
+                               const TableArgs &ta, const Replay &rp, const void *vals, const unsigned *sbits,
+                               const OutDst &dst, Counters *ctr, int wpc, hipStream_t st) {
     const bool batch = ba.count > 0;
     if (vd > 4 || d < 1 || d > 10 || !rp.rec) return hipErrorInvalidValue;
     int64_t tiles = ntiles;
     if (!batch) tiles = pixel_blocks(n, row_w, tile_w);
     if (tiles <= 0) return hipSuccess;
     if (tiles * kBlock > rp.stride) return hipErrorInvalidValue;   // not the splat's decomposition
-#define GT_RP(D, T, LVD, B)                                                                                  \
-    hipLaunchKernelGGL((k_slice_replay<D, T, LVD, kTileKeys, B>), dim3((unsigned)tiles), dim3(kBlock),        \
-                       occupancy_pad((size_t)kTileKeys * LVD * 4, wpc), st, src, n, row_w, tile_w, ba, vd, kp, \
-                       ta, rp, (const T *)vals, sbits, dst, ctr)
-#define GT_RP_B(D, T, LVD)                 \
-    {                                      \
-        if (batch) GT_RP(D, T, LVD, true); \
-        else GT_RP(D, T, LVD, false);      \
+#define GT_RP(D, P, T, LVD, B)                                                                               \
+    hipLaunchKernelGGL((k_slice_replay<D, P, T, LVD, kTileKeys, B>), dim3((unsigned)tiles), dim3(kBlock),     \
+                       occupancy_pad((size_t)kTileKeys * LVD * 4, wpc), st, src, n, row_w, tile_w, ba, vd, lc, \
+                       kp, ta, rp, (const T *)vals, sbits, dst, ctr)
+#define GT_RP_B(D, T, LVD)                          \
+    {                                               \
+        if (batch) GT_RP(D, true, T, LVD, true);    \
+        else if (bilat) GT_RP(D, true, T, LVD, false); \
+        else GT_RP(D, false, T, LVD, false);        \
     }
 #define GT_RP_V(D, T)                    \
     {                                    \

@@ -52,7 +52,7 @@ size_t footprint(const gt_request &r) {
     return (size_t)(2 * r.C * n * 4                       // planes in / out
                     + cap * (8 + 8 * vd + 4)              // keys, fixed-point sums, slot ids
                     + m * (4 + 16 + 4 + 8 + 2 * vs * sb + 8 * (d + 1))    // compaction, sort, values, nbr
-                    + np * (4 * 16 + 4 * (d + 1)) + np / 256 * (8 * 512 + 4));   // replay entries, tile lists
+                    + np * (16 + 4 * (d + 1)) + np / 256 * (8 * 512 + 4));   // replay entries, tile lists
 }
 
 struct Req {
