@@ -124,6 +124,7 @@ struct gt_ctx {
     DevBuf grid_marks;                 // bilateral grid: NaN/Inf marks (calls with such values only)
     DevBuf patch_w, patch_guide;       // non-local means: reordered filters, projected guide
     DevBuf rp_rec, rp_gs, rp_tent, rp_tcnt;   // the tiled splat's replay record
+    DevBuf tags;                               // per-slot key tags (hash ids, wide keys)
     gt::Counters *h_ctr = nullptr;   // pinned
     float *h_mm = nullptr;           // pinned, 3 * 2 * kMaxDH
     // capacity memory: last lattice size seen for (n, d), and vertices per
@@ -756,6 +757,15 @@ int run(gt_ctx *c, const gt::PixSrc &src, int n, int row_w, int d, int vd, int s
         L.ta.tnf = (uint32_t *)c->tnf.p;
         L.ta.mask = cap - 1;
         L.ta.slot_id = (int *)c->slot_id.p;
+        // Tags for the neighbour lookups of wide keys (hash ids): d = 16,
+        // 1024^2 (17.8 M vertices, 2 GB of key words; 94% of lookups miss):
+        // neighbours 12.1 -> 8.1 ms probing tag by tag, 6.5 ms eight tags
+        // at a time.
+        L.ta.tags = nullptr;
+        if (L.hashnbr && L.W > 1) {
+            GT_TRY(ensure(c->tags, (size_t)cap));
+            L.ta.tags = (uint8_t *)c->tags.p;
+        }
         // One-word keys leave room above the key for the vertex id: fold it in
         // at the finalize, so lookups after it read key and id in one word.
         {
@@ -1211,7 +1221,7 @@ void gt_ctx_destroy(gt_ctx *c) {
                       &c->mm_part,     &c->mm_out,    &c->stage_in, &c->stage_out, &c->view_a,    &c->view_b,
                       &c->view_o,      &c->grid_acc,  &c->grid_a,   &c->grid_b,    &c->grid_marks,
                       &c->patch_w,     &c->patch_guide, &c->rp_rec, &c->rp_gs,
-                      &c->rp_tent,     &c->rp_tcnt};
+                      &c->rp_tent,     &c->rp_tcnt,   &c->tags};
     for (DevBuf *b : bufs) free_buf(*b);
     for (auto &e : c->ev)
         if (e) (void)hipEventDestroy(e);

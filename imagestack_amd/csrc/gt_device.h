@@ -235,17 +235,25 @@ __device__ __forceinline__ bool packed_vertex_keys(const Simplex<D> &s, const Ke
     return ok;
 }
 
-template <int W>
-__device__ __forceinline__ uint64_t hash_key(const uint64_t (&k)[W]) {
-    uint64_t h = 0x9E3779B97F4A7C15ull;
-#pragma unroll
-    for (int w = 0; w < W; w++) {
-        h ^= k[w];
-        h *= 0xff51afd7ed558ccdull;
-        h ^= h >> 33;
-    }
+// Key hash: the words mixed one by one and summed, then a final mix.  A
+// sum, so that a key that differs from a known one in a single word (a
+// lattice neighbour: the same constant added to every field, then one field
+// moved) hashes with one word mix instead of W (k_nbr_hash_vertex).
+__device__ __forceinline__ uint64_t hash_word(uint64_t x, int w) {
+    x ^= 0x9E3779B97F4A7C15ull * (uint64_t)(2 * w + 1);
+    x *= 0xff51afd7ed558ccdull;
+    return x ^ (x >> 33);
+}
+__device__ __forceinline__ uint64_t hash_final(uint64_t h) {
     h *= 0xc4ceb9fe1a85ec53ull;
     return h ^ (h >> 29);
+}
+template <int W>
+__device__ __forceinline__ uint64_t hash_key(const uint64_t (&k)[W]) {
+    uint64_t h = 0;
+#pragma unroll
+    for (int w = 0; w < W; w++) h += hash_word(k[w], w);
+    return hash_final(h);
 }
 
 // Insert-or-find in the global open-addressing table (linear probing; slot s
@@ -282,6 +290,50 @@ __device__ __forceinline__ int table_insert(uint64_t *__restrict__ tkeys, uint32
         }
         if (match) return (int)s;
         s = (s + 1) & mask;
+    }
+    return -1;
+}
+
+// Tag of a key with 64-bit hash h (TableArgs::tags): never 0 (empty).
+__device__ __forceinline__ uint8_t key_tag8(uint64_t h) { return (uint8_t)(0x80u | (uint32_t)(h >> 57)); }
+
+// Lookup through the tags (TableArgs::tags) of a complete table, from home
+// slot s: eight tags at a time (two aligned 8-byte loads), the empty ones and
+// the ones equal to `want` found by byte-parallel masks, key words compared
+// only at candidates before the first empty slot, in probe order.  The
+// zero-byte test may also flag a byte just above a zero one; a flagged byte
+// is a candidate only, checked against the key words, so the result is
+// exact.  Returns the slot (w0: its first key word) or -1.
+template <int W>
+__device__ __forceinline__ int tag_find(const TableArgs &ta, const uint64_t (&k)[W], uint64_t kmask, uint32_t s,
+                                        uint8_t want, uint64_t &w0) {
+    constexpr uint64_t kOnes = 0x0101010101010101ull, kTops = 0x8080808080808080ull;
+    const uint64_t wv = kOnes * want;
+    for (uint32_t n = 0; n < kMaxProbes; n += 8) {
+        const uint32_t a = s & ~7u;
+        const uint64_t lo = *reinterpret_cast<const uint64_t *>(ta.tags + a);
+        const uint64_t hi = *reinterpret_cast<const uint64_t *>(ta.tags + ((a + 8) & ta.mask));
+        const int sh = 8 * (int)(s & 7u);
+        const uint64_t t = sh ? (lo >> sh) | (hi << (64 - sh)) : lo;   // tags of slots s .. s+7
+        const uint64_t empty = ~t & kTops;
+        const uint64_t x = t ^ wv;
+        uint64_t cand = (x - kOnes) & ~x & kTops;
+        if (empty) cand &= (empty & (0ull - empty)) - 1;   // before the first empty slot
+        while (cand) {
+            const uint32_t slot = (s + (uint32_t)(__builtin_ctzll(cand) >> 3)) & ta.mask;
+            const uint64_t *kw = ta.tkeys + (size_t)slot * W;
+            const uint64_t v0 = kw[0];
+            bool match = (v0 & kmask) == k[0];
+#pragma unroll
+            for (int w = 1; w < W; w++) match = match && kw[w] == k[w];
+            if (match) {
+                w0 = v0;
+                return (int)slot;
+            }
+            cand &= cand - 1;
+        }
+        if (empty) return -1;
+        s = (s + 8) & ta.mask;
     }
     return -1;
 }

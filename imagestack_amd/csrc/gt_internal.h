@@ -109,6 +109,13 @@ struct TableArgs {
     const BatchImage *part_img;    // a batch: per-image part slices, by the key's image tag
     int part_lg;                   // log2 of the part count P (>= 8)
     int part_shift;                // log2(capacity / P)
+    // capacity bytes or null: after the finalize (ids in compaction order),
+    // slot s's tag -- 0 empty, else 0x80 | 7 bits of its key's hash
+    // (key_tag8).  A lookup probes the tags and reads key words only where the
+    // tag matches: a lattice of wide keys is gigabytes of key words, and most
+    // neighbour lookups of a sparse one miss, so the probes stay in the tags of
+    // the XCD's own table parts, in its L2.
+    uint8_t *tags;
 };
 
 // Replay record of the LDS-tiled splat, so that a slice at the splat's own

@@ -647,11 +647,16 @@ __global__ __launch_bounds__(kBlock) void k_write_final(TableArgs ta, uint32_t c
             off += (w < wave) ? ws[w] : 0u;
             tot += ws[w];
         }
+        uint8_t tg = 0;
         if (occ) {
             const unsigned pos = off + (unsigned)__popcll(m & lanemask_lt());
-            vkeys[(size_t)pos * W] = w0;
+            uint64_t kw[W];
+            kw[0] = w0;
 #pragma unroll
-            for (int w = 1; w < W; w++) vkeys[(size_t)pos * W + w] = ta.tkeys[s * W + w];
+            for (int w = 1; w < W; w++) kw[w] = ta.tkeys[s * W + w];
+#pragma unroll
+            for (int w = 0; w < W; w++) vkeys[(size_t)pos * W + w] = kw[w];
+            if (ta.tags) tg = key_tag8(hash_key<W>(kw));
             float o[kMaxVD];
             take_sums(ta, vsc, s, vd, o);
             store_row_rt<T>(vals, pos, vs, o);
@@ -659,6 +664,7 @@ __global__ __launch_bounds__(kBlock) void k_write_final(TableArgs ta, uint32_t c
             if (W == 1 && ta.idshift > 0) ta.tkeys[s] = w0 | ((uint64_t)pos << ta.idshift);
             else ta.slot_id[s] = (int)pos;
         }
+        if (ta.tags && s < cap) ta.tags[s] = tg;
         run += tot;
         __syncthreads();
     }
@@ -1055,7 +1061,8 @@ __global__ __launch_bounds__(kBlock) void k_nbr_hash_vertex(const uint64_t *__re
         uint64_t k[W], one[W];
 #pragma unroll
         for (int w = 0; w < W; w++) {
-            k[w] = vk[(size_t)i * W + w];
+            // (streamed: the table's tags and key lines are what should stay in L2)
+            k[w] = __builtin_nontemporal_load(vk + (size_t)i * W + w);
             one[w] = 0;
         }
         uint32_t no_up1 = 0, no_dn1 = 0, ok_upD = 0, ok_dnD = 0;   // bit q: field q
@@ -1075,10 +1082,25 @@ __global__ __launch_bounds__(kBlock) void k_nbr_hash_vertex(const uint64_t *__re
             if (f + D <= hi) ok_upD |= 1u << q;
             if (f >= lo + D) ok_dnD |= 1u << q;
         }
+        // Targets: every word moves by +-one (the same for all axes), then
+        // axis j < D moves word j/4 (packed: the one word) by -+big.  Their
+        // hashes are the sum of the word hashes of k +- one, with word j/4's
+        // replaced.
+        uint64_t gw[2][W], gsum[2];
+#pragma unroll
+        for (int sgn = 0; sgn < 2; sgn++) {
+            gsum[sgn] = 0;
+#pragma unroll
+            for (int w = 0; w < W; w++) {
+                gw[sgn][w] = hash_word(sgn ? k[w] - one[w] : k[w] + one[w], w);
+                gsum[sgn] += gw[sgn][w];
+            }
+        }
         for (int j = 0; j <= D; j++) {
             uint64_t t[2][W];
             bool ok[2];
             int tsum[2];
+            uint64_t hsum[2] = {gsum[0], gsum[1]};
 #pragma unroll
             for (int w = 0; w < W; w++) {
                 t[0][w] = k[w] + one[w];
@@ -1091,12 +1113,25 @@ __global__ __launch_bounds__(kBlock) void k_nbr_hash_vertex(const uint64_t *__re
                 const int wj = PACKED ? 0 : j / 4;
                 const int shj = PACKED ? kp.shift[j] : 48 - 16 * (j % 4);
                 const uint64_t big = (uint64_t)(D + 1) << shj;
+                uint64_t m0 = t[0][0], m1 = t[1][0], g0 = gw[0][0], g1 = gw[1][0];
+#pragma unroll
+                for (int w = 1; w < W; w++)
+                    if (w == wj) {
+                        m0 = t[0][w];
+                        m1 = t[1][w];
+                        g0 = gw[0][w];
+                        g1 = gw[1][w];
+                    }
+                m0 -= big;
+                m1 += big;
 #pragma unroll
                 for (int w = 0; w < W; w++)
                     if (w == wj) {
-                        t[0][w] -= big;
-                        t[1][w] += big;
+                        t[0][w] = m0;
+                        t[1][w] = m1;
                     }
+                hsum[0] += hash_word(m0, wj) - g0;   // one word rehashed
+                hsum[1] += hash_word(m1, wj) - g1;
                 tsum[0] = sum - 1;
                 tsum[1] = sum + 1;
             } else {
@@ -1109,13 +1144,22 @@ __global__ __launch_bounds__(kBlock) void k_nbr_hash_vertex(const uint64_t *__re
             bool done[2];
             int slot[2] = {-1, -1};
             uint64_t hit[2] = {0, 0};
+            uint8_t want[2];
 #pragma unroll
             for (int sgn = 0; sgn < 2; sgn++) {
-                const uint32_t h = (uint32_t)hash_key<W>(t[sgn]);
+                const uint64_t h64 = hash_final(hsum[sgn]);   // = hash_key<W>(t[sgn])
+                const uint32_t h = (uint32_t)h64;
                 s[sgn] = ta.part_mul ? slot_in_part(ta, h, tsum[sgn], tag) : (h & ta.mask);
+                want[sgn] = key_tag8(h64);
                 done[sgn] = !ok[sgn];
             }
-            for (uint32_t it = 0; it < kMaxProbes && !(done[0] && done[1]); it++) {
+            // through the tags: key words only where a tag matches
+            if (ta.tags) {
+#pragma unroll
+                for (int sgn = 0; sgn < 2; sgn++)
+                    if (!done[sgn]) slot[sgn] = tag_find<W>(ta, t[sgn], kmask, s[sgn], want[sgn], hit[sgn]);
+            }
+            for (uint32_t it = 0; !ta.tags && it < kMaxProbes && !(done[0] && done[1]); it++) {
                 uint64_t w0[2];
 #pragma unroll
                 for (int sgn = 0; sgn < 2; sgn++) w0[sgn] = done[sgn] ? kEmpty : ta.tkeys[(size_t)s[sgn] * W];
@@ -1142,7 +1186,8 @@ __global__ __launch_bounds__(kBlock) void k_nbr_hash_vertex(const uint64_t *__re
 #pragma unroll
             for (int sgn = 0; sgn < 2; sgn++)
                 res[sgn] = slot[sgn] < 0 ? -1 : fold ? (int)(hit[sgn] >> ta.idshift) : ta.slot_id[slot[sgn]];
-            *((int2 *)nbr + (size_t)j * stride + i) = make_int2(res[0], res[1]);
+            const u32x2 pair = {(unsigned)res[0], (unsigned)res[1]};
+            __builtin_nontemporal_store(pair, reinterpret_cast<u32x2 *>(nbr) + (size_t)j * stride + i);
         }
     }
 }

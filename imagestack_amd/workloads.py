@@ -16,7 +16,8 @@ nlmeans : non-local means on a noisy 2048x2048 RGB image (7x7 patches, 6 PCA
           dimensions, sigma_s 16, patch sigma 0.4; d = 8 lattice).
 
 The pipeline runs as a replayed graph unless --eager (launch by launch, with
-per-stage HIP events).  Prints one JSON line with the mean wall time per call.
+per-stage HIP events).  --tune takes gt_tuning fields as JSON, e.g.
+'{"load_factor": 0.5}'.  Prints one JSON line with the mean wall time per call.
 """
 import argparse
 import json
@@ -35,7 +36,9 @@ def main() -> None:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--storage", default="f32", choices=["f32", "bf16", "f16"])
     ap.add_argument("--eager", action="store_true")
+    ap.add_argument("--tune", default=None, help="gt_tuning fields as JSON")
     a = ap.parse_args()
+    tune = json.loads(a.tune) if a.tune else None
 
     import numpy as np
     import torch
@@ -52,7 +55,8 @@ def main() -> None:
         npx = im.shape[-1] * im.shape[-2]
 
         def call():
-            return ctx.joint_bilateral(d_im, d_g, ss, ss, 0.0, sr, out=out, storage=a.storage, stats=True)[1]
+            return ctx.joint_bilateral(d_im, d_g, ss, ss, 0.0, sr, out=out, storage=a.storage, tune=tune,
+                                       stats=True)[1]
     elif a.workload == "highdim":
         side = 1024
         pos = synth.highdim_positions(side, side, nfeat=14, sigma_s=12.0, seed=3)
@@ -64,7 +68,7 @@ def main() -> None:
         npx = side * side
 
         def call():
-            return ctx.gauss_transform(dp, dv, inv, row_w=side, storage=a.storage, stats=True)[1]
+            return ctx.gauss_transform(dp, dv, inv, row_w=side, storage=a.storage, tune=tune, stats=True)[1]
     elif a.workload == "batch":
         imgs = synth.batch_images()
         dev = [torch.from_numpy(im[:, None]).cuda() for im, _, _ in imgs]
@@ -72,7 +76,7 @@ def main() -> None:
         npx = sum(im.shape[1] * im.shape[2] for im, _, _ in imgs)
 
         def call():
-            return ctx.joint_bilateral_batch(items, storage=a.storage, stats=True)
+            return ctx.joint_bilateral_batch(items, storage=a.storage, tune=tune, stats=True)
     else:
         from imagestack_amd import nlmeans as NL
         im = synth.noisy_rect_image(2048, 2048, 3, K=64, amp=0.2, seed=46)
@@ -82,7 +86,7 @@ def main() -> None:
         npx = 2048 * 2048
 
         def call():
-            return ctx.nlmeans(d_im, d_f, 16.0, 0.4, storage=a.storage, stats=True)[1]
+            return ctx.nlmeans(d_im, d_f, 16.0, 0.4, storage=a.storage, tune=tune, stats=True)[1]
 
     for _ in range(a.warmup):
         call()
@@ -93,7 +97,7 @@ def main() -> None:
     torch.cuda.synchronize()
     ms = (time.perf_counter() - t0) * 1e3 / a.steps
     ctx.close()
-    print(json.dumps({"workload": a.workload, "storage": a.storage, "eager": a.eager, "steps": a.steps,
+    print(json.dumps({"workload": a.workload, "storage": a.storage, "eager": a.eager, "tune": tune, "steps": a.steps,
                       "ms_per_call": round(ms, 4), "mpix_s": round(npx / ms / 1e3, 2), "M": st["M"],
                       "capacity": st["capacity"],
                       "stages_ms": {k: round(st[k], 4) for k in ("ms_splat", "ms_sort", "ms_nbr", "ms_blur",
