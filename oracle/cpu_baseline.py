@@ -17,8 +17,9 @@ Run as a child process so the OpenMP thread count is fixed before the
 library loads:
     OMP_NUM_THREADS=1 python -m oracle.cpu_baseline --side 2048 --reps 3
 Prints one JSON object.  With --procs P, P processes each filter their own
-copy concurrently (independent images, one per core) and the aggregate rate
-is reported.
+copy concurrently (independent images, one per core; each loads and warms
+up, then all start timing at the same moment) and the aggregate rate is
+reported.
 """
 import argparse
 import json
@@ -47,7 +48,7 @@ def nlmeans_workload(side: int):
     return im, NL.patch_pca(im, 1.0, 6, seed=0), 16.0, 0.4
 
 
-def run_one(side: int, reps: int, kind: str, workload: str = "lattice") -> dict:
+def run_one(side: int, reps: int, kind: str, workload: str = "lattice", start_at: float = 0.0) -> dict:
     from imagestack_amd import synth as S
     from oracle import oracle as O
     if workload == "nlmeans":
@@ -71,6 +72,9 @@ def run_one(side: int, reps: int, kind: str, workload: str = "lattice") -> dict:
         else:
             fn = lambda: O.joint_bilateral(im, guide, ss, ss, 0.0, sr, mode=O.F32)  # noqa: E731
     fn()  # warm-up
+    # concurrent processes start their timed runs together
+    if start_at > 0.0:
+        time.sleep(max(0.0, start_at - time.time()))
     times = []
     for _ in range(reps):
         t0 = time.perf_counter()
@@ -87,19 +91,22 @@ def main():
     ap.add_argument("--kind", default="auto", choices=["auto", "reference", "port"])
     ap.add_argument("--procs", type=int, default=1)
     ap.add_argument("--workload", default="lattice", choices=["lattice", "grid", "nlmeans"])
+    ap.add_argument("--start-at", type=float, default=0.0, help="wall-clock time (epoch s) to start timing at")
     a = ap.parse_args()
     from oracle import oracle as O
     kind = a.kind
     if kind == "auto":
         kind = "reference" if O.ref_available("fast") else "port"
     if a.procs <= 1:
-        r = run_one(a.side, a.reps, kind, a.workload)
+        r = run_one(a.side, a.reps, kind, a.workload, a.start_at)
         r.update({"kind": kind, "cores": 1, "side": a.side, "workload": a.workload})
         print(json.dumps(r))
         return
     env = dict(os.environ, OMP_NUM_THREADS="1")
+    # every process loads and warms up first, then all time their reps in the
+    # same window (set well past the slowest start-up)
     cmd = [sys.executable, "-m", "oracle.cpu_baseline", "--side", str(a.side), "--reps", str(a.reps),
-           "--kind", kind, "--workload", a.workload]
+           "--kind", kind, "--workload", a.workload, "--start-at", str(time.time() + 30.0)]
     t0 = time.perf_counter()
     procs = [subprocess.Popen(cmd, cwd=ROOT, env=env, stdout=subprocess.PIPE, text=True) for _ in range(a.procs)]
     outs = [json.loads(p.communicate()[0].strip().splitlines()[-1]) for p in procs]
