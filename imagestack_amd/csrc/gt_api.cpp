@@ -789,6 +789,7 @@ int run(gt_ctx *c, const gt::PixSrc &src, int n, int row_w, int d, int vd, int s
         if (hc.flags & gt::kFlagBad)
             return fail(GT_ERANGE, "position is NaN/Inf or a lattice key leaves the int16 range");
         if (hc.flags & gt::kFlagSort) return fail(GT_EDEVICE, "radix sort look-back timed out");
+        if (hc.flags & gt::kFlagWait) return fail(GT_EDEVICE, "a hash table key word never appeared");
         if (hc.flags & gt::kFlagRange) {
             // Keys outside the cached plan: bound these positions again.
             if (++retries > 6) return fail(GT_EDEVICE, "inputs keep leaving the computed bounds");

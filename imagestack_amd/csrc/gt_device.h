@@ -257,16 +257,61 @@ __device__ __forceinline__ uint64_t hash_key(const uint64_t (&k)[W]) {
 }
 
 // Insert-or-find in the global open-addressing table (linear probing; slot s
-// owns words tkeys[s*W .. s*W+W-1]).  Each word is claimed by its own 64-bit
-// CAS from kEmpty; a thread whose word is already there moves to the next
-// word, and a mismatch on any word sends it to the next slot.  Words are
-// written once, so every thread with a given key agrees on its slot, with no
-// locks or spinning (no intra-wave deadlock).  The CAS that completes the
-// last word of a slot creates the vertex.  Returns the slot, or -1 after
-// kMaxProbes slots (table full).
+// owns words tkeys[s*W .. s*W+W-1]), words written once.  Returns the slot,
+// -1 after kMaxProbes slots (table full), or -2 when a wide key's words
+// never appeared (kFlagWait).
+//
+// One-word keys: a workgroup-scope (cache-friendly) read, then a CAS from
+// kEmpty only where the word looks empty.
+//
+// Wide keys: the slot belongs to whoever claims word 0 by CAS; the claimer
+// then stores words 1..W-1 (device-coherent stores, no atomics).  A thread
+// that finds word 0 equal to its own reads the other words, waiting while
+// one still reads kEmpty -- its claimer has already issued the store: in a
+// wave the claiming lanes store before any lane reaches the wait, and a
+// claimer elsewhere stores right after its CAS returned -- then compares;
+// any mismatch moves on to the next slot.  A new vertex costs one CAS and
+// W-1 stores instead of W dependent read-and-CAS pairs (d = 16: 17 new
+// vertices a pixel).  The wait is bounded: past kWaitSpins reads it reports
+// -2 rather than spin forever.
+constexpr uint32_t kWaitSpins = 1u << 20;
 template <int W>
 __device__ __forceinline__ int table_insert(uint64_t *__restrict__ tkeys, uint32_t mask, const uint64_t (&k)[W],
                                             uint32_t s, uint32_t &probes) {
+    if constexpr (W > 1) {
+        for (uint32_t it = 0; it < kMaxProbes; it++) {
+            probes++;
+            uint64_t *a = tkeys + (size_t)s * W;
+            uint64_t cur = __hip_atomic_load(a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (cur == kEmpty) {
+                uint64_t expect = kEmpty;
+                __hip_atomic_compare_exchange_strong(a, &expect, k[0], __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_AGENT);
+                if (expect == kEmpty) {   // claimed: the other words are this key's
+#pragma unroll
+                    for (int w = 1; w < W; w++)
+                        __hip_atomic_store(a + w, k[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    return (int)s;
+                }
+                cur = expect;
+            }
+            if (cur == k[0]) {
+                bool match = true;
+#pragma unroll
+                for (int w = 1; w < W; w++) {
+                    uint64_t v = __hip_atomic_load(a + w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    for (uint32_t n = 0; v == kEmpty; n++) {
+                        if (n >= kWaitSpins) return -2;
+                        v = __hip_atomic_load(a + w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    }
+                    match = match && v == k[w];
+                }
+                if (match) return (int)s;
+            }
+            s = (s + 1) & mask;
+        }
+        return -1;
+    }
     for (uint32_t it = 0; it < kMaxProbes; it++) {
         probes++;
         bool match = true;
@@ -293,6 +338,9 @@ __device__ __forceinline__ int table_insert(uint64_t *__restrict__ tkeys, uint32
     }
     return -1;
 }
+
+// Flag of a failed table_insert.
+__device__ __forceinline__ unsigned insert_fail_flag(int slot) { return slot == -2 ? kFlagWait : kFlagFull; }
 
 // Tag of a key with 64-bit hash h (TableArgs::tags): never 0 (empty).
 __device__ __forceinline__ uint8_t key_tag8(uint64_t h) { return (uint8_t)(0x80u | (uint32_t)(h >> 57)); }

@@ -28,6 +28,7 @@ constexpr unsigned kFlagBad = 2u;     // NaN/Inf position or key outside int16
 constexpr unsigned kFlagRange = 4u;   // a key outside the packing plan: replan and redo
 constexpr unsigned kFlagSort = 8u;    // a radix-sort look-back gave up waiting (a bug, never expected)
 constexpr unsigned kFlagNonFinite = 16u;   // the values hold NaN/Inf (marks in TableArgs::tnf, not an error)
+constexpr unsigned kFlagWait = 32u;   // a table key word awaited from its writer never came (a bug, never expected)
 
 // Where pixel positions and values come from.
 struct PixSrc {

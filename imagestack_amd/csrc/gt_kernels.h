@@ -274,7 +274,7 @@ __device__ __forceinline__ void mark_nonfinite(const Simplex<D> &s, const KeyPla
         uint64_t k[W];
         if (!K::make(s, r, kp, tag, k)) continue;   // reported by the caller
         const int slot = table_insert<W>(ta.tkeys, ta.mask, k, home_slot<D, PACKED, BATCH>(ta, kp, k), probes);
-        if (slot < 0) { flags |= kFlagFull; continue; }
+        if (slot < 0) { flags |= insert_fail_flag(slot); continue; }
         uint32_t bits = 0;
         for (int c = 0; c < vd; c++) {
             if (!((nfmask >> c) & 1u)) continue;
@@ -412,7 +412,7 @@ __global__ __launch_bounds__(kBlock) void k_splat_tile(PixSrc src0, int n0, int 
                     const int slot = table_insert<W>(ta.tkeys, ta.mask, k, home_slot<D, PACKED, BATCH>(ta, kp, k), probes);
                     inserts++;
                     if (rp.rec) rp.gs[r * rp.stride + rq] = (uint32_t)slot;
-                    if (slot < 0) { flags |= kFlagFull; continue; }
+                    if (slot < 0) { flags |= insert_fail_flag(slot); continue; }
                     unsigned long long *dst = ta.tacc + (size_t)slot * vd;
 #pragma unroll
                     for (int c = 0; c < LVD; c++)
@@ -450,7 +450,7 @@ __global__ __launch_bounds__(kBlock) void k_splat_tile(PixSrc src0, int n0, int 
         for (int w = 0; w < W; w++) k[w] = lk[ls * W + w];
         const int slot = table_insert<W>(ta.tkeys, ta.mask, k, home_slot<D, PACKED, BATCH>(ta, kp, k), probes);
         inserts++;
-        if (slot < 0) flags |= kFlagFull;
+        if (slot < 0) flags |= insert_fail_flag(slot);
         lk[ls * W] = (uint64_t)(int64_t)slot;
         if (rp.rec) rp.tent[(size_t)tile * LT + j] = make_uint2((unsigned)ls, (unsigned)slot);
     }
@@ -521,7 +521,7 @@ __global__ __launch_bounds__(kBlock) void k_splat_direct(PixSrc src, int n, int 
                 }
                 const int slot = table_insert<W>(ta.tkeys, ta.mask, k, home_slot<D, PACKED, false>(ta, kp, k), probes);
                 inserts++;
-                if (slot < 0) { flags |= kFlagFull; continue; }
+                if (slot < 0) { flags |= insert_fail_flag(slot); continue; }
                 unsigned long long *dst = ta.tacc + (size_t)slot * vd;
 #pragma unroll
                 for (int c = 0; c < kMaxVD; c++)
