@@ -576,15 +576,14 @@ __device__ __forceinline__ void publish_max(unsigned *track, const float (&m)[kM
 }
 
 // The fixed-point word of a product already in fixed-point units, y = (w v) s
-// with |y| < 2^62, computed as w * (v s) (s a power of two, so the product
-// rounds once, as (w v) s would): hi = floor(y / 2^32) and lo = y - hi 2^32
-// in FP32.  Both are exact for |y| >= 2^31 (y then has no bits below 2^8);
-// below that the word lies within 2^8 units of y.  Deterministic either way,
-// so the integer sums still do not depend on the order of the atomics.
+// with |y| < 2^51, computed as w * (v s) (s a power of two, so the product
+// rounds once, as (w v) s would): rint(y) as a two's-complement word.  In
+// FP64, 1.5 * 2^52 + y lies in [2^52, 2^53), where the spacing is 1, so the
+// sum rounds y to the nearest integer and its bit pattern is that of
+// 1.5 * 2^52 plus rint(y): one conversion, one add, one 64-bit subtract.
 __device__ __forceinline__ unsigned long long fixed_word(float y) {
-    const float hi = floorf(y * 2.3283064365386963e-10f);   // 2^-32
-    const float lo = fmaf(hi, -4294967296.0f, y);
-    return ((unsigned long long)(uint32_t)(int)hi << 32) | (uint32_t)lo;
+    const double t = (double)y + 6755399441055744.0;   // 1.5 * 2^52
+    return (unsigned long long)__double_as_longlong(t) - 0x4338000000000000ull;
 }
 
 __device__ __forceinline__ uint64_t lanemask_lt() {

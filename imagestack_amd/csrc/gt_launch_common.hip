@@ -242,9 +242,10 @@ __global__ __launch_bounds__(kBlock) void k_value_bounds(const float *__restrict
 // This call's fixed-point scales from the per-workgroup bounds of the nval
 // value planes (a workgroup that saw no finite value holds min > max); a
 // homogeneous channel (homog) has magnitude 1.  For n contributions of
-// magnitude <= vmax the scale is 2^p with p = 62 - ceil(log2(n vmax 1.001)),
-// clamped to [-120, 120]: every sum stays below 2^62.  ceil(log2) is exact
-// through frexp.  One workgroup, a wave per channel.
+// magnitude <= vmax the scale is 2^p with p = min(62 - ceil(log2(n vmax
+// 1.001)), 50 - ceil(log2(vmax 1.001))), clamped to [-120, 120]: every sum
+// stays below 2^62 and every product below 2^50 (fixed_word's range).
+// ceil(log2) is exact through frexp.  One workgroup, a wave per channel.
 __global__ __launch_bounds__(kBlock) void k_value_scale(const float *__restrict__ partial, int nblk, int nval,
                                                         int homog, int n, int vd, ValScale *__restrict__ out,
                                                         Counters *ctr) {
@@ -273,11 +274,16 @@ __global__ __launch_bounds__(kBlock) void k_value_scale(const float *__restrict_
         if (lane == 0) {
             float s = 0.0f, inv = 0.0f;
             if (c < vd) {
+                // every sum below 2^62 (n contributions of weight <= 1), every
+                // single product below 2^50 (the splat's FP64 rounding of a
+                // product to an integer is exact below 2^51)
                 const double bound = fmax(1e-30, (double)n * (double)vmax * 1.001);
                 int e = 0;
-                const double f = frexp(bound, &e);   // bound = f 2^e, f in [0.5, 1)
+                double f = frexp(bound, &e);   // bound = f 2^e, f in [0.5, 1)
                 const int lg = (f == 0.5) ? e - 1 : e;   // ceil(log2(bound))
-                const int p = min(120, max(-120, 62 - lg));
+                f = frexp(fmax(1e-30, (double)vmax * 1.001), &e);
+                const int lg1 = (f == 0.5) ? e - 1 : e;
+                const int p = min(120, max(-120, min(62 - lg, 50 - lg1)));
                 s = ldexpf(1.0f, p);
                 inv = ldexpf(1.0f, -p);
             }
