@@ -66,6 +66,9 @@ uint32_t next_pow2(uint64_t v) {
 constexpr int kLookbackTiles = 256;
 
 int value_stride(int vd) { return vd <= 2 ? 2 : vd <= 4 ? 4 : vd <= 8 ? 8 : 16; }
+
+// Compaction chunks of a table of cap slots.
+unsigned chunks_of(uint32_t cap) { return (cap + gt::kCompactSlots - 1) / gt::kCompactSlots; }
 size_t store_bytes(int storage) { return storage == GT_STORE_F32 ? 4 : 2; }
 
 } // namespace
@@ -345,14 +348,15 @@ int enqueue(gt_ctx *c, const Launch &L, const gt::PixSrc &src, const SliceSpec &
     hipStream_t s = c->stream;
     gt::Counters *ctr = (gt::Counters *)c->ctr.p;
     const gt::TableArgs &ta = L.ta;
-    GT_HIP(gt::launch_clear(ta.tkeys, (size_t)L.cap * L.W, ta.tacc, L.clear_sums ? (size_t)L.cap * L.vd : 0, ta.tnf,
-                            L.clear_marks ? (size_t)L.cap : 0, ctr, s));
-    // this call's value bounds -> fixed-point scales, on the device
+    // this call's value bounds, then the table cleared and the bounds turned
+    // into fixed-point scales, on the device
+    const gt::ClearArgs cl{ta.tkeys, (size_t)L.cap * L.W, ta.tacc, L.clear_sums ? (size_t)L.cap * L.vd : 0, ta.tnf,
+                           L.clear_marks ? (size_t)L.cap : 0};
     if (batch)
-        GT_HIP(gt::launch_value_scale_batch(batch->ba, batch->ntiles, L.tile_w, batch->C, L.n, L.vd, L.vs_part,
+        GT_HIP(gt::launch_value_scale_batch(cl, batch->ba, batch->ntiles, L.tile_w, batch->C, L.n, L.vd, L.vs_part,
                                             L.vs_nblk, L.vsc, ctr, s));
     else
-        GT_HIP(gt::launch_value_scale(src.val, src.val_cs, src.nval, src.homog, L.n, L.vd, L.vs_part, L.vs_nblk,
+        GT_HIP(gt::launch_value_scale(cl, src.val, src.val_cs, src.nval, src.homog, L.n, L.vd, L.vs_part, L.vs_nblk,
                                       L.vsc, ctr, s));
     if (c->profiling) GT_HIP(mark(c, 0, s));
     if (batch)
@@ -364,7 +368,8 @@ int enqueue(gt_ctx *c, const Launch &L, const gt::PixSrc &src, const SliceSpec &
     if (c->profiling) GT_HIP(mark(c, 1, s));
     *passes = 0;
     if (L.hashnbr) {   // ids in compaction order: the finalize rides along with the compaction
-        GT_HIP(gt::launch_compact_final(L.W, L.storage, ta, L.cap, (uint32_t *)c->counts.p, L.vsc, L.vd, L.vs,
+        GT_HIP(gt::launch_compact_final(L.W, L.storage, ta, L.cap, (uint32_t *)c->counts.p,
+                                        (uint32_t *)c->counts.p + chunks_of(L.cap), L.vsc, L.vd, L.vs,
                                         (uint64_t *)c->vkeys.p, c->vals_a.p, ctr, L.m_max, s));
     } else {
         GT_HIP(gt::launch_compact(L.W, ta, L.cap, (uint32_t *)c->counts.p, (uint32_t *)c->slot_list.p,
@@ -401,11 +406,12 @@ int enqueue(gt_ctx *c, const Launch &L, const gt::PixSrc &src, const SliceSpec &
     const unsigned *sbits = L.storage ? &L.vsc->vmax_bits[L.d][0] : nullptr;   // the last pass's scale
     if (L.rp.rec && batch)
         GT_HIP(gt::launch_slice_replay(L.d, true, L.storage, batch->tmpl, 0, 0, L.tile_w, batch->ba, batch->ntiles,
-                                       L.vd, L.lc, L.kp, ta, L.rp, cur, sbits, batch->tmpl_dst, ctr, L.wpc, s));
+                                       L.vd, L.lc, L.kp, ta, L.rp, cur, sbits, batch->tmpl_dst, ctr, c->h_ctr,
+                                       L.wpc, s));
     else if (L.rp.rec)
         GT_HIP(gt::launch_slice_replay(L.d, sl.src.ref != nullptr, L.storage, sl.src, sl.n, L.row_w, L.tile_w,
                                        gt::BatchArgs{nullptr, 0}, 0, L.vd, L.lc, L.kp, ta, L.rp, cur, sbits, sl.dst,
-                                       ctr, L.wpc, s));
+                                       ctr, c->h_ctr, L.wpc, s));
     else if (batch)
         GT_HIP(gt::launch_slice_batch(L.d, L.storage, batch->tmpl, batch->ba, batch->ntiles, L.tile_w, L.vd, L.lc,
                                       L.kp, ta, cur, sbits, batch->tmpl_dst, ctr, L.count, L.wpc, s));
@@ -413,7 +419,8 @@ int enqueue(gt_ctx *c, const Launch &L, const gt::PixSrc &src, const SliceSpec &
         GT_HIP(gt::launch_slice(L.d, L.packed, sl.src.ref != nullptr, L.dedup, L.storage, sl.src, sl.n, L.row_w,
                                 L.tile_w, L.vd, L.vs, L.lc, L.kp, ta, cur, sbits, sl.dst, ctr, L.count, L.wpc, s));
     if (c->profiling) GT_HIP(mark(c, 5, s));
-    GT_HIP(hipMemcpyAsync(c->h_ctr, ctr, sizeof(gt::Counters), hipMemcpyDeviceToHost, s));
+    if (!L.rp.rec)   // (the replay slice writes them itself)
+        GT_HIP(hipMemcpyAsync(c->h_ctr, ctr, sizeof(gt::Counters), hipMemcpyDeviceToHost, s));
     *vals_out = cur;
     return GT_OK;
 }
@@ -710,7 +717,7 @@ int run(gt_ctx *c, const gt::PixSrc &src, int n, int row_w, int d, int vd, int s
         L.vsc = (gt::ValScale *)c->vscale.p;
         L.vs_part = (float *)c->vs_part.p;
         GT_TRY(ensure(c->ctr, sizeof(gt::Counters)));
-        GT_TRY(ensure(c->counts, (size_t)ncount * 4));
+        GT_TRY(ensure(c->counts, (size_t)ncount * 8));   // counts, then offsets (a self-scanning finalize)
         GT_TRY(ensure(c->slot_id, (size_t)cap * 4));
         if (!L.hashnbr) {   // the slot list and the radix sort's buffers
             GT_TRY(ensure(c->slot_list, (size_t)L.m_max * 4));
@@ -776,7 +783,8 @@ int run(gt_ctx *c, const gt::PixSrc &src, int n, int row_w, int d, int vd, int s
             L.ta.keymask = fold ? (1ull << L.kp.bits) - 1 : ~0ull;
         }
         if (L.hashnbr && L.ta.part_mul)
-            L.pm = {(const uint32_t *)c->counts.p, (int)(cap / 8 / gt::kCompactSlots)};
+            L.pm = {(const uint32_t *)c->counts.p + (ncount <= (unsigned)gt::kSelfScanChunks ? ncount : 0u),
+                    (int)(cap / 8 / gt::kCompactSlots)};
 
         void *cur = nullptr;
         GT_TRY(execute(c, L, src, sl, batch, tu.graph && !keep_splat && c->profiling == 0, &passes, &cur));
@@ -1751,7 +1759,7 @@ constexpr size_t kNaKeys = na_round((size_t)kNaCap * kNaKeyWords * 8);
 constexpr size_t kNaAcc = na_round((size_t)kNaCap * (kNaMaxC + 1) * 8);
 constexpr size_t kNaMarks = na_round((size_t)kNaCap * 4);
 constexpr size_t kNaSlotId = na_round((size_t)kNaCap * 4);
-constexpr size_t kNaCounts = na_round((size_t)kNaCap / gt::kCompactSlots * 4);
+constexpr size_t kNaCounts = na_round((size_t)kNaCap / gt::kCompactSlots * 8);   // counts, then offsets
 constexpr size_t kNaVkeys = na_round((size_t)kNaVerts * kNaKeyWords * 8);
 constexpr size_t kNaVals = na_round((size_t)kNaVerts * 4 * 4);   // value stride 4, FP32 rows
 constexpr size_t kNaNbr = na_round((size_t)(kNaMaxD + 1) * kNaVerts * 8);

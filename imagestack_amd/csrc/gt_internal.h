@@ -175,7 +175,7 @@ struct PartMap {
 // finite magnitude in this call, weights <= 1, at most n contributions).
 // Integer sums are exact and independent of the order of the atomics;
 // finalize converts them back with inv_s[c].  The lattice pipeline computes
-// it on the device at the start of every call (k_value_scale), so the scales
+// it on the device at the start of every call (k_clear_scale), so the scales
 // depend on the call's own values only.  nonfinite: some value is NaN/Inf;
 // those contributions are kept as marks (TableArgs::tnf), not sums.
 //
@@ -205,20 +205,28 @@ hipError_t launch_minmax(const float *pa, int64_t csa, int na, const float *pb, 
                          float *partial, int nblk, float *out, hipStream_t st);
 hipError_t launch_bounds_batch(const BatchArgs &ba, int ntiles, int tile_w, int Cr, int C, float *partial, int nblk,
                                float *out, hipStream_t st);
-// Largest finite magnitude and a non-finite indicator of each of the nval
-// value planes (n pixels each), then this call's fixed-point scales for vd
-// channels (a homogeneous channel of ones after the planes when homog) into
-// the device ValScale; kFlagNonFinite is raised in ctr when a value is NaN or
-// Inf.  partial holds nblk * nval * 3 floats.
-hipError_t launch_value_scale(const float *val, int64_t cs, int nval, int homog, int n, int vd, float *partial,
-                              int nblk, ValScale *out, Counters *ctr, hipStream_t st);
+// What a lattice call clears before its splat: nwords table words to
+// kEmpty, nacc sums and nmarks non-finite marks to zero.
+struct ClearArgs {
+    uint64_t *tkeys;
+    size_t nwords;
+    unsigned long long *tacc;
+    size_t nacc;
+    uint32_t *tnf;
+    size_t nmarks;
+};
+// The start of a lattice call in two launches: the largest finite magnitude
+// and a non-finite indicator of each of the nval value planes (n pixels
+// each); then the table cleared (ClearArgs), the counters zeroed and this
+// call's fixed-point scales for vd channels (a homogeneous channel of ones
+// after the planes when homog) written to the device ValScale, kFlagNonFinite
+// raised in ctr when a value is NaN or Inf.  partial holds nblk * nval * 3
+// floats.
+hipError_t launch_value_scale(const ClearArgs &cl, const float *val, int64_t cs, int nval, int homog, int n, int vd,
+                              float *partial, int nblk, ValScale *out, Counters *ctr, hipStream_t st);
 // The same over the image planes of a batch (n: the batch's pixel count).
-hipError_t launch_value_scale_batch(const BatchArgs &ba, int ntiles, int tile_w, int C, int n, int vd,
-                                    float *partial, int nblk, ValScale *out, Counters *ctr, hipStream_t st);
-// Table words to kEmpty, nacc sums and nmarks non-finite marks to zero, the
-// counters to zero.
-hipError_t launch_clear(uint64_t *tkeys, size_t nwords, unsigned long long *tacc, size_t nacc, uint32_t *tnf,
-                        size_t nmarks, Counters *ctr, hipStream_t st);
+hipError_t launch_value_scale_batch(const ClearArgs &cl, const BatchArgs &ba, int ntiles, int tile_w, int C, int n,
+                                    int vd, float *partial, int nblk, ValScale *out, Counters *ctr, hipStream_t st);
 // rp: the tiled splat's replay record (rp.rec == nullptr: none; the direct
 // splat writes none).
 hipError_t launch_splat(int d, bool packed, bool bilat, bool dedup, const PixSrc &src, int n, int row_w, int tile_w,
@@ -234,6 +242,19 @@ hipError_t launch_slice_batch(int d, int storage, const PixSrc &tmpl, const Batc
                               int vd, const LatConst &lc, const KeyPlan &kp, const TableArgs &ta, const void *vals,
                               const unsigned *sbits, const OutDst &tmpl_dst, Counters *ctr, bool count, int wpc,
                               hipStream_t st);
+// A finalize that scans the compaction's chunk counts itself instead of a
+// separate one-workgroup scan (tables of at most kSelfScanChunks chunks):
+// each workgroup sums the counts of the chunks before its own; offs receives
+// those offsets (the PartMap then reads them), wctr the vertex count M (0
+// after a full table or over `limit`) and the filled count.  offs == nullptr:
+// the counts were scanned already.
+struct SelfScan {
+    uint32_t *offs;
+    Counters *wctr;
+    int nchunks;
+    unsigned limit;
+};
+constexpr int kSelfScanChunks = 2048;
 // limit: the per-vertex buffers' size; a table holding more vertices counts as
 // full (kFlagFull, M = 0).
 hipError_t launch_compact(int words, const TableArgs &ta, uint32_t capacity, uint32_t *counts, uint32_t *slot_list,
@@ -249,9 +270,12 @@ hipError_t launch_neighbors(int d, bool packed, const KeyPlan &kp, const uint64_
                             int m_max, int2 *nbr, int wpc, hipStream_t st);
 // Compaction with the finalize fused in, for ids in compaction order (no
 // sort, no slot list): count, scan, then k_write_final.
+// offs: where the chunk offsets go when the table has at most
+// kSelfScanChunks chunks (the finalize scans the counts itself); else they
+// replace the counts in place.
 hipError_t launch_compact_final(int words, int storage, const TableArgs &ta, uint32_t capacity, uint32_t *counts,
-                                const ValScale *vsc, int vd, int vs, uint64_t *vkeys, void *vals, Counters *ctr,
-                                int limit, hipStream_t st);
+                                uint32_t *offs, const ValScale *vsc, int vd, int vs, uint64_t *vkeys, void *vals,
+                                Counters *ctr, int limit, hipStream_t st);
 // The same table by hash lookups in the complete global table, for vertex ids
 // in compaction order (no sort).
 // With pm.cpp > 0, workgroups on XCD x take the vertices of table part x.
@@ -270,10 +294,12 @@ hipError_t launch_blur(int storage, int vs, int pass, const ValScale *vsc, const
 // Slice from the tiled splat's replay record (the splat's own positions,
 // d <= 10, vd <= 4; batch: ba.count > 0, over ntiles tiles); the same tile
 // decomposition as that splat (n, row_w, tile_w or the batch's tiles).
+// hctr: host memory (pinned, device-visible) that receives the counters, or
+// null.
 hipError_t launch_slice_replay(int d, bool bilat, int storage, const PixSrc &src, int n, int row_w, int tile_w,
                                const BatchArgs &ba, int ntiles, int vd, const LatConst &lc, const KeyPlan &kp,
                                const TableArgs &ta, const Replay &rp, const void *vals, const unsigned *sbits,
-                               const OutDst &dst, Counters *ctr, int wpc, hipStream_t st);
+                               const OutDst &dst, Counters *ctr, Counters *hctr, int wpc, hipStream_t st);
 hipError_t launch_slice(int d, bool packed, bool bilat, bool tiled, int storage, const PixSrc &src, int n, int row_w,
                         int tile_w, int vd, int vs, const LatConst &lc, const KeyPlan &kp, const TableArgs &ta,
                         const void *vals, const unsigned *sbits, const OutDst &dst, Counters *ctr, bool count,

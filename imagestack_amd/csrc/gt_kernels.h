@@ -623,16 +623,51 @@ __global__ __launch_bounds__(kBlock) void k_write_final(TableArgs ta, uint32_t c
                                                         const uint32_t *__restrict__ offsets, const Counters *ctr,
                                                         const ValScale *__restrict__ vsc, int vd, int vs,
                                                         uint64_t *__restrict__ vkeys, T *__restrict__ vals,
-                                                        unsigned *__restrict__ track) {
+                                                        unsigned *__restrict__ track, SelfScan sc) {
     constexpr int NW = kBlock / kWave;
     __shared__ unsigned ws[NW];
-    if (*(volatile const unsigned *)&ctr->flags & kFlagFull) return;   // the same for every thread
+    const int wave = threadIdx.x / kWave;
+    unsigned run;
+    if (sc.offs) {
+        // offsets holds the chunks' counts: this chunk's offset and the total
+        __shared__ unsigned spre[NW], stot[NW];
+        unsigned pre = 0, tot = 0;
+        for (int i = threadIdx.x; i < sc.nchunks; i += kBlock) {
+            const unsigned v = offsets[i];
+            tot += v;
+            pre += i < (int)blockIdx.x ? v : 0u;
+        }
+        pre = wave_sum(pre);
+        tot = wave_sum(tot);
+        if (__lane_id() == 0) {
+            spre[wave] = pre;
+            stot[wave] = tot;
+        }
+        __syncthreads();
+        pre = tot = 0;
+#pragma unroll
+        for (int w = 0; w < NW; w++) {
+            pre += spre[w];
+            tot += stot[w];
+        }
+        const bool over = tot > sc.limit;
+        const bool full = over || (*(volatile const unsigned *)&ctr->flags & kFlagFull);   // the same everywhere
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            if (over) atomicOr(&sc.wctr->flags, kFlagFull);
+            sc.wctr->M = full ? 0u : tot;
+            sc.wctr->filled = tot;
+        }
+        if (threadIdx.x == 0) sc.offs[blockIdx.x] = pre;
+        if (full) return;
+        run = pre;
+    } else {
+        if (*(volatile const unsigned *)&ctr->flags & kFlagFull) return;   // the same for every thread
+        run = offsets[blockIdx.x];
+    }
     float mx[kMaxVD];   // track: largest finite magnitude per channel (a narrow storage's first scale)
 #pragma unroll
     for (int c = 0; c < kMaxVD; c++) mx[c] = 0.0f;
     const size_t base = (size_t)blockIdx.x * kCompactSlots;
-    unsigned run = offsets[blockIdx.x];
-    const int wave = threadIdx.x / kWave;
     for (int k = 0; k < kCompactSlots / kBlock; k++) {
         const size_t s = base + (size_t)k * kBlock + threadIdx.x;
         uint64_t w0 = kEmpty;
@@ -1420,10 +1455,20 @@ __global__ __launch_bounds__(kBlock) void k_slice_replay(PixSrc src0, int n0, in
                                                          int vd, LatConst lc, KeyPlan kp, TableArgs ta, Replay rp,
                                                          const T *__restrict__ vals,
                                                          const unsigned *__restrict__ sbits, OutDst dst0,
-                                                         Counters *ctr) {
+                                                         Counters *ctr, Counters *hctr) {
     constexpr bool kScaled = !std::is_same<T, float>::value;
     constexpr int RW = replay_dwords(D);
     __shared__ float lv[LT * LVD];
+    // The counters are final here (this kernel changes none): hand them to
+    // the host's pinned copy, which spares the call a copy launch.
+    if (hctr && blockIdx.x == 0 && threadIdx.x == 0) {
+        hctr->M = ctr->M;
+        hctr->flags = ctr->flags;
+        hctr->filled = ctr->filled;
+        hctr->probes = ctr->probes;
+        hctr->inserts = ctr->inserts;
+        hctr->atomics = ctr->atomics;
+    }
     if (*(volatile const unsigned *)&ctr->flags & kFlagFull) return;   // redone with a larger table
     const unsigned M = ctr->M;
     const int tile = BATCH ? (int)blockIdx.x : xcd_tile(blockIdx.x, gridDim.x);

@@ -154,26 +154,6 @@ hipError_t launch_bounds_batch(const BatchArgs &ba, int ntiles, int tile_w, int 
     return hipGetLastError();
 }
 
-// ---------------------------------------------------------------------------
-// clear: table words to kEmpty, sums, non-finite marks and counters to zero
-// (a fill kernel rather than hipMemsetAsync, so the sequence can be
-// graph-captured).
-__global__ __launch_bounds__(kBlock) void k_clear(uint64_t *__restrict__ tkeys, size_t nwords,
-                                                  unsigned long long *__restrict__ tacc, size_t nacc,
-                                                  uint32_t *__restrict__ tnf, size_t nmarks, Counters *ctr) {
-    const size_t stride = (size_t)gridDim.x * kBlock;
-    const size_t t0 = (size_t)blockIdx.x * kBlock + threadIdx.x;
-    for (size_t i = t0; i < nwords; i += stride) tkeys[i] = kEmpty;
-    for (size_t i = t0; i < nacc; i += stride) tacc[i] = 0ull;
-    for (size_t i = t0; i < nmarks; i += stride) tnf[i] = 0u;
-    if (t0 == 0) {
-        ctr->M = 0;
-        ctr->flags = 0;
-        ctr->probes = 0;
-        ctr->inserts = 0;
-        ctr->atomics = 0;
-    }
-}
 
 // ---------------------------------------------------------------------------
 // Value bounds of this call, for the fixed-point scales.  Per workgroup and
@@ -246,9 +226,8 @@ __global__ __launch_bounds__(kBlock) void k_value_bounds(const float *__restrict
 // 1.001)), 50 - ceil(log2(vmax 1.001))), clamped to [-120, 120]: every sum
 // stays below 2^62 and every product below 2^50 (fixed_word's range).
 // ceil(log2) is exact through frexp.  One workgroup, a wave per channel.
-__global__ __launch_bounds__(kBlock) void k_value_scale(const float *__restrict__ partial, int nblk, int nval,
-                                                        int homog, int n, int vd, ValScale *__restrict__ out,
-                                                        Counters *ctr) {
+__device__ void value_scale_wg(const float *__restrict__ partial, int nblk, int nval, int homog, int n, int vd,
+                               ValScale *__restrict__ out, Counters *ctr) {
     __shared__ unsigned s_nf;
     if (threadIdx.x == 0) s_nf = 0;
     // a narrow storage's per-pass magnitudes start from zero in every call
@@ -297,6 +276,30 @@ __global__ __launch_bounds__(kBlock) void k_value_scale(const float *__restrict_
         out->nonfinite = s_nf;
         if (s_nf) atomicOr(&ctr->flags, kFlagNonFinite);
     }
+}
+
+// The table cleared (grid-stride), the counters zeroed, and this call's
+// fixed-point scales computed by workgroup 0 from the value bounds' partials
+// (the kernel before).  One launch for what needs no grid-wide order.
+__global__ __launch_bounds__(kBlock) void k_clear_scale(ClearArgs cl, Counters *ctr, const float *__restrict__ partial,
+                                                        int nblk, int nval, int homog, int n, int vd,
+                                                        ValScale *__restrict__ out) {
+    const size_t stride = (size_t)gridDim.x * kBlock;
+    const size_t t0 = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    for (size_t i = t0; i < cl.nwords; i += stride) cl.tkeys[i] = kEmpty;
+    for (size_t i = t0; i < cl.nacc; i += stride) cl.tacc[i] = 0ull;
+    for (size_t i = t0; i < cl.nmarks; i += stride) cl.tnf[i] = 0u;
+    if (blockIdx.x != 0) return;
+    if (threadIdx.x == 0) {
+        ctr->M = 0;
+        ctr->flags = 0;
+        ctr->filled = 0;
+        ctr->probes = 0;
+        ctr->inserts = 0;
+        ctr->atomics = 0;
+    }
+    __syncthreads();
+    value_scale_wg(partial, nblk, nval, homog, n, vd, out, ctr);
 }
 
 // In-place exclusive scan of n counts in one workgroup; the total goes to
@@ -373,34 +376,33 @@ hipError_t launch_minmax(const float *pa, int64_t csa, int na, const float *pb, 
     return hipGetLastError();
 }
 
-hipError_t launch_clear(uint64_t *tkeys, size_t nwords, unsigned long long *tacc, size_t nacc, uint32_t *tnf,
-                        size_t nmarks, Counters *ctr, hipStream_t st) {
-    size_t m = nwords > nacc ? nwords : nacc;
-    if (nmarks > m) m = nmarks;
+static unsigned clear_blocks(const ClearArgs &cl) {
+    size_t m = cl.nwords > cl.nacc ? cl.nwords : cl.nacc;
+    if (cl.nmarks > m) m = cl.nmarks;
     unsigned g = blocks_for((int64_t)m, kBlock);
     if (g > 8192) g = 8192;
-    if (g == 0) g = 1;
-    hipLaunchKernelGGL(k_clear, dim3(g), dim3(kBlock), 0, st, tkeys, nwords, tacc, nacc, tnf, nmarks, ctr);
-    return hipGetLastError();
+    return g == 0 ? 1u : g;
 }
 
-hipError_t launch_value_scale(const float *val, int64_t cs, int nval, int homog, int n, int vd, float *partial,
-                              int nblk, ValScale *out, Counters *ctr, hipStream_t st) {
+hipError_t launch_value_scale(const ClearArgs &cl, const float *val, int64_t cs, int nval, int homog, int n, int vd,
+                              float *partial, int nblk, ValScale *out, Counters *ctr, hipStream_t st) {
     if (nval > 0 && n > 0)
         hipLaunchKernelGGL(k_value_bounds, dim3(nblk), dim3(kBlock), 0, st, val, cs, nval, n, partial);
     else
         nval = 0;
-    hipLaunchKernelGGL(k_value_scale, dim3(1), dim3(kBlock), 0, st, partial, nblk, nval, homog, n, vd, out, ctr);
+    hipLaunchKernelGGL(k_clear_scale, dim3(clear_blocks(cl)), dim3(kBlock), 0, st, cl, ctr, partial, nblk, nval,
+                       homog, n, vd, out);
     return hipGetLastError();
 }
 
-hipError_t launch_value_scale_batch(const BatchArgs &ba, int ntiles, int tile_w, int C, int n, int vd,
-                                    float *partial, int nblk, ValScale *out, Counters *ctr, hipStream_t st) {
+hipError_t launch_value_scale_batch(const ClearArgs &cl, const BatchArgs &ba, int ntiles, int tile_w, int C, int n,
+                                    int vd, float *partial, int nblk, ValScale *out, Counters *ctr, hipStream_t st) {
     if (C > 0 && ntiles > 0)
         hipLaunchKernelGGL(k_bounds_batch, dim3(nblk), dim3(kBlock), 0, st, ba, ntiles, tile_w, 0, C, partial);
     else
         C = 0;
-    hipLaunchKernelGGL(k_value_scale, dim3(1), dim3(kBlock), 0, st, partial, nblk, C, 1, n, vd, out, ctr);
+    hipLaunchKernelGGL(k_clear_scale, dim3(clear_blocks(cl)), dim3(kBlock), 0, st, cl, ctr, partial, nblk, C, 1, n,
+                       vd, out);
     return hipGetLastError();
 }
 
@@ -465,11 +467,11 @@ hipError_t launch_finalize(int words, int storage, const TableArgs &ta, const Va
 template <typename T>
 static void launch_write_final_t(int words, const TableArgs &ta, uint32_t capacity, unsigned nblk,
                                  const uint32_t *offsets, const Counters *ctr, const ValScale *vsc, int vd, int vs,
-                                 uint64_t *vkeys, void *vals, unsigned *track, hipStream_t st) {
+                                 uint64_t *vkeys, void *vals, unsigned *track, const SelfScan &sc, hipStream_t st) {
     T *v = (T *)vals;
 #define GT_WF(W)                                                                                                 \
     hipLaunchKernelGGL((k_write_final<W, T>), dim3(nblk), dim3(kBlock), 0, st, ta, capacity, offsets, ctr, vsc,  \
-                       vd, vs, vkeys, v, track)
+                       vd, vs, vkeys, v, track, sc)
     switch (words) {
     case 1: GT_WF(1); break;
     case 2: GT_WF(2); break;
@@ -480,8 +482,8 @@ static void launch_write_final_t(int words, const TableArgs &ta, uint32_t capaci
 }
 
 hipError_t launch_compact_final(int words, int storage, const TableArgs &ta, uint32_t capacity, uint32_t *counts,
-                                const ValScale *vsc, int vd, int vs, uint64_t *vkeys, void *vals, Counters *ctr,
-                                int limit, hipStream_t st) {
+                                uint32_t *offs, const ValScale *vsc, int vd, int vs, uint64_t *vkeys, void *vals,
+                                Counters *ctr, int limit, hipStream_t st) {
     if (words < 1 || words > 4) return hipErrorInvalidValue;
     const unsigned nblk = blocks_for(capacity, kCompactSlots);
     switch (words) {
@@ -490,11 +492,14 @@ hipError_t launch_compact_final(int words, int storage, const TableArgs &ta, uin
     case 3: hipLaunchKernelGGL(k_count<3>, dim3(nblk), dim3(kBlock), 0, st, ta.tkeys, capacity, counts); break;
     default: hipLaunchKernelGGL(k_count<4>, dim3(nblk), dim3(kBlock), 0, st, ta.tkeys, capacity, counts); break;
     }
-    hipLaunchKernelGGL(k_scan_1wg, dim3(1), dim3(1024), 0, st, counts, (int)nblk, &ctr->M, &ctr->flags,
-                       &ctr->filled, (unsigned)limit);
+    SelfScan sc{nullptr, ctr, (int)nblk, (unsigned)limit};
+    if (nblk <= (unsigned)kSelfScanChunks) sc.offs = offs;
+    else
+        hipLaunchKernelGGL(k_scan_1wg, dim3(1), dim3(1024), 0, st, counts, (int)nblk, &ctr->M, &ctr->flags,
+                           &ctr->filled, (unsigned)limit);
     // FP32 rows for every storage (see launch_finalize)
     unsigned *track = storage == 0 ? nullptr : const_cast<unsigned *>(&vsc->vmax_bits[0][0]);
-    launch_write_final_t<float>(words, ta, capacity, nblk, counts, ctr, vsc, vd, vs, vkeys, vals, track, st);
+    launch_write_final_t<float>(words, ta, capacity, nblk, counts, ctr, vsc, vd, vs, vkeys, vals, track, sc, st);
     return hipGetLastError();
 }
 

@@ -9,7 +9,7 @@ namespace gt {
 hipError_t launch_slice_replay(int d, bool bilat, int storage, const PixSrc &src, int n, int row_w, int tile_w,
                                const BatchArgs &ba, int ntiles, int vd, const LatConst &lc, const KeyPlan &kp,
                                const TableArgs &ta, const Replay &rp, const void *vals, const unsigned *sbits,
-                               const OutDst &dst, Counters *ctr, int wpc, hipStream_t st) {
+                               const OutDst &dst, Counters *ctr, Counters *hctr, int wpc, hipStream_t st) {
     const bool batch = ba.count > 0;
     if (vd > 4 || d < 1 || d > 10 || !rp.rec) return hipErrorInvalidValue;
     int64_t tiles = ntiles;
@@ -19,7 +19,7 @@ hipError_t launch_slice_replay(int d, bool bilat, int storage, const PixSrc &src
 #define GT_RP(D, P, T, LVD, B)                                                                               \
     hipLaunchKernelGGL((k_slice_replay<D, P, T, LVD, kTileKeys, B>), dim3((unsigned)tiles), dim3(kBlock),     \
                        occupancy_pad((size_t)kTileKeys * LVD * 4, wpc), st, src, n, row_w, tile_w, ba, vd, lc, \
-                       kp, ta, rp, (const T *)vals, sbits, dst, ctr)
+                       kp, ta, rp, (const T *)vals, sbits, dst, ctr, hctr)
 #define GT_RP_B(D, T, LVD)                          \
     {                                               \
         if (batch) GT_RP(D, true, T, LVD, true);    \
