@@ -323,6 +323,7 @@ void sig_add(std::string &k, const T &v) { k.append((const char *)&v, sizeof(T))
 // blur, and (optionally) slice, enqueued on the context's stream.
 struct Launch {
     int d, vd, storage, W, pw, m_max, vs, n, row_w, tile_w, wpc;
+    int m_grid;   // vertices the hash-neighbour and blur grids are sized for (those kernels loop past it)
     bool packed, bilat, dedup, count, hashnbr;
     bool clear_sums;   // clear the fixed-point sums too (not known to be zero)
     bool clear_marks;  // clear the non-finite marks too (a call that stopped early may have left some)
@@ -390,15 +391,15 @@ int enqueue(gt_ctx *c, const Launch &L, const gt::PixSrc &src, const SliceSpec &
     if (c->profiling) GT_HIP(mark(c, 2, s));
     int2 *nbr = (int2 *)c->nbr.p;
     if (L.hashnbr)
-        GT_HIP(gt::launch_neighbors_hash(L.d, L.packed, L.kp, (const uint64_t *)c->vkeys.p, ctr, L.m_max, ta, L.pm,
-                                         L.nbr_per_vertex, nbr, s));
+        GT_HIP(gt::launch_neighbors_hash(L.d, L.packed, L.kp, (const uint64_t *)c->vkeys.p, ctr, L.m_max, L.m_grid,
+                                         ta, L.pm, L.nbr_per_vertex, nbr, s));
     else
         GT_HIP(gt::launch_neighbors(L.d, L.packed, L.kp, (const uint64_t *)c->vkeys.p, ctr, L.m_max, nbr, L.wpc,
                                     s));
     if (c->profiling) GT_HIP(mark(c, 3, s));
     void *cur = c->vals_a.p, *nxt = c->vals_b.p;
     for (int j = 0; j <= L.d; j++) {
-        GT_HIP(gt::launch_blur(L.storage, L.vs, j, L.vsc, nbr + (size_t)j * L.m_max, ctr, L.m_max, L.pm, cur, nxt,
+        GT_HIP(gt::launch_blur(L.storage, L.vs, j, L.vsc, nbr + (size_t)j * L.m_max, ctr, L.m_grid, L.pm, cur, nxt,
                                s));
         std::swap(cur, nxt);
     }
@@ -689,6 +690,11 @@ int run(gt_ctx *c, const gt::PixSrc &src, int n, int row_w, int d, int vd, int s
         L.nbr_per_vertex = d >= 7 || est >= (1u << 18);
         L.m_max = (int)std::min<uint64_t>(cap, (uint64_t)n * (d + 1));
         if (tu.vert_limit > 0) L.m_max = std::min(L.m_max, tu.vert_limit);
+        // Grids for twice the expected count, not the table's capacity (a
+        // batch's 2^27 slots for 31 M vertices start 4x the workgroups
+        // needed, in every blur pass); the margin keeps the table part with
+        // the most vertices within one pass of its workgroups.
+        L.m_grid = (int)std::min<uint64_t>((uint64_t)L.m_max, 2 * est);
         L.pw = L.packed ? 1 : std::max(1, (L.kp.bits + 63) / 64);
         L.vs = value_stride(vd);
         const unsigned ncount = (cap + 2047) / 2048;

@@ -281,12 +281,15 @@ hipError_t launch_compact_final(int words, int storage, const TableArgs &ta, uin
 // With pm.cpp > 0, workgroups on XCD x take the vertices of table part x.
 // per_vertex: one thread per vertex for all axes (k_nbr_hash_vertex), else
 // one per (vertex, axis) (k_nbr_hash, more threads for small lattices).
+// m_max: the neighbour table's stride; m_grid (<= m_max): the vertices the
+// grid is sized for (the kernel loops past it, so any M <= m_max is covered).
 hipError_t launch_neighbors_hash(int d, bool packed, const KeyPlan &kp, const uint64_t *vkeys, const Counters *ctr,
-                                 int m_max, const TableArgs &ta, const PartMap &pm, bool per_vertex, int2 *nbr,
-                                 hipStream_t st);
+                                 int m_max, int m_grid, const TableArgs &ta, const PartMap &pm, bool per_vertex,
+                                 int2 *nbr, hipStream_t st);
 // Blur pass `pass` (0..d): in holds FP32 for pass 0 of a narrow storage (the
 // finalize writes FP32 there) and the storage type otherwise; vsc supplies the
 // narrow storages' scales.
+// m_max: the vertices the grid is sized for (the kernel loops past it).
 hipError_t launch_blur(int storage, int vs, int pass, const ValScale *vsc, const int2 *nbr, const Counters *ctr,
                        int m_max, const PartMap &pm, const void *in, void *out, hipStream_t st);
 // sbits: the scale source of a narrow storage's last blur pass

@@ -527,11 +527,12 @@ static unsigned vertex_blocks(int m_max, const PartMap &pm) {
 }
 
 hipError_t launch_neighbors_hash(int d, bool packed, const KeyPlan &kp, const uint64_t *vkeys, const Counters *ctr,
-                                 int m_max, const TableArgs &ta, const PartMap &pm, bool per_vertex, int2 *nbr,
-                                 hipStream_t st) {
+                                 int m_max, int m_grid, const TableArgs &ta, const PartMap &pm, bool per_vertex,
+                                 int2 *nbr, hipStream_t st) {
     if (m_max <= 0) return hipSuccess;
+    if (m_grid <= 0 || m_grid > m_max) m_grid = m_max;
     if (per_vertex) {
-        const dim3 g1(vertex_blocks(m_max, pm)), b1(kBlock);
+        const dim3 g1(vertex_blocks(m_grid, pm)), b1(kBlock);
 #define GT_NBRV(D)                                                                                                  \
     {                                                                                                               \
         if (packed) hipLaunchKernelGGL((k_nbr_hash_vertex<D, true>), g1, b1, 0, st, vkeys, kp, ctr, m_max, ta, pm,  \
@@ -549,7 +550,7 @@ hipError_t launch_neighbors_hash(int d, bool packed, const KeyPlan &kp, const ui
 #undef GT_NBRV
         return hipGetLastError();
     }
-    const dim3 g(vertex_blocks(m_max, pm), d + 1), b(kBlock);
+    const dim3 g(vertex_blocks(m_grid, pm), d + 1), b(kBlock);
 #define GT_NBRH(D)                                                                                                  \
     {                                                                                                               \
         if (packed)                                                                                                 \
