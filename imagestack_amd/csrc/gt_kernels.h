@@ -618,6 +618,26 @@ __device__ __forceinline__ void take_sums(const TableArgs &ta, const ValScale *_
     }
 }
 
+// take_sums with the sums already in registers (vd <= 4).
+__device__ __forceinline__ void take_sums4(const TableArgs &ta, const ValScale *__restrict__ vsc, size_t s, int vd,
+                                           const unsigned long long (&a)[4], float (&o)[kMaxVD]) {
+#pragma unroll
+    for (int c = 0; c < kMaxVD; c++)
+        o[c] = (c < 4 && c < vd) ? (float)((double)(long long)a[c < 4 ? c : 0] * (double)vsc->inv_s[c]) : 0.0f;
+    for (int c = 0; c < vd; c++) ta.tacc[s * vd + c] = 0ull;
+    if (vsc->nonfinite) {
+        const uint32_t m = ta.tnf[s];
+        if (m) {
+#pragma unroll
+            for (int c = 0; c < kMaxVD; c++) {
+                const uint32_t b = (m >> (2 * c)) & 3u;
+                o[c] = b == 3u ? __builtin_nanf("") : b == 1u ? INFINITY : b == 2u ? -INFINITY : o[c];
+            }
+            ta.tnf[s] = 0u;
+        }
+    }
+}
+
 template <int W, typename T>
 __global__ __launch_bounds__(kBlock) void k_write_final(TableArgs ta, uint32_t cap,
                                                         const uint32_t *__restrict__ offsets, const Counters *ctr,
@@ -625,7 +645,6 @@ __global__ __launch_bounds__(kBlock) void k_write_final(TableArgs ta, uint32_t c
                                                         uint64_t *__restrict__ vkeys, T *__restrict__ vals,
                                                         unsigned *__restrict__ track, SelfScan sc) {
     constexpr int NW = kBlock / kWave;
-    __shared__ unsigned ws[NW];
     const int wave = threadIdx.x / kWave;
     unsigned run;
     if (sc.offs) {
@@ -668,40 +687,62 @@ __global__ __launch_bounds__(kBlock) void k_write_final(TableArgs ta, uint32_t c
 #pragma unroll
     for (int c = 0; c < kMaxVD; c++) mx[c] = 0.0f;
     const size_t base = (size_t)blockIdx.x * kCompactSlots;
-    for (int k = 0; k < kCompactSlots / kBlock; k++) {
+    // Each thread's KI key words first (all in flight at once) and one
+    // barrier for the ranks of all of them; then row by row, with the next
+    // row's sums (vd <= 4) loaded before this row's stores, so the chunk's
+    // memory round trips overlap instead of following one another.
+    constexpr int KI = kCompactSlots / kBlock;
+    __shared__ unsigned wsk[KI][NW];
+    uint64_t w0[KI], mk[KI];
+#pragma unroll
+    for (int k = 0; k < KI; k++) {
         const size_t s = base + (size_t)k * kBlock + threadIdx.x;
-        uint64_t w0 = kEmpty;
-        if (s < cap) w0 = ta.tkeys[s * W];
-        const bool occ = w0 != kEmpty;
-        const uint64_t m = __ballot(occ);
-        if (__lane_id() == 0) ws[wave] = (unsigned)__popcll(m);
-        __syncthreads();
+        w0[k] = s < cap ? ta.tkeys[s * W] : kEmpty;
+    }
+#pragma unroll
+    for (int k = 0; k < KI; k++) {
+        mk[k] = __ballot(w0[k] != kEmpty);
+        if (__lane_id() == 0) wsk[k][wave] = (unsigned)__popcll(mk[k]);
+    }
+    __syncthreads();
+    const bool pre = vd <= 4;
+    unsigned long long acc[2][4];   // row k's sums, and row k+1's in flight
+    auto load_sums = [&](int k, unsigned long long(&a)[4]) {
+        const size_t s = base + (size_t)k * kBlock + threadIdx.x;
+#pragma unroll
+        for (int c = 0; c < 4; c++) a[c] = (pre && w0[k] != kEmpty && c < vd) ? ta.tacc[s * vd + c] : 0ull;
+    };
+    load_sums(0, acc[0]);
+#pragma unroll
+    for (int k = 0; k < KI; k++) {
+        if (k + 1 < KI) load_sums(k + 1, acc[(k + 1) & 1]);
+        const size_t s = base + (size_t)k * kBlock + threadIdx.x;
         unsigned off = run, tot = 0;
 #pragma unroll
         for (int w = 0; w < NW; w++) {
-            off += (w < wave) ? ws[w] : 0u;
-            tot += ws[w];
+            off += (w < wave) ? wsk[k][w] : 0u;
+            tot += wsk[k][w];
         }
         uint8_t tg = 0;
-        if (occ) {
-            const unsigned pos = off + (unsigned)__popcll(m & lanemask_lt());
+        if (w0[k] != kEmpty) {
+            const unsigned pos = off + (unsigned)__popcll(mk[k] & lanemask_lt());
             uint64_t kw[W];
-            kw[0] = w0;
+            kw[0] = w0[k];
 #pragma unroll
             for (int w = 1; w < W; w++) kw[w] = ta.tkeys[s * W + w];
 #pragma unroll
             for (int w = 0; w < W; w++) vkeys[(size_t)pos * W + w] = kw[w];
             if (ta.tags) tg = key_tag8(hash_key<W>(kw));
             float o[kMaxVD];
-            take_sums(ta, vsc, s, vd, o);
+            if (pre) take_sums4(ta, vsc, s, vd, acc[k & 1], o);
+            else take_sums(ta, vsc, s, vd, o);
             store_row_rt<T>(vals, pos, vs, o);
             if (track) take_max<kMaxVD>(mx, o);
-            if (W == 1 && ta.idshift > 0) ta.tkeys[s] = w0 | ((uint64_t)pos << ta.idshift);
+            if (W == 1 && ta.idshift > 0) ta.tkeys[s] = w0[k] | ((uint64_t)pos << ta.idshift);
             else ta.slot_id[s] = (int)pos;
         }
         if (ta.tags && s < cap) ta.tags[s] = tg;
         run += tot;
-        __syncthreads();
     }
     if (track) publish_max<kMaxVD>(track, mx, vd);
 }
