@@ -328,6 +328,7 @@ struct Launch {
     bool clear_sums;   // clear the fixed-point sums too (not known to be zero)
     bool clear_marks;  // clear the non-finite marks too (a call that stopped early may have left some)
     bool nbr_per_vertex;   // hash neighbours one thread per vertex (else per vertex and axis)
+    bool nbr_half;         // ... looking up one sign and scattering the other (entries set to -1 by the finalize)
     uint32_t cap;
     gt::LatConst lc;
     gt::KeyPlan kp;
@@ -371,7 +372,8 @@ int enqueue(gt_ctx *c, const Launch &L, const gt::PixSrc &src, const SliceSpec &
     if (L.hashnbr) {   // ids in compaction order: the finalize rides along with the compaction
         GT_HIP(gt::launch_compact_final(L.W, L.storage, ta, L.cap, (uint32_t *)c->counts.p,
                                         (uint32_t *)c->counts.p + chunks_of(L.cap), L.vsc, L.vd, L.vs,
-                                        (uint64_t *)c->vkeys.p, c->vals_a.p, ctr, L.m_max, s));
+                                        (uint64_t *)c->vkeys.p, c->vals_a.p, ctr, L.m_max,
+                                        gt::NbrInit{L.nbr_half ? (int2 *)c->nbr.p : nullptr, L.m_max, L.d + 1}, s));
     } else {
         GT_HIP(gt::launch_compact(L.W, ta, L.cap, (uint32_t *)c->counts.p, (uint32_t *)c->slot_list.p,
                                   (uint64_t *)c->pk_a.p, L.packed, ctr, L.m_max, s));
@@ -392,7 +394,7 @@ int enqueue(gt_ctx *c, const Launch &L, const gt::PixSrc &src, const SliceSpec &
     int2 *nbr = (int2 *)c->nbr.p;
     if (L.hashnbr)
         GT_HIP(gt::launch_neighbors_hash(L.d, L.packed, L.kp, (const uint64_t *)c->vkeys.p, ctr, L.m_max, L.m_grid,
-                                         ta, L.pm, L.nbr_per_vertex, nbr, s));
+                                         ta, L.pm, L.nbr_per_vertex, L.nbr_half, nbr, s));
     else
         GT_HIP(gt::launch_neighbors(L.d, L.packed, L.kp, (const uint64_t *)c->vkeys.p, ctr, L.m_max, nbr, L.wpc,
                                     s));
@@ -688,6 +690,14 @@ int run(gt_ctx *c, const gt::PixSrc &src, int n, int row_w, int d, int vd, int s
         // 35 us; 2048^2 joint bilateral, 0.41 M: 57 against 55 us; 4096^2
         // RGB at sigma_s 4, 28.9 M: 5.6 against 4.3 ms).
         L.nbr_per_vertex = d >= 7 || est >= (1u << 18);
+        // Each lattice edge looked up from one end only: u = k + 1 - (d+1)e_j
+        // found from k gives both u (k's first entry) and k (u's second),
+        // for the finalize's -1 fill of every entry and a scattered store.
+        // That pays where lookups are dear (d >= 7: wide keys; d = 16
+        // transform 16.8 -> 15.8 ms, non-local means 2.30 -> 2.27 ms); at
+        // d = 5 the 2048^2 joint bilateral is unchanged and the 64-image
+        // batch slower (16.4 -> 17.6 ms: its scattered stores miss L2).
+        L.nbr_half = L.hashnbr && L.nbr_per_vertex && d >= 7;
         L.m_max = (int)std::min<uint64_t>(cap, (uint64_t)n * (d + 1));
         if (tu.vert_limit > 0) L.m_max = std::min(L.m_max, tu.vert_limit);
         // Grids for twice the expected count, not the table's capacity (a

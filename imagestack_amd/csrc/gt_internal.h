@@ -248,6 +248,13 @@ hipError_t launch_slice_batch(int d, int storage, const PixSrc &tmpl, const Batc
 // those offsets (the PartMap then reads them), wctr the vertex count M (0
 // after a full table or over `limit`) and the filled count.  offs == nullptr:
 // the counts were scanned already.
+// Neighbour entries the finalize initialises to {-1, -1} (nbr == nullptr: none).
+struct NbrInit {
+    int2 *nbr;
+    int stride;   // entries between axes
+    int axes;
+};
+
 struct SelfScan {
     uint32_t *offs;
     Counters *wctr;
@@ -273,9 +280,12 @@ hipError_t launch_neighbors(int d, bool packed, const KeyPlan &kp, const uint64_
 // offs: where the chunk offsets go when the table has at most
 // kSelfScanChunks chunks (the finalize scans the counts itself); else they
 // replace the counts in place.
+// ninit.nbr != nullptr: every vertex's neighbour entries ({-1, -1} on each
+// of ninit.axes axes, stride ninit.stride) are written too, for the
+// neighbour form that fills the second half of each entry by scatter.
 hipError_t launch_compact_final(int words, int storage, const TableArgs &ta, uint32_t capacity, uint32_t *counts,
                                 uint32_t *offs, const ValScale *vsc, int vd, int vs, uint64_t *vkeys, void *vals,
-                                Counters *ctr, int limit, hipStream_t st);
+                                Counters *ctr, int limit, const NbrInit &ninit, hipStream_t st);
 // The same table by hash lookups in the complete global table, for vertex ids
 // in compaction order (no sort).
 // With pm.cpp > 0, workgroups on XCD x take the vertices of table part x.
@@ -283,9 +293,13 @@ hipError_t launch_compact_final(int words, int storage, const TableArgs &ta, uin
 // one per (vertex, axis) (k_nbr_hash, more threads for small lattices).
 // m_max: the neighbour table's stride; m_grid (<= m_max): the vertices the
 // grid is sized for (the kernel loops past it, so any M <= m_max is covered).
+// half (per_vertex only): each vertex looks up its sign-0 neighbour u on
+// every axis, writes it and writes itself as u's sign-1 neighbour (the
+// relation is symmetric); the entries must hold {-1, -1} beforehand
+// (launch_compact_final's ninit).
 hipError_t launch_neighbors_hash(int d, bool packed, const KeyPlan &kp, const uint64_t *vkeys, const Counters *ctr,
                                  int m_max, int m_grid, const TableArgs &ta, const PartMap &pm, bool per_vertex,
-                                 int2 *nbr, hipStream_t st);
+                                 bool half, int2 *nbr, hipStream_t st);
 // Blur pass `pass` (0..d): in holds FP32 for pass 0 of a narrow storage (the
 // finalize writes FP32 there) and the storage type otherwise; vsc supplies the
 // narrow storages' scales.

@@ -643,9 +643,10 @@ __global__ __launch_bounds__(kBlock) void k_write_final(TableArgs ta, uint32_t c
                                                         const uint32_t *__restrict__ offsets, const Counters *ctr,
                                                         const ValScale *__restrict__ vsc, int vd, int vs,
                                                         uint64_t *__restrict__ vkeys, T *__restrict__ vals,
-                                                        unsigned *__restrict__ track, SelfScan sc) {
+                                                        unsigned *__restrict__ track, SelfScan sc, NbrInit ninit) {
     constexpr int NW = kBlock / kWave;
     const int wave = threadIdx.x / kWave;
+    const int chunk = (int)blockIdx.x;
     unsigned run;
     if (sc.offs) {
         // offsets holds the chunks' counts: this chunk's offset and the total
@@ -654,7 +655,7 @@ __global__ __launch_bounds__(kBlock) void k_write_final(TableArgs ta, uint32_t c
         for (int i = threadIdx.x; i < sc.nchunks; i += kBlock) {
             const unsigned v = offsets[i];
             tot += v;
-            pre += i < (int)blockIdx.x ? v : 0u;
+            pre += i < chunk ? v : 0u;
         }
         pre = wave_sum(pre);
         tot = wave_sum(tot);
@@ -676,17 +677,17 @@ __global__ __launch_bounds__(kBlock) void k_write_final(TableArgs ta, uint32_t c
             sc.wctr->M = full ? 0u : tot;
             sc.wctr->filled = tot;
         }
-        if (threadIdx.x == 0) sc.offs[blockIdx.x] = pre;
+        if (threadIdx.x == 0) sc.offs[chunk] = pre;
         if (full) return;
         run = pre;
     } else {
         if (*(volatile const unsigned *)&ctr->flags & kFlagFull) return;   // the same for every thread
-        run = offsets[blockIdx.x];
+        run = offsets[chunk];
     }
     float mx[kMaxVD];   // track: largest finite magnitude per channel (a narrow storage's first scale)
 #pragma unroll
     for (int c = 0; c < kMaxVD; c++) mx[c] = 0.0f;
-    const size_t base = (size_t)blockIdx.x * kCompactSlots;
+    const size_t base = (size_t)chunk * kCompactSlots;
     // Each thread's KI key words first (all in flight at once) and one
     // barrier for the ranks of all of them; then row by row, with the next
     // row's sums (vd <= 4) loaded before this row's stores, so the chunk's
@@ -740,6 +741,11 @@ __global__ __launch_bounds__(kBlock) void k_write_final(TableArgs ta, uint32_t c
             if (track) take_max<kMaxVD>(mx, o);
             if (W == 1 && ta.idshift > 0) ta.tkeys[s] = w0[k] | ((uint64_t)pos << ta.idshift);
             else ta.slot_id[s] = (int)pos;
+            if (ninit.nbr) {
+                const u32x2 none = {~0u, ~0u};
+                for (int j = 0; j < ninit.axes; j++)
+                    __builtin_nontemporal_store(none, reinterpret_cast<u32x2 *>(ninit.nbr) + (size_t)j * ninit.stride + pos);
+            }
         }
         if (ta.tags && s < cap) ta.tags[s] = tg;
         run += tot;
@@ -1123,11 +1129,16 @@ __global__ __launch_bounds__(kBlock) void k_nbr_hash(const uint64_t *__restrict_
 // -(D+1) (sign 0) or +(D+1) (sign 1) on field j -- as integer arithmetic,
 // exact whenever no field leaves its range (the checks).  The fields of a
 // target sum to the vertex's sum -1 / +1 (j < D) or +D / -D (j = D).
-template <int D, bool PACKED>
+// HALF: only sign 0 is looked up.  u = id(k + 1 - (D+1) e_j) has this vertex
+// as its sign-1 neighbour on axis j (u - 1 + (D+1) e_j = k), so the thread
+// writes entry i's first half and entry u's second half; every second half
+// without a writer keeps the -1 the finalize put there.
+template <int D, bool PACKED, bool HALF>
 __global__ __launch_bounds__(kBlock) void k_nbr_hash_vertex(const uint64_t *__restrict__ vk, KeyPlan kp,
                                                             const Counters *ctr, int stride, TableArgs ta, PartMap pm,
                                                             int *__restrict__ nbr) {
     constexpr int W = KeyOps<D, PACKED>::W;
+    constexpr int NS = HALF ? 1 : 2;   // signs looked up
     const unsigned M = ctr->M;
     unsigned i, end, step;
     part_walk(pm, M, blockIdx.x, gridDim.x, i, end, step);
@@ -1162,9 +1173,9 @@ __global__ __launch_bounds__(kBlock) void k_nbr_hash_vertex(const uint64_t *__re
         // axis j < D moves word j/4 (packed: the one word) by -+big.  Their
         // hashes are the sum of the word hashes of k +- one, with word j/4's
         // replaced.
-        uint64_t gw[2][W], gsum[2];
+        uint64_t gw[2][W], gsum[2] = {0, 0};
 #pragma unroll
-        for (int sgn = 0; sgn < 2; sgn++) {
+        for (int sgn = 0; sgn < NS; sgn++) {
             gsum[sgn] = 0;
 #pragma unroll
             for (int w = 0; w < W; w++) {
@@ -1207,7 +1218,7 @@ __global__ __launch_bounds__(kBlock) void k_nbr_hash_vertex(const uint64_t *__re
                         t[1][w] = m1;
                     }
                 hsum[0] += hash_word(m0, wj) - g0;   // one word rehashed
-                hsum[1] += hash_word(m1, wj) - g1;
+                if (!HALF) hsum[1] += hash_word(m1, wj) - g1;
                 tsum[0] = sum - 1;
                 tsum[1] = sum + 1;
             } else {
@@ -1216,13 +1227,13 @@ __global__ __launch_bounds__(kBlock) void k_nbr_hash_vertex(const uint64_t *__re
                 tsum[0] = sum + D;
                 tsum[1] = sum - D;
             }
-            uint32_t s[2];
-            bool done[2];
+            uint32_t s[2] = {0, 0};
+            bool done[2] = {true, true};   // (sign 1 stays done under HALF)
             int slot[2] = {-1, -1};
             uint64_t hit[2] = {0, 0};
-            uint8_t want[2];
+            uint8_t want[2] = {0, 0};
 #pragma unroll
-            for (int sgn = 0; sgn < 2; sgn++) {
+            for (int sgn = 0; sgn < NS; sgn++) {
                 const uint64_t h64 = hash_final(hsum[sgn]);   // = hash_key<W>(t[sgn])
                 const uint32_t h = (uint32_t)h64;
                 s[sgn] = ta.part_mul ? slot_in_part(ta, h, tsum[sgn], tag) : (h & ta.mask);
@@ -1232,15 +1243,15 @@ __global__ __launch_bounds__(kBlock) void k_nbr_hash_vertex(const uint64_t *__re
             // through the tags: key words only where a tag matches
             if (ta.tags) {
 #pragma unroll
-                for (int sgn = 0; sgn < 2; sgn++)
+                for (int sgn = 0; sgn < NS; sgn++)
                     if (!done[sgn]) slot[sgn] = tag_find<W>(ta, t[sgn], kmask, s[sgn], want[sgn], hit[sgn]);
             }
             for (uint32_t it = 0; !ta.tags && it < kMaxProbes && !(done[0] && done[1]); it++) {
                 uint64_t w0[2];
 #pragma unroll
-                for (int sgn = 0; sgn < 2; sgn++) w0[sgn] = done[sgn] ? kEmpty : ta.tkeys[(size_t)s[sgn] * W];
+                for (int sgn = 0; sgn < NS; sgn++) w0[sgn] = done[sgn] ? kEmpty : ta.tkeys[(size_t)s[sgn] * W];
 #pragma unroll
-                for (int sgn = 0; sgn < 2; sgn++) {
+                for (int sgn = 0; sgn < NS; sgn++) {
                     if (done[sgn]) continue;
                     if (w0[sgn] == kEmpty) {
                         done[sgn] = true;
@@ -1258,12 +1269,17 @@ __global__ __launch_bounds__(kBlock) void k_nbr_hash_vertex(const uint64_t *__re
                     }
                 }
             }
-            int res[2];
+            int res[2] = {-1, -1};
 #pragma unroll
-            for (int sgn = 0; sgn < 2; sgn++)
+            for (int sgn = 0; sgn < NS; sgn++)
                 res[sgn] = slot[sgn] < 0 ? -1 : fold ? (int)(hit[sgn] >> ta.idshift) : ta.slot_id[slot[sgn]];
-            const u32x2 pair = {(unsigned)res[0], (unsigned)res[1]};
-            __builtin_nontemporal_store(pair, reinterpret_cast<u32x2 *>(nbr) + (size_t)j * stride + i);
+            if (HALF) {
+                nbr[2 * ((size_t)j * stride + i)] = res[0];
+                if (res[0] >= 0) nbr[2 * ((size_t)j * stride + (unsigned)res[0]) + 1] = (int)i;
+            } else {
+                const u32x2 pair = {(unsigned)res[0], (unsigned)res[1]};
+                __builtin_nontemporal_store(pair, reinterpret_cast<u32x2 *>(nbr) + (size_t)j * stride + i);
+            }
         }
     }
 }

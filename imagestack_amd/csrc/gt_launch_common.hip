@@ -467,11 +467,12 @@ hipError_t launch_finalize(int words, int storage, const TableArgs &ta, const Va
 template <typename T>
 static void launch_write_final_t(int words, const TableArgs &ta, uint32_t capacity, unsigned nblk,
                                  const uint32_t *offsets, const Counters *ctr, const ValScale *vsc, int vd, int vs,
-                                 uint64_t *vkeys, void *vals, unsigned *track, const SelfScan &sc, hipStream_t st) {
+                                 uint64_t *vkeys, void *vals, unsigned *track, const SelfScan &sc,
+                                 const NbrInit &ninit, hipStream_t st) {
     T *v = (T *)vals;
 #define GT_WF(W)                                                                                                 \
     hipLaunchKernelGGL((k_write_final<W, T>), dim3(nblk), dim3(kBlock), 0, st, ta, capacity, offsets, ctr, vsc,  \
-                       vd, vs, vkeys, v, track, sc)
+                       vd, vs, vkeys, v, track, sc, ninit)
     switch (words) {
     case 1: GT_WF(1); break;
     case 2: GT_WF(2); break;
@@ -483,8 +484,9 @@ static void launch_write_final_t(int words, const TableArgs &ta, uint32_t capaci
 
 hipError_t launch_compact_final(int words, int storage, const TableArgs &ta, uint32_t capacity, uint32_t *counts,
                                 uint32_t *offs, const ValScale *vsc, int vd, int vs, uint64_t *vkeys, void *vals,
-                                Counters *ctr, int limit, hipStream_t st) {
+                                Counters *ctr, int limit, const NbrInit &ninit, hipStream_t st) {
     if (words < 1 || words > 4) return hipErrorInvalidValue;
+    if (ninit.nbr && (ninit.stride < limit || ninit.axes < 1 || ninit.axes > kMaxD + 1)) return hipErrorInvalidValue;
     const unsigned nblk = blocks_for(capacity, kCompactSlots);
     switch (words) {
     case 1: hipLaunchKernelGGL(k_count<1>, dim3(nblk), dim3(kBlock), 0, st, ta.tkeys, capacity, counts); break;
@@ -499,7 +501,8 @@ hipError_t launch_compact_final(int words, int storage, const TableArgs &ta, uin
                            &ctr->filled, (unsigned)limit);
     // FP32 rows for every storage (see launch_finalize)
     unsigned *track = storage == 0 ? nullptr : const_cast<unsigned *>(&vsc->vmax_bits[0][0]);
-    launch_write_final_t<float>(words, ta, capacity, nblk, counts, ctr, vsc, vd, vs, vkeys, vals, track, sc, st);
+    launch_write_final_t<float>(words, ta, capacity, nblk, counts, ctr, vsc, vd, vs, vkeys, vals, track, sc, ninit,
+                                st);
     return hipGetLastError();
 }
 
@@ -528,17 +531,22 @@ static unsigned vertex_blocks(int m_max, const PartMap &pm) {
 
 hipError_t launch_neighbors_hash(int d, bool packed, const KeyPlan &kp, const uint64_t *vkeys, const Counters *ctr,
                                  int m_max, int m_grid, const TableArgs &ta, const PartMap &pm, bool per_vertex,
-                                 int2 *nbr, hipStream_t st) {
+                                 bool half, int2 *nbr, hipStream_t st) {
     if (m_max <= 0) return hipSuccess;
     if (m_grid <= 0 || m_grid > m_max) m_grid = m_max;
     if (per_vertex) {
         const dim3 g1(vertex_blocks(m_grid, pm)), b1(kBlock);
-#define GT_NBRV(D)                                                                                                  \
+#define GT_NBRV_H(D, P)                                                                                             \
     {                                                                                                               \
-        if (packed) hipLaunchKernelGGL((k_nbr_hash_vertex<D, true>), g1, b1, 0, st, vkeys, kp, ctr, m_max, ta, pm,  \
-                                       (int *)nbr);                                                                 \
-        else hipLaunchKernelGGL((k_nbr_hash_vertex<D, false>), g1, b1, 0, st, vkeys, kp, ctr, m_max, ta, pm,        \
+        if (half) hipLaunchKernelGGL((k_nbr_hash_vertex<D, P, true>), g1, b1, 0, st, vkeys, kp, ctr, m_max, ta, pm, \
+                                     (int *)nbr);                                                                   \
+        else hipLaunchKernelGGL((k_nbr_hash_vertex<D, P, false>), g1, b1, 0, st, vkeys, kp, ctr, m_max, ta, pm,     \
                                 (int *)nbr);                                                                        \
+    }
+#define GT_NBRV(D)                 \
+    {                              \
+        if (packed) GT_NBRV_H(D, true) \
+        else GT_NBRV_H(D, false)   \
     } break;
         switch (d) {
         case 1: GT_NBRV(1) case 2: GT_NBRV(2) case 3: GT_NBRV(3) case 4: GT_NBRV(4) case 5: GT_NBRV(5)
@@ -548,6 +556,7 @@ hipError_t launch_neighbors_hash(int d, bool packed, const KeyPlan &kp, const ui
         default: return hipErrorInvalidValue;
         }
 #undef GT_NBRV
+#undef GT_NBRV_H
         return hipGetLastError();
     }
     const dim3 g(vertex_blocks(m_grid, pm), d + 1), b(kBlock);
