@@ -373,7 +373,9 @@ int enqueue(gt_ctx *c, const Launch &L, const gt::PixSrc &src, const SliceSpec &
         GT_HIP(gt::launch_compact_final(L.W, L.storage, ta, L.cap, (uint32_t *)c->counts.p,
                                         (uint32_t *)c->counts.p + chunks_of(L.cap), L.vsc, L.vd, L.vs,
                                         (uint64_t *)c->vkeys.p, c->vals_a.p, ctr, L.m_max,
-                                        gt::NbrInit{L.nbr_half ? (int2 *)c->nbr.p : nullptr, L.m_max, L.d + 1}, s));
+                                        gt::NbrInit{L.nbr_half ? (int2 *)c->nbr.p : nullptr, L.m_max, L.d + 1,
+                                                    L.m_grid},
+                                        s));
     } else {
         GT_HIP(gt::launch_compact(L.W, ta, L.cap, (uint32_t *)c->counts.p, (uint32_t *)c->slot_list.p,
                                   (uint64_t *)c->pk_a.p, L.packed, ctr, L.m_max, s));
@@ -692,7 +694,7 @@ int run(gt_ctx *c, const gt::PixSrc &src, int n, int row_w, int d, int vd, int s
         L.nbr_per_vertex = d >= 7 || est >= (1u << 18);
         // Each lattice edge looked up from one end only: u = k + 1 - (d+1)e_j
         // found from k gives both u (k's first entry) and k (u's second),
-        // for the finalize's -1 fill of every entry and a scattered store.
+        // for a -1 fill of every entry and a scattered store.
         // That pays where lookups are dear (d >= 7: wide keys; d = 16
         // transform 16.8 -> 15.8 ms, non-local means 2.30 -> 2.27 ms); at
         // d = 5 the 2048^2 joint bilateral is unchanged and the 64-image

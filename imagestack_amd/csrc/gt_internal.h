@@ -248,11 +248,12 @@ hipError_t launch_slice_batch(int d, int storage, const PixSrc &tmpl, const Batc
 // those offsets (the PartMap then reads them), wctr the vertex count M (0
 // after a full table or over `limit`) and the filled count.  offs == nullptr:
 // the counts were scanned already.
-// Neighbour entries the finalize initialises to {-1, -1} (nbr == nullptr: none).
+// Neighbour entries set to {-1, -1} after the finalize (nbr == nullptr: none).
 struct NbrInit {
     int2 *nbr;
     int stride;   // entries between axes
     int axes;
+    int grid;     // vertices the fill's grid is sized for (it loops to M)
 };
 
 struct SelfScan {
@@ -280,8 +281,8 @@ hipError_t launch_neighbors(int d, bool packed, const KeyPlan &kp, const uint64_
 // offs: where the chunk offsets go when the table has at most
 // kSelfScanChunks chunks (the finalize scans the counts itself); else they
 // replace the counts in place.
-// ninit.nbr != nullptr: every vertex's neighbour entries ({-1, -1} on each
-// of ninit.axes axes, stride ninit.stride) are written too, for the
+// ninit.nbr != nullptr: then the M vertices' neighbour entries are set to
+// {-1, -1} on each of ninit.axes axes (stride ninit.stride), for the
 // neighbour form that fills the second half of each entry by scatter.
 hipError_t launch_compact_final(int words, int storage, const TableArgs &ta, uint32_t capacity, uint32_t *counts,
                                 uint32_t *offs, const ValScale *vsc, int vd, int vs, uint64_t *vkeys, void *vals,

@@ -643,7 +643,7 @@ __global__ __launch_bounds__(kBlock) void k_write_final(TableArgs ta, uint32_t c
                                                         const uint32_t *__restrict__ offsets, const Counters *ctr,
                                                         const ValScale *__restrict__ vsc, int vd, int vs,
                                                         uint64_t *__restrict__ vkeys, T *__restrict__ vals,
-                                                        unsigned *__restrict__ track, SelfScan sc, NbrInit ninit) {
+                                                        unsigned *__restrict__ track, SelfScan sc) {
     constexpr int NW = kBlock / kWave;
     const int wave = threadIdx.x / kWave;
     const int chunk = (int)blockIdx.x;
@@ -741,11 +741,6 @@ __global__ __launch_bounds__(kBlock) void k_write_final(TableArgs ta, uint32_t c
             if (track) take_max<kMaxVD>(mx, o);
             if (W == 1 && ta.idshift > 0) ta.tkeys[s] = w0[k] | ((uint64_t)pos << ta.idshift);
             else ta.slot_id[s] = (int)pos;
-            if (ninit.nbr) {
-                const u32x2 none = {~0u, ~0u};
-                for (int j = 0; j < ninit.axes; j++)
-                    __builtin_nontemporal_store(none, reinterpret_cast<u32x2 *>(ninit.nbr) + (size_t)j * ninit.stride + pos);
-            }
         }
         if (ta.tags && s < cap) ta.tags[s] = tg;
         run += tot;

@@ -464,15 +464,24 @@ hipError_t launch_finalize(int words, int storage, const TableArgs &ta, const Va
     return hipGetLastError();
 }
 
+// Neighbour entries [0, M) of axis blockIdx.y set to {-1, -1} (the one-end
+// neighbour form fills their second halves by scatter).
+__global__ __launch_bounds__(kBlock) void k_nbr_fill(int2 *__restrict__ nbr, int stride, const Counters *ctr) {
+    const unsigned M = ctr->M;
+    u32x2 *row = reinterpret_cast<u32x2 *>(nbr) + (size_t)blockIdx.y * stride;
+    const u32x2 none = {~0u, ~0u};
+    for (unsigned i = blockIdx.x * kBlock + threadIdx.x; i < M; i += gridDim.x * kBlock)
+        __builtin_nontemporal_store(none, row + i);
+}
+
 template <typename T>
 static void launch_write_final_t(int words, const TableArgs &ta, uint32_t capacity, unsigned nblk,
                                  const uint32_t *offsets, const Counters *ctr, const ValScale *vsc, int vd, int vs,
-                                 uint64_t *vkeys, void *vals, unsigned *track, const SelfScan &sc,
-                                 const NbrInit &ninit, hipStream_t st) {
+                                 uint64_t *vkeys, void *vals, unsigned *track, const SelfScan &sc, hipStream_t st) {
     T *v = (T *)vals;
 #define GT_WF(W)                                                                                                 \
     hipLaunchKernelGGL((k_write_final<W, T>), dim3(nblk), dim3(kBlock), 0, st, ta, capacity, offsets, ctr, vsc,  \
-                       vd, vs, vkeys, v, track, sc, ninit)
+                       vd, vs, vkeys, v, track, sc)
     switch (words) {
     case 1: GT_WF(1); break;
     case 2: GT_WF(2); break;
@@ -486,7 +495,8 @@ hipError_t launch_compact_final(int words, int storage, const TableArgs &ta, uin
                                 uint32_t *offs, const ValScale *vsc, int vd, int vs, uint64_t *vkeys, void *vals,
                                 Counters *ctr, int limit, const NbrInit &ninit, hipStream_t st) {
     if (words < 1 || words > 4) return hipErrorInvalidValue;
-    if (ninit.nbr && (ninit.stride < limit || ninit.axes < 1 || ninit.axes > kMaxD + 1)) return hipErrorInvalidValue;
+    if (ninit.nbr && (ninit.stride < limit || ninit.axes < 1 || ninit.axes > kMaxD + 1 || ninit.grid < 1))
+        return hipErrorInvalidValue;
     const unsigned nblk = blocks_for(capacity, kCompactSlots);
     switch (words) {
     case 1: hipLaunchKernelGGL(k_count<1>, dim3(nblk), dim3(kBlock), 0, st, ta.tkeys, capacity, counts); break;
@@ -501,8 +511,10 @@ hipError_t launch_compact_final(int words, int storage, const TableArgs &ta, uin
                            &ctr->filled, (unsigned)limit);
     // FP32 rows for every storage (see launch_finalize)
     unsigned *track = storage == 0 ? nullptr : const_cast<unsigned *>(&vsc->vmax_bits[0][0]);
-    launch_write_final_t<float>(words, ta, capacity, nblk, counts, ctr, vsc, vd, vs, vkeys, vals, track, sc, ninit,
-                                st);
+    launch_write_final_t<float>(words, ta, capacity, nblk, counts, ctr, vsc, vd, vs, vkeys, vals, track, sc, st);
+    if (ninit.nbr)
+        hipLaunchKernelGGL(k_nbr_fill, dim3(blocks_for(ninit.grid, kBlock), ninit.axes), dim3(kBlock), 0, st,
+                           ninit.nbr, ninit.stride, ctr);
     return hipGetLastError();
 }
 
