@@ -694,12 +694,13 @@ int run(gt_ctx *c, const gt::PixSrc &src, int n, int row_w, int d, int vd, int s
         L.nbr_per_vertex = d >= 7 || est >= (1u << 18);
         // Each lattice edge looked up from one end only: u = k + 1 - (d+1)e_j
         // found from k gives both u (k's first entry) and k (u's second),
-        // for a -1 fill of every entry and a scattered store.
-        // That pays where lookups are dear (d >= 7: wide keys; d = 16
-        // transform 16.8 -> 15.8 ms, non-local means 2.30 -> 2.27 ms); at
-        // d = 5 the 2048^2 joint bilateral is unchanged and the 64-image
-        // batch slower (16.4 -> 17.6 ms: its scattered stores miss L2).
-        L.nbr_half = L.hashnbr && L.nbr_per_vertex && d >= 7;
+        // for a -1 fill of every entry and a scattered store.  That pays
+        // where lookups are dear (d >= 7: wide keys; d = 16 transform 16.8
+        // -> 14.8 ms, non-local means 2.30 -> 2.23 ms) and where the
+        // scattered stores stay in L2 (2048^2 joint bilateral: neighbours
+        // 52 -> 41 us plus a 7 us fill); not for large d = 5 lattices (a
+        // 64-image batch, 31 M vertices: 16.4 -> 17.6 ms).
+        L.nbr_half = L.hashnbr && L.nbr_per_vertex && (d >= 7 || est * (d + 1) * 8 <= (32u << 20));
         L.m_max = (int)std::min<uint64_t>(cap, (uint64_t)n * (d + 1));
         if (tu.vert_limit > 0) L.m_max = std::min(L.m_max, tu.vert_limit);
         // Grids for twice the expected count, not the table's capacity (a
