@@ -328,7 +328,8 @@ struct Launch {
     bool clear_sums;   // clear the fixed-point sums too (not known to be zero)
     bool clear_marks;  // clear the non-finite marks too (a call that stopped early may have left some)
     bool nbr_per_vertex;   // hash neighbours one thread per vertex (else per vertex and axis)
-    bool nbr_half;         // ... looking up one sign and scattering the other (entries set to -1 by the finalize)
+    bool nbr_half;         // ... looking up one sign and scattering the other (entries set to -1 by a fill)
+    bool blur_pairs;       // blur passes two per launch
     uint32_t cap;
     gt::LatConst lc;
     gt::KeyPlan kp;
@@ -402,9 +403,16 @@ int enqueue(gt_ctx *c, const Launch &L, const gt::PixSrc &src, const SliceSpec &
                                     s));
     if (c->profiling) GT_HIP(mark(c, 3, s));
     void *cur = c->vals_a.p, *nxt = c->vals_b.p;
-    for (int j = 0; j <= L.d; j++) {
-        GT_HIP(gt::launch_blur(L.storage, L.vs, j, L.vsc, nbr + (size_t)j * L.m_max, ctr, L.m_grid, L.pm, cur, nxt,
-                               s));
+    for (int j = 0; j <= L.d;) {
+        if (L.blur_pairs && j + 1 <= L.d) {
+            GT_HIP(gt::launch_blur2(L.vs, nbr + (size_t)j * L.m_max, nbr + (size_t)(j + 1) * L.m_max, ctr, L.m_grid,
+                                    L.pm, (const float *)cur, (float *)nxt, s));
+            j += 2;
+        } else {
+            GT_HIP(gt::launch_blur(L.storage, L.vs, j, L.vsc, nbr + (size_t)j * L.m_max, ctr, L.m_grid, L.pm, cur,
+                                   nxt, s));
+            j++;
+        }
         std::swap(cur, nxt);
     }
     if (c->profiling) GT_HIP(mark(c, 4, s));
@@ -701,6 +709,13 @@ int run(gt_ctx *c, const gt::PixSrc &src, int n, int row_w, int d, int vd, int s
         // 52 -> 41 us plus a 7 us fill); not for large d = 5 lattices (a
         // 64-image batch, 31 M vertices: 16.4 -> 17.6 ms).
         L.nbr_half = L.hashnbr && L.nbr_per_vertex && (d >= 7 || est * (d + 1) * 8 <= (32u << 20));
+        // Two blur passes per launch (FP32 values only: a narrow storage
+        // rounds between passes) where the lattice is sparse, more vertices
+        // than (d+1)/2 per pixel, so that most neighbours are missing and the
+        // nine reads of a fused pair are few: d = 16 transform 14.8 -> 14.3
+        // ms.  Dense lattices read more than the two launches do (2048^2
+        // joint bilateral 0.376 -> 0.384 ms, 64-image batch 16.4 -> 17.0 ms).
+        L.blur_pairs = storage == 0 && vd <= 4 && est * 2 > (uint64_t)n * (d + 1);
         L.m_max = (int)std::min<uint64_t>(cap, (uint64_t)n * (d + 1));
         if (tu.vert_limit > 0) L.m_max = std::min(L.m_max, tu.vert_limit);
         // Grids for twice the expected count, not the table's capacity (a

@@ -307,6 +307,10 @@ hipError_t launch_neighbors_hash(int d, bool packed, const KeyPlan &kp, const ui
 // m_max: the vertices the grid is sized for (the kernel loops past it).
 hipError_t launch_blur(int storage, int vs, int pass, const ValScale *vsc, const int2 *nbr, const Counters *ctr,
                        int m_max, const PartMap &pm, const void *in, void *out, hipStream_t st);
+// Passes j and j+1 in one launch (FP32 storage, vs <= 4): nbr_j, nbr_k their
+// neighbour rows; the same result as two launch_blur calls.
+hipError_t launch_blur2(int vs, const int2 *nbr_j, const int2 *nbr_k, const Counters *ctr, int m_max,
+                        const PartMap &pm, const float *in, float *out, hipStream_t st);
 // sbits: the scale source of a narrow storage's last blur pass
 // (ValScale::vmax_bits[d]); ignored for FP32.
 // Slice from the tiled splat's replay record (the splat's own positions,

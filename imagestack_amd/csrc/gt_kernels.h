@@ -1326,6 +1326,46 @@ __global__ __launch_bounds__(kBlock) void k_blur(const int2 *__restrict__ nbr, c
     if (SCALED) publish_max<VS>(out_max, mx, VS);
 }
 
+// Two blur passes, along axes j and k = j+1, in one launch (FP32 storage):
+// out(v) = (0.25 P(u) + 0.5 P(v)) + 0.25 P(w), u and w v's neighbours on
+// axis k, with P(x) = (0.25 in(a) + 0.5 in(x)) + 0.25 in(b), a and b x's
+// neighbours on axis j, a missing vertex contributing 0 -- the FP32
+// operations of the two passes one after the other, without the array
+// between them (each P is formed three times, by the threads of v's row).
+template <int VS>
+__global__ __launch_bounds__(kBlock) void k_blur2(const int2 *__restrict__ nbr_j, const int2 *__restrict__ nbr_k,
+                                                  const Counters *ctr, PartMap pm, const float *__restrict__ in,
+                                                  float *__restrict__ out) {
+    const unsigned M = ctr->M;
+    unsigned i, end, step;
+    part_walk(pm, M, blockIdx.x, gridDim.x, i, end, step);
+    for (; i < end; i += step) {
+        const int2 nk = nbr_k[i];
+        const int xs[3] = {nk.x, (int)i, nk.y};
+        int2 nj[3];
+#pragma unroll
+        for (int q = 0; q < 3; q++) nj[q] = xs[q] >= 0 ? nbr_j[xs[q]] : make_int2(-1, -1);
+        float P[3][kMaxVD];
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            float a[kMaxVD], o[kMaxVD], b[kMaxVD];
+#pragma unroll
+            for (int c = 0; c < VS; c++) { a[c] = 0.0f; o[c] = 0.0f; b[c] = 0.0f; }
+            if (xs[q] >= 0) {
+                if (nj[q].x >= 0) load_row<float, VS>(in, nj[q].x, a);
+                load_row<float, VS>(in, xs[q], o);
+                if (nj[q].y >= 0) load_row<float, VS>(in, nj[q].y, b);
+            }
+#pragma unroll
+            for (int c = 0; c < VS; c++) P[q][c] = xs[q] >= 0 ? (0.25f * a[c] + 0.5f * o[c]) + 0.25f * b[c] : 0.0f;
+        }
+        float r[kMaxVD];
+#pragma unroll
+        for (int c = 0; c < VS; c++) r[c] = (0.25f * P[0][c] + 0.5f * P[1][c]) + 0.25f * P[2][c];
+        store_row<float, VS>(out, i, r);
+    }
+}
+
 // Final write of one pixel: raw vd channels, or the first vd-1 divided by the
 // homogeneous last one (true division, src/GaussTransform.cpp:673-676).
 template <int NV>

@@ -614,6 +614,16 @@ static hipError_t launch_blur_narrow(int vs, int pass, const ValScale *vsc, cons
                                      out_max, st);
 }
 
+hipError_t launch_blur2(int vs, const int2 *nbr_j, const int2 *nbr_k, const Counters *ctr, int m_max,
+                        const PartMap &pm, const float *in, float *out, hipStream_t st) {
+    if (m_max <= 0) return hipSuccess;
+    const dim3 g(vertex_blocks(m_max, pm)), b(kBlock);
+    if (vs == 2) hipLaunchKernelGGL(k_blur2<2>, g, b, 0, st, nbr_j, nbr_k, ctr, pm, in, out);
+    else if (vs == 4) hipLaunchKernelGGL(k_blur2<4>, g, b, 0, st, nbr_j, nbr_k, ctr, pm, in, out);
+    else return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
 hipError_t launch_blur(int storage, int vs, int pass, const ValScale *vsc, const int2 *nbr, const Counters *ctr,
                        int m_max, const PartMap &pm, const void *in, void *out, hipStream_t st) {
     if (m_max <= 0) return hipSuccess;

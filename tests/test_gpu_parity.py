@@ -170,6 +170,31 @@ def test_highdim_d16_small(gpu_ctx, oracle_lib):
     assert np.array_equal(g["nbr"], o["nbr"])
 
 
+def test_highdim_d16_repeat_same_bits(oracle_lib):
+    """A sparse lattice (d = 16, about d+1 vertices per pixel) on a fresh
+    context: the first call blurs one pass per launch, the second, which
+    knows the vertex density, two passes per launch; both give the same
+    bits, within the FP64 gate."""
+    from imagestack_amd import gtlattice as G
+    O = oracle_lib
+    case("d=16 gauss transform 96x96, twice on a fresh context")
+    pos = planar(S.highdim_positions(96, 96, nfeat=14, sigma_s=12.0, seed=5))
+    s = S.Stream(6)
+    val = s.uniform(4 * 96 * 96).reshape(4, -1).astype(np.float32)
+    inv = np.ones(16, np.float32)
+    ctx = G.Context(0)
+    try:
+        first, st = ctx.gauss_transform(dev(pos), dev(val), inv, row_w=96, stats=True)
+        second = ctx.gauss_transform(dev(pos), dev(val), inv, row_w=96)
+        first, second = first.cpu().numpy(), second.cpu().numpy()
+    finally:
+        ctx.close()
+    assert st["M"] * 2 > 96 * 96 * 17   # sparse: the second call pairs its passes
+    assert np.array_equal(first, second)
+    o64, _ = O.gauss_transform(pos, val, inv, mode=O.F64_LAT)
+    assert rel_err(second, o64) < FP32_REL_TOL
+
+
 def test_zero_sigma_rows(gpu_ctx, oracle_lib):
     """sigma_y = 0 filters each row independently (src/GaussTransform.cpp:524-530)."""
     O = oracle_lib
