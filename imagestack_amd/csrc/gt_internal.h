@@ -242,12 +242,6 @@ hipError_t launch_slice_batch(int d, int storage, const PixSrc &tmpl, const Batc
                               int vd, const LatConst &lc, const KeyPlan &kp, const TableArgs &ta, const void *vals,
                               const unsigned *sbits, const OutDst &tmpl_dst, Counters *ctr, bool count, int wpc,
                               hipStream_t st);
-// A finalize that scans the compaction's chunk counts itself instead of a
-// separate one-workgroup scan (tables of at most kSelfScanChunks chunks):
-// each workgroup sums the counts of the chunks before its own; offs receives
-// those offsets (the PartMap then reads them), wctr the vertex count M (0
-// after a full table or over `limit`) and the filled count.  offs == nullptr:
-// the counts were scanned already.
 // Neighbour entries set to {-1, -1} after the finalize (nbr == nullptr: none).
 struct NbrInit {
     int2 *nbr;
@@ -256,6 +250,12 @@ struct NbrInit {
     int grid;     // vertices the fill's grid is sized for (it loops to M)
 };
 
+// A finalize that scans the compaction's chunk counts itself instead of a
+// separate one-workgroup scan (tables of at most kSelfScanChunks chunks):
+// each workgroup sums the counts of the chunks before its own; offs receives
+// those offsets (the PartMap then reads them), wctr the vertex count M (0
+// after a full table or over `limit`) and the filled count.  offs == nullptr:
+// the counts were scanned already.
 struct SelfScan {
     uint32_t *offs;
     Counters *wctr;

@@ -545,6 +545,7 @@ hipError_t launch_neighbors_hash(int d, bool packed, const KeyPlan &kp, const ui
                                  int m_max, int m_grid, const TableArgs &ta, const PartMap &pm, bool per_vertex,
                                  bool half, int2 *nbr, hipStream_t st) {
     if (m_max <= 0) return hipSuccess;
+    if (half && !per_vertex) return hipErrorInvalidValue;
     if (m_grid <= 0 || m_grid > m_max) m_grid = m_max;
     if (per_vertex) {
         const dim3 g1(vertex_blocks(m_grid, pm)), b1(kBlock);
