@@ -330,6 +330,7 @@ struct Launch {
     bool nbr_per_vertex;   // hash neighbours one thread per vertex (else per vertex and axis)
     bool nbr_half;         // ... looking up one sign and scattering the other (entries set to -1 by a fill)
     bool blur_pairs;       // blur passes two per launch
+    int nbr_fill_grid;     // one-end form: 0, the finalize fills the entries; > 0, a fill kernel's grid
     uint32_t cap;
     gt::LatConst lc;
     gt::KeyPlan kp;
@@ -375,7 +376,7 @@ int enqueue(gt_ctx *c, const Launch &L, const gt::PixSrc &src, const SliceSpec &
                                         (uint32_t *)c->counts.p + chunks_of(L.cap), L.vsc, L.vd, L.vs,
                                         (uint64_t *)c->vkeys.p, c->vals_a.p, ctr, L.m_max,
                                         gt::NbrInit{L.nbr_half ? (int2 *)c->nbr.p : nullptr, L.m_max, L.d + 1,
-                                                    L.m_grid},
+                                                    L.nbr_fill_grid},
                                         s));
     } else {
         GT_HIP(gt::launch_compact(L.W, ta, L.cap, (uint32_t *)c->counts.p, (uint32_t *)c->slot_list.p,
@@ -723,6 +724,12 @@ int run(gt_ctx *c, const gt::PixSrc &src, int n, int row_w, int d, int vd, int s
         // needed, in every blur pass); the margin keeps the table part with
         // the most vertices within one pass of its workgroups.
         L.m_grid = (int)std::min<uint64_t>((uint64_t)L.m_max, 2 * est);
+        // The one-end form's -1 fill: by the finalize's workgroups, each its
+        // chunk's ids, while the entries are few (2048^2 joint bilateral,
+        // 20 MB: 3 us there against a 7 us launch); by a kernel of its own,
+        // coalesced over whole rows, for large tables (d = 16, 2.4 GB: 0.3 ms
+        // less).
+        L.nbr_fill_grid = est * (d + 1) * 8 <= (64u << 20) ? 0 : std::max(L.m_grid, 1);
         L.pw = L.packed ? 1 : std::max(1, (L.kp.bits + 63) / 64);
         L.vs = value_stride(vd);
         const unsigned ncount = (cap + 2047) / 2048;

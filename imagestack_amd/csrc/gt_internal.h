@@ -242,12 +242,13 @@ hipError_t launch_slice_batch(int d, int storage, const PixSrc &tmpl, const Batc
                               int vd, const LatConst &lc, const KeyPlan &kp, const TableArgs &ta, const void *vals,
                               const unsigned *sbits, const OutDst &tmpl_dst, Counters *ctr, bool count, int wpc,
                               hipStream_t st);
-// Neighbour entries set to {-1, -1} after the finalize (nbr == nullptr: none).
+// Neighbour entries set to {-1, -1} with the finalize (nbr == nullptr: none).
 struct NbrInit {
     int2 *nbr;
     int stride;   // entries between axes
     int axes;
-    int grid;     // vertices the fill's grid is sized for (it loops to M)
+    int grid;     // > 0: by a fill kernel after the finalize, its grid sized for this many vertices
+                  // (it loops to M); 0: by the finalize's workgroups, each its chunk's ids
 };
 
 // A finalize that scans the compaction's chunk counts itself instead of a
@@ -281,7 +282,7 @@ hipError_t launch_neighbors(int d, bool packed, const KeyPlan &kp, const uint64_
 // offs: where the chunk offsets go when the table has at most
 // kSelfScanChunks chunks (the finalize scans the counts itself); else they
 // replace the counts in place.
-// ninit.nbr != nullptr: then the M vertices' neighbour entries are set to
+// ninit.nbr != nullptr: the M vertices' neighbour entries are also set to
 // {-1, -1} on each of ninit.axes axes (stride ninit.stride), for the
 // neighbour form that fills the second half of each entry by scatter.
 hipError_t launch_compact_final(int words, int storage, const TableArgs &ta, uint32_t capacity, uint32_t *counts,

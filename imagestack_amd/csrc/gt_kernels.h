@@ -643,7 +643,7 @@ __global__ __launch_bounds__(kBlock) void k_write_final(TableArgs ta, uint32_t c
                                                         const uint32_t *__restrict__ offsets, const Counters *ctr,
                                                         const ValScale *__restrict__ vsc, int vd, int vs,
                                                         uint64_t *__restrict__ vkeys, T *__restrict__ vals,
-                                                        unsigned *__restrict__ track, SelfScan sc) {
+                                                        unsigned *__restrict__ track, SelfScan sc, NbrInit ninit) {
     constexpr int NW = kBlock / kWave;
     const int wave = threadIdx.x / kWave;
     const int chunk = (int)blockIdx.x;
@@ -687,6 +687,7 @@ __global__ __launch_bounds__(kBlock) void k_write_final(TableArgs ta, uint32_t c
     float mx[kMaxVD];   // track: largest finite magnitude per channel (a narrow storage's first scale)
 #pragma unroll
     for (int c = 0; c < kMaxVD; c++) mx[c] = 0.0f;
+    const unsigned run0 = run;   // this chunk's ids are [run0, run at the end)
     const size_t base = (size_t)chunk * kCompactSlots;
     // Each thread's KI key words first (all in flight at once) and one
     // barrier for the ranks of all of them; then row by row, with the next
@@ -744,6 +745,12 @@ __global__ __launch_bounds__(kBlock) void k_write_final(TableArgs ta, uint32_t c
         }
         if (ta.tags && s < cap) ta.tags[s] = tg;
         run += tot;
+    }
+    if (ninit.nbr && ninit.grid == 0) {   // this chunk's neighbour entries, {-1, -1} on every axis
+        const u32x2 none = {~0u, ~0u};
+        for (int j = 0; j < ninit.axes; j++)
+            for (unsigned i = run0 + threadIdx.x; i < run; i += kBlock)
+                __builtin_nontemporal_store(none, reinterpret_cast<u32x2 *>(ninit.nbr) + (size_t)j * ninit.stride + i);
     }
     if (track) publish_max<kMaxVD>(track, mx, vd);
 }

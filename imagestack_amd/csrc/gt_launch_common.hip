@@ -477,11 +477,12 @@ __global__ __launch_bounds__(kBlock) void k_nbr_fill(int2 *__restrict__ nbr, int
 template <typename T>
 static void launch_write_final_t(int words, const TableArgs &ta, uint32_t capacity, unsigned nblk,
                                  const uint32_t *offsets, const Counters *ctr, const ValScale *vsc, int vd, int vs,
-                                 uint64_t *vkeys, void *vals, unsigned *track, const SelfScan &sc, hipStream_t st) {
+                                 uint64_t *vkeys, void *vals, unsigned *track, const SelfScan &sc,
+                                 const NbrInit &ninit, hipStream_t st) {
     T *v = (T *)vals;
 #define GT_WF(W)                                                                                                 \
     hipLaunchKernelGGL((k_write_final<W, T>), dim3(nblk), dim3(kBlock), 0, st, ta, capacity, offsets, ctr, vsc,  \
-                       vd, vs, vkeys, v, track, sc)
+                       vd, vs, vkeys, v, track, sc, ninit)
     switch (words) {
     case 1: GT_WF(1); break;
     case 2: GT_WF(2); break;
@@ -495,7 +496,7 @@ hipError_t launch_compact_final(int words, int storage, const TableArgs &ta, uin
                                 uint32_t *offs, const ValScale *vsc, int vd, int vs, uint64_t *vkeys, void *vals,
                                 Counters *ctr, int limit, const NbrInit &ninit, hipStream_t st) {
     if (words < 1 || words > 4) return hipErrorInvalidValue;
-    if (ninit.nbr && (ninit.stride < limit || ninit.axes < 1 || ninit.axes > kMaxD + 1 || ninit.grid < 1))
+    if (ninit.nbr && (ninit.stride < limit || ninit.axes < 1 || ninit.axes > kMaxD + 1 || ninit.grid < 0))
         return hipErrorInvalidValue;
     const unsigned nblk = blocks_for(capacity, kCompactSlots);
     switch (words) {
@@ -511,8 +512,9 @@ hipError_t launch_compact_final(int words, int storage, const TableArgs &ta, uin
                            &ctr->filled, (unsigned)limit);
     // FP32 rows for every storage (see launch_finalize)
     unsigned *track = storage == 0 ? nullptr : const_cast<unsigned *>(&vsc->vmax_bits[0][0]);
-    launch_write_final_t<float>(words, ta, capacity, nblk, counts, ctr, vsc, vd, vs, vkeys, vals, track, sc, st);
-    if (ninit.nbr)
+    launch_write_final_t<float>(words, ta, capacity, nblk, counts, ctr, vsc, vd, vs, vkeys, vals, track, sc, ninit,
+                                st);
+    if (ninit.nbr && ninit.grid > 0)
         hipLaunchKernelGGL(k_nbr_fill, dim3(blocks_for(ninit.grid, kBlock), ninit.axes), dim3(kBlock), 0, st,
                            ninit.nbr, ninit.stride, ctr);
     return hipGetLastError();
