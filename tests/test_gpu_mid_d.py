@@ -69,3 +69,25 @@ def test_hash_and_merge_neighbours_agree(gpu_ctx, oracle_lib, d):
     assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
     o64, _ = oracle_lib.gauss_transform(pos, val, inv, mode=oracle_lib.F64_LAT)
     assert rel_err(outs[1], o64) < FP32_REL_TOL
+
+
+def test_sparse_one_end_neighbours_with_fill_kernel(gpu_ctx, oracle_lib):
+    """A sparse d = 8 lattice large enough (about 9 vertices per pixel, 1.1 M
+    vertices) that, once its vertex density is known, the hash-id form looks
+    neighbours up from one end of each edge and fills the entries with a
+    kernel of its own: the same output, bit for bit, as sorted ids with
+    merged neighbours, and within the FP64 gate."""
+    print("\n[case] sparse d=8 352x352, one-end neighbours + fill kernel vs merge", flush=True)
+    W = H = 352
+    pos = S.highdim_positions(W, H, nfeat=6, sigma_s=2.0, seed=11).reshape(8, -1)
+    rng = np.random.default_rng(12)
+    val = np.concatenate([rng.uniform(size=(3, W * H)), np.ones((1, W * H))]).astype(np.float32)
+    inv = np.ones(8, np.float32)
+    first, st = gpu_ctx.gauss_transform(dev(pos), dev(val), inv, row_w=W, stats=True)   # learns the density
+    assert st["M"] * 8 * 9 > (64 << 20) // 1.05   # large enough for the fill kernel
+    hashed = gpu_ctx.gauss_transform(dev(pos), dev(val), inv, row_w=W, tune={"neighbours": 2}).cpu().numpy()
+    merged = gpu_ctx.gauss_transform(dev(pos), dev(val), inv, row_w=W, tune={"neighbours": 1}).cpu().numpy()
+    assert np.array_equal(hashed, merged)
+    assert np.array_equal(hashed, first.cpu().numpy())
+    o64, _ = oracle_lib.gauss_transform(pos, val, inv, mode=oracle_lib.F64_LAT)
+    assert rel_err(hashed, o64) < FP32_REL_TOL
