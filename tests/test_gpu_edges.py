@@ -61,6 +61,38 @@ def test_all_pixels_collide(gpu_ctx, oracle_lib):
     assert g["M"] == 4   # d + 1 vertices
 
 
+@pytest.mark.parametrize("d", [3, 5])
+def test_exact_ties_key_set(gpu_ctx, oracle_lib, d):
+    """Positions on the half-integer grid of the lattice's own scale
+    (a_i / (2 sf_i), a_i in -4..4): many elevate to exact ties -- greedy
+    roundings halfway between multiples of d+1 (ties go down,
+    src/Permutohedral.h:312) and equal differentials (the later index ranks
+    higher, :321-324) -- which show as vertices of weight exactly 0.  The
+    rank tie rule decides which zero-weight vertex is inserted: ranking ties
+    to the earlier index gives another key set for these positions (checked
+    with a restatement on the CPU), so the key set and the neighbour table
+    must match the oracle bit for bit.  (The greedy tie direction is undone
+    by the walk back onto the hyperplane here: both directions give the same
+    simplex.)"""
+    print(f"\n[case] exact-tie positions, d={d}", flush=True)
+    O = oracle_lib
+    sf = O.scale_factors(d).astype(np.float32)
+    rng = np.random.default_rng(40 + d)
+    n = 4096
+    a = rng.integers(-4, 5, size=(d, n)).astype(np.float32)
+    pos = (a / (np.float32(2.0) * sf[:, None])).astype(np.float32)
+    inv = np.ones(d, np.float32)
+    _, w = O.simplex_batch(pos, inv)
+    ties = int((w == 0).any(axis=1).sum())
+    print("pixels with a zero-weight vertex", ties, "of", n, flush=True)
+    assert ties > n // 8
+    val = np.ones((1, n), np.float32)
+    g = gpu_ctx.build_lattice(dev(pos), dev(val), inv)
+    o = O.lattice(pos, val, inv)
+    assert np.array_equal(g["keys"], o["keys"])
+    assert np.array_equal(g["nbr"], o["nbr"])
+
+
 def test_keys_beyond_int16(gpu_ctx):
     print("\n[case] keys beyond int16", flush=True)
     pos = np.zeros((3, 64), np.float32)

@@ -107,8 +107,10 @@ def test_tie_rules_match_reference_semantics(oracle_lib):
     giving g = (0, -2), sum = -1; the rank tie diff0 == diff1 increments the
     later index (:324), rank = (0, 1); the fix-up (:337-347) lifts coordinate
     0: g = (2, -2), rank = (1, 0).  Worked by hand: vertex 0 is key 2 with
-    weight 0 and vertex 1 is key 1 with weight 1.  Rounding ties up instead
-    would create a different zero-weight vertex (key 0)."""
+    weight 0 and vertex 1 is key 1 with weight 1.  (Rounding both ties up
+    gives g = (2, 0), sum = 1, and the walk back reaches the same g and rank;
+    ranking the diff tie to the earlier index instead gives rank (1, 0)
+    before the walk and another simplex.)"""
     sf = np.float32(oracle_lib.scale_factors(1)[0])
     p = np.float32(1.0) / sf
     found = None
