@@ -355,7 +355,7 @@ int enqueue(gt_ctx *c, const Launch &L, const gt::PixSrc &src, const SliceSpec &
     // this call's value bounds, then the table cleared and the bounds turned
     // into fixed-point scales, on the device
     const gt::ClearArgs cl{ta.tkeys, (size_t)L.cap * L.W, ta.tacc, L.clear_sums ? (size_t)L.cap * L.vd : 0, ta.tnf,
-                           L.clear_marks ? (size_t)L.cap : 0};
+                           L.clear_marks ? (size_t)L.cap : 0, ta.tags, ta.tags ? (size_t)L.cap : 0};
     if (batch)
         GT_HIP(gt::launch_value_scale_batch(cl, batch->ba, batch->ntiles, L.tile_w, batch->C, L.n, L.vd, L.vs_part,
                                             L.vs_nblk, L.vsc, ctr, s));

@@ -345,6 +345,16 @@ __device__ __forceinline__ unsigned insert_fail_flag(int slot) { return slot == 
 // Tag of a key with 64-bit hash h (TableArgs::tags): never 0 (empty).
 __device__ __forceinline__ uint8_t key_tag8(uint64_t h) { return (uint8_t)(0x80u | (uint32_t)(h >> 57)); }
 
+// The tag of key k at its slot (tables with tags: wide keys), written by
+// every insert that found or claimed the slot (the same byte each time), so
+// that the count and the finalize can read a slot's occupancy from one byte.
+template <int W>
+__device__ __forceinline__ void write_tag(const TableArgs &ta, int slot, const uint64_t (&k)[W]) {
+    if constexpr (W > 1) {
+        if (ta.tags && slot >= 0) ta.tags[slot] = key_tag8(hash_key<W>(k));
+    }
+}
+
 // Lookup through the tags (TableArgs::tags) of a complete table, from home
 // slot s: eight tags at a time (two aligned 8-byte loads), the empty ones and
 // the ones equal to `want` found by byte-parallel masks, key words compared

@@ -214,6 +214,8 @@ struct ClearArgs {
     size_t nacc;
     uint32_t *tnf;
     size_t nmarks;
+    uint8_t *tags;   // the per-slot tags (nullptr: none), ntags bytes, a multiple of 4
+    size_t ntags;
 };
 // The start of a lattice call in two launches: the largest finite magnitude
 // and a non-finite indicator of each of the nval value planes (n pixels

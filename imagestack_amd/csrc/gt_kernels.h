@@ -275,6 +275,7 @@ __device__ __forceinline__ void mark_nonfinite(const Simplex<D> &s, const KeyPla
         if (!K::make(s, r, kp, tag, k)) continue;   // reported by the caller
         const int slot = table_insert<W>(ta.tkeys, ta.mask, k, home_slot<D, PACKED, BATCH>(ta, kp, k), probes);
         if (slot < 0) { flags |= insert_fail_flag(slot); continue; }
+        write_tag<W>(ta, slot, k);
         uint32_t bits = 0;
         for (int c = 0; c < vd; c++) {
             if (!((nfmask >> c) & 1u)) continue;
@@ -413,6 +414,7 @@ __global__ __launch_bounds__(kBlock) void k_splat_tile(PixSrc src0, int n0, int 
                     inserts++;
                     if (rp.rec) rp.gs[r * rp.stride + rq] = (uint32_t)slot;
                     if (slot < 0) { flags |= insert_fail_flag(slot); continue; }
+                    write_tag<W>(ta, slot, k);
                     unsigned long long *dst = ta.tacc + (size_t)slot * vd;
 #pragma unroll
                     for (int c = 0; c < LVD; c++)
@@ -451,6 +453,7 @@ __global__ __launch_bounds__(kBlock) void k_splat_tile(PixSrc src0, int n0, int 
         const int slot = table_insert<W>(ta.tkeys, ta.mask, k, home_slot<D, PACKED, BATCH>(ta, kp, k), probes);
         inserts++;
         if (slot < 0) flags |= insert_fail_flag(slot);
+        write_tag<W>(ta, slot, k);
         lk[ls * W] = (uint64_t)(int64_t)slot;
         if (rp.rec) rp.tent[(size_t)tile * LT + j] = make_uint2((unsigned)ls, (unsigned)slot);
     }
@@ -522,6 +525,7 @@ __global__ __launch_bounds__(kBlock) void k_splat_direct(PixSrc src, int n, int 
                 const int slot = table_insert<W>(ta.tkeys, ta.mask, k, home_slot<D, PACKED, false>(ta, kp, k), probes);
                 inserts++;
                 if (slot < 0) { flags |= insert_fail_flag(slot); continue; }
+                write_tag<W>(ta, slot, k);
                 unsigned long long *dst = ta.tacc + (size_t)slot * vd;
 #pragma unroll
                 for (int c = 0; c < kMaxVD; c++)
@@ -538,14 +542,25 @@ __global__ __launch_bounds__(kBlock) void k_splat_direct(PixSrc src, int n, int 
 // compaction: occupied slots -> list, in slot order, by count / scan / write.
 template <int W>
 __global__ __launch_bounds__(kBlock) void k_count(const uint64_t *__restrict__ tkeys, uint32_t cap,
-                                                  uint32_t *__restrict__ counts) {
+                                                  uint32_t *__restrict__ counts, const uint8_t *__restrict__ tags) {
     __shared__ unsigned ws[kBlock / kWave];
     const size_t base = (size_t)blockIdx.x * kCompactSlots;
     unsigned c = 0;
+    if (tags) {   // occupancy from the tags: one byte per slot instead of a key line
 #pragma unroll
-    for (int k = 0; k < kCompactSlots / kBlock; k++) {
-        const size_t s = base + (size_t)k * kBlock + threadIdx.x;
-        c += (s < cap && tkeys[s * W] != kEmpty) ? 1u : 0u;
+        for (int k = 0; k < kCompactSlots / (4 * kBlock); k++) {
+            const size_t s = base + 4 * ((size_t)k * kBlock + threadIdx.x);
+            if (s < cap) {
+                const uint32_t t = *reinterpret_cast<const uint32_t *>(tags + s);
+                c += (unsigned)__popc(t & 0x80808080u);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < kCompactSlots / kBlock; k++) {
+            const size_t s = base + (size_t)k * kBlock + threadIdx.x;
+            c += (s < cap && tkeys[s * W] != kEmpty) ? 1u : 0u;
+        }
     }
     c = wave_sum(c);
     if (__lane_id() == 0) ws[threadIdx.x / kWave] = c;
@@ -697,9 +712,9 @@ __global__ __launch_bounds__(kBlock) void k_write_final(TableArgs ta, uint32_t c
     __shared__ unsigned wsk[KI][NW];
     uint64_t w0[KI], mk[KI];
 #pragma unroll
-    for (int k = 0; k < KI; k++) {
+    for (int k = 0; k < KI; k++) {   // (with tags: an occupied slot's tag is set, so only those keys are read)
         const size_t s = base + (size_t)k * kBlock + threadIdx.x;
-        w0[k] = s < cap ? ta.tkeys[s * W] : kEmpty;
+        w0[k] = (s < cap && (!ta.tags || ta.tags[s] != 0)) ? ta.tkeys[s * W] : kEmpty;
     }
 #pragma unroll
     for (int k = 0; k < KI; k++) {
@@ -725,7 +740,6 @@ __global__ __launch_bounds__(kBlock) void k_write_final(TableArgs ta, uint32_t c
             off += (w < wave) ? wsk[k][w] : 0u;
             tot += wsk[k][w];
         }
-        uint8_t tg = 0;
         if (w0[k] != kEmpty) {
             const unsigned pos = off + (unsigned)__popcll(mk[k] & lanemask_lt());
             uint64_t kw[W];
@@ -734,7 +748,7 @@ __global__ __launch_bounds__(kBlock) void k_write_final(TableArgs ta, uint32_t c
             for (int w = 1; w < W; w++) kw[w] = ta.tkeys[s * W + w];
 #pragma unroll
             for (int w = 0; w < W; w++) vkeys[(size_t)pos * W + w] = kw[w];
-            if (ta.tags) tg = key_tag8(hash_key<W>(kw));
+
             float o[kMaxVD];
             if (pre) take_sums4(ta, vsc, s, vd, acc[k & 1], o);
             else take_sums(ta, vsc, s, vd, o);
@@ -743,7 +757,6 @@ __global__ __launch_bounds__(kBlock) void k_write_final(TableArgs ta, uint32_t c
             if (W == 1 && ta.idshift > 0) ta.tkeys[s] = w0[k] | ((uint64_t)pos << ta.idshift);
             else ta.slot_id[s] = (int)pos;
         }
-        if (ta.tags && s < cap) ta.tags[s] = tg;
         run += tot;
     }
     if (ninit.nbr && ninit.grid == 0) {   // this chunk's neighbour entries, {-1, -1} on every axis

@@ -289,6 +289,7 @@ __global__ __launch_bounds__(kBlock) void k_clear_scale(ClearArgs cl, Counters *
     for (size_t i = t0; i < cl.nwords; i += stride) cl.tkeys[i] = kEmpty;
     for (size_t i = t0; i < cl.nacc; i += stride) cl.tacc[i] = 0ull;
     for (size_t i = t0; i < cl.nmarks; i += stride) cl.tnf[i] = 0u;
+    for (size_t i = t0; i < cl.ntags / 4; i += stride) reinterpret_cast<uint32_t *>(cl.tags)[i] = 0u;
     if (blockIdx.x != 0) return;
     if (threadIdx.x == 0) {
         ctr->M = 0;
@@ -410,10 +411,10 @@ hipError_t launch_compact(int words, const TableArgs &ta, uint32_t capacity, uin
                           uint64_t *sort_keys, bool packed, Counters *ctr, int limit, hipStream_t st) {
     const unsigned nblk = blocks_for(capacity, kCompactSlots);
     switch (words) {
-    case 1: hipLaunchKernelGGL(k_count<1>, dim3(nblk), dim3(kBlock), 0, st, ta.tkeys, capacity, counts); break;
-    case 2: hipLaunchKernelGGL(k_count<2>, dim3(nblk), dim3(kBlock), 0, st, ta.tkeys, capacity, counts); break;
-    case 3: hipLaunchKernelGGL(k_count<3>, dim3(nblk), dim3(kBlock), 0, st, ta.tkeys, capacity, counts); break;
-    case 4: hipLaunchKernelGGL(k_count<4>, dim3(nblk), dim3(kBlock), 0, st, ta.tkeys, capacity, counts); break;
+    case 1: hipLaunchKernelGGL(k_count<1>, dim3(nblk), dim3(kBlock), 0, st, ta.tkeys, capacity, counts, nullptr); break;
+    case 2: hipLaunchKernelGGL(k_count<2>, dim3(nblk), dim3(kBlock), 0, st, ta.tkeys, capacity, counts, nullptr); break;
+    case 3: hipLaunchKernelGGL(k_count<3>, dim3(nblk), dim3(kBlock), 0, st, ta.tkeys, capacity, counts, nullptr); break;
+    case 4: hipLaunchKernelGGL(k_count<4>, dim3(nblk), dim3(kBlock), 0, st, ta.tkeys, capacity, counts, nullptr); break;
     default: return hipErrorInvalidValue;
     }
     hipLaunchKernelGGL(k_scan_1wg, dim3(1), dim3(1024), 0, st, counts, (int)nblk, &ctr->M, &ctr->flags,
@@ -500,10 +501,10 @@ hipError_t launch_compact_final(int words, int storage, const TableArgs &ta, uin
         return hipErrorInvalidValue;
     const unsigned nblk = blocks_for(capacity, kCompactSlots);
     switch (words) {
-    case 1: hipLaunchKernelGGL(k_count<1>, dim3(nblk), dim3(kBlock), 0, st, ta.tkeys, capacity, counts); break;
-    case 2: hipLaunchKernelGGL(k_count<2>, dim3(nblk), dim3(kBlock), 0, st, ta.tkeys, capacity, counts); break;
-    case 3: hipLaunchKernelGGL(k_count<3>, dim3(nblk), dim3(kBlock), 0, st, ta.tkeys, capacity, counts); break;
-    default: hipLaunchKernelGGL(k_count<4>, dim3(nblk), dim3(kBlock), 0, st, ta.tkeys, capacity, counts); break;
+    case 1: hipLaunchKernelGGL(k_count<1>, dim3(nblk), dim3(kBlock), 0, st, ta.tkeys, capacity, counts, ta.tags); break;
+    case 2: hipLaunchKernelGGL(k_count<2>, dim3(nblk), dim3(kBlock), 0, st, ta.tkeys, capacity, counts, ta.tags); break;
+    case 3: hipLaunchKernelGGL(k_count<3>, dim3(nblk), dim3(kBlock), 0, st, ta.tkeys, capacity, counts, ta.tags); break;
+    default: hipLaunchKernelGGL(k_count<4>, dim3(nblk), dim3(kBlock), 0, st, ta.tkeys, capacity, counts, ta.tags); break;
     }
     SelfScan sc{nullptr, ctr, (int)nblk, (unsigned)limit};
     if (nblk <= (unsigned)kSelfScanChunks) sc.offs = offs;
