@@ -71,7 +71,8 @@ typedef struct {
     int waves_per_cu;    /* resident waves per CU of the tiled and neighbour kernels:
                             12..64 caps occupancy through LDS; 0 = no cap         */
     int dedup;           /* LDS tile dedup / pre-reduction: 0 auto, 1 on, -1 off     */
-    int tile_w;          /* pixel-tile width of a workgroup: 16, 32 (default), 64    */
+    int tile_w;          /* pixel-tile width of a workgroup: 16, 32, 64; 0 auto
+                            (32, or 16 for lattices of 7 or more dimensions)    */
     int graph;           /* 0: replay the pipeline as a cached hipGraph; -1: don't  */
     int neighbours;      /* vertex ids and neighbour table: 0 auto (hash with row-band
                             table parts, else for d >= 11 or an expected lattice
