@@ -237,10 +237,30 @@ __device__ void value_scale_wg(const float *__restrict__ partial, int nblk, int 
     for (int c = threadIdx.x / kWave; c < kMaxDH; c += kBlock / kWave) {
         float vmax = 0.0f, nf = 0.0f;
         if (c < nval) {
-            for (int b = lane; b < nblk; b += kWave) {
-                const float *o = partial + ((size_t)b * nval + c) * 3;
-                if (o[0] <= o[1]) vmax = fmaxf(vmax, fmaxf(-o[0], o[1]));
-                nf = fmaxf(nf, o[2]);
+            // sixteen workgroups' bounds per lane in flight at once: 1024
+            // partials are one round of loads, not sixteen
+            constexpr int U = 16;
+            for (int b0 = lane; b0 < nblk; b0 += U * kWave) {
+                float lo[U], hi[U], fl[U];
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    const int b = b0 + u * kWave;
+                    if (b < nblk) {
+                        const float *o = partial + ((size_t)b * nval + c) * 3;
+                        lo[u] = o[0];
+                        hi[u] = o[1];
+                        fl[u] = o[2];
+                    } else {
+                        lo[u] = 1.0f;   // min > max: no finite value
+                        hi[u] = 0.0f;
+                        fl[u] = 0.0f;
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    if (lo[u] <= hi[u]) vmax = fmaxf(vmax, fmaxf(-lo[u], hi[u]));
+                    nf = fmaxf(nf, fl[u]);
+                }
             }
         } else if (c == nval && homog) {
             vmax = 1.0f;
