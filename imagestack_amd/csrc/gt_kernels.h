@@ -667,10 +667,20 @@ __global__ __launch_bounds__(kBlock) void k_write_final(TableArgs ta, uint32_t c
         // offsets holds the chunks' counts: this chunk's offset and the total
         __shared__ unsigned spre[NW], stot[NW];
         unsigned pre = 0, tot = 0;
-        for (int i = threadIdx.x; i < sc.nchunks; i += kBlock) {
-            const unsigned v = offsets[i];
-            tot += v;
-            pre += i < chunk ? v : 0u;
+        {   // all of a thread's counts in flight at once (nchunks <= kSelfScanChunks)
+            constexpr int U = kSelfScanChunks / kBlock;
+            unsigned v[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int i = (int)threadIdx.x + u * kBlock;
+                v[u] = i < sc.nchunks ? offsets[i] : 0u;
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int i = (int)threadIdx.x + u * kBlock;
+                tot += v[u];
+                pre += i < chunk ? v[u] : 0u;
+            }
         }
         pre = wave_sum(pre);
         tot = wave_sum(tot);
@@ -1608,16 +1618,32 @@ __global__ __launch_bounds__(kBlock) void k_slice_replay(PixSrc src0, int n0, in
         load_position<D, BILAT>(tv.src, p, px, prow, pos);
     }
     const unsigned cnt = min(rp.tcnt[tile], (unsigned)LT);
-    for (unsigned j = threadIdx.x; j < cnt; j += kBlock) {
-        const uint2 te = rp.tent[(size_t)tile * LT + j];
-        if (te.x >= (unsigned)LT) continue;
-        const int id = te.y <= ta.mask ? slot_vertex(ta, te.y) : -1;
-        float row[kMaxVD];
+    {   // a thread's entries (LT / kBlock) step by step together: three
+        // rounds of dependent loads (entry, id, row) instead of three each
+        constexpr int SU = (LT + kBlock - 1) / kBlock;
+        uint2 te[SU];
+        int id[SU];
 #pragma unroll
-        for (int c = 0; c < LVD; c++) row[c] = 0.0f;
-        if (id >= 0 && (unsigned)id < M) load_row<T, LVD>(vals, id, row);
+        for (int u = 0; u < SU; u++) {
+            const unsigned j = threadIdx.x + (unsigned)(u * kBlock);
+            te[u] = j < cnt ? rp.tent[(size_t)tile * LT + j] : make_uint2((unsigned)LT, 0u);
+        }
 #pragma unroll
-        for (int c = 0; c < LVD; c++) lv[c * LT + te.x] = kScaled ? row[c] * un[c] : row[c];
+        for (int u = 0; u < SU; u++)
+            id[u] = (te[u].x < (unsigned)LT && te[u].y <= ta.mask) ? slot_vertex(ta, te[u].y) : -1;
+        float row[SU][kMaxVD];
+#pragma unroll
+        for (int u = 0; u < SU; u++) {
+#pragma unroll
+            for (int c = 0; c < LVD; c++) row[u][c] = 0.0f;
+            if (id[u] >= 0 && (unsigned)id[u] < M) load_row<T, LVD>(vals, id[u], row[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < SU; u++) {
+            if (te[u].x >= (unsigned)LT) continue;
+#pragma unroll
+            for (int c = 0; c < LVD; c++) lv[c * LT + te[u].x] = kScaled ? row[u][c] * un[c] : row[u][c];
+        }
     }
     Simplex<D> s;
     if (p >= 0) simplex<D>(pos, lc.sf, s);   // (before the barrier: overlaps the staging)
