@@ -1,6 +1,6 @@
 // Route 1 check: ImageStack with the in-process GPU method case
 // (GaussTransform::PERMUTOHEDRAL_GPU, src/GaussTransform.h:8 and
-// src/GaussTransform.cpp:134, :566, patched as INTEGRATION.md shows) calls
+// src/GaussTransform.cpp:134, :566, patched by the Makefile's sed recipe) calls
 // the library from inside GaussTransform::apply and JointBilateral::apply.
 // This driver runs both methods on the same images -- the reference's own
 // PERMUTOHEDRAL on the CPU and the patched PERMUTOHEDRAL_GPU -- and prints

@@ -3,7 +3,7 @@
 // (src/GaussTransform.cpp:227-297, taps = 5).
 //
 // TEST INFRASTRUCTURE ONLY: the oracle of the GPU grid path (tests/, the smoke
-// check and bench.py's CPU leg call it as the checker); the product library
+// check and bench.py's CPU leg call it to check results); the product library
 // never links it.  Included by lattice_oracle.cpp and compiled with the same
 // strict flags (-O2 -ffp-contract=off, no fast math).
 //

@@ -1,7 +1,7 @@
 // CPU restatement of ImageStack's permutohedral-lattice Gauss transform.
 //
 // TEST INFRASTRUCTURE ONLY.  This file is the oracle: tests/, the smoke check
-// and bench.py's cpu_baseline leg call it as the checker (and as the scalar
+// and bench.py's cpu_baseline leg call it to check results (and as the scalar
 // CPU baseline).  The product path (imagestack_amd/libgtlattice.so) never
 // links or calls it.
 //

@@ -1,7 +1,7 @@
 """ctypes bindings for the test oracle (liboracle.so, our CPU restatement of
 the reference lattice) and for the reference itself built from its sources
 (oracle/_ref/libref_*.so).  TEST INFRASTRUCTURE ONLY: imported by tests/, the
-smoke check and bench.py's cpu_baseline leg as the checker, never by the
+smoke check and bench.py's cpu_baseline leg to check results, never by the
 product path.
 
 Arrays are dense planar numpy float32 images of shape (C, F, H, W) or (C, N).

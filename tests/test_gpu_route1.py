@@ -1,9 +1,10 @@
 """Route 1: the in-process GaussTransform::PERMUTOHEDRAL_GPU method case.
 
-The build patches the reference's GaussTransform.{h,cpp} as INTEGRATION.md
-shows (src/GaussTransform.h:8, src/GaussTransform.cpp:134 and :566) and links
+The build patches the reference's GaussTransform.{h,cpp} with a sed recipe in
+imagestack_amd/Makefile (src/GaussTransform.h:8, src/GaussTransform.cpp:134
+and :566) and links
 imagestack_amd/host/route1_check.cpp with the reference's objects and
-libgtlattice.so.  The driver compares GaussTransform::apply and
+libgtlattice.so.  That program compares GaussTransform::apply and
 JointBilateral::apply (with and without a zero spatial sigma) between the
 reference's PERMUTOHEDRAL and the patched PERMUTOHEDRAL_GPU on the same images.
 """
