@@ -90,31 +90,47 @@ __global__ __launch_bounds__(kBlock) void k_minmax_final(const float *__restrict
     }
 }
 
-// Bounds of a batch, one tile per workgroup iteration: per guide channel the
-// min/max of ref * ic (the scaled colour position, ic per image) and per image
-// channel the min/max of the value, each with a non-finite indicator, as
-// partial[blk][plane][3] for k_minmax_final.
+// Bounds of a batch: per guide channel the min/max of ref * ic (the scaled
+// colour position, ic per image) and per image channel the min/max of the
+// value, each with a non-finite indicator, as partial[blk][plane][3] for
+// k_minmax_final.  A workgroup's tiles rise, so the image of its next tile
+// is found by stepping forward from the last one (the last image whose first
+// tile is at or before it), and eight tiles' loads are in flight at a time
+// (64 images of 30.8 M pixels: 488 -> 358 us against one tile and one binary
+// search per step).
 __global__ __launch_bounds__(kBlock) void k_bounds_batch(BatchArgs ba, int ntiles, int tile_w, int Cr, int C,
                                                          float *__restrict__ partial) {
     __shared__ float red[3][kBlock / kWave];
     const int np = Cr + C;
+    constexpr int U = 8;
     for (int c = 0; c < np; c++) {
         float mn = INFINITY, mx = -INFINITY, nf = 0.0f;
-        for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-            int lo = 0, hi = ba.count - 1;
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (ba.img[mid].tile0 <= tile) lo = mid;
-                else hi = mid - 1;
+        int img = 0;   // tile 0 starts image 0
+        for (int t0 = blockIdx.x; t0 < ntiles; t0 += U * gridDim.x) {
+            float v[U];
+            bool have[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int tile = t0 + u * (int)gridDim.x;
+                have[u] = false;
+                v[u] = 0.0f;
+                if (tile < ntiles) {
+                    while (img + 1 < ba.count && ba.img[img + 1].tile0 <= tile) img++;
+                    const BatchImage &bi = ba.img[img];
+                    const int n = bi.W * bi.H;
+                    const int p = pixel_index(tile - bi.tile0, threadIdx.x, n, bi.W, tile_w);
+                    if (p >= 0) {
+                        have[u] = true;
+                        v[u] = c < Cr ? bi.ref[(int64_t)c * n + p] * bi.ic : bi.im[(int64_t)(c - Cr) * n + p];
+                    }
+                }
             }
-            const BatchImage &bi = ba.img[lo];
-            const int n = bi.W * bi.H;
-            const int p = pixel_index(tile - bi.tile0, threadIdx.x, n, bi.W, tile_w);
-            if (p >= 0) {
-                const float v = c < Cr ? bi.ref[(int64_t)c * n + p] * bi.ic : bi.im[(int64_t)(c - Cr) * n + p];
-                if (isfinite(v)) {
-                    mn = fminf(mn, v);
-                    mx = fmaxf(mx, v);
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                if (!have[u]) continue;
+                if (isfinite(v[u])) {
+                    mn = fminf(mn, v[u]);
+                    mx = fmaxf(mx, v[u]);
                 } else {
                     nf = 1.0f;
                 }
